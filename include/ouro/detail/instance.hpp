@@ -1,0 +1,1042 @@
+// One Ouroboros instance: NQ size classes over chunks of SMALLEST << (NQ-1) bytes, with one of the
+// six queue organisations. Device side only; the host half lives in ouroboros.hpp.
+//
+// Every routine here is called by all active lanes of a wavefront together ("wave op"). Work that
+// touches shared counters is grouped by match<G> and issued once per group by the group's lowest
+// lane, so results inside one wave are assigned in lane order. G is the aggregation width:
+//   W = 1  : the reference's own aggregation only. BulkSemaphore::wait groups a warp
+//            (BulkSemaphore_impl.cuh:207-299) and atomicAggInc groups front_/back_
+//            (Utility.cuh:76-95); those become 64-lane groups. Everything else is per lane:
+//            semaphore signals, chunk-queue back_/count_, ChunkAccess claims and releases.
+//   W = 16 : everything grouped inside 16-lane segments.
+//   W = 64 : everything grouped across the wavefront, chunk pages claimed and released in
+//            64-bit mask words, one semaphore signal per class group.
+#pragma once
+#include "device.hpp"
+#include "layout.hpp"
+
+namespace ouro {
+
+template <Kind K, uint32_t SMALLEST, uint32_t NQ>
+struct Instance {
+  static constexpr Kind kKind = K;
+  static constexpr uint32_t kSmallest = SMALLEST;
+  static constexpr uint32_t kNQ = NQ;
+  static constexpr uint32_t kSmallLog2 = log2_floor(SMALLEST);
+  static constexpr uint32_t kLargest = SMALLEST << (NQ - 1);       // LargestPageSize_
+  static constexpr uint32_t kChunkSize = kLargest;                  // ChunkSize_ (Ouroboros.cuh:61)
+  static constexpr uint32_t kAdd = kChunkSize / kBaseChunk;         // ChunkAddFactor_ (:62)
+  static constexpr bool kIsBase = kAdd == 1;
+  static constexpr bool kChunks = is_chunk_kind(K);
+  static_assert(kChunkSize % kBaseChunk == 0, "instance chunk must be a multiple of the base chunk");
+  static_assert(kChunkSize / SMALLEST <= 512, "at most 512 pages per chunk (9 page bits)");
+
+  ReuseRing reuse;   // d_chunk_reuse_queue
+  ClassQueue q[NQ];  // d_storage_reuse_queue
+};
+
+namespace detail {
+
+using namespace dev;
+
+__device__ __forceinline__ void set_error(Memory& m, uint32_t bits) { or_(&m.error, bits); }
+
+__device__ __forceinline__ uint8_t* chunk_addr(const Memory& m, uint32_t c) {
+  return m.d_data + (uint64_t)c * kStride;
+}
+template <class T>
+__device__ __forceinline__ T* hdr(const Memory& m, uint32_t c) {
+  return reinterpret_cast<T*>(chunk_addr(m, c));
+}
+__device__ __forceinline__ uint32_t* qc_slots(const Memory& m, uint32_t c) {
+  return reinterpret_cast<uint32_t*>(chunk_addr(m, c) + kMeta);
+}
+
+// A chunk index read back from device memory is only dereferenced when it names a pool chunk.
+__device__ __forceinline__ bool chunk_ok(const Memory& m, uint32_t c) { return c < m.max_chunks; }
+
+// Broadcast from the group leader of each lane (every lane of the wave must be active).
+__device__ __forceinline__ int leader_of(uint64_t grp) { return first(grp); }
+
+// ------------------------------------------------------------------ reuse ring (Queue_impl.cuh)
+// IndexQueue::enqueue (Queue_impl.cuh:20-38) for every lane with want=true, positions in lane order.
+__device__ inline void ring_put(Memory& m, ReuseRing& r, bool want, uint32_t value) {
+  const uint64_t mask = ballot(want);
+  if (!mask) return;
+  const int l = first(mask);
+  const uint32_t n = popc(mask);
+  uint32_t fill = 0, pos = 0;
+  if ((int)lane() == l) {
+    const int32_t f = add(&r.count, (int32_t)n);
+    const int32_t room = (int32_t)r.size - f;
+    const uint32_t acc = room <= 0 ? 0u : (room >= (int32_t)n ? n : (uint32_t)room);
+    if (acc < n) {
+      sub(&r.count, (int32_t)(n - acc));
+      set_error(m, ERR_REUSE_FULL);
+    }
+    fill = acc;
+    pos = acc ? add(&r.back, acc) : 0u;
+  }
+  fill = readlane(fill, l);
+  pos = readlane(pos, l);
+  if (want) {
+    const uint32_t rk = rank_in(mask);
+    if (rk < fill) {
+      uint32_t* slot = &r.ring[(pos + rk) & (r.size - 1)];
+      Spin it;
+      while (cas(slot, kMarker, value) != kMarker) {
+        if (it.expired()) { set_error(m, ERR_TIMEOUT); break; }
+        backoff(it.it);
+      }
+    }
+  }
+}
+
+// IndexQueue::dequeue (Queue_impl.cuh:54-67) for up to k entries, issued by lane `l` (uniform).
+// Returns the number obtained; the j-th active lane (j < result) receives entry j in `out`.
+__device__ inline uint32_t ring_take(Memory& m, ReuseRing& r, uint32_t k, int l, uint32_t me, uint32_t& out) {
+  uint32_t t = 0, pos = 0;
+  if ((int)lane() == l) {
+    const int32_t old = sub(&r.count, (int32_t)k);
+    t = old <= 0 ? 0u : ((uint32_t)old >= k ? k : (uint32_t)old);
+    if (t < k) add(&r.count, (int32_t)(k - t));
+    pos = t ? add(&r.front, t) : 0u;
+  }
+  t = readlane(t, l);
+  pos = readlane(pos, l);
+  if (me < t) {
+    uint32_t* slot = &r.ring[(pos + me) & (r.size - 1)];
+    Spin it;
+    uint32_t v;
+    while ((v = xchg(slot, kMarker)) == kMarker) {
+      if (it.expired()) { set_error(m, ERR_TIMEOUT); break; }
+      backoff(it.it);
+    }
+    out = v;
+  }
+  return t;
+}
+
+// allocateChunk<QUEUECHUNK> (Ouroboros.cuh:108-136) for k chunks at once: the reuse ring first
+// (FIFO), then the bump pointer, exactly the order k sequential calls would produce. The j-th active
+// lane (j < k) receives chunk j. Validity follows the reference's predicate chunk + add < maxChunks
+// (:133); unlike the reference, an invalid chunk is never handed out. Returns the valid prefix length.
+__device__ inline uint32_t alloc_chunks(Memory& m, ReuseRing& r, uint32_t add_factor, uint32_t k, uint32_t& chunk) {
+  const uint64_t act = active();
+  const uint32_t me = rank_in(act);
+  const int l = first(act);
+  uint32_t got = 0;
+  const uint32_t t = ring_take(m, r, k, l, me, got);
+  uint32_t base = 0;
+  if ((int)lane() == l && k > t) base = add(&m.next_free_chunk, (k - t) * add_factor);
+  base = readlane(base, l);
+  bool valid = false;
+  if (me < t) {
+    chunk = got;
+    valid = chunk_ok(m, got);
+  } else if (me < k) {
+    chunk = base + (me - t) * add_factor;
+    valid = (uint64_t)chunk + add_factor < (uint64_t)m.max_chunks;
+    if (valid) or_(&m.locator[chunk >> 5], 1u << (chunk & 31u));  // ChunkLocator::initChunkIndex
+  }
+  const uint64_t okm = ballot(me < k && valid);
+  // valid chunks form a prefix in active-rank order
+  const uint32_t k_ok = popc(okm);
+  if (k_ok < k && (int)lane() == l) set_error(m, ERR_OOM);
+  return k_ok;
+}
+
+// Lane id of the j-th active lane.
+__device__ __forceinline__ int lane_of_rank(uint64_t act, uint32_t j) { return (int)nth_set_bit(act, j); }
+
+// ------------------------------------------------------------------ queue chunks (VA / VL)
+// QueueChunk::initializeChunk (QueueChunk.cuh:221-225) plus a wave-cooperative clean of its 2048
+// slots (d_cleanChunks, MemoryInitialization.cuh:21-38, does this for the whole pool at start-up).
+__device__ inline void qc_init_lanes(Memory& m, bool mine, uint32_t qc, uint32_t vstart) {
+  const uint64_t act = active();
+  const uint32_t me = rank_in(act), nact = popc(act);
+  uint64_t todo = ballot(mine);
+  while (todo) {
+    const int l = first(todo);
+    todo &= todo - 1;
+    const uint32_t c = readlane(qc, l);
+    uint32_t* s = qc_slots(m, c);
+    for (uint32_t i = me; i < kQCSpots; i += nact) st(&s[i], kMarker);
+  }
+  if (mine) {
+    QueueChunkHdr* h = hdr<QueueChunkHdr>(m, qc);
+    st(&h->page_size, 0u);
+    st(&h->count, 0u);
+    st(&h->virtual_start, vstart);
+    st(&h->next, kMarker);
+    st(&h->chunk_index, qc);
+  }
+  order();
+}
+
+// Empty <=> A == 2048 and B == 0 (QueueChunk.cuh:185-203).
+__device__ __forceinline__ bool qc_empty_after_enqueue(uint32_t old, uint32_t added) {
+  return old + added == kQCSpots;
+}
+__device__ __forceinline__ bool qc_empty_after_dequeue(uint32_t old) { return old - (1u << 16) == kQCSpots; }
+
+template <class I>
+struct Ctx {
+  Memory& m;
+  I& inst;
+  ReuseRing& base_ring;  // d_base_chunk_reuse_queue: queue chunks always come from the base ring
+};
+
+// ---- VL pointer maintenance (QueueChunk_impl.cuh:440-551)
+template <class I>
+__device__ inline void vl_set_back(Ctx<I>& x, ClassQueue* cq, uint32_t c) {
+  Spin it;
+  while (true) {
+    QueueChunkHdr* h = hdr<QueueChunkHdr>(x.m, c);
+    const uint32_t nx = ld(&h->next);
+    if (!chunk_ok(x.m, nx)) break;
+    if (cas(&cq->back_qc, c, nx) != c) break;
+    if ((ld(&hdr<QueueChunkHdr>(x.m, nx)->count) & 0xFFFFu) == kQCSpots) c = nx;
+    else break;
+    if (it.expired()) { set_error(x.m, ERR_TIMEOUT); break; }
+  }
+}
+template <class I>
+__device__ inline uint32_t vl_set_front(Ctx<I>& x, ClassQueue* cq, uint32_t c) {
+  uint32_t moved = 0;
+  while (true) {
+    QueueChunkHdr* h = hdr<QueueChunkHdr>(x.m, c);
+    const uint32_t nx = ld(&h->next);
+    if (!chunk_ok(x.m, nx)) break;
+    if (cas(&cq->front_qc, c, nx) != c) break;
+    ++moved;
+    if (ld(&hdr<QueueChunkHdr>(x.m, nx)->count) == kQCSpots) c = nx;
+    else break;
+  }
+  return moved;
+}
+// setOldPointer: hold back the LARGEST_OLD_COUNT_VALUE most recently emptied queue chunks, recycle
+// the rest through the base reuse ring. Wave-uniform loop so ring_put stays lane ordered.
+template <class I>
+__device__ inline void vl_set_old(Ctx<I>& x, ClassQueue* cq, bool want, uint32_t free_count) {
+  if (want && free_count) {
+    uint32_t cur_old = ld(&cq->old_count);
+    if (cur_old < kOldHold) {
+      cur_old = add(&cq->old_count, free_count);
+      if (cur_old < kOldHold) {
+        if (cur_old + free_count > kOldHold) {
+          free_count = cur_old + free_count - kOldHold;
+          sub(&cq->old_count, free_count);
+        } else {
+          free_count = 0;
+        }
+      }
+    }
+  } else {
+    free_count = 0;
+  }
+  uint32_t cur = free_count ? ld(&cq->old_qc) : 0u;
+  Spin it;
+  while (ballot(free_count > 0)) {
+    bool put = false;
+    uint32_t victim = 0;
+    if (free_count > 0) {
+      const uint32_t nx = chunk_ok(x.m, cur) ? ld(&hdr<QueueChunkHdr>(x.m, cur)->next) : kMarker;
+      const uint32_t seen = !chunk_ok(x.m, nx) ? ~cur : cas(&cq->old_qc, cur, nx);
+      if (seen == cur) {
+        victim = cur;
+        put = true;
+        cur = nx;
+        --free_count;
+      } else if (chunk_ok(x.m, nx)) {
+        cur = seen;
+      }
+      if (it.expired()) { set_error(x.m, ERR_TIMEOUT); free_count = 0; }
+    }
+    ring_put(x.m, x.base_ring, put, victim);
+  }
+}
+
+// Walk the linked queue chunks from `start` to the one holding virtual position v
+// (QueueChunk::locateQueueChunkForPosition, QueueChunk_impl.cuh:386-414).
+template <class I>
+__device__ inline uint32_t vl_locate(Ctx<I>& x, uint32_t start, uint32_t v) {
+  uint32_t c = start;
+  Spin it;
+  if (!chunk_ok(x.m, c)) {
+    set_error(x.m, ERR_VSTART);
+    return kMarker;
+  }
+  while (ld(&hdr<QueueChunkHdr>(x.m, c)->virtual_start) / kQCSpots != v / kQCSpots) {
+    uint32_t nx;
+    while ((nx = ld(&hdr<QueueChunkHdr>(x.m, c)->next)) == kMarker) {
+      if (it.expired()) { set_error(x.m, ERR_TIMEOUT); return kMarker; }
+      backoff(it.it);
+    }
+    if (!chunk_ok(x.m, nx)) {
+      set_error(x.m, ERR_VSTART);
+      return kMarker;
+    }
+    c = nx;
+    if (it.expired()) { set_error(x.m, ERR_TIMEOUT); return kMarker; }
+  }
+  return c;
+}
+
+// VA: queue chunk that holds virtual position v (accessQueueElement, PageQueueVA_impl.cuh:231-248).
+template <class I>
+__device__ inline uint32_t va_wait_qc(Ctx<I>& x, ClassQueue* cq, uint32_t v) {
+  const uint32_t id = (v / kQCSpots) & (cq->ring_size - 1);
+  uint32_t qc;
+  Spin it;
+  while ((qc = ld(&cq->ring[id])) == kMarker) {
+    if (it.expired()) { set_error(x.m, ERR_TIMEOUT); return kMarker; }
+    backoff(it.it);
+  }
+  if (!chunk_ok(x.m, qc) || ld(&hdr<QueueChunkHdr>(x.m, qc)->virtual_start) / kQCSpots != v / kQCSpots) {
+    set_error(x.m, ERR_VSTART);
+    return kMarker;
+  }
+  return qc;
+}
+
+// ------------------------------------------------------------------ slot sequences
+// put_lanes: every lane with want=true stores `val` at virtual position v of its queue cq.
+//   PQ/CQ   : ring slot CAS (PageQueue_impl.cuh:22-36, ChunkQueue_impl.cuh:20-38)
+//   VAPQ/VACQ: queue chunk via the pointer ring; position 0 of a queue chunk pre-allocates the next
+//              one (PageQueueVA_impl.cuh:196-229, ChunkQueueVA_impl.cuh:276-306)
+//   VLPQ/VLCQ: linked queue chunks (QueueChunk_impl.cuh:137-185), with the lockstep traversal of
+//              guaranteeWarpSyncPerChunk (:7-72) so a lane never waits on a link that a lane of
+//              the same wave sets later.
+// Queue-chunk allocation happens in a pre-phase in lane order, so chunk indices are handed out
+// in the same order a sequential replay would.
+template <class I>
+__device__ inline void put_lanes(Ctx<I>& x, ClassQueue* cq, bool want, uint32_t v, uint32_t val) {
+  constexpr Kind K = I::kKind;
+  if constexpr (is_std(K)) {
+    if (want) {
+      uint32_t* slot = &cq->ring[v & (cq->ring_size - 1)];
+      Spin it;
+      while (cas(slot, kMarker, val) != kMarker) {
+        if (it.expired()) { set_error(x.m, ERR_TIMEOUT); break; }
+        backoff(it.it);
+      }
+    }
+  } else {
+    // pre-phase: position 0 of a queue chunk allocates the next queue chunk
+    const bool need = want && (v % kQCSpots) == 0;
+    uint32_t qc = kMarker;
+    if (ballot(need)) {
+      uint32_t got = kMarker;
+      // lanes that need one form a sub-wave; alloc_chunks hands them out in lane order
+      if (need) {
+        const uint32_t k = popc(ballot(true));
+        const uint32_t ok = alloc_chunks(x.m, x.base_ring, 1u, k, got);
+        (void)ok;
+        qc = got;
+        if (rank_in(ballot(true)) >= ok) qc = kMarker;
+      }
+      const bool have = need && qc != kMarker;
+      qc_init_lanes(x.m, have, qc, v + kQCSpots);
+      if constexpr (is_va(K)) {
+        if (have) xchg(&cq->ring[((v / kQCSpots) + 1) & (cq->ring_size - 1)], qc);
+      }
+    }
+    if constexpr (is_va(K)) {
+      bool reclaim = false;
+      uint32_t victim = 0;
+      if (want) {
+        const uint32_t c = va_wait_qc(x, cq, v);
+        if (c != kMarker) {
+          uint32_t* slot = &qc_slots(x.m, c)[v % kQCSpots];
+          Spin it;
+          while (cas(slot, kMarker, val) != kMarker) {
+            if (it.expired()) { set_error(x.m, ERR_TIMEOUT); break; }
+            backoff(it.it);
+          }
+          const uint32_t old = add(&hdr<QueueChunkHdr>(x.m, c)->count, 0x10001u);
+          if (qc_empty_after_enqueue(old, 0x10001u)) {
+            victim = xchg(&cq->ring[(v / kQCSpots) & (cq->ring_size - 1)], kMarker);
+            reclaim = victim != kMarker;
+          }
+        }
+      }
+      ring_put(x.m, x.base_ring, reclaim, victim);
+    } else {
+      // VL lockstep walk from back_ptr_
+      uint32_t cur = want ? ld(&cq->back_qc) : 0u;
+      bool todo = want && chunk_ok(x.m, cur);
+      if (want && !todo) set_error(x.m, ERR_VSTART);
+      bool old_want = false;
+      uint32_t old_free = 0;
+      Spin it;
+      while (ballot(todo)) {
+        if (todo) {
+          QueueChunkHdr* h = hdr<QueueChunkHdr>(x.m, cur);
+          if (ld(&h->virtual_start) / kQCSpots == v / kQCSpots) {
+            if (v % kQCSpots == 0 && qc != kMarker) xchg(&h->next, qc);
+            xchg(&qc_slots(x.m, cur)[v % kQCSpots], val);
+            const uint32_t old = add(&h->count, 0x10001u);
+            if ((old & 0xFFFFu) + 1u == kQCSpots) vl_set_back(x, cq, cur);
+            if (qc_empty_after_enqueue(old, 0x10001u)) {
+              old_free = vl_set_front(x, cq, cur);
+              old_want = true;
+            }
+            todo = false;
+          }
+        }
+        if (todo) {
+          uint32_t nx;
+          while ((nx = ld(&hdr<QueueChunkHdr>(x.m, cur)->next)) == kMarker) {
+            if (it.expired()) break;
+            backoff(it.it);
+          }
+          if (!chunk_ok(x.m, nx)) {
+            set_error(x.m, nx == kMarker ? ERR_TIMEOUT : ERR_VSTART);
+            todo = false;
+          } else {
+            cur = nx;
+          }
+        }
+      }
+      vl_set_old(x, cq, old_want, old_free);
+    }
+  }
+}
+
+// take_lanes: every lane with want=true removes the element at virtual position v (spinning until
+// it has been published). PageQueue::dequeue (PageQueue_impl.cuh:84-104), PageQueueVA::allocPage
+// (:202-242), QueueChunk::dequeue (QueueChunk_impl.cuh:323-382).
+template <class I>
+__device__ inline uint32_t take_lanes(Ctx<I>& x, ClassQueue* cq, bool want, uint32_t v) {
+  constexpr Kind K = I::kKind;
+  uint32_t val = kMarker;
+  if constexpr (is_std(K)) {
+    if (want) {
+      uint32_t* slot = &cq->ring[v & (cq->ring_size - 1)];
+      Spin it;
+      while ((val = xchg(slot, kMarker)) == kMarker) {
+        if (it.expired()) { set_error(x.m, ERR_TIMEOUT); break; }
+        backoff(it.it);
+      }
+    }
+  } else if constexpr (is_va(K)) {
+    bool reclaim = false;
+    uint32_t victim = 0;
+    if (want) {
+      const uint32_t c = va_wait_qc(x, cq, v);
+      if (c != kMarker) {
+        uint32_t* slot = &qc_slots(x.m, c)[v % kQCSpots];
+        Spin it;
+        while ((val = xchg(slot, kMarker)) == kMarker) {
+          if (it.expired()) { set_error(x.m, ERR_TIMEOUT); break; }
+          backoff(it.it);
+        }
+        const uint32_t old = sub(&hdr<QueueChunkHdr>(x.m, c)->count, 1u << 16);
+        if (qc_empty_after_dequeue(old)) {
+          victim = xchg(&cq->ring[(v / kQCSpots) & (cq->ring_size - 1)], kMarker);
+          reclaim = victim != kMarker;
+        }
+      }
+    }
+    ring_put(x.m, x.base_ring, reclaim, victim);
+  } else {
+    bool old_want = false;
+    uint32_t old_free = 0;
+    if (want) {
+      const uint32_t c = vl_locate(x, ld(&cq->front_qc), v);
+      if (c != kMarker) {
+        uint32_t* slot = &qc_slots(x.m, c)[v % kQCSpots];
+        Spin it;
+        while ((val = xchg(slot, kMarker)) == kMarker) {
+          if (it.expired()) { set_error(x.m, ERR_TIMEOUT); break; }
+          backoff(it.it);
+        }
+        const uint32_t old = sub(&hdr<QueueChunkHdr>(x.m, c)->count, 1u << 16);
+        if (qc_empty_after_dequeue(old)) {
+          old_free = vl_set_front(x, cq, c);
+          old_want = true;
+        }
+      }
+    }
+    vl_set_old(x, cq, old_want, old_free);
+  }
+  return val;
+}
+
+// ------------------------------------------------------------------ semaphore
+// BulkSemaphore::wait (BulkSemaphore_impl.cuh:199-300), one loop per group leader. A carve decision
+// reserves `k` chunks' worth of expected pages at once (the reference loops one chunk per pass);
+// carves then run wave-cooperatively, one leader at a time in lane order. Returns the group's grant.
+template <class I, class Carve>
+__device__ inline bool sem_acquire(Ctx<I>& x, ClassQueue* cq, uint64_t grp, bool want, Carve&& carve) {
+  const int L = leader_of(grp);
+  const bool leader = want && (int)lane() == L;
+  const int64_t num = (int64_t)popc(grp);
+  bool pending = leader, ok = false;
+  Spin it;
+  while (ballot(pending)) {
+    uint32_t carve_k = 0;
+    bool reserve = false;
+    if (pending) {
+      uint64_t* s = &cq->sem;
+      uint64_t v = ld(s);
+      if (Sem::count(v) >= num) {
+        const uint64_t old = add(s, (uint64_t)(-num));
+        if (Sem::count(old) >= num) {
+          pending = false;
+          ok = true;
+        } else {
+          add(s, (uint64_t)num);
+        }
+      }
+      if (pending) {
+        v = ld(s);
+        const int64_t ppc = cq->ppc;
+        while (true) {
+          const int64_t c = Sem::count(v), e = (int64_t)Sem::expected(v), r = (int64_t)Sem::reserved(v);
+          uint64_t nv;
+          uint32_t k = 0;
+          bool take = false, res = false;
+          if (c + e - r < num) {
+            k = (uint32_t)((num - (c + e - r) + ppc - 1) / ppc);
+            nv = Sem::make(c, (uint64_t)(e + (int64_t)k * ppc), (uint64_t)r);
+          } else if (c >= num) {
+            nv = Sem::make(c - num, (uint64_t)e, (uint64_t)r);
+            take = true;
+          } else {
+            nv = Sem::make(c, (uint64_t)e, (uint64_t)(r + num));
+            res = true;
+          }
+          const uint64_t seen = cas(s, v, nv);
+          if (seen == v) {
+            carve_k = k;
+            reserve = res;
+            if (take) {
+              pending = false;
+              ok = true;
+            }
+            break;
+          }
+          v = seen;
+        }
+      }
+    }
+    // wave-cooperative carves, leaders in lane order
+    uint64_t carvers = ballot(carve_k > 0);
+    while (carvers) {
+      const int l = first(carvers);
+      carvers &= carvers - 1;
+      ClassQueue* lq = reinterpret_cast<ClassQueue*>(readlane64(reinterpret_cast<uint64_t>(cq), l));
+      const uint32_t k = readlane(carve_k, l);
+      const uint32_t k_ok = carve(lq, k);
+      if ((int)lane() == l && k_ok < k) pending = false;  // out of chunks: the group gets nullptr
+    }
+    if (reserve) {
+      uint64_t* s = &cq->sem;
+      Spin spin;
+      while (true) {
+        const uint64_t v = ld(s);
+        const int64_t c = Sem::count(v);
+        if (!(c < num && (int64_t)Sem::reserved(v) < c + (int64_t)Sem::expected(v))) break;
+        if (spin.expired()) {
+          set_error(x.m, ERR_TIMEOUT);
+          pending = false;
+          break;
+        }
+        backoff(spin.it);
+      }
+      add(s, (uint64_t)0 - ((uint64_t)num << Sem::kResShift));
+    }
+    if (it.expired()) {
+      if (pending) set_error(x.m, ERR_TIMEOUT);
+      pending = false;
+    }
+  }
+  return shfl((uint32_t)ok, L) != 0;
+}
+
+// signalExpected (BulkSemaphore_impl.cuh:305-308) for n pages carved out of k_req * ppc expected.
+__device__ __forceinline__ void sem_signal_expected(ClassQueue* cq, uint32_t n, uint32_t expected_removed) {
+  add(&cq->sem, (uint64_t)n - ((uint64_t)expected_removed << Sem::kExpShift));
+}
+__device__ __forceinline__ void sem_signal(ClassQueue* cq, uint32_t n) { add(&cq->sem, (uint64_t)n); }
+// tryReduce (BulkSemaphore_impl.cuh:7-17)
+__device__ __forceinline__ bool sem_try_reduce(ClassQueue* cq, uint32_t n) {
+  const uint64_t old = add(&cq->sem, (uint64_t)0 - (uint64_t)n);
+  if (Sem::count(old) < (int64_t)n) {
+    add(&cq->sem, (uint64_t)n);
+    return false;
+  }
+  return true;
+}
+
+// Positions for a queue operation: lanes grouped by queue (match<G>), one fetch-add per group,
+// rank order inside the group (atomicAggInc, Utility.cuh:76-95, when G = 64).
+template <int G>
+__device__ __forceinline__ uint32_t fetch_positions(uint32_t* counter_of_lane, bool want, uint32_t& n_out) {
+  uint32_t pos = 0;
+  const uint64_t key = want ? reinterpret_cast<uint64_t>(counter_of_lane) : ~0ull;
+  const uint64_t grp = match<G>(key);
+  const int L = first(grp);
+  if (want && (int)lane() == L) pos = add(counter_of_lane, popc(grp));
+  pos = shfl(pos, L) + rank_in(grp);
+  n_out = popc(grp);
+  return pos;
+}
+
+// ------------------------------------------------------------------ page-based instance
+// Carve k chunks for class queue cq (PageQueue::allocPage's allocation lambda,
+// PageQueue_impl.cuh:108-133, and enqueueChunk, :58-79), all active lanes cooperating.
+template <class I>
+__device__ inline uint32_t carve_pages(Ctx<I>& x, ClassQueue* cq, uint32_t k) {
+  const uint64_t act = active();
+  const uint32_t me = rank_in(act), nact = popc(act);
+  const int l0 = first(act);
+  const uint32_t ppc = cq->ppc, psize = cq->page_size;
+  uint32_t chunk = kMarker;
+  uint32_t k_ok = alloc_chunks(x.m, x.inst.reuse, I::kAdd, k, chunk);
+  // capacity guard: never carve more pages into a class than its queue can hold
+  uint32_t cap_ok = 1;
+  if ((int)lane() == l0 && k_ok) {
+    const uint64_t cap = is_vl(I::kKind) ? 0xFFFFFFFFull
+                         : is_va(I::kKind) ? (uint64_t)cq->ring_size * kQCSpots - 2ull * kQCSpots
+                                            : (uint64_t)cq->ring_size;
+    const uint32_t before = add(&cq->carved, k_ok * ppc);
+    if ((uint64_t)before + (uint64_t)k_ok * ppc > cap) {
+      sub(&cq->carved, k_ok * ppc);
+      set_error(x.m, ERR_QUEUE_FULL);
+      cap_ok = 0;
+    }
+  }
+  if (!readlane(cap_ok, l0)) k_ok = 0;
+  if (me < k_ok) {
+    PageChunkHdr* h = hdr<PageChunkHdr>(x.m, chunk);
+    st(&h->page_size, psize);
+    st(&h->number_pages, ppc);
+  }
+  order();
+  const uint32_t n = k_ok * ppc;
+  uint32_t v0 = 0;
+  if ((int)lane() == l0 && n) v0 = add(&cq->back, n);
+  v0 = readlane(v0, l0);
+  for (uint32_t e0 = 0; e0 < n; e0 += nact) {
+    const uint32_t e = e0 + me;
+    const uint32_t j = (e < n ? e : n - 1) / ppc;
+    const uint32_t c = shfl(chunk, lane_of_rank(act, j));
+    put_lanes(x, cq, e < n, v0 + e, make_index(c, e % ppc));
+  }
+  if ((int)lane() == l0) sem_signal_expected(cq, n, k * ppc);
+  return k_ok;
+}
+
+__device__ __forceinline__ uint32_t page_ppc_for(uint32_t chunk_size, uint32_t page_size) { return chunk_size / page_size; }
+
+template <int W, class I>
+__device__ inline void* alloc_pages(Ctx<I>& x, uint32_t qi, bool want) {
+  constexpr int GS = W == 1 ? 64 : W;  // semaphore + front/back grouping
+  ClassQueue* cq = &x.inst.q[want ? qi : 0];
+  const uint64_t key = want ? (uint64_t)qi : ~0ull;
+  const uint64_t grp = match<GS>(key);
+  bool ok = false;
+  {
+    auto carve = [&](ClassQueue* lq, uint32_t k) { return carve_pages(x, lq, k); };
+    ok = sem_acquire(x, cq, grp, want, carve);
+  }
+  ok = ok && want;
+  uint32_t n;
+  const uint32_t v = fetch_positions<GS>(&cq->front, ok, n);
+  const uint32_t idx = take_lanes(x, cq, ok, v);
+  if (!ok || idx == kMarker) return nullptr;
+  if (!chunk_ok(x.m, index_chunk(idx)) || index_page(idx) >= cq->ppc) {
+    set_error(x.m, ERR_CORRUPT);
+    return nullptr;
+  }
+  return chunk_addr(x.m, index_chunk(idx)) + kMeta + (uint64_t)index_page(idx) * cq->page_size;
+}
+
+template <int W, class I>
+__device__ inline void free_pages(Ctx<I>& x, uint32_t qi, bool want, uint32_t chunk, uint32_t page) {
+  constexpr int GS = W == 1 ? 64 : W;
+  ClassQueue* cq = &x.inst.q[want ? qi : 0];
+  uint32_t n;
+  const uint32_t v = fetch_positions<GS>(&cq->back, want, n);
+  put_lanes(x, cq, want, v, make_index(chunk, page));
+  // publish, then signal (the reference signals first, PageQueue_impl.cuh:24-31)
+  if constexpr (W == 1) {
+    if (want) sem_signal(cq, 1);
+  } else {
+    const uint64_t key = want ? (uint64_t)qi : ~0ull;
+    const uint64_t grp = match<W>(key);
+    if (want && (int)lane() == first(grp)) sem_signal(cq, popc(grp));
+  }
+}
+
+// ------------------------------------------------------------------ chunk-based instance
+__device__ __forceinline__ uint32_t mask_words(uint32_t ppc) {
+  const uint32_t w = (ppc + 63u) / 64u;
+  return w == 0 ? 1u : (w > kMaskWords ? kMaskWords : w);
+}
+
+// Enqueue chunk indices (ChunkQueue::enqueue, ChunkQueue_impl.cuh:20-38; ChunkQueueVA::enqueue,
+// :344-376; ChunkQueueVL::enqueue, :264-273): count_ / back_ per queue group, queue_pos recorded.
+template <int G, class I>
+__device__ inline void enqueue_chunks(Ctx<I>& x, ClassQueue* cq, bool want, uint32_t chunk) {
+  uint32_t n;
+  // count_ (VL keeps no count_ in the reference, but tracking it costs nothing here)
+  const uint64_t key = want ? reinterpret_cast<uint64_t>(cq) : ~0ull;
+  const uint64_t grp = match<G>(key);
+  const int L = first(grp);
+  if (want && (int)lane() == L) add(&cq->count, (int32_t)popc(grp));
+  const uint32_t v = fetch_positions<G>(&cq->back, want, n);
+  if (want) st(&hdr<ChunkHdr>(x.m, chunk)->queue_pos, v);
+  order();
+  put_lanes(x, cq, want, v, chunk);
+}
+
+template <class I>
+__device__ inline uint32_t carve_chunks(Ctx<I>& x, ClassQueue* cq, uint32_t k) {
+  const uint64_t act = active();
+  const uint32_t me = rank_in(act);
+  const int l0 = first(act);
+  const uint32_t ppc = cq->ppc, psize = cq->page_size;
+  uint32_t chunk = kMarker;
+  uint32_t k_ok = alloc_chunks(x.m, x.inst.reuse, I::kAdd, k, chunk);
+  uint32_t cap_ok = 1;
+  if ((int)lane() == l0 && k_ok) {
+    const uint64_t cap = is_vl(I::kKind) ? 0xFFFFFFFFull
+                         : is_va(I::kKind) ? (uint64_t)cq->ring_size * kQCSpots - 2ull * kQCSpots
+                                            : (uint64_t)cq->ring_size;
+    const uint32_t before = add(&cq->carved, k_ok);
+    if ((uint64_t)before + k_ok > cap) {
+      sub(&cq->carved, k_ok);
+      set_error(x.m, ERR_QUEUE_FULL);
+      cap_ok = 0;
+    }
+  }
+  if (!readlane(cap_ok, l0)) k_ok = 0;
+  if (me < k_ok) {
+    // ChunkIndexChunk::initializeChunk with all pages free (ChunkAccess ctor, ChunkAccess.cuh:69-89)
+    ChunkHdr* h = hdr<ChunkHdr>(x.m, chunk);
+    st(&h->page_size, psize);
+    st(&h->size, ppc);
+    st(&h->count, (int32_t)ppc);
+    st(&h->queue_pos, kMarker);
+    for (uint32_t w = 0; w < kMaskWords; ++w) {
+      const uint32_t lo = w * 64u;
+      uint64_t bits = 0;
+      if (ppc > lo) bits = (ppc - lo >= 64u) ? ~0ull : ((1ull << (ppc - lo)) - 1ull);
+      st(&h->mask[w], bits);
+    }
+  }
+  order();
+  // chunks enqueue in lane order: one group, positions = active ranks
+  enqueue_chunks<64>(x, cq, me < k_ok, chunk);
+  if ((int)lane() == l0) sem_signal_expected(cq, k_ok * ppc, k * ppc);
+  return k_ok;
+}
+
+// Read the chunk index at virtual position v without removing it (ChunkQueue scan,
+// ChunkQueue_impl.cuh:105-145; QueueChunk::access, QueueChunk.cuh:111-114; accessLinked).
+template <class I>
+__device__ inline uint32_t peek_slot(Ctx<I>& x, ClassQueue* cq, uint32_t v, uint32_t& vl_cursor) {
+  constexpr Kind K = I::kKind;
+  if constexpr (is_std(K)) {
+    return ld(&cq->ring[v & (cq->ring_size - 1)]);
+  } else if constexpr (is_va(K)) {
+    const uint32_t qc = ld(&cq->ring[(v / kQCSpots) & (cq->ring_size - 1)]);
+    if (!chunk_ok(x.m, qc)) return kMarker;
+    if (ld(&hdr<QueueChunkHdr>(x.m, qc)->virtual_start) / kQCSpots != v / kQCSpots) return kMarker;
+    return ld(&qc_slots(x.m, qc)[v % kQCSpots]);
+  } else {
+    if (vl_cursor == kMarker) vl_cursor = ld(&cq->front_qc);
+    const uint32_t c = vl_locate(x, vl_cursor, v);
+    if (c == kMarker) return kMarker;
+    vl_cursor = c;
+    return ld(&qc_slots(x.m, c)[v % kQCSpots]);
+  }
+}
+
+// Remove the chunk at virtual position v from the queue (DEQUEUE_CHUNK branches:
+// ChunkQueue_impl.cuh:429-437, ChunkQueueVA_impl.cuh:209-243, ChunkQueueVL_impl.cuh:176-187).
+// Queue-chunk reclamation is returned through `reclaim` (VA) or done inline (VL).
+template <class I>
+__device__ inline void erase_slot(Ctx<I>& x, ClassQueue* cq, uint32_t v, uint32_t& vl_cursor, uint32_t& reclaim) {
+  constexpr Kind K = I::kKind;
+  max_(&cq->front, v + 1);
+  if constexpr (is_std(K)) {
+    xchg(&cq->ring[v & (cq->ring_size - 1)], kMarker);
+  } else if constexpr (is_va(K)) {
+    const uint32_t id = (v / kQCSpots) & (cq->ring_size - 1);
+    const uint32_t qc = ld(&cq->ring[id]);
+    if (chunk_ok(x.m, qc)) {
+      xchg(&qc_slots(x.m, qc)[v % kQCSpots], kMarker);
+      const uint32_t old = sub(&hdr<QueueChunkHdr>(x.m, qc)->count, 1u << 16);
+      if (qc_empty_after_dequeue(old)) {
+        const uint32_t victim = xchg(&cq->ring[id], kMarker);
+        if (victim != kMarker) reclaim = victim;
+      }
+    }
+  } else {
+    if (vl_cursor == kMarker) vl_cursor = ld(&cq->front_qc);
+    const uint32_t c = vl_locate(x, vl_cursor, v);
+    if (c != kMarker) {
+      xchg(&qc_slots(x.m, c)[v % kQCSpots], kMarker);
+      const uint32_t old = sub(&hdr<QueueChunkHdr>(x.m, c)->count, 1u << 16);
+      if (qc_empty_after_dequeue(old)) reclaim = vl_set_front(x, cq, c) | 0x80000000u;
+    }
+  }
+  sub(&cq->count, 1);
+}
+
+// Finish deferred queue-chunk reclamation in lane order.
+template <class I>
+__device__ inline void finish_reclaim(Ctx<I>& x, ClassQueue* cq, bool want, uint32_t reclaim) {
+  constexpr Kind K = I::kKind;
+  if constexpr (is_va(K)) {
+    ring_put(x.m, x.base_ring, want && reclaim != kMarker, reclaim);
+  } else if constexpr (is_vl(K)) {
+    const bool w = want && reclaim != kMarker;
+    vl_set_old(x, cq, w, w ? (reclaim & 0x7FFFFFFFu) : 0u);
+  }
+}
+
+// ChunkAccess::allocPage (ChunkAccess_impl.cuh:38-135), per lane, with 64-bit mask words.
+enum class AccessMode { SUCCESS, DEQUEUE, RE_ENQUEUE, CONTINUE, ERROR };
+
+template <class I>
+__device__ inline AccessMode access_alloc_lane(Ctx<I>& x, ChunkHdr* h, uint32_t& page) {
+  int32_t cur = 0;
+  AccessMode mode = AccessMode::SUCCESS;
+  Spin it;
+  while ((cur = sub(&h->count, 1)) <= 0) {
+    if ((cur = add(&h->count, 1)) < 0) return AccessMode::CONTINUE;
+    if (cur == 0) mode = AccessMode::RE_ENQUEUE;
+    if (it.expired()) { set_error(x.m, ERR_TIMEOUT); return AccessMode::ERROR; }
+  }
+  if (cur == 1) mode = AccessMode::DEQUEUE;
+  const uint32_t nw = mask_words(ld(&h->size));
+  const uint32_t offset = (threadIdx.x + blockIdx.x) % 64u;
+  uint32_t wi = threadIdx.x;
+  while (true) {
+    ++wi;
+    uint64_t* word = &h->mask[wi % nw];
+    uint64_t cm = ld(word);
+    while (cm) {
+      const uint64_t rot = offset ? ((cm >> offset) | (cm << (64u - offset))) : cm;
+      const uint32_t b = ((uint32_t)__builtin_ctzll(rot) + offset) % 64u;
+      const uint64_t old = and_(word, ~(1ull << b));
+      if ((old >> b) & 1ull) {
+        page = (wi % nw) * 64u + b;
+        return mode;
+      }
+      cm = old & ~(1ull << b);
+    }
+    if (it.expired()) { set_error(x.m, ERR_TIMEOUT); return AccessMode::ERROR; }
+  }
+}
+
+template <int W, class I>
+__device__ inline void* alloc_chunk_pages(Ctx<I>& x, uint32_t qi, bool want) {
+  constexpr int GS = W == 1 ? 64 : W;
+  ClassQueue* cq = &x.inst.q[want ? qi : 0];
+  const uint64_t key = want ? (uint64_t)qi : ~0ull;
+  const uint64_t grp = match<GS>(key);
+  bool ok;
+  {
+    auto carve = [&](ClassQueue* lq, uint32_t k) { return carve_chunks(x, lq, k); };
+    ok = sem_acquire(x, cq, grp, want, carve);
+  }
+  ok = ok && want;
+  uint32_t chunk = kMarker, page = 0;
+  uint32_t reclaim = kMarker;
+  bool got = false;
+  if constexpr (W == 1) {
+    // reference protocol, every lane on its own
+    bool reenq = false;
+    if (ok) {
+      uint32_t v = ld(&cq->front);
+      uint32_t cursor = kMarker;
+      Spin it;
+      while (true) {
+        const uint32_t c = peek_slot(x, cq, v, cursor);
+        if (c != kMarker && (uint64_t)c < x.m.max_chunks) {
+          ChunkHdr* h = hdr<ChunkHdr>(x.m, c);
+          const AccessMode md = access_alloc_lane(x, h, page);
+          if (md == AccessMode::ERROR) break;
+          if (md != AccessMode::CONTINUE) {
+            chunk = c;
+            got = true;
+            if (md == AccessMode::RE_ENQUEUE) reenq = true;
+            if (md == AccessMode::DEQUEUE) {
+              st(&h->queue_pos, kMarker);
+              erase_slot(x, cq, v, cursor, reclaim);
+            }
+            break;
+          }
+        }
+        ++v;
+        if (v - ld(&cq->back) < 0x80000000u) {  // ran past back_: restart from front_
+          v = ld(&cq->front);
+          cursor = kMarker;
+        }
+        if (it.expired()) { set_error(x.m, ERR_TIMEOUT); break; }
+        backoff(it.it > 64 ? it.it : 0);
+      }
+    }
+    // RE_ENQUEUE_CHUNK: put the chunk back at the end (ChunkQueue_impl.cuh:423-428)
+    enqueue_chunks<1>(x, cq, reenq, chunk);
+  } else {
+    // group protocol: the leader reserves pages chunk by chunk with a CAS on the chunk count and
+    // claims them from the chunk's 64-bit mask words; lanes take them in rank order.
+    const int L = first(grp);
+    const bool leader = ok && (int)lane() == L;
+    const uint32_t num = popc(grp);
+    uint32_t v = leader ? ld(&cq->front) : 0u;
+    uint32_t cursor = kMarker;
+    uint32_t need = leader ? num : 0u, assigned = 0, t_rem = 0, w = 0, c = kMarker;
+    Spin it;
+    bool done = !ok, gave_up = false;
+    while (ballot(!done)) {
+      uint32_t bc = kMarker, r0 = 0;
+      uint64_t bits = 0;
+      if (leader && assigned < num && !gave_up) {
+        if (t_rem == 0) {
+          const uint32_t s = peek_slot(x, cq, v, cursor);
+          bool advance = true;
+          if (s != kMarker && (uint64_t)s < x.m.max_chunks) {
+            ChunkHdr* h = hdr<ChunkHdr>(x.m, s);
+            int32_t cnt = ld(&h->count);
+            uint32_t t = 0;
+            while (cnt > 0) {
+              t = (uint32_t)cnt < need ? (uint32_t)cnt : need;
+              const int32_t seen = cas(&h->count, cnt, cnt - (int32_t)t);
+              if (seen == cnt) break;
+              cnt = seen;
+            }
+            if (cnt > 0) {
+              c = s;
+              t_rem = t;
+              need -= t;
+              const uint32_t size = ld(&h->size);
+              w = ((size - (uint32_t)cnt) / 64u) % mask_words(size);
+              if (cnt == (int32_t)t) {
+                st(&h->queue_pos, kMarker);
+                uint32_t rc = kMarker;
+                erase_slot(x, cq, v, cursor, rc);
+                if (rc != kMarker) {
+                  if (reclaim != kMarker) finish_reclaim(x, cq, true, reclaim);
+                  reclaim = rc;
+                }
+              } else {
+                advance = false;
+              }
+            }
+          }
+          if (advance && t_rem == 0) {
+            ++v;
+            if (v - ld(&cq->back) < 0x80000000u) {
+              v = ld(&cq->front);
+              cursor = kMarker;
+            }
+          } else if (advance) {
+            ++v;
+          }
+        }
+        if (t_rem > 0) {
+          ChunkHdr* h = hdr<ChunkHdr>(x.m, c);
+          const uint32_t nw = mask_words(ld(&h->size));
+          const uint64_t mw = ld(&h->mask[w]);
+          const uint64_t take = lowest_bits(mw, t_rem);
+          if (take == 0) {
+            w = (w + 1) % nw;
+          } else {
+            const uint64_t old = and_(&h->mask[w], ~take);
+            const uint64_t gotb = old & take;
+            bits = gotb;
+            bc = c | (w << 29);  // word index rides in the top bits (chunk < 2^23)
+            r0 = assigned;
+            assigned += popc(gotb);
+            t_rem -= popc(gotb);
+            if (gotb == take && t_rem > 0) w = (w + 1) % nw;
+          }
+        }
+        if (it.expired()) {
+          set_error(x.m, ERR_TIMEOUT);
+          gave_up = true;
+        }
+      }
+      const uint32_t bcl = shfl(bc, L);
+      const uint64_t bl = shfl64(bits, L);
+      const uint32_t r0l = shfl(r0, L);
+      const uint32_t gul = shfl((uint32_t)gave_up, L);
+      if (!done) {
+        const uint32_t r = rank_in(grp);
+        if (bcl != kMarker && r >= r0l && r < r0l + popc(bl)) {
+          chunk = bcl & 0x1FFFFFFFu;
+          page = ((bcl >> 29) * 64u) + nth_set_bit(bl, r - r0l);
+          got = true;
+          done = true;
+        } else if (gul) {
+          done = true;  // the leader gave up: this lane gets nullptr
+        }
+      }
+    }
+  }
+  finish_reclaim(x, cq, true, reclaim);
+  if (!got) return nullptr;
+  return chunk_addr(x.m, chunk) + kMeta + (uint64_t)page * cq->page_size;
+}
+
+template <int W, class I>
+__device__ inline void free_chunk_pages(Ctx<I>& x, uint32_t qi, bool want, uint32_t chunk, uint32_t page) {
+  ClassQueue* cq = &x.inst.q[want ? qi : 0];
+  ChunkHdr* h = hdr<ChunkHdr>(x.m, want ? chunk : 0u);
+  // ChunkAccess::freePage (ChunkAccess_impl.cuh:7-25): set the bit, then count
+  if (want) or_(&h->mask[page / 64u], 1ull << (page % 64u));
+  order();
+  constexpr int G = W;  // W = 1: per lane
+  const uint64_t key = want ? (uint64_t)chunk : ~0ull;
+  const uint64_t grp = match<G>(key);
+  const int L = first(grp);
+  const uint32_t n = popc(grp);
+  bool first_free = false;
+  uint32_t sig = 0;
+  if (want && (int)lane() == L) {
+    const int32_t old = add(&h->count, (int32_t)n);
+    sig = n;
+    const uint32_t size = ld(&h->size);
+    if (old == 0) {
+      first_free = true;
+    } else if ((uint32_t)(old + (int32_t)n) == size && I::kKind == Kind::CQ &&
+               ld(&cq->count) > (int32_t)x.m.lower_fill_level) {
+      // flash the chunk back to the reuse ring (ChunkQueue_impl.cuh:163-199)
+      if (sem_try_reduce(cq, size - n)) {
+        if (cas(&h->count, (int32_t)size, 0) == (int32_t)size) {
+          uint32_t qp;
+          Spin spin;
+          while ((qp = ld(&h->queue_pos)) == kMarker) {
+            if (spin.expired()) { set_error(x.m, ERR_TIMEOUT); break; }
+            backoff(spin.it);
+          }
+          if (qp != kMarker) {
+            xchg(&cq->ring[qp & (cq->ring_size - 1)], kMarker);
+            sub(&cq->count, 1);
+            sub(&cq->carved, size);
+          }
+          sig = 0;
+        } else {
+          sig = size;
+        }
+      }
+    }
+  }
+  const bool flashed = want && (int)lane() == L && sig == 0;
+  // FIRST_FREE: back into the queue before the pages are signalled
+  enqueue_chunks<W>(x, cq, first_free, chunk);
+  ring_put(x.m, x.inst.reuse, flashed, chunk);
+  if (want && (int)lane() == L && sig) sem_signal(cq, sig);
+}
+
+}  // namespace detail
+}  // namespace ouro

@@ -1,0 +1,189 @@
+// Pool geometry, chunk headers and the device-resident manager state.
+//
+// The pool keeps the reference's partition (device/MemoryInitialization.cuh:156-187,42-58):
+//   [manager state, padded to 8 KiB][max_chunks x 8320 B chunks][reuse rings + class queues][locator]
+// so the chunk count, the chunk addresses and every page address are the reference's.
+// What changes is the manager state itself: every counter that is hit by atomics sits on its own
+// 128-byte line (the reference packs ten queue structs back to back, Ouroboros.cuh:75/171), and
+// queue-chunk links are 32-bit chunk indices instead of 64-bit pointers.
+#pragma once
+#include <stdint.h>
+#include <stddef.h>
+
+#ifndef OURO_HD
+#if defined(__HIPCC__)
+#define OURO_HD __host__ __device__
+#else
+#define OURO_HD
+#endif
+#endif
+
+namespace ouro {
+
+// ------------------------------------------------------------------ constants (Parameters.h)
+constexpr uint32_t kBaseChunk = 8192;          // CHUNK_SIZE (Parameters.h:18)
+constexpr uint32_t kMeta = 128;                // CHUNK_METADATA_SIZE (Parameters.h:19)
+constexpr uint32_t kStride = kBaseChunk + kMeta;  // Chunk<8192>::size() (Chunk.cuh:23-26)
+constexpr uint32_t kPageBits = 9;              // NUM_BITS_FOR_PAGE (Parameters.h:31)
+constexpr uint32_t kPageMask = (1u << kPageBits) - 1u;
+constexpr uint32_t kReuseEntries = 65536;      // chunk_queue_size (Parameters.h:10)
+constexpr uint32_t kStdEntries = 1u << 23;     // page_queue_size (Parameters.h:11)
+constexpr uint32_t kVAEntries = 16384;         // virtual_queue_size (Parameters.h:12)
+constexpr uint32_t kQCSpots = kBaseChunk / 4;  // QueueChunk::num_spots_ (QueueChunk.cuh:27)
+constexpr uint32_t kOldHold = 10;              // LARGEST_OLD_COUNT_VALUE (Parameters.h:13)
+constexpr uint32_t kMaskWords = 8;             // 512 pages / 64-bit words (ChunkAccess.cuh:13 uses 16 x u32)
+constexpr uint32_t kStateBytes = 8192;         // Ouroboros::size_() for every chain here (Ouroboros.cuh:260)
+
+// Queue organisation of one instance (InstanceDefinitions.cuh:1-15).
+enum class Kind : uint32_t { PQ = 0, CQ = 1, VAPQ = 2, VACQ = 3, VLPQ = 4, VLCQ = 5 };
+OURO_HD constexpr bool is_chunk_kind(Kind k) { return k == Kind::CQ || k == Kind::VACQ || k == Kind::VLCQ; }
+OURO_HD constexpr bool is_va(Kind k) { return k == Kind::VAPQ || k == Kind::VACQ; }
+OURO_HD constexpr bool is_vl(Kind k) { return k == Kind::VLPQ || k == Kind::VLCQ; }
+OURO_HD constexpr bool is_std(Kind k) { return k == Kind::PQ || k == Kind::CQ; }
+
+// MemoryIndex (MemoryIndex.cuh:6-52): 23 chunk bits | 9 page bits.
+OURO_HD constexpr uint32_t make_index(uint32_t chunk, uint32_t page) { return (chunk << kPageBits) + page; }
+OURO_HD constexpr uint32_t index_chunk(uint32_t idx) { return idx >> kPageBits; }
+OURO_HD constexpr uint32_t index_page(uint32_t idx) { return idx & kPageMask; }
+
+OURO_HD constexpr uint64_t align_up(uint64_t v, uint64_t a) { return (v + a - 1) / a * a; }
+OURO_HD constexpr uint32_t log2_floor(uint32_t v) { return v <= 1 ? 0 : 1 + log2_floor(v >> 1); }
+
+// Size class, QueueIndex::getQueueIndex (Helper.cuh:42-45): max(ceil(log2(size)) - log2(smallest), 0).
+OURO_HD inline uint32_t class_of(uint64_t size, uint32_t smallest_log2) {
+  if (size <= 1) return 0;
+  uint32_t l = 64 - (uint32_t)__builtin_clzll(size - 1);  // ceil(log2(size))
+  return l > smallest_log2 ? l - smallest_log2 : 0;
+}
+
+// ------------------------------------------------------------------ semaphore word
+// BulkSemaphore (BulkSemaphore.cuh:5-84) packs count | expected | reserved into one u64 with a
+// 25/25/14-bit split. The 14-bit reserved field wraps once more than 16383 pages are reserved,
+// which a 64-lane aggregated wave on a chip with 524,288 resident lanes can reach; the split here
+// is 26/19/19 (count biased by 2^25), identical in behaviour whenever the reference's fields fit.
+struct Sem {
+  static constexpr uint32_t kCountBits = 26;
+  static constexpr uint64_t kBias = 1ull << 25;
+  static constexpr uint32_t kExpShift = 26;
+  static constexpr uint32_t kResShift = 45;
+  static constexpr uint64_t kCountMask = (1ull << kCountBits) - 1ull;
+  static constexpr uint64_t kFieldMask = (1ull << 19) - 1ull;
+  OURO_HD static constexpr uint64_t make(int64_t count, uint64_t expected, uint64_t reserved) {
+    return (uint64_t)(count + (int64_t)kBias) | (expected << kExpShift) | (reserved << kResShift);
+  }
+  OURO_HD static constexpr int64_t count(uint64_t v) { return (int64_t)(v & kCountMask) - (int64_t)kBias; }
+  OURO_HD static constexpr uint64_t expected(uint64_t v) { return (v >> kExpShift) & kFieldMask; }
+  OURO_HD static constexpr uint64_t reserved(uint64_t v) { return (v >> kResShift) & kFieldMask; }
+  static constexpr uint64_t kInit = kBias;
+};
+
+// ------------------------------------------------------------------ chunk headers (128 B)
+// PageChunk (PageIndexChunk.cuh:5-20): {page_size, number_pages}.
+struct PageChunkHdr {
+  uint32_t page_size;
+  uint32_t number_pages;
+};
+// ChunkIndexChunk (ChunkIndexChunk.cuh:5-20) + ChunkAccess (ChunkAccess.cuh:26-28). The
+// availability mask is eight 64-bit words (one wave-wide ballot per word) instead of 16 x u32.
+struct ChunkHdr {
+  uint32_t page_size;
+  uint32_t size;       // pages on this chunk
+  int32_t count;       // free pages
+  uint32_t queue_pos;  // position in the class queue, or marker
+  uint64_t pad;
+  uint64_t mask[kMaskWords];  // bit set = page free
+};
+static_assert(sizeof(ChunkHdr) <= kMeta, "chunk header exceeds metadata area");
+// QueueChunk (QueueChunk.cuh:37-42): slots live in the 8 KiB data area.
+struct QueueChunkHdr {
+  uint32_t page_size;      // always 0 (CommonChunk)
+  uint32_t count;          // (B << 16) | A: A = enqueued, B = enqueued - dequeued
+  uint32_t virtual_start;  // first virtual position held by this chunk
+  uint32_t next;           // chunk index of the next queue chunk (linked mode), or marker
+  uint32_t chunk_index;
+};
+
+// ------------------------------------------------------------------ device-resident state
+// Memory (Ouroboros.cuh:8-29): shared by every instance of a chain.
+struct alignas(128) Memory {
+  uint8_t* d_memory;
+  uint8_t* d_data;
+  uint32_t* locator;        // ChunkLocator::d_chunk_flags
+  uint64_t allocation_size;
+  uint64_t locator_bytes;
+  uint32_t max_chunks;
+  uint32_t variant;
+  uint32_t initialized;
+  uint32_t lower_fill_level;  // ChunkQueue flash threshold (ChunkQueue.cuh:27)
+  alignas(128) uint32_t next_free_chunk;
+  alignas(128) uint32_t error;
+  alignas(128) uint32_t stats[4];  // pageAlloc, pageFree, chunkAlloc, chunkReuse (Statistics.h:5-17)
+};
+
+// IndexQueue (queues/Queue.h:5-24): chunk reuse ring, 65,536 entries per instance.
+struct alignas(128) ReuseRing {
+  uint32_t* ring;
+  uint32_t size;
+  int32_t count;
+  alignas(128) uint32_t front;
+  alignas(128) uint32_t back;
+};
+
+// One size class: the per-class queue struct of every queue kind
+// (PageQueue.cuh:18-23, ChunkQueue.cuh:20-26, PageQueueVA.cuh:20-26, PageQueueVL.cuh:20-29, ...).
+struct alignas(128) ClassQueue {
+  uint64_t sem;          // BulkSemaphore
+  uint32_t* ring;        // std: element ring; VA: queue-chunk index ring; VL: unused
+  uint32_t ring_size;    // power of two
+  uint32_t page_size;
+  uint32_t ppc;          // pages per chunk
+  uint32_t qindex;
+  alignas(128) uint32_t front;
+  uint32_t front_qc;     // VL front_ptr_
+  alignas(128) uint32_t back;
+  uint32_t back_qc;      // VL back_ptr_
+  alignas(128) int32_t count;   // chunk queues: chunks in the queue (count_)
+  uint32_t carved;       // pages ever carved into this class (capacity guard)
+  uint32_t old_qc;       // VL old_ptr_
+  uint32_t old_count;    // VL old_count_
+};
+
+// ------------------------------------------------------------------ pool layout
+// Restates Ouroboros::initialize (MemoryInitialization.cuh:156-166) and initNew (:42-58).
+struct PoolLayout {
+  uint64_t max_chunks;
+  uint64_t allocation_size;
+  uint64_t total_required;
+  uint64_t locator_bytes;
+  uint64_t data_offset;  // d_data - d_memory
+  uint64_t queue_offset; // lowest byte of the queue region
+};
+
+struct InstanceGeometry {
+  uint32_t kind;         // Kind
+  uint32_t num_classes;
+  uint64_t queue_entries;  // entries per class queue array (0 for VL)
+};
+
+OURO_HD inline uint64_t queue_bytes(const InstanceGeometry& g) {
+  return align_up(g.queue_entries * 4ull, kBaseChunk);
+}
+OURO_HD inline uint64_t reuse_bytes() { return align_up((uint64_t)kReuseEntries * 4ull, kBaseChunk); }
+
+OURO_HD inline PoolLayout compute_layout(uint64_t instantiation_size, const InstanceGeometry* g, int n,
+                                         uint64_t state_bytes = kStateBytes) {
+  PoolLayout L{};
+  uint64_t mm = 0;
+  for (int i = 0; i < n; ++i) mm += reuse_bytes() + queue_bytes(g[i]) * g[i].num_classes;
+  L.total_required = align_up(state_bytes + mm, kStride);
+  const uint64_t aligned = align_up(instantiation_size, kStride);
+  L.max_chunks = aligned > L.total_required ? (aligned - L.total_required) / kStride : 0;
+  const uint64_t adjacency = align_up(L.max_chunks * kStride, 128);
+  L.locator_bytes = align_up((L.max_chunks + 31) / 32 * 4, kStride);
+  L.allocation_size = align_up(L.total_required + adjacency + L.locator_bytes, kStride);
+  L.data_offset = state_bytes;
+  L.queue_offset = L.allocation_size - L.locator_bytes - mm;
+  return L;
+}
+
+}  // namespace ouro

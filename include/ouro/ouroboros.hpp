@@ -1,0 +1,342 @@
+// Ouroboros for MI355X: the public device API (malloc / free from any kernel thread) and its host
+// half (initialize / getDeviceMemoryManager), header-only like the reference.
+//
+// Reference surface kept: Ouroboros<Instances...> with
+//   void  initialize(size_t instantiation_size, size_t additionalSizeBeginning = 0,
+//                    size_t additionalSizeEnd = 0)                   (device/Ouroboros.cuh:281)
+//   MyType* getDeviceMemoryManager()                                  (device/Ouroboros.cuh:297)
+//   __device__ void* malloc(size_t size)                              (device/Ouroboros_impl.cuh:83-91)
+//   __device__ void  free(void* ptr)                                  (device/Ouroboros_impl.cuh:95-113)
+//   __device__ void  enqueueInitialChunk(queue, chunk, avail, ppc)    (device/Ouroboros.cuh:291-295)
+// malloc/free take an optional aggregation width template argument (1, 16 or 64); the default
+// OURO_DEFAULT_AGG is 64.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stddef.h>
+#include <string.h>
+#include <utility>
+#include <vector>
+
+#include "detail/device.hpp"
+#include "detail/layout.hpp"
+#include "detail/instance.hpp"
+
+#ifndef OURO_DEFAULT_AGG
+#define OURO_DEFAULT_AGG 64
+#endif
+
+namespace ouro {
+
+// ------------------------------------------------------------------ instance chain storage
+template <class... Is>
+struct Chain;
+template <>
+struct Chain<> {};
+template <class I, class... Rest>
+struct Chain<I, Rest...> {
+  I inst;
+  Chain<Rest...> next;
+};
+
+namespace detail {
+
+// ChunkLocator::getChunkIndex (ChunkLocator.cuh:29-51): highest run-start bit <= idx. Bits are set
+// once and never cleared, so a set bit seen through a plain (possibly stale) load is final; only a
+// miss needs the device-coherent walk.
+__device__ inline uint32_t locate_run(Memory& m, uint32_t idx) {
+  const uint32_t* loc = m.locator;
+  if ((loc[idx >> 5] >> (idx & 31u)) & 1u) return idx;
+  int64_t w = idx >> 5;
+  uint32_t mask = (uint32_t)((2ull << (idx & 31u)) - 1ull);
+  while (true) {
+    const uint32_t bits = dev::ld(&loc[w]) & mask;
+    if (bits) return (uint32_t)(w * 32 + (31 - __builtin_clz(bits)));
+    if (w == 0) return kMarker;
+    --w;
+    mask = 0xFFFFFFFFu;
+  }
+}
+
+template <int W, class I>
+__device__ inline void* inst_malloc(Memory& m, I& inst, ReuseRing& base, uint64_t size, bool want) {
+  Ctx<I> x{m, inst, base};
+  const uint32_t qi = want ? class_of(size, I::kSmallLog2) : 0u;
+  if constexpr (I::kChunks) return alloc_chunk_pages<W>(x, qi, want);
+  else return alloc_pages<W>(x, qi, want);
+}
+
+template <int W, class I>
+__device__ inline void inst_free(Memory& m, I& inst, ReuseRing& base, uint32_t qi, bool want, uint32_t chunk,
+                                 uint32_t page) {
+  Ctx<I> x{m, inst, base};
+  if constexpr (I::kChunks) free_chunk_pages<W>(x, qi, want, chunk, page);
+  else free_pages<W>(x, qi, want, chunk, page);
+}
+
+// Walk the chain: lanes whose size fits instance I allocate there, the rest move on.
+template <int W, class I, class... Rest>
+__device__ inline void* chain_malloc(Memory& m, Chain<I, Rest...>& ch, ReuseRing& base, uint64_t size, bool want) {
+  const bool here = want && size <= I::kLargest;
+  void* r = nullptr;
+  if (dev::ballot(here)) {
+    void* p = inst_malloc<W>(m, ch.inst, base, size, here);
+    if (here) r = p;
+  }
+  if constexpr (sizeof...(Rest) > 0) {
+    const bool later = want && !here;
+    if (dev::ballot(later)) {
+      void* p = chain_malloc<W>(m, ch.next, base, size, later);
+      if (later) r = p;
+    }
+  } else {
+    if (want && !here) set_error(m, ERR_TOO_LARGE);  // the reference falls back to the device heap
+  }
+  return r;
+}
+
+// freePageRecursive (Ouroboros_impl.cuh:117-125): route by the chunk's page size.
+template <int W, class I, class... Rest>
+__device__ inline void chain_free(Memory& m, Chain<I, Rest...>& ch, ReuseRing& base, uint32_t page_size,
+                                  bool want, uint32_t chunk, uint64_t offset) {
+  const bool here = want && page_size <= I::kLargest;
+  if (dev::ballot(here)) {
+    uint32_t qi = 0, page = 0;
+    bool ok = here;
+    if (here) {
+      // page sizes are powers of two between kSmallest and kLargest
+      ok = page_size >= I::kSmallest && (page_size & (page_size - 1)) == 0;
+      if (ok) {
+        qi = log2_floor(page_size) - I::kSmallLog2;
+        page = (uint32_t)(offset >> log2_floor(page_size));
+        ok = (offset & (page_size - 1)) == 0 && page < I::kChunkSize / page_size;
+      }
+      if (!ok) set_error(m, ERR_BAD_FREE);
+    }
+    inst_free<W>(m, ch.inst, base, qi, ok, chunk, page);
+  }
+  if constexpr (sizeof...(Rest) > 0) {
+    const bool later = want && !here;
+    if (dev::ballot(later)) chain_free<W>(m, ch.next, base, page_size, later, chunk, offset);
+  } else {
+    if (want && !here) set_error(m, ERR_BAD_FREE);
+  }
+}
+
+template <class I>
+inline void host_instance_geometry(InstanceGeometry& g) {
+  g.kind = (uint32_t)I::kKind;
+  g.num_classes = I::kNQ;
+  g.queue_entries = is_std(I::kKind) ? kStdEntries : (is_va(I::kKind) ? kVAEntries : 0);
+}
+
+}  // namespace detail
+
+template <class... Is>
+struct Ouroboros {
+  using MyType = Ouroboros<Is...>;
+  static constexpr int kNumInstances = sizeof...(Is);
+
+  Memory memory;
+  Chain<Is...> chain;
+
+  // ---------------------------------------------------------------- device API
+  template <int W = OURO_DEFAULT_AGG>
+  __device__ void* malloc(size_t size) {
+    static_assert(W == 1 || W == 16 || W == 64, "aggregation width must be 1, 16 or 64");
+    return detail::chain_malloc<W>(memory, chain, chain.inst.reuse, (uint64_t)size, true);
+  }
+
+  template <int W = OURO_DEFAULT_AGG>
+  __device__ void free(void* ptr) {
+    static_assert(W == 1 || W == 16 || W == 64, "aggregation width must be 1, 16 or 64");
+    const uint8_t* p = reinterpret_cast<const uint8_t*>(ptr);
+    bool want = p != nullptr;
+    uint32_t chunk = 0, page_size = 0;
+    uint64_t offset = 0;
+    if (want) {
+      // validOuroborosPointer (Ouroboros.cuh:350-356), tightened to the chunk region
+      const uint64_t span = (uint64_t)memory.max_chunks * kStride;
+      want = p >= memory.d_data && p < memory.d_data + span;
+      if (want) {
+        const uint64_t diff = (uint64_t)(p - memory.d_data);
+        const uint32_t c = detail::locate_run(memory, (uint32_t)(diff / kStride));
+        want = c != kMarker;
+        if (want) {
+          chunk = c;
+          page_size = dev::ld(detail::hdr<uint32_t>(memory, c));  // CommonChunk::page_size
+          const uint64_t base = (uint64_t)c * kStride + kMeta;
+          want = diff >= base && page_size != 0;
+          offset = diff - base;
+        }
+      }
+      if (!want) detail::set_error(memory, ERR_BAD_FREE);
+    }
+    if (dev::ballot(want)) detail::chain_free<W>(memory, chain, chain.inst.reuse, page_size, want, chunk, offset);
+  }
+
+  // Bulk preload of a chunk's pages into a class queue (Ouroboros.cuh:291-295). Single thread,
+  // before any concurrent use, like the reference.
+  __device__ void enqueueInitialChunk(uint32_t queue_index, uint32_t chunk_index, int available_pages,
+                                      uint32_t pages_per_chunk) {
+    auto& inst = chain.inst;
+    ClassQueue* cq = &inst.q[queue_index];
+    using I = decltype(chain.inst);
+    detail::Ctx<I> x{memory, inst, inst.reuse};
+    if constexpr (I::kChunks) {
+      ChunkHdr* h = detail::hdr<ChunkHdr>(memory, chunk_index);
+      h->page_size = cq->page_size;
+      h->size = pages_per_chunk;
+      h->count = available_pages;
+      const uint32_t taken = pages_per_chunk - (uint32_t)available_pages;
+      for (uint32_t w = 0; w < kMaskWords; ++w) {
+        uint64_t bits = 0;
+        for (uint32_t b = 0; b < 64; ++b) {
+          const uint32_t pg = w * 64 + b;
+          if (pg >= taken && pg < pages_per_chunk) bits |= 1ull << b;
+        }
+        h->mask[w] = bits;
+      }
+      detail::enqueue_chunks<1>(x, cq, true, chunk_index);
+      dev::add(&cq->carved, 1u);
+    } else {
+      PageChunkHdr* h = detail::hdr<PageChunkHdr>(memory, chunk_index);
+      h->page_size = cq->page_size;
+      h->number_pages = pages_per_chunk;
+      for (uint32_t p = pages_per_chunk - (uint32_t)available_pages; p < pages_per_chunk; ++p) {
+        const uint32_t v = dev::add(&cq->back, 1u);
+        detail::put_lanes(x, cq, true, v, make_index(chunk_index, p));
+      }
+      dev::add(&cq->carved, (uint32_t)available_pages);
+    }
+    dev::order();
+    detail::sem_signal(cq, (uint32_t)available_pages);
+  }
+
+  // ---------------------------------------------------------------- host API
+  Ouroboros() { memset(static_cast<void*>(this), 0, sizeof(*this)); }
+  ~Ouroboros() { release(); }
+  Ouroboros(const Ouroboros&) = delete;  // the reference's Memory double-frees on copy
+  Ouroboros& operator=(const Ouroboros&) = delete;
+
+  MyType* getDeviceMemoryManager() { return reinterpret_cast<MyType*>(memory.d_memory); }
+
+  static PoolLayout layout(size_t instantiation_size, size_t add_beg = 0, size_t add_end = 0) {
+    InstanceGeometry g[kNumInstances];
+    fill_geometry<0, Is...>(g);
+    return compute_layout(instantiation_size, g, kNumInstances, kStateBytes + add_beg + add_end);
+  }
+
+  // Ouroboros::initialize (MemoryInitialization.cuh:123-219). Idempotent; returns hipSuccess or the
+  // first HIP error instead of exiting.
+  hipError_t initialize(size_t instantiation_size, size_t additionalSizeBeginning = 0, size_t additionalSizeEnd = 0,
+                        uint32_t lower_fill_level = (uint32_t)(kStdEntries * 0.1f)) {
+    if (memory.initialized) return hipSuccess;
+    static_assert(sizeof(MyType) <= kStateBytes, "manager state exceeds its 8 KiB slot");
+    const PoolLayout L = layout(instantiation_size, additionalSizeBeginning, additionalSizeEnd);
+    if (L.max_chunks == 0) return hipErrorInvalidValue;
+    uint8_t* d = nullptr;
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&d), L.allocation_size);
+    if (e != hipSuccess) return e;
+    memory.d_memory = d;
+    memory.d_data = d + L.data_offset;
+    memory.allocation_size = L.allocation_size;
+    memory.max_chunks = (uint32_t)L.max_chunks;
+    memory.locator_bytes = L.locator_bytes;
+    memory.locator = reinterpret_cast<uint32_t*>(d + L.allocation_size - L.locator_bytes);
+    memory.lower_fill_level = lower_fill_level;
+    memory.next_free_chunk = 0;
+    memory.error = 0;
+    // queue region: rings and class queues are placed downward from the locator (initNew, :42-58)
+    uint8_t* end = d + L.allocation_size - L.locator_bytes;
+    std::vector<uint32_t> qcs;  // initial queue chunks of VA / VL queues, in allocation order
+    std::vector<std::pair<uint32_t*, uint32_t>> va_heads;  // ring entry 0 of each VA queue
+    place<0, Is...>(chain, end, qcs, va_heads);
+    // every queue array and ring to the empty marker, the locator to zero
+    const uint64_t qbytes = (uint64_t)(d + L.allocation_size - L.locator_bytes - end);
+    e = hipMemsetD32(reinterpret_cast<hipDeviceptr_t>(end), kMarker, qbytes / 4);
+    if (e == hipSuccess) e = hipMemset(memory.locator, 0, L.locator_bytes);
+    // initial queue chunks (PageQueueVA::init etc. take chunks 0, 1, 2, ... via allocateChunk<true>)
+    if (e == hipSuccess && !qcs.empty()) {
+      std::vector<uint32_t> loc((L.locator_bytes + 3) / 4, 0u);
+      std::vector<uint8_t> qc(kStride);
+      for (uint32_t i = 0; i < qcs.size() && e == hipSuccess; ++i) {
+        memset(qc.data(), 0xFF, qc.size());
+        QueueChunkHdr h{0u, 0u, 0u, kMarker, i};
+        memset(qc.data(), 0, kMeta);
+        memcpy(qc.data(), &h, sizeof(h));
+        e = hipMemcpy(memory.d_data + (uint64_t)i * kStride, qc.data(), kStride, hipMemcpyHostToDevice);
+        loc[i >> 5] |= 1u << (i & 31u);
+      }
+      if (e == hipSuccess) e = hipMemcpy(memory.locator, loc.data(), ((qcs.size() + 31) / 32) * 4, hipMemcpyHostToDevice);
+      for (size_t i = 0; i < va_heads.size() && e == hipSuccess; ++i)
+        e = hipMemcpy(va_heads[i].first, &va_heads[i].second, 4, hipMemcpyHostToDevice);
+      memory.next_free_chunk = (uint32_t)qcs.size();
+    }
+    memory.initialized = 1;
+    if (e == hipSuccess) e = hipMemcpy(d, this, sizeof(MyType), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e != hipSuccess) release();
+    return e;
+  }
+
+  // Device -> host copy of the manager state (updateMemoryManagerHost, Ouroboros_impl.cuh:137-144).
+  hipError_t updateHost(MyType& out) const {
+    return hipMemcpy(&out, memory.d_memory, sizeof(MyType), hipMemcpyDeviceToHost);
+  }
+
+  void release() {
+    if (memory.d_memory) (void)hipFree(memory.d_memory);
+    memory.d_memory = nullptr;
+    memory.initialized = 0;
+  }
+
+ private:
+  template <int N>
+  static void fill_geometry(InstanceGeometry*) {}
+  template <int N, class I, class... Rest>
+  static void fill_geometry(InstanceGeometry* g) {
+    detail::host_instance_geometry<I>(g[N]);
+    fill_geometry<N + 1, Rest...>(g);
+  }
+
+  template <int N>
+  static void place(Chain<>&, uint8_t*&, std::vector<uint32_t>&, std::vector<std::pair<uint32_t*, uint32_t>>&) {}
+  template <int N, class I, class... Rest>
+  static void place(Chain<I, Rest...>& ch, uint8_t*& end, std::vector<uint32_t>& qcs,
+                    std::vector<std::pair<uint32_t*, uint32_t>>& va_heads) {
+    InstanceGeometry g;
+    detail::host_instance_geometry<I>(g);
+    I& inst = ch.inst;
+    end -= reuse_bytes();
+    inst.reuse.ring = reinterpret_cast<uint32_t*>(end);
+    inst.reuse.size = kReuseEntries;
+    inst.reuse.count = 0;
+    inst.reuse.front = inst.reuse.back = 0;
+    for (uint32_t i = 0; i < I::kNQ; ++i) {
+      ClassQueue& q = inst.q[i];
+      const uint64_t qb = queue_bytes(g);
+      end -= qb;
+      q.ring = qb ? reinterpret_cast<uint32_t*>(end) : nullptr;
+      q.ring_size = (uint32_t)g.queue_entries;
+      q.page_size = I::kSmallest << i;
+      q.ppc = I::kChunkSize / q.page_size;
+      q.qindex = i;
+      q.sem = Sem::kInit;
+      q.front = q.back = 0;
+      q.count = 0;
+      q.carved = 0;
+      q.old_count = 0;
+      q.front_qc = q.back_qc = q.old_qc = kMarker;
+      if (!is_std(I::kKind)) {
+        const uint32_t c = (uint32_t)qcs.size();
+        qcs.push_back(c);
+        if (is_vl(I::kKind)) q.front_qc = q.back_qc = q.old_qc = c;
+        if (is_va(I::kKind)) va_heads.emplace_back(q.ring, c);
+      }
+    }
+    place<N + 1, Rest...>(ch.next, end, qcs, va_heads);
+  }
+};
+
+}  // namespace ouro

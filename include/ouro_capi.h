@@ -1,0 +1,174 @@
+/*
+ * C ABI of the MI355X Ouroboros allocator (libouro.so).
+ *
+ * The reference ships templates only: its host interface is Ouroboros<...>::initialize /
+ * getDeviceMemoryManager (include/device/Ouroboros.cuh:281,297) and its only caller is the harness
+ * in src/main.cu. Device code cannot cross a C ABI, so kernels that call malloc/free from their own
+ * threads include include/ouro/instances.hpp (the header-only device API). This library wraps the
+ * six MultiOuro* instantiations behind plain pointers and sizes for host drivers, tests and
+ * benchmarks. Every function returns 0 on success and never exits the process.
+ */
+#ifndef OURO_CAPI_H
+#define OURO_CAPI_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ouro_manager* ouro_handle;
+
+/* Instantiations: MultiOuroPQ ... MultiOuroVLCQ (include/InstanceDefinitions.cuh:17-34). */
+enum {
+  OURO_PQ = 0,   /* page-based, standard array queue        */
+  OURO_CQ = 1,   /* chunk-based, standard array queue       */
+  OURO_VAPQ = 2, /* page-based, virtualized array queue     */
+  OURO_VACQ = 3, /* chunk-based, virtualized array queue    */
+  OURO_VLPQ = 4, /* page-based, virtualized linked list     */
+  OURO_VLCQ = 5  /* chunk-based, virtualized linked list    */
+};
+
+/* Status codes. */
+enum {
+  OURO_OK = 0,
+  OURO_E_INVALID = 1, /* bad argument                                  */
+  OURO_E_HIP = 2,     /* a HIP runtime call failed                     */
+  OURO_E_NOMEM = 3    /* the pool (or a staging buffer) could not be allocated */
+};
+
+/* Sticky device error bits reported in ouro_state.error / ouro_trace_result.error_word. */
+enum {
+  OURO_ERR_OOM = 1u << 0,
+  OURO_ERR_QUEUE_FULL = 1u << 1,
+  OURO_ERR_VSTART = 1u << 2,
+  OURO_ERR_TIMEOUT = 1u << 3,
+  OURO_ERR_BAD_FREE = 1u << 4,
+  OURO_ERR_TOO_LARGE = 1u << 5,
+  OURO_ERR_REUSE_FULL = 1u << 6,
+  OURO_ERR_LOCATOR = 1u << 7,
+  OURO_ERR_CORRUPT = 1u << 8
+};
+
+#define OURO_MAX_CLASSES 16
+
+/* Snapshot of the device-resident manager state (printFreeResources, device/MemoryQueries.cuh:8-61,
+ * as data rather than printf). Classes of all instances are listed in chain order. */
+typedef struct {
+  uint64_t allocation_size;
+  uint64_t max_chunks;
+  uint64_t data_offset;
+  uint32_t next_free_chunk;
+  uint32_t error;
+  uint32_t variant;
+  uint32_t num_classes;
+  uint32_t page_size[OURO_MAX_CLASSES];
+  uint32_t pages_per_chunk[OURO_MAX_CLASSES];
+  int64_t sem_count[OURO_MAX_CLASSES];
+  uint32_t sem_expected[OURO_MAX_CLASSES];
+  uint32_t sem_reserved[OURO_MAX_CLASSES];
+  uint32_t front[OURO_MAX_CLASSES];
+  uint32_t back[OURO_MAX_CLASSES];
+  int32_t queue_count[OURO_MAX_CLASSES];
+  uint32_t carved[OURO_MAX_CLASSES];
+  uint32_t front_qc[OURO_MAX_CLASSES];   /* linked-list queues: first queue chunk */
+  uint32_t kind[OURO_MAX_CLASSES];       /* OURO_PQ ... OURO_VLCQ of the class's instance */
+  int32_t reuse_count[2];
+} ouro_state;
+
+/* Quiescent-state invariant check: after every block has been freed, each class's semaphore,
+ * queue and chunk bitmasks must describe exactly the carved pages, all free. */
+typedef struct {
+  uint64_t violations;      /* total number of violated invariants      */
+  uint64_t chunks_checked;
+  uint64_t pages_free;      /* pages found free in carved chunks        */
+  int32_t first_class;      /* class of the first violation, or -1      */
+  int32_t first_chunk;      /* chunk of the first violation, or -1      */
+  char message[160];        /* description of the first violation       */
+} ouro_check_result;
+
+/* One alloc / write / verify / free round per "round", over `threads` device threads
+ * (src/main.cu:146-167, with per-thread sizes). */
+typedef struct {
+  uint32_t threads;
+  uint32_t rounds;
+  uint32_t agg;                /* aggregation width: 1 (reference), 16 or 64 */
+  uint32_t block;              /* threads per block (0 = 256)                  */
+  uint32_t check_rounds;       /* 1 = host invariant check of the live set every round */
+  uint32_t latency;            /* 1 = record malloc latency on lane 0 of every wave */
+  uint32_t hold_permille;      /* blocks whose free is deferred by one round (per mille) */
+  uint32_t pad;
+  uint64_t hold_seed;
+  const uint32_t* sizes;       /* host [threads] sizes, fixed across rounds ...          */
+  const uint32_t* round_sizes; /* ... or host [rounds * threads], overrides `sizes`     */
+  uint64_t* offsets_out;       /* optional host [rounds * threads]: byte offset of each
+                                  block from the data region, UINT64_MAX for nullptr     */
+} ouro_trace_config;
+
+typedef struct {
+  double alloc_ms;             /* summed over rounds, hipEvent-timed malloc kernels */
+  double free_ms;              /* summed over rounds, hipEvent-timed free kernels   */
+  double write_ms;
+  double verify_ms;
+  uint64_t mallocs;
+  uint64_t frees;
+  uint64_t null_returns;
+  uint64_t verify_errors;      /* words that did not read back the writer's id        */
+  uint64_t bounds_errors;      /* blocks outside the data region                      */
+  uint64_t overlap_errors;     /* live blocks that overlap (check_rounds only)         */
+  uint64_t align_errors;       /* blocks not on a page boundary of their class         */
+  uint64_t touch_bytes;        /* algorithmic bytes of the write + verify passes        */
+  uint64_t peak_chunks;        /* highest next_free_chunk seen                        */
+  uint64_t peak_live_chunks;   /* most chunks holding a live block at a round end (check_rounds) */
+  uint64_t ideal_chunks;       /* sum over classes of ceil(live / pages per chunk) at that round */
+  uint64_t lat_samples;
+  double lat_p50_ns;
+  double lat_p99_ns;
+  double lat_max_ns;
+  uint32_t error_word;
+  uint32_t pad;
+} ouro_trace_result;
+
+/* ---- lifetime ------------------------------------------------------------------------------ */
+/* Ouroboros::initialize(pool_bytes) (device/MemoryInitialization.cuh:123-219) on the current device. */
+int ouro_create(int variant, uint64_t pool_bytes, ouro_handle* out);
+int ouro_destroy(ouro_handle h);
+/* getDeviceMemoryManager() (device/Ouroboros.cuh:297): pass to kernels built on include/ouro. */
+void* ouro_device_manager(ouro_handle h);
+/* Start of the chunk region (Memory::d_data, device/Ouroboros.cuh:18). */
+void* ouro_data_base(ouro_handle h);
+/* Pool arithmetic without a device (device/MemoryInitialization.cuh:156-166). */
+int ouro_layout(int variant, uint64_t pool_bytes, uint64_t* max_chunks, uint64_t* allocation_size);
+
+/* ---- batched device calls ------------------------------------------------------------------- */
+/* d_testAllocation (src/main.cu:10-18): d_out[i] = malloc(d_sizes[i]) on n device threads. */
+int ouro_malloc_batch(ouro_handle h, const uint32_t* d_sizes, void** d_out, uint32_t n, int agg, void* stream);
+/* d_testFree (src/main.cu:55-63): free(d_ptrs[i]) on n device threads (nullptr entries are skipped). */
+int ouro_free_batch(ouro_handle h, void* const* d_ptrs, uint32_t n, int agg, void* stream);
+
+/* ---- state and checks ----------------------------------------------------------------------- */
+int ouro_query(ouro_handle h, ouro_state* out);
+int ouro_check_quiescent(ouro_handle h, ouro_check_result* out);
+/* Copy the whole pool (the complete allocator state) to dst (device or host memory, allocation_size bytes). */
+int ouro_snapshot(ouro_handle h, void* dst);
+
+/* ---- harness -------------------------------------------------------------------------------- */
+int ouro_run_trace(ouro_handle h, const ouro_trace_config* cfg, ouro_trace_result* out);
+/* Host trace generator (SplitMix64): dist 0 = fixed size `param`, 1 = uniform over the class
+ * sizes 16 << q (q < classes), 2 = Zipf(param) over class ranks. */
+int ouro_gen_sizes(uint64_t seed, uint32_t n, int dist, double param, uint32_t classes, uint32_t* out);
+
+/* ---- measurement helpers (no reference counterpart) ---------------------------------------- */
+/* Peak device-scope atomic throughput: `threads` threads each issue `iters` u32 fetch-adds with
+ * return. pattern 0 = every lane its own address (distinct cache lines), 1 = one address per wave,
+ * 2 = one address for the whole grid. Returns atomics per second in *ops_per_s. */
+int ouro_bench_atomics(uint32_t threads, uint32_t iters, int pattern, double* ops_per_s);
+/* Plain device copy of `bytes` bytes (16-byte vector loads/stores), `reps` times; returns GB/s.
+ * Used to calibrate the FETCH_SIZE / WRITE_SIZE counters against known traffic. */
+int ouro_bench_copy(uint64_t bytes, uint32_t reps, double* gb_per_s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* OURO_CAPI_H */

@@ -1,0 +1,983 @@
+/*
+ * Sequential host simulation of the Ouroboros queue logic -- the oracle.
+ *
+ * TEST INFRASTRUCTURE ONLY (see ouro_sim.h). It restates the reference's algorithms with plain,
+ * single-threaded state. Every function names the reference lines it follows. Where the MI355X
+ * device code deliberately differs from the reference (64-bit page masks, group claims, batched
+ * carves, publish-before-signal), the restatement follows the documented device rule and says so.
+ */
+#include "ouro_sim.h"
+
+#include <stdlib.h>
+#include <string.h>
+
+#define MARKER 0xFFFFFFFFu
+#define STRIDE 8320u          /* Chunk<8192>::size() (Chunk.cuh:23-26) */
+#define META 128u             /* CHUNK_METADATA_SIZE (Parameters.h:19) */
+#define BASE_CHUNK 8192u
+#define PAGE_BITS 9u          /* NUM_BITS_FOR_PAGE (Parameters.h:31) */
+#define STD_ENTRIES (1u << 23) /* page_queue_size (Parameters.h:11) */
+#define VA_ENTRIES 16384u     /* virtual_queue_size (Parameters.h:12) */
+#define REUSE_ENTRIES 65536u  /* chunk_queue_size (Parameters.h:10) */
+#define QC_SPOTS 2048u        /* QueueChunk::num_spots_ (QueueChunk.cuh:27) */
+#define OLD_HOLD 10u          /* LARGEST_OLD_COUNT_VALUE (Parameters.h:13) */
+#define STATE_BYTES 8192u
+#define BLOCK 4096u
+
+enum { ERR_OOM = 1, ERR_QUEUE_FULL = 2, ERR_VSTART = 4, ERR_TIMEOUT = 8, ERR_BAD_FREE = 16, ERR_TOO_LARGE = 32,
+       ERR_REUSE_FULL = 64, ERR_CORRUPT = 256 };
+enum { K_PQ = 0, K_CQ = 1, K_VAPQ = 2, K_VACQ = 3, K_VLPQ = 4, K_VLCQ = 5 };
+
+static int is_chunk_kind(int k) { return k == K_CQ || k == K_VACQ || k == K_VLCQ; }
+static int is_va(int k) { return k == K_VAPQ || k == K_VACQ; }
+static int is_vl(int k) { return k == K_VLPQ || k == K_VLCQ; }
+static int is_std(int k) { return k == K_PQ || k == K_CQ; }
+
+typedef struct {
+  int type; /* 0 unused, 1 page chunk, 2 chunk-queue chunk, 3 queue chunk */
+  uint32_t page_size, size;
+  int32_t count;
+  uint32_t queue_pos;
+  uint64_t mask[8];
+  /* queue chunk view */
+  uint32_t qc_count, vstart, next;
+  uint32_t* slots;
+} chunk_t;
+
+typedef struct {
+  int64_t cnt;
+  int64_t exp, res;
+  uint32_t front, back, carved, old_count, front_qc, back_qc, old_qc;
+  int32_t count;
+  uint32_t page_size, ppc, ring_size;
+  uint32_t** blocks; /* std ring, allocated in 4096-entry blocks */
+  uint32_t* va;      /* VA queue-chunk ring */
+} cls_t;
+
+typedef struct {
+  uint32_t* ring;
+  uint32_t size, front, back;
+  int32_t count;
+} reuse_t;
+
+typedef struct {
+  int kind;
+  uint32_t smallest, nq, largest, chunk_size, add;
+  reuse_t reuse;
+  cls_t q[10];
+} inst_t;
+
+struct ouro_sim {
+  int variant;
+  uint64_t max_chunks, alloc_size;
+  uint32_t next_free, error, lower_fill;
+  uint32_t* locator;
+  chunk_t* chunks;
+  inst_t inst[2];
+  int ninst;
+};
+
+/* ------------------------------------------------------------------ closed forms */
+/* QueueIndex::getQueueIndex (Helper.cuh:42-45): max(ceil(log2 size) - log2 smallest, 0). */
+int sim_class_of(uint32_t size, uint32_t smallest) {
+  int l = 0, s = 0;
+  while ((1u << s) < smallest) ++s;
+  while (l < 32 && (1ull << l) < size) ++l;
+  return l > s ? l - s : 0;
+}
+static uint32_t make_index(uint32_t c, uint32_t p) { return (c << PAGE_BITS) + p; }
+
+/* BulkSemaphore (BulkSemaphore.cuh:5-84) with the widened 26/19/19 split. */
+uint64_t sim_sem_make(int64_t count, uint64_t expected, uint64_t reserved) {
+  return (uint64_t)(count + (1ll << 25)) | (expected << 26) | (reserved << 45);
+}
+int64_t sim_sem_count(uint64_t v) { return (int64_t)(v & ((1ull << 26) - 1)) - (1ll << 25); }
+uint64_t sim_sem_expected(uint64_t v) { return (v >> 26) & ((1ull << 19) - 1); }
+uint64_t sim_sem_reserved(uint64_t v) { return (v >> 45) & ((1ull << 19) - 1); }
+
+/* ChunkLocator::getChunkIndex (ChunkLocator.cuh:29-51). */
+uint32_t sim_locate(const uint32_t* bits, uint32_t nwords, uint32_t idx) {
+  int64_t w = idx >> 5;
+  uint32_t mask = (uint32_t)((2ull << (idx & 31u)) - 1ull);
+  if ((uint32_t)w >= nwords) return MARKER;
+  for (;;) {
+    uint32_t b = bits[w] & mask;
+    if (b) {
+      int hi = 31;
+      while (!((b >> hi) & 1u)) --hi;
+      return (uint32_t)(w * 32 + hi);
+    }
+    if (w == 0) return MARKER;
+    --w;
+    mask = 0xFFFFFFFFu;
+  }
+}
+
+/* Pool arithmetic (MemoryInitialization.cuh:156-166, initNew :42-58). */
+static uint64_t align_up(uint64_t v, uint64_t a) { return (v + a - 1) / a * a; }
+static uint64_t entries_of(int kind) { return is_std(kind) ? STD_ENTRIES : (is_va(kind) ? VA_ENTRIES : 0); }
+void sim_layout(int variant, uint64_t pool_bytes, uint64_t* max_chunks, uint64_t* allocation_size) {
+  const uint64_t qb = align_up(entries_of(variant) * 4ull, BASE_CHUNK);
+  const uint64_t rb = align_up((uint64_t)REUSE_ENTRIES * 4ull, BASE_CHUNK);
+  const uint64_t mm = (rb + qb * 10) + (rb + qb * 2);
+  const uint64_t total_required = align_up(STATE_BYTES + mm, STRIDE);
+  const uint64_t aligned = align_up(pool_bytes, STRIDE);
+  const uint64_t mc = aligned > total_required ? (aligned - total_required) / STRIDE : 0;
+  const uint64_t adjacency = align_up(mc * STRIDE, 128);
+  const uint64_t loc = align_up((mc + 31) / 32 * 4, STRIDE);
+  if (max_chunks) *max_chunks = mc;
+  if (allocation_size) *allocation_size = align_up(total_required + adjacency + loc, STRIDE);
+}
+
+/* ------------------------------------------------------------------ storage helpers */
+static uint32_t* std_slot(cls_t* q, uint32_t pos) {
+  const uint32_t p = pos & (q->ring_size - 1);
+  uint32_t** b = &q->blocks[p / BLOCK];
+  if (!*b) {
+    *b = (uint32_t*)malloc(BLOCK * 4);
+    memset(*b, 0xFF, BLOCK * 4);
+  }
+  return &(*b)[p % BLOCK];
+}
+static chunk_t* ch(ouro_sim* s, uint32_t c) { return &s->chunks[c]; }
+static int chunk_ok(const ouro_sim* s, uint32_t c) { return c < s->max_chunks; }
+
+/* IndexQueue::enqueue (Queue_impl.cuh:20-38) */
+static void ring_put(ouro_sim* s, reuse_t* r, uint32_t value) {
+  const int32_t f = r->count++;
+  if (f >= (int32_t)r->size) {
+    r->count--;
+    s->error |= ERR_REUSE_FULL;
+    return;
+  }
+  const uint32_t pos = r->back++ & (r->size - 1);
+  if (r->ring[pos] != MARKER) s->error |= ERR_TIMEOUT;
+  r->ring[pos] = value;
+}
+/* IndexQueue::dequeue (Queue_impl.cuh:54-67); NO value -> MARKER */
+static uint32_t ring_take(reuse_t* r) {
+  if (r->count <= 0) return MARKER;
+  r->count--;
+  const uint32_t pos = r->front++ & (r->size - 1);
+  const uint32_t v = r->ring[pos];
+  r->ring[pos] = MARKER;
+  return v;
+}
+
+/* allocateChunk<QUEUECHUNK> (Ouroboros.cuh:108-136) for k chunks: reuse ring first, then bump.
+ * Validity follows the reference's predicate chunk + add < maxChunks (:133). Returns the valid count. */
+static uint32_t alloc_chunks(ouro_sim* s, reuse_t* r, uint32_t add, uint32_t k, uint32_t* out) {
+  uint32_t t = 0;
+  while (t < k && r->count > 0) out[t++] = ring_take(r);
+  uint32_t ok = 0;
+  for (uint32_t j = 0; j < t; ++j)
+    if (chunk_ok(s, out[j])) ++ok;
+  if (k > t) {
+    const uint32_t base = s->next_free;
+    s->next_free += (k - t) * add;
+    for (uint32_t j = t; j < k; ++j) {
+      const uint32_t c = base + (j - t) * add;
+      out[j] = c;
+      if ((uint64_t)c + add < s->max_chunks) {
+        s->locator[c >> 5] |= 1u << (c & 31u);
+        ++ok;
+      }
+    }
+  }
+  if (ok < k) s->error |= ERR_OOM;
+  return ok;
+}
+
+/* QueueChunk::initializeChunk (QueueChunk.cuh:221-225) with cleaned slots. */
+static void qc_init(ouro_sim* s, uint32_t c, uint32_t vstart) {
+  chunk_t* q = ch(s, c);
+  q->type = 3;
+  q->page_size = 0;
+  q->qc_count = 0;
+  q->vstart = vstart;
+  q->next = MARKER;
+  if (!q->slots) q->slots = (uint32_t*)malloc(QC_SPOTS * 4);
+  memset(q->slots, 0xFF, QC_SPOTS * 4);
+}
+
+/* ------------------------------------------------------------------ VL pointer maintenance */
+/* QueueChunk::setBackPointer (QueueChunk_impl.cuh:440-467) */
+static void vl_set_back(ouro_sim* s, cls_t* q, uint32_t c) {
+  for (;;) {
+    const uint32_t nx = ch(s, c)->next;
+    if (!chunk_ok(s, nx) || q->back_qc != c) break;
+    q->back_qc = nx;
+    if ((ch(s, nx)->qc_count & 0xFFFFu) == QC_SPOTS) c = nx;
+    else break;
+  }
+}
+/* QueueChunk::setFrontPointer (QueueChunk_impl.cuh:471-495) */
+static uint32_t vl_set_front(ouro_sim* s, cls_t* q, uint32_t c) {
+  uint32_t moved = 0;
+  for (;;) {
+    const uint32_t nx = ch(s, c)->next;
+    if (!chunk_ok(s, nx) || q->front_qc != c) break;
+    q->front_qc = nx;
+    ++moved;
+    if (ch(s, nx)->qc_count == QC_SPOTS) c = nx;
+    else break;
+  }
+  return moved;
+}
+/* QueueChunk::setOldPointer (QueueChunk_impl.cuh:499-551): hold back OLD_HOLD emptied queue chunks,
+ * recycle the rest through the base reuse ring. */
+static void vl_set_old(ouro_sim* s, cls_t* q, uint32_t free_count) {
+  if (q->old_count < OLD_HOLD) {
+    const uint32_t cur = q->old_count;
+    q->old_count += free_count;
+    if (cur < OLD_HOLD) {
+      if (cur + free_count > OLD_HOLD) {
+        free_count = cur + free_count - OLD_HOLD;
+        q->old_count -= free_count;
+      } else {
+        free_count = 0;
+      }
+    }
+  }
+  while (free_count > 0) {
+    const uint32_t cur = q->old_qc;
+    const uint32_t nx = ch(s, cur)->next;
+    if (!chunk_ok(s, nx)) {
+      s->error |= ERR_TIMEOUT;
+      break;
+    }
+    q->old_qc = nx;
+    --free_count;
+    ring_put(s, &s->inst[0].reuse, cur);
+  }
+}
+
+static uint32_t vl_locate(ouro_sim* s, uint32_t c, uint32_t v) {
+  while (chunk_ok(s, c) && ch(s, c)->vstart / QC_SPOTS != v / QC_SPOTS) c = ch(s, c)->next;
+  if (!chunk_ok(s, c)) {
+    s->error |= ERR_VSTART;
+    return MARKER;
+  }
+  return c;
+}
+
+/* ------------------------------------------------------------------ slot sequences */
+/* put for a set of lanes already sorted in lane order: positions v[i], values val[i].
+ * Queue chunks needed by positions at a chunk boundary are allocated first, in lane order
+ * (PageQueueVA_impl.cuh:196-229, QueueChunk_impl.cuh:137-185 with the device's pre-phase). */
+static void put_lanes(ouro_sim* s, inst_t* I, cls_t** qs, int n, const uint32_t* v, const uint32_t* val) {
+  const int K = I->kind;
+  if (is_std(K)) {
+    for (int i = 0; i < n; ++i) {
+      uint32_t* slot = std_slot(qs[i], v[i]);
+      if (*slot != MARKER) s->error |= ERR_TIMEOUT;
+      else *slot = val[i];
+    }
+    return;
+  }
+  uint32_t qc[64];
+  int need[64], nneed = 0;
+  for (int i = 0; i < n; ++i) {
+    qc[i] = MARKER;
+    if (v[i] % QC_SPOTS == 0) need[nneed++] = i;
+  }
+  if (nneed) {
+    uint32_t got[64];
+    const uint32_t ok = alloc_chunks(s, &s->inst[0].reuse, 1, (uint32_t)nneed, got);
+    for (int j = 0; j < nneed; ++j)
+      if ((uint32_t)j < ok) {
+        qc[need[j]] = got[j];
+        qc_init(s, got[j], v[need[j]] + QC_SPOTS);
+        if (is_va(K)) qs[need[j]]->va[((v[need[j]] / QC_SPOTS) + 1) & (qs[need[j]]->ring_size - 1)] = got[j];
+      }
+  }
+  if (is_va(K)) {
+    uint32_t victims[64];
+    int nv = 0;
+    for (int i = 0; i < n; ++i) {
+      cls_t* q = qs[i];
+      const uint32_t id = (v[i] / QC_SPOTS) & (q->ring_size - 1);
+      const uint32_t c = q->va[id];
+      if (!chunk_ok(s, c) || ch(s, c)->vstart / QC_SPOTS != v[i] / QC_SPOTS) {
+        s->error |= ERR_VSTART;
+        continue;
+      }
+      chunk_t* x = ch(s, c);
+      if (x->slots[v[i] % QC_SPOTS] != MARKER) s->error |= ERR_TIMEOUT;
+      x->slots[v[i] % QC_SPOTS] = val[i];
+      const uint32_t old = x->qc_count;
+      x->qc_count += 0x10001u;
+      if (old + 0x10001u == QC_SPOTS) {
+        victims[nv++] = q->va[id];
+        q->va[id] = MARKER;
+      }
+    }
+    for (int i = 0; i < nv; ++i) ring_put(s, &s->inst[0].reuse, victims[i]);
+    return;
+  }
+  /* VL: walk, link at local position 0, write, count */
+  uint32_t oldf[64];
+  int oldw[64];
+  for (int i = 0; i < n; ++i) {
+    cls_t* q = qs[i];
+    oldw[i] = 0;
+    oldf[i] = 0;
+    const uint32_t c = vl_locate(s, q->back_qc, v[i]);
+    if (c == MARKER) continue;
+    chunk_t* x = ch(s, c);
+    if (v[i] % QC_SPOTS == 0 && qc[i] != MARKER) x->next = qc[i];
+    x->slots[v[i] % QC_SPOTS] = val[i];
+    const uint32_t old = x->qc_count;
+    x->qc_count += 0x10001u;
+    if ((old & 0xFFFFu) + 1u == QC_SPOTS) vl_set_back(s, q, c);
+    if (old + 0x10001u == QC_SPOTS) {
+      oldf[i] = vl_set_front(s, q, c);
+      oldw[i] = 1;
+    }
+  }
+  for (int i = 0; i < n; ++i)
+    if (oldw[i]) vl_set_old(s, qs[i], oldf[i]);
+}
+
+/* take for lanes in lane order (PageQueue::dequeue PageQueue_impl.cuh:84-104, PageQueueVA::allocPage
+ * :202-242, QueueChunk::dequeue QueueChunk_impl.cuh:323-382). */
+static void take_lanes(ouro_sim* s, inst_t* I, cls_t** qs, int n, const uint32_t* v, uint32_t* out) {
+  const int K = I->kind;
+  if (is_std(K)) {
+    for (int i = 0; i < n; ++i) {
+      uint32_t* slot = std_slot(qs[i], v[i]);
+      out[i] = *slot;
+      if (*slot == MARKER) s->error |= ERR_TIMEOUT;
+      *slot = MARKER;
+    }
+    return;
+  }
+  if (is_va(K)) {
+    uint32_t victims[64];
+    int nv = 0;
+    for (int i = 0; i < n; ++i) {
+      cls_t* q = qs[i];
+      out[i] = MARKER;
+      const uint32_t id = (v[i] / QC_SPOTS) & (q->ring_size - 1);
+      const uint32_t c = q->va[id];
+      if (!chunk_ok(s, c) || ch(s, c)->vstart / QC_SPOTS != v[i] / QC_SPOTS) {
+        s->error |= ERR_VSTART;
+        continue;
+      }
+      chunk_t* x = ch(s, c);
+      out[i] = x->slots[v[i] % QC_SPOTS];
+      if (out[i] == MARKER) s->error |= ERR_TIMEOUT;
+      x->slots[v[i] % QC_SPOTS] = MARKER;
+      const uint32_t old = x->qc_count;
+      x->qc_count -= 0x10000u;
+      if (old - 0x10000u == QC_SPOTS) {
+        victims[nv++] = q->va[id];
+        q->va[id] = MARKER;
+      }
+    }
+    for (int i = 0; i < nv; ++i) ring_put(s, &s->inst[0].reuse, victims[i]);
+    return;
+  }
+  uint32_t oldf[64];
+  int oldw[64];
+  for (int i = 0; i < n; ++i) {
+    cls_t* q = qs[i];
+    out[i] = MARKER;
+    oldw[i] = 0;
+    oldf[i] = 0;
+    const uint32_t c = vl_locate(s, q->front_qc, v[i]);
+    if (c == MARKER) continue;
+    chunk_t* x = ch(s, c);
+    out[i] = x->slots[v[i] % QC_SPOTS];
+    if (out[i] == MARKER) s->error |= ERR_TIMEOUT;
+    x->slots[v[i] % QC_SPOTS] = MARKER;
+    const uint32_t old = x->qc_count;
+    x->qc_count -= 0x10000u;
+    if (old - 0x10000u == QC_SPOTS) {
+      oldf[i] = vl_set_front(s, q, c);
+      oldw[i] = 1;
+    }
+  }
+  for (int i = 0; i < n; ++i)
+    if (oldw[i]) vl_set_old(s, qs[i], oldf[i]);
+}
+
+static uint32_t peek(ouro_sim* s, inst_t* I, cls_t* q, uint32_t v, uint32_t* cursor) {
+  if (is_std(I->kind)) return *std_slot(q, v);
+  if (is_va(I->kind)) {
+    const uint32_t c = q->va[(v / QC_SPOTS) & (q->ring_size - 1)];
+    if (!chunk_ok(s, c) || ch(s, c)->vstart / QC_SPOTS != v / QC_SPOTS) return MARKER;
+    return ch(s, c)->slots[v % QC_SPOTS];
+  }
+  if (*cursor == MARKER) *cursor = q->front_qc;
+  const uint32_t c = vl_locate(s, *cursor, v);
+  if (c == MARKER) return MARKER;
+  *cursor = c;
+  return ch(s, c)->slots[v % QC_SPOTS];
+}
+
+/* DEQUEUE_CHUNK erase (ChunkQueue_impl.cuh:429-437, ChunkQueueVA_impl.cuh:209-243,
+ * ChunkQueueVL_impl.cuh:176-187). A queue chunk emptied here is reclaimed after the scan. */
+static void erase(ouro_sim* s, inst_t* I, cls_t* q, uint32_t v, uint32_t* cursor, uint32_t* reclaim) {
+  if (q->front < v + 1) q->front = v + 1;
+  if (is_std(I->kind)) {
+    *std_slot(q, v) = MARKER;
+  } else if (is_va(I->kind)) {
+    const uint32_t id = (v / QC_SPOTS) & (q->ring_size - 1);
+    const uint32_t c = q->va[id];
+    if (chunk_ok(s, c)) {
+      chunk_t* x = ch(s, c);
+      x->slots[v % QC_SPOTS] = MARKER;
+      const uint32_t old = x->qc_count;
+      x->qc_count -= 0x10000u;
+      if (old - 0x10000u == QC_SPOTS) {
+        *reclaim = c;
+        q->va[id] = MARKER;
+      }
+    }
+  } else {
+    if (*cursor == MARKER) *cursor = q->front_qc;
+    const uint32_t c = vl_locate(s, *cursor, v);
+    if (c != MARKER) {
+      chunk_t* x = ch(s, c);
+      x->slots[v % QC_SPOTS] = MARKER;
+      const uint32_t old = x->qc_count;
+      x->qc_count -= 0x10000u;
+      if (old - 0x10000u == QC_SPOTS) *reclaim = vl_set_front(s, q, c) | 0x80000000u;
+    }
+  }
+  q->count--;
+}
+
+/* ------------------------------------------------------------------ carves */
+static uint64_t capacity(int kind, const cls_t* q) {
+  if (is_vl(kind)) return 0xFFFFFFFFull;
+  if (is_va(kind)) return (uint64_t)q->ring_size * QC_SPOTS - 2ull * QC_SPOTS;
+  return q->ring_size;
+}
+
+/* Enqueue chunk indices in lane order (ChunkQueue::enqueue ChunkQueue_impl.cuh:20-38). */
+static void enqueue_chunks(ouro_sim* s, inst_t* I, cls_t** qs, int n, const uint32_t* chunks) {
+  uint32_t v[64];
+  for (int i = 0; i < n; ++i) {
+    qs[i]->count++;
+    v[i] = qs[i]->back++;
+    ch(s, chunks[i])->queue_pos = v[i];
+  }
+  put_lanes(s, I, qs, n, v, chunks);
+}
+
+/* PageQueue allocation lambda + enqueueChunk (PageQueue_impl.cuh:108-133, :58-79), k chunks. */
+static uint32_t carve(ouro_sim* s, inst_t* I, cls_t* q, uint32_t k) {
+  uint32_t chunks[64];
+  uint32_t ok = alloc_chunks(s, &I->reuse, I->add, k, chunks);
+  const uint32_t unit = is_chunk_kind(I->kind) ? 1u : q->ppc;
+  if (ok) {
+    if ((uint64_t)q->carved + (uint64_t)ok * unit > capacity(I->kind, q)) {
+      s->error |= ERR_QUEUE_FULL;
+      ok = 0;
+    } else {
+      q->carved += ok * unit;
+    }
+  }
+  for (uint32_t j = 0; j < ok; ++j) {
+    chunk_t* x = ch(s, chunks[j]);
+    x->page_size = q->page_size;
+    x->size = q->ppc;
+    if (is_chunk_kind(I->kind)) {
+      /* ChunkIndexChunk with all pages free (ChunkAccess.cuh:69-89), 64-bit words */
+      x->type = 2;
+      x->count = (int32_t)q->ppc;
+      x->queue_pos = MARKER;
+      for (uint32_t w = 0; w < 8; ++w) {
+        const uint32_t lo = w * 64;
+        x->mask[w] = q->ppc > lo ? ((q->ppc - lo >= 64) ? ~0ull : ((1ull << (q->ppc - lo)) - 1ull)) : 0ull;
+      }
+    } else {
+      x->type = 1; /* PageChunk (PageIndexChunk.cuh:74-78) */
+    }
+  }
+  if (is_chunk_kind(I->kind)) {
+    cls_t* qs[64];
+    for (uint32_t j = 0; j < ok; ++j) qs[j] = q;
+    enqueue_chunks(s, I, qs, (int)ok, chunks);
+  } else {
+    const uint32_t n = ok * q->ppc;
+    const uint32_t v0 = q->back;
+    q->back += n;
+    /* the device writes in batches of up to 64 positions; queue-chunk allocation order is the
+       position order either way */
+    for (uint32_t e0 = 0; e0 < n; e0 += 64) {
+      cls_t* qs[64];
+      uint32_t v[64], val[64];
+      int m = 0;
+      for (uint32_t e = e0; e < n && e < e0 + 64; ++e, ++m) {
+        qs[m] = q;
+        v[m] = v0 + e;
+        val[m] = make_index(chunks[e / q->ppc], e % q->ppc);
+      }
+      put_lanes(s, I, qs, m, v, val);
+    }
+  }
+  /* signalExpected (BulkSemaphore_impl.cuh:305-308) after publication */
+  q->cnt += (int64_t)ok * (is_chunk_kind(I->kind) ? q->ppc : q->ppc);
+  q->exp -= (int64_t)k * q->ppc;
+  return ok;
+}
+
+/* ------------------------------------------------------------------ wave ops */
+typedef struct {
+  int lanes[64];
+  int n;
+  uint32_t qi;
+} group_t;
+
+/* Groups of lanes with equal class, ordered by their lowest lane (match<64> + peeling). */
+static int make_groups(const int* lanes, int n, const uint32_t* cls, group_t* g) {
+  int ng = 0;
+  for (int i = 0; i < n; ++i) {
+    int found = -1;
+    for (int j = 0; j < ng; ++j)
+      if (g[j].qi == cls[i]) found = j;
+    if (found < 0) {
+      found = ng++;
+      g[found].n = 0;
+      g[found].qi = cls[i];
+    }
+    g[found].lanes[g[found].n++] = lanes[i];
+  }
+  return ng;
+}
+
+/* BulkSemaphore::wait (BulkSemaphore_impl.cuh:199-300) for every group leader of one wave: each
+ * pass decides for all pending leaders, then carves run in leader order. */
+static void acquire(ouro_sim* s, inst_t* I, group_t* g, int ng, int* ok) {
+  int pending[64];
+  for (int j = 0; j < ng; ++j) {
+    pending[j] = 1;
+    ok[j] = 0;
+  }
+  for (int pass = 0; pass < 64; ++pass) {
+    uint32_t carve_k[64];
+    int any = 0;
+    for (int j = 0; j < ng; ++j) {
+      carve_k[j] = 0;
+      if (!pending[j]) continue;
+      any = 1;
+      cls_t* q = &I->q[g[j].qi];
+      const int64_t num = g[j].n;
+      if (q->cnt >= num) {
+        q->cnt -= num;
+        pending[j] = 0;
+        ok[j] = 1;
+        continue;
+      }
+      if (q->cnt + q->exp - q->res < num) {
+        const int64_t k = (num - (q->cnt + q->exp - q->res) + q->ppc - 1) / q->ppc;
+        q->exp += k * q->ppc;
+        carve_k[j] = (uint32_t)k;
+      } else if (q->cnt >= num) {
+        q->cnt -= num;
+        pending[j] = 0;
+        ok[j] = 1;
+      } else {
+        /* reserve: impossible inside a single wavefront, as no carve is in flight */
+        s->error |= ERR_TIMEOUT;
+        pending[j] = 0;
+      }
+    }
+    if (!any) break;
+    for (int j = 0; j < ng; ++j)
+      if (carve_k[j]) {
+        const uint32_t got = carve(s, I, &I->q[g[j].qi], carve_k[j]);
+        if (got < carve_k[j]) pending[j] = 0;
+      }
+  }
+}
+
+static uint64_t page_offset(const ouro_sim* s, uint32_t c, uint32_t page, uint32_t page_size) {
+  (void)s;
+  return (uint64_t)c * STRIDE + META + (uint64_t)page * page_size;
+}
+
+static uint32_t mask_words(uint32_t ppc) {
+  uint32_t w = (ppc + 63) / 64;
+  return w == 0 ? 1 : (w > 8 ? 8 : w);
+}
+
+/* Group claim for chunk-based queues: reserve pages chunk by chunk on the chunk count, take them
+ * from the 64-bit mask words, lowest bits first, starting at word (size - count) / 64.
+ * (ChunkQueue::allocPage ChunkQueue_impl.cuh:88-151 and ChunkAccess::allocPage
+ * ChunkAccess_impl.cuh:38-135, as the device's W = 64 rule.) */
+static void claim_group(ouro_sim* s, inst_t* I, group_t* g, uint64_t* out, uint32_t* reclaim) {
+  cls_t* q = &I->q[g->qi];
+  uint32_t v = q->front, cursor = MARKER, need = (uint32_t)g->n, assigned = 0;
+  int guard = 0;
+  *reclaim = MARKER;
+  while (assigned < (uint32_t)g->n) {
+    if (++guard > 10000000) {
+      s->error |= ERR_TIMEOUT;
+      return;
+    }
+    const uint32_t sl = peek(s, I, q, v, &cursor);
+    int advance = 1;
+    if (sl != MARKER && chunk_ok(s, sl)) {
+      chunk_t* x = ch(s, sl);
+      const int32_t cnt = x->count;
+      if (cnt > 0) {
+        const uint32_t t = (uint32_t)cnt < need ? (uint32_t)cnt : need;
+        x->count -= (int32_t)t;
+        need -= t;
+        const uint32_t nw = mask_words(x->size);
+        uint32_t w = ((x->size - (uint32_t)cnt) / 64u) % nw;
+        if (cnt == (int32_t)t) {
+          x->queue_pos = MARKER;
+          uint32_t rc = MARKER;
+          erase(s, I, q, v, &cursor, &rc);
+          if (rc != MARKER) {
+            if (*reclaim != MARKER) {
+              /* a second reclaim in one scan is issued immediately */
+              if (is_va(I->kind)) ring_put(s, &s->inst[0].reuse, *reclaim);
+              else vl_set_old(s, q, *reclaim & 0x7FFFFFFFu);
+            }
+            *reclaim = rc;
+          }
+        } else {
+          advance = 0;
+        }
+        uint32_t t_rem = t;
+        int wguard = 0;
+        while (t_rem > 0 && wguard++ < 64) {
+          const uint64_t mw = x->mask[w];
+          uint64_t take = 0;
+          uint32_t c = 0;
+          for (uint32_t b = 0; b < 64 && c < t_rem; ++b)
+            if ((mw >> b) & 1ull) {
+              take |= 1ull << b;
+              ++c;
+            }
+          if (take == 0) {
+            w = (w + 1) % nw;
+            continue;
+          }
+          x->mask[w] &= ~take;
+          for (uint32_t b = 0; b < 64; ++b)
+            if ((take >> b) & 1ull) {
+              const int lane = g->lanes[assigned++];
+              out[lane] = page_offset(s, sl, w * 64 + b, q->page_size);
+            }
+          t_rem -= c;
+          if (t_rem > 0) w = (w + 1) % nw;
+        }
+        if (t_rem) s->error |= ERR_TIMEOUT;
+      }
+    }
+    if (advance) {
+      ++v;
+      if (need > 0 && v - q->back < 0x80000000u) {
+        v = q->front;
+        cursor = MARKER;
+      }
+    }
+  }
+}
+
+static void finish_reclaim(ouro_sim* s, inst_t* I, cls_t* q, uint32_t reclaim) {
+  if (reclaim == MARKER) return;
+  if (is_va(I->kind)) ring_put(s, &s->inst[0].reuse, reclaim);
+  else if (is_vl(I->kind)) vl_set_old(s, q, reclaim & 0x7FFFFFFFu);
+}
+
+static void inst_malloc(ouro_sim* s, inst_t* I, const int* lanes, int n, const uint32_t* sizes, uint64_t* out) {
+  uint32_t cls[64];
+  for (int i = 0; i < n; ++i) cls[i] = (uint32_t)sim_class_of(sizes[lanes[i]], I->smallest);
+  group_t g[64];
+  const int ng = make_groups(lanes, n, cls, g);
+  int ok[64];
+  acquire(s, I, g, ng, ok);
+  if (!is_chunk_kind(I->kind)) {
+    /* dequeue positions: front fetch-add per group, rank order (atomicAggInc) */
+    cls_t* qs[64];
+    uint32_t v[64], got[64];
+    int owner[64], m = 0;
+    /* lanes in lane order, each with its group's front position */
+    uint32_t base[64];
+    for (int j = 0; j < ng; ++j)
+      if (ok[j]) {
+        base[j] = I->q[g[j].qi].front;
+        I->q[g[j].qi].front += (uint32_t)g[j].n;
+      }
+    for (int i = 0; i < n; ++i) {
+      const int lane = lanes[i];
+      for (int j = 0; j < ng; ++j) {
+        if (!ok[j]) continue;
+        for (int r = 0; r < g[j].n; ++r)
+          if (g[j].lanes[r] == lane) {
+            qs[m] = &I->q[g[j].qi];
+            v[m] = base[j] + (uint32_t)r;
+            owner[m] = lane;
+            ++m;
+          }
+      }
+    }
+    take_lanes(s, I, qs, m, v, got);
+    for (int i = 0; i < m; ++i) {
+      const uint32_t c = got[i] >> PAGE_BITS, p = got[i] & 511u;
+      if (got[i] == MARKER || !chunk_ok(s, c) || p >= qs[i]->ppc) {
+        if (got[i] != MARKER) s->error |= ERR_CORRUPT;
+        continue;
+      }
+      out[owner[i]] = page_offset(s, c, p, qs[i]->page_size);
+    }
+    return;
+  }
+  uint32_t rec[64];
+  for (int j = 0; j < ng; ++j) {
+    rec[j] = MARKER;
+    if (ok[j]) claim_group(s, I, &g[j], out, &rec[j]);
+  }
+  /* deferred queue-chunk reclamation, by leader lane */
+  for (int j = 0; j < ng; ++j) finish_reclaim(s, I, &I->q[g[j].qi], rec[j]);
+}
+
+void sim_wave_malloc(ouro_sim* s, int n, const uint32_t* sizes, uint64_t* out) {
+  int rest[64], nr = 0;
+  for (int i = 0; i < n; ++i) {
+    out[i] = UINT64_MAX;
+    if (sizes[i]) rest[nr++] = i;
+  }
+  for (int k = 0; k < s->ninst && nr; ++k) {
+    inst_t* I = &s->inst[k];
+    int here[64], nh = 0, later[64], nl = 0;
+    for (int i = 0; i < nr; ++i) {
+      if (sizes[rest[i]] <= I->largest) here[nh++] = rest[i];
+      else later[nl++] = rest[i];
+    }
+    if (nh) inst_malloc(s, I, here, nh, sizes, out);
+    memcpy(rest, later, sizeof(int) * (size_t)nl);
+    nr = nl;
+  }
+  if (nr) s->error |= ERR_TOO_LARGE;
+}
+
+/* ---- free */
+static void inst_free(ouro_sim* s, inst_t* I, const int* lanes, int n, const uint32_t* qi, const uint32_t* chunk,
+                      const uint32_t* page) {
+  if (!is_chunk_kind(I->kind)) {
+    /* back positions per class group in lane order, publish, then signal (PageQueue_impl.cuh:137-143) */
+    cls_t* qs[64];
+    uint32_t v[64], val[64];
+    uint32_t cls[64];
+    for (int i = 0; i < n; ++i) cls[i] = qi[i];
+    group_t g[64];
+    const int ng = make_groups(lanes, n, cls, g);
+    uint32_t base[64];
+    for (int j = 0; j < ng; ++j) {
+      base[j] = I->q[g[j].qi].back;
+      I->q[g[j].qi].back += (uint32_t)g[j].n;
+    }
+    for (int i = 0; i < n; ++i) {
+      for (int j = 0; j < ng; ++j)
+        for (int r = 0; r < g[j].n; ++r)
+          if (g[j].lanes[r] == lanes[i]) {
+            qs[i] = &I->q[g[j].qi];
+            v[i] = base[j] + (uint32_t)r;
+          }
+      val[i] = make_index(chunk[i], page[i]);
+    }
+    put_lanes(s, I, qs, n, v, val);
+    for (int i = 0; i < n; ++i) qs[i]->cnt += 1;
+    return;
+  }
+  /* ChunkAccess::freePage (ChunkAccess_impl.cuh:7-25) grouped by chunk */
+  for (int i = 0; i < n; ++i) ch(s, chunk[i])->mask[page[i] / 64] |= 1ull << (page[i] % 64);
+  group_t g[64];
+  const int ng = make_groups(lanes, n, chunk, g); /* groups keyed by chunk index */
+  uint32_t ff_chunk[64];
+  cls_t* ff_q[64];
+  int nff = 0;
+  uint32_t flashed[64];
+  int nfl = 0;
+  int64_t sig[64];
+  for (int j = 0; j < ng; ++j) {
+    const uint32_t c = g[j].qi;
+    int idx = 0;
+    for (int i = 0; i < n; ++i)
+      if (lanes[i] == g[j].lanes[0]) idx = i;
+    cls_t* q = &I->q[qi[idx]];
+    chunk_t* x = ch(s, c);
+    const int32_t old = x->count;
+    x->count += g[j].n;
+    sig[j] = g[j].n;
+    if (old == 0) {
+      ff_chunk[nff] = c;
+      ff_q[nff++] = q;
+    } else if ((uint32_t)(old + g[j].n) == x->size && I->kind == K_CQ && q->count > (int32_t)s->lower_fill) {
+      /* flash (ChunkQueue_impl.cuh:163-199) */
+      const int64_t red = (int64_t)x->size - g[j].n;
+      if (q->cnt >= red) {
+        q->cnt -= red;
+        x->count = 0;
+        *std_slot(q, x->queue_pos) = MARKER;
+        q->count--;
+        q->carved -= x->size;
+        flashed[nfl++] = c;
+        sig[j] = 0;
+      }
+    }
+  }
+  /* FIRST_FREE: back into the queue (positions per queue in lane order), before signalling */
+  if (nff) enqueue_chunks(s, I, ff_q, nff, ff_chunk);
+  for (int i = 0; i < nfl; ++i) ring_put(s, &I->reuse, flashed[i]);
+  for (int j = 0; j < ng; ++j) {
+    int idx = 0;
+    for (int i = 0; i < n; ++i)
+      if (lanes[i] == g[j].lanes[0]) idx = i;
+    I->q[qi[idx]].cnt += sig[j];
+  }
+}
+
+void sim_wave_free(ouro_sim* s, int n, const uint64_t* offsets) {
+  int lanes1[2][64], n1[2] = {0, 0};
+  uint32_t qi[2][64], chunk[2][64], page[2][64];
+  const uint32_t nwords = (uint32_t)((s->max_chunks + 31) / 32);
+  for (int i = 0; i < n; ++i) {
+    const uint64_t off = offsets[i];
+    if (off == UINT64_MAX) continue;
+    if (off >= s->max_chunks * STRIDE) {
+      s->error |= ERR_BAD_FREE;
+      continue;
+    }
+    const uint32_t c = sim_locate(s->locator, nwords, (uint32_t)(off / STRIDE));
+    if (c == MARKER) {
+      s->error |= ERR_BAD_FREE;
+      continue;
+    }
+    const uint32_t ps = ch(s, c)->page_size;
+    const uint64_t base = (uint64_t)c * STRIDE + META;
+    if (ps == 0 || off < base) {
+      s->error |= ERR_BAD_FREE;
+      continue;
+    }
+    int k = 0;
+    while (k < s->ninst && ps > s->inst[k].largest) ++k;
+    if (k == s->ninst) {
+      s->error |= ERR_BAD_FREE;
+      continue;
+    }
+    inst_t* I = &s->inst[k];
+    const uint64_t rel = off - base;
+    if (ps < I->smallest || (ps & (ps - 1)) || rel % ps || rel / ps >= I->chunk_size / ps) {
+      s->error |= ERR_BAD_FREE;
+      continue;
+    }
+    lanes1[k][n1[k]] = i;
+    qi[k][n1[k]] = (uint32_t)sim_class_of(ps, I->smallest);
+    chunk[k][n1[k]] = c;
+    page[k][n1[k]] = (uint32_t)(rel / ps);
+    n1[k]++;
+  }
+  for (int k = 0; k < s->ninst; ++k)
+    if (n1[k]) inst_free(s, &s->inst[k], lanes1[k], n1[k], qi[k], chunk[k], page[k]);
+}
+
+/* ------------------------------------------------------------------ create / state */
+ouro_sim* sim_create(int variant, uint64_t pool_bytes) {
+  if (variant < 0 || variant > 5) return NULL;
+  ouro_sim* s = (ouro_sim*)calloc(1, sizeof(ouro_sim));
+  s->variant = variant;
+  sim_layout(variant, pool_bytes, &s->max_chunks, &s->alloc_size);
+  s->chunks = (chunk_t*)calloc(s->max_chunks + 4, sizeof(chunk_t));
+  s->locator = (uint32_t*)calloc((s->max_chunks + 31) / 32 + 1, 4);
+  s->lower_fill = (uint32_t)(STD_ENTRIES * 0.1f);
+  s->ninst = 2;
+  uint32_t qc_next = 0;
+  for (int k = 0; k < 2; ++k) {
+    inst_t* I = &s->inst[k];
+    I->kind = variant;
+    I->smallest = k == 0 ? 16u : (16u << 10);
+    I->nq = k == 0 ? 10u : 2u;
+    I->largest = I->smallest << (I->nq - 1);
+    I->chunk_size = I->largest;
+    I->add = I->chunk_size / BASE_CHUNK;
+    I->reuse.size = REUSE_ENTRIES;
+    I->reuse.ring = (uint32_t*)malloc(REUSE_ENTRIES * 4);
+    memset(I->reuse.ring, 0xFF, REUSE_ENTRIES * 4);
+    for (uint32_t i = 0; i < I->nq; ++i) {
+      cls_t* q = &I->q[i];
+      q->page_size = I->smallest << i;
+      q->ppc = I->chunk_size / q->page_size;
+      q->ring_size = (uint32_t)entries_of(variant);
+      q->front_qc = q->back_qc = q->old_qc = MARKER;
+      if (is_std(variant)) q->blocks = (uint32_t**)calloc(q->ring_size / BLOCK, sizeof(uint32_t*));
+      if (!is_std(variant)) {
+        /* PageQueueVA::init etc. take queue chunks 0, 1, 2, ... (PageQueueVA_impl.cuh:13-28) */
+        const uint32_t c = qc_next++;
+        qc_init(s, c, 0);
+        s->locator[c >> 5] |= 1u << (c & 31u);
+        if (is_va(variant)) {
+          q->va = (uint32_t*)malloc(q->ring_size * 4);
+          memset(q->va, 0xFF, q->ring_size * 4);
+          q->va[0] = c;
+        } else {
+          q->front_qc = q->back_qc = q->old_qc = c;
+        }
+      }
+    }
+  }
+  s->next_free = qc_next;
+  return s;
+}
+
+void sim_destroy(ouro_sim* s) {
+  if (!s) return;
+  for (int k = 0; k < 2; ++k) {
+    inst_t* I = &s->inst[k];
+    free(I->reuse.ring);
+    for (uint32_t i = 0; i < I->nq; ++i) {
+      cls_t* q = &I->q[i];
+      if (q->blocks) {
+        for (uint32_t b = 0; b < q->ring_size / BLOCK; ++b) free(q->blocks[b]);
+        free(q->blocks);
+      }
+      free(q->va);
+    }
+  }
+  for (uint64_t c = 0; c < s->max_chunks + 4; ++c) free(s->chunks[c].slots);
+  free(s->chunks);
+  free(s->locator);
+  free(s);
+}
+
+void sim_get_state(const ouro_sim* s, sim_state* out) {
+  memset(out, 0, sizeof(*out));
+  out->next_free_chunk = s->next_free;
+  out->error = s->error;
+  uint32_t k = 0;
+  for (int i = 0; i < s->ninst; ++i) {
+    const inst_t* I = &s->inst[i];
+    out->reuse_count[i] = I->reuse.count;
+    out->reuse_front[i] = I->reuse.front;
+    out->reuse_back[i] = I->reuse.back;
+    for (uint32_t j = 0; j < I->nq; ++j, ++k) {
+      const cls_t* q = &I->q[j];
+      out->sem_count[k] = q->cnt;
+      out->sem_expected[k] = (uint32_t)q->exp;
+      out->sem_reserved[k] = (uint32_t)q->res;
+      out->front[k] = q->front;
+      out->back[k] = q->back;
+      out->queue_count[k] = q->count;
+      out->carved[k] = q->carved;
+    }
+  }
+  out->num_classes = k;
+}
+
+int sim_chunk_info(const ouro_sim* s, uint32_t c, uint32_t* page_size, int32_t* count, uint64_t* mask8) {
+  if (c >= s->max_chunks) return -1;
+  const chunk_t* x = &s->chunks[c];
+  if (page_size) *page_size = x->page_size;
+  if (count) *count = x->count;
+  if (mask8) memcpy(mask8, x->mask, sizeof(x->mask));
+  return x->type;
+}

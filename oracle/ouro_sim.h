@@ -1,0 +1,70 @@
+/*
+ * Sequential host simulation of the Ouroboros queue logic -- the oracle.
+ *
+ * TEST INFRASTRUCTURE ONLY: used by tests/, __graft_entry__.smoke() and bench.py's CPU leg as the
+ * checker. The product (libouro.so, include/ouro) never links or calls it.
+ *
+ * It replays one wavefront's malloc / free calls with the ordering rules of the device code:
+ * lanes requesting the same class form one group led by the lowest lane; chunk carves run one
+ * group at a time in lane order; queue positions inside a group follow lane order; queue-chunk
+ * allocation and reclamation are issued in lane order. With a single wavefront there is no other
+ * concurrency, so a single-wave device trace must equal this replay bit for bit.
+ */
+#ifndef OURO_SIM_H
+#define OURO_SIM_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ouro_sim ouro_sim;
+
+/* variant: 0 PQ, 1 CQ, 2 VAPQ, 3 VACQ, 4 VLPQ, 5 VLCQ (MultiOuro* chains). */
+ouro_sim* sim_create(int variant, uint64_t pool_bytes);
+void sim_destroy(ouro_sim* s);
+/* MemoryInitialization.cuh:156-166 */
+void sim_layout(int variant, uint64_t pool_bytes, uint64_t* max_chunks, uint64_t* allocation_size);
+
+/* One wavefront of n <= 64 lanes. sizes[i] == 0 marks an inactive lane. out[i] = byte offset of
+ * the block from the chunk region, or UINT64_MAX for nullptr. */
+void sim_wave_malloc(ouro_sim* s, int n, const uint32_t* sizes, uint64_t* out);
+/* offsets[i] == UINT64_MAX marks a lane that frees nullptr (or is inactive). */
+void sim_wave_free(ouro_sim* s, int n, const uint64_t* offsets);
+
+typedef struct {
+  uint32_t next_free_chunk;
+  uint32_t error;
+  uint32_t num_classes;
+  int64_t sem_count[16];
+  uint32_t sem_expected[16];
+  uint32_t sem_reserved[16];
+  uint32_t front[16];
+  uint32_t back[16];
+  int32_t queue_count[16];
+  uint32_t carved[16];
+  int32_t reuse_count[2];
+  uint32_t reuse_front[2];
+  uint32_t reuse_back[2];
+} sim_state;
+void sim_get_state(const ouro_sim* s, sim_state* out);
+
+/* Chunk-level views for tests: 0 unused, 1 page chunk, 2 chunk-queue chunk, 3 queue chunk. */
+int sim_chunk_info(const ouro_sim* s, uint32_t chunk, uint32_t* page_size, int32_t* count, uint64_t* mask8);
+
+/* Class index / page size / pages-per-chunk closed forms (Helper.cuh:42-65) for a request size. */
+int sim_class_of(uint32_t size, uint32_t smallest);
+
+/* BulkSemaphore packed-word arithmetic with the 26/19/19 split used on the device. */
+uint64_t sim_sem_make(int64_t count, uint64_t expected, uint64_t reserved);
+int64_t sim_sem_count(uint64_t v);
+uint64_t sim_sem_expected(uint64_t v);
+uint64_t sim_sem_reserved(uint64_t v);
+
+/* ChunkLocator::getChunkIndex (ChunkLocator.cuh:29-51) over a caller-supplied bitmap. */
+uint32_t sim_locate(const uint32_t* bits, uint32_t nwords, uint32_t idx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

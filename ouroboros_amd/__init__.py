@@ -1,0 +1,296 @@
+"""Python host driver for the MI355X Ouroboros allocator (libouro.so, C ABI in include/ouro_capi.h).
+
+The allocator itself is device code: kernels call malloc/free through include/ouro/instances.hpp.
+This module drives the library's batched entry points and its trace harness from Python, with the
+reference's host vocabulary (Ouroboros::initialize, getDeviceMemoryManager,
+include/device/Ouroboros.cuh:281,297). The library must be built (ouroboros_amd/build.py); there
+is no fallback path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+from pathlib import Path
+
+import numpy as np
+
+_LIB_PATH = Path(__file__).resolve().parent / "libouro.so"
+
+VARIANTS = {"PQ": 0, "CQ": 1, "VAPQ": 2, "VACQ": 3, "VLPQ": 4, "VLCQ": 5}
+VARIANT_NAMES = {v: k for k, v in VARIANTS.items()}
+MAX_CLASSES = 16
+
+ERR_BITS = {
+    1 << 0: "OOM",
+    1 << 1: "QUEUE_FULL",
+    1 << 2: "VSTART",
+    1 << 3: "TIMEOUT",
+    1 << 4: "BAD_FREE",
+    1 << 5: "TOO_LARGE",
+    1 << 6: "REUSE_FULL",
+    1 << 7: "LOCATOR",
+    1 << 8: "CORRUPT",
+}
+
+
+class OuroError(RuntimeError):
+    pass
+
+
+class State(C.Structure):
+    _fields_ = [
+        ("allocation_size", C.c_uint64),
+        ("max_chunks", C.c_uint64),
+        ("data_offset", C.c_uint64),
+        ("next_free_chunk", C.c_uint32),
+        ("error", C.c_uint32),
+        ("variant", C.c_uint32),
+        ("num_classes", C.c_uint32),
+        ("page_size", C.c_uint32 * MAX_CLASSES),
+        ("pages_per_chunk", C.c_uint32 * MAX_CLASSES),
+        ("sem_count", C.c_int64 * MAX_CLASSES),
+        ("sem_expected", C.c_uint32 * MAX_CLASSES),
+        ("sem_reserved", C.c_uint32 * MAX_CLASSES),
+        ("front", C.c_uint32 * MAX_CLASSES),
+        ("back", C.c_uint32 * MAX_CLASSES),
+        ("queue_count", C.c_int32 * MAX_CLASSES),
+        ("carved", C.c_uint32 * MAX_CLASSES),
+        ("front_qc", C.c_uint32 * MAX_CLASSES),
+        ("kind", C.c_uint32 * MAX_CLASSES),
+        ("reuse_count", C.c_int32 * 2),
+    ]
+
+
+class CheckResult(C.Structure):
+    _fields_ = [
+        ("violations", C.c_uint64),
+        ("chunks_checked", C.c_uint64),
+        ("pages_free", C.c_uint64),
+        ("first_class", C.c_int32),
+        ("first_chunk", C.c_int32),
+        ("message", C.c_char * 160),
+    ]
+
+
+class TraceConfig(C.Structure):
+    _fields_ = [
+        ("threads", C.c_uint32),
+        ("rounds", C.c_uint32),
+        ("agg", C.c_uint32),
+        ("block", C.c_uint32),
+        ("check_rounds", C.c_uint32),
+        ("latency", C.c_uint32),
+        ("hold_permille", C.c_uint32),
+        ("pad", C.c_uint32),
+        ("hold_seed", C.c_uint64),
+        ("sizes", C.POINTER(C.c_uint32)),
+        ("round_sizes", C.POINTER(C.c_uint32)),
+        ("offsets_out", C.POINTER(C.c_uint64)),
+    ]
+
+
+class TraceResult(C.Structure):
+    _fields_ = [
+        ("alloc_ms", C.c_double),
+        ("free_ms", C.c_double),
+        ("write_ms", C.c_double),
+        ("verify_ms", C.c_double),
+        ("mallocs", C.c_uint64),
+        ("frees", C.c_uint64),
+        ("null_returns", C.c_uint64),
+        ("verify_errors", C.c_uint64),
+        ("bounds_errors", C.c_uint64),
+        ("overlap_errors", C.c_uint64),
+        ("align_errors", C.c_uint64),
+        ("touch_bytes", C.c_uint64),
+        ("peak_chunks", C.c_uint64),
+        ("peak_live_chunks", C.c_uint64),
+        ("ideal_chunks", C.c_uint64),
+        ("lat_samples", C.c_uint64),
+        ("lat_p50_ns", C.c_double),
+        ("lat_p99_ns", C.c_double),
+        ("lat_max_ns", C.c_double),
+        ("error_word", C.c_uint32),
+        ("pad", C.c_uint32),
+    ]
+
+    def as_dict(self) -> dict:
+        d = {name: getattr(self, name) for name, _ in self._fields_ if name != "pad"}
+        d["errors"] = decode_errors(self.error_word)
+        return d
+
+
+# Entry points and their C signatures (include/ouro_capi.h).
+SIGNATURES = {
+    "ouro_create": (C.c_int, [C.c_int, C.c_uint64, C.POINTER(C.c_void_p)]),
+    "ouro_destroy": (C.c_int, [C.c_void_p]),
+    "ouro_device_manager": (C.c_void_p, [C.c_void_p]),
+    "ouro_data_base": (C.c_void_p, [C.c_void_p]),
+    "ouro_layout": (C.c_int, [C.c_int, C.c_uint64, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+    "ouro_malloc_batch": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_int, C.c_void_p]),
+    "ouro_free_batch": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_int, C.c_void_p]),
+    "ouro_query": (C.c_int, [C.c_void_p, C.POINTER(State)]),
+    "ouro_check_quiescent": (C.c_int, [C.c_void_p, C.POINTER(CheckResult)]),
+    "ouro_snapshot": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "ouro_run_trace": (C.c_int, [C.c_void_p, C.POINTER(TraceConfig), C.POINTER(TraceResult)]),
+    "ouro_gen_sizes": (C.c_int, [C.c_uint64, C.c_uint32, C.c_int, C.c_double, C.c_uint32, C.POINTER(C.c_uint32)]),
+    "ouro_bench_atomics": (C.c_int, [C.c_uint32, C.c_uint32, C.c_int, C.POINTER(C.c_double)]),
+    "ouro_bench_copy": (C.c_int, [C.c_uint64, C.c_uint32, C.POINTER(C.c_double)]),
+}
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """Load libouro.so; raises if it has not been built (no fallback)."""
+    global _lib
+    if _lib is None:
+        if not _LIB_PATH.exists():
+            raise OuroError(f"{_LIB_PATH} is missing: build it with ouroboros_amd/build.py")
+        _lib = C.CDLL(str(_LIB_PATH))
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(_lib, name)
+            fn.restype = res
+            fn.argtypes = args
+    return _lib
+
+
+def decode_errors(word: int) -> list[str]:
+    return [name for bit, name in ERR_BITS.items() if word & bit]
+
+
+def _check(rc: int, what: str) -> None:
+    if rc != 0:
+        raise OuroError(f"{what} failed with status {rc}")
+
+
+def layout(variant: str | int, pool_bytes: int) -> tuple[int, int]:
+    """(max_chunks, allocation_size) for a pool (MemoryInitialization.cuh:156-166); no device needed."""
+    v = VARIANTS[variant] if isinstance(variant, str) else variant
+    mc, asz = C.c_uint64(), C.c_uint64()
+    _check(lib().ouro_layout(v, pool_bytes, C.byref(mc), C.byref(asz)), "ouro_layout")
+    return mc.value, asz.value
+
+
+def gen_sizes(seed: int, n: int, dist: str = "uniform", param: float = 0.0, classes: int = 10) -> np.ndarray:
+    """Host trace generator: 'fixed' (param = bytes), 'uniform' over classes, 'zipf' (param = s)."""
+    d = {"fixed": 0, "uniform": 1, "zipf": 2}[dist]
+    out = np.empty(n, dtype=np.uint32)
+    _check(
+        lib().ouro_gen_sizes(seed, n, d, float(param), classes, out.ctypes.data_as(C.POINTER(C.c_uint32))),
+        "ouro_gen_sizes",
+    )
+    return out
+
+
+def bench_atomics(threads: int = 1 << 20, iters: int = 64, pattern: str = "distinct") -> float:
+    """Device-scope u32 fetch-add-with-return rate (atomics/s)."""
+    p = {"distinct": 0, "per_wave": 1, "same": 2}[pattern]
+    r = C.c_double()
+    _check(lib().ouro_bench_atomics(threads, iters, p, C.byref(r)), "ouro_bench_atomics")
+    return r.value
+
+
+def bench_copy(nbytes: int = 1 << 30, reps: int = 10) -> float:
+    """Device copy bandwidth in GB/s (read + write bytes)."""
+    r = C.c_double()
+    _check(lib().ouro_bench_copy(nbytes, reps, C.byref(r)), "ouro_bench_copy")
+    return r.value
+
+
+@dataclass
+class TraceOutput:
+    result: dict
+    offsets: np.ndarray | None
+
+
+class MemoryManager:
+    """One allocator pool on the current HIP device (MultiOuro<variant>)."""
+
+    def __init__(self, variant: str = "PQ", pool_bytes: int = 512 << 20):
+        self.variant = variant
+        self.pool_bytes = pool_bytes
+        h = C.c_void_p()
+        _check(lib().ouro_create(VARIANTS[variant], pool_bytes, C.byref(h)), f"ouro_create({variant})")
+        self._h = h
+
+    # reference host API names
+    def getDeviceMemoryManager(self) -> int:  # noqa: N802 - reference name
+        return lib().ouro_device_manager(self._h)
+
+    def data_base(self) -> int:
+        return lib().ouro_data_base(self._h)
+
+    def close(self) -> None:
+        if self._h:
+            lib().ouro_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def query(self) -> State:
+        st = State()
+        _check(lib().ouro_query(self._h, C.byref(st)), "ouro_query")
+        return st
+
+    def check_quiescent(self) -> CheckResult:
+        r = CheckResult()
+        _check(lib().ouro_check_quiescent(self._h, C.byref(r)), "ouro_check_quiescent")
+        return r
+
+    def malloc_batch(self, d_sizes: int, d_out: int, n: int, agg: int = 64, stream: int = 0) -> None:
+        _check(lib().ouro_malloc_batch(self._h, d_sizes, d_out, n, agg, stream), "ouro_malloc_batch")
+
+    def free_batch(self, d_ptrs: int, n: int, agg: int = 64, stream: int = 0) -> None:
+        _check(lib().ouro_free_batch(self._h, d_ptrs, n, agg, stream), "ouro_free_batch")
+
+    def run_trace(
+        self,
+        sizes: np.ndarray,
+        rounds: int,
+        agg: int = 64,
+        block: int = 256,
+        check_rounds: bool = False,
+        latency: bool = False,
+        keep_offsets: bool = False,
+        hold_permille: int = 0,
+        hold_seed: int = 0,
+    ) -> TraceOutput:
+        """Run `rounds` alloc / write / verify / free rounds (src/main.cu:146-167).
+
+        `sizes` is either [threads] (fixed across rounds) or [rounds, threads]."""
+        sizes = np.ascontiguousarray(sizes, dtype=np.uint32)
+        per_round = sizes.ndim == 2
+        n = sizes.shape[-1]
+        cfg = TraceConfig()
+        cfg.threads = n
+        cfg.rounds = rounds
+        cfg.agg = agg
+        cfg.block = block
+        cfg.check_rounds = int(check_rounds)
+        cfg.latency = int(latency)
+        cfg.hold_permille = hold_permille
+        cfg.hold_seed = hold_seed
+        ptr = sizes.ctypes.data_as(C.POINTER(C.c_uint32))
+        if per_round:
+            assert sizes.shape[0] == rounds
+            cfg.round_sizes = ptr
+        else:
+            cfg.sizes = ptr
+        offsets = None
+        if keep_offsets:
+            offsets = np.empty((rounds, n), dtype=np.uint64)
+            cfg.offsets_out = offsets.ctypes.data_as(C.POINTER(C.c_uint64))
+        res = TraceResult()
+        _check(lib().ouro_run_trace(self._h, C.byref(cfg), C.byref(res)), "ouro_run_trace")
+        return TraceOutput(res.as_dict(), offsets)

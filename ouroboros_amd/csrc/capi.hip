@@ -1,0 +1,588 @@
+// extern "C" entry points of libouro.so (declared in include/ouro_capi.h), the trace harness that
+// mirrors src/main.cu's alloc / write / verify / free rounds, and the host-side invariant checkers.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <cmath>
+#include <memory>
+#include <new>
+#include <unordered_set>
+#include <vector>
+
+#include "manager.hpp"
+
+using namespace ouro_lib;
+
+struct ouro_manager {
+  std::unique_ptr<IManager> impl;
+};
+
+namespace {
+
+IManager* make_variant(int v) {
+  switch (v) {
+    case OURO_PQ: return make_pq();
+    case OURO_CQ: return make_cq();
+    case OURO_VAPQ: return make_vapq();
+    case OURO_VACQ: return make_vacq();
+    case OURO_VLPQ: return make_vlpq();
+    case OURO_VLCQ: return make_vlcq();
+    default: return nullptr;
+  }
+}
+
+inline uint64_t splitmix64(uint64_t x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+// Independent stream per (seed, index).
+inline uint64_t keyed(uint64_t seed, uint64_t i) { return splitmix64(splitmix64(seed) ^ (i * 0xD1B54A32D192ED03ull)); }
+inline double unit(uint64_t r) { return (double)(r >> 11) * (1.0 / 9007199254740992.0); }
+
+// Page size a request of `size` bytes receives from the MultiOuro chain.
+inline uint32_t page_of(uint32_t size) {
+  uint32_t p = 16;
+  while (p < size) p <<= 1;
+  return p;
+}
+
+// ------------------------------------------------------------------ touch pass (write / verify)
+// Writes (or checks) the owning thread's id into every 32-bit word of its block, as
+// d_testWriteToMemory / d_testReadFromMemory (src/main.cu:20-53). Blocks up to 256 B are handled by
+// their own lane; larger blocks by the whole wavefront with coalesced 16-byte accesses. Pointers
+// outside [lo, hi) are counted, never dereferenced.
+__global__ void __launch_bounds__(256) k_touch(void* const* __restrict__ ptrs, const uint32_t* __restrict__ sizes,
+                                               uint32_t n, const uint8_t* lo, const uint8_t* hi, int verify,
+                                               unsigned long long* __restrict__ errs) {
+  const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool in = tid < n;
+  const uint8_t* p = in ? reinterpret_cast<const uint8_t*>(ptrs[tid]) : nullptr;
+  const uint32_t size = in ? sizes[tid] : 0u;
+  const uint32_t bytes = size & ~3u;
+  const bool ok = p != nullptr && p >= lo && p + bytes <= hi && (reinterpret_cast<uintptr_t>(p) & 15u) == 0;
+  if (p != nullptr && !ok) atomicAdd(&errs[1], 1ull);
+  unsigned long long bad = 0;
+  if (ok && bytes <= 256) {
+    uint32_t* w = reinterpret_cast<uint32_t*>(const_cast<uint8_t*>(p));
+    for (uint32_t i = 0; i < bytes / 4; ++i) {
+      if (!verify) w[i] = tid;
+      else bad += (w[i] != tid);
+    }
+  }
+  uint64_t todo = __ballot(ok && bytes > 256);
+  const uint32_t ln = __lane_id();
+  while (todo) {
+    const int l = __ffsll((unsigned long long)todo) - 1;
+    todo &= todo - 1;
+    const uint64_t addr = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(reinterpret_cast<uint64_t>(p) >> 32), l)
+                           << 32) |
+                          (uint32_t)__builtin_amdgcn_readlane((int)reinterpret_cast<uint64_t>(p), l);
+    const uint32_t nb = (uint32_t)__builtin_amdgcn_readlane((int)bytes, l);
+    const uint32_t val = (uint32_t)__builtin_amdgcn_readlane((int)tid, l);
+    uint4* b = reinterpret_cast<uint4*>(addr);
+    const uint32_t nv = nb / 16;
+    for (uint32_t i = ln; i < nv; i += 64) {
+      if (!verify) {
+        b[i] = make_uint4(val, val, val, val);
+      } else {
+        const uint4 x = b[i];
+        bad += (x.x != val) + (x.y != val) + (x.z != val) + (x.w != val);
+      }
+    }
+  }
+  if (verify && bad) atomicAdd(&errs[0], bad);
+}
+
+struct DevBuf {
+  void* p = nullptr;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  hipError_t alloc(size_t bytes) { return hipMalloc(&p, bytes ? bytes : 4); }
+  template <class T>
+  T* as() const { return reinterpret_cast<T*>(p); }
+};
+
+#define OURO_TRY(x)                          \
+  do {                                       \
+    hipError_t _e = (x);                     \
+    if (_e != hipSuccess) return OURO_E_HIP; \
+  } while (0)
+
+// Host-side check of the live set: alignment to the page grid of the request's class, no overlap,
+// and the fragmentation figures (chunks holding a live block vs. the ideal packing).
+void check_live(const std::vector<uint64_t>& offs, const std::vector<uint32_t>& sizes, ouro_trace_result* r) {
+  struct Iv {
+    uint64_t a, b;
+  };
+  std::vector<Iv> iv;
+  iv.reserve(offs.size());
+  uint64_t live_q[16] = {0};
+  std::unordered_set<uint64_t> chunks;
+  for (size_t i = 0; i < offs.size(); ++i) {
+    if (offs[i] == UINT64_MAX) continue;
+    const uint32_t ps = page_of(sizes[i]);
+    iv.push_back({offs[i], offs[i] + ps});
+    if (ps <= 8192) {
+      const uint64_t c = offs[i] / 8320, in = offs[i] - c * 8320;
+      if (in < 128 || (in - 128) % ps != 0 || in - 128 + ps > 8192) r->align_errors++;
+      chunks.insert(c);
+      uint32_t q = 0;
+      while ((16u << q) < ps) ++q;
+      live_q[q]++;
+    } else if ((offs[i] & 15u) != 0) {
+      r->align_errors++;
+    }
+  }
+  std::sort(iv.begin(), iv.end(), [](const Iv& x, const Iv& y) { return x.a < y.a; });
+  for (size_t i = 1; i < iv.size(); ++i)
+    if (iv[i].a < iv[i - 1].b) r->overlap_errors++;
+  uint64_t ideal = 0;
+  for (uint32_t q = 0; q < 10; ++q) ideal += (live_q[q] + (512u >> q) - 1) / (512u >> q);
+  if (chunks.size() > r->peak_live_chunks) {
+    r->peak_live_chunks = chunks.size();
+    r->ideal_chunks = ideal;
+  }
+}
+
+// ------------------------------------------------------------------ measurement kernels
+__global__ void __launch_bounds__(256) k_atomics(uint32_t* buf, uint32_t iters, int pattern, uint32_t* sink) {
+  const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t* p;
+  if (pattern == 0) p = buf + (size_t)tid * 32u;            // one 128-byte line per lane
+  else if (pattern == 1) p = buf + (size_t)(tid >> 6) * 32u; // one line per wave
+  else p = buf;
+  uint32_t acc = 0;
+  for (uint32_t i = 0; i < iters; ++i)
+    acc += __hip_atomic_fetch_add(p, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (acc == 0xFFFFFFFFu) sink[0] = acc;
+}
+
+__global__ void __launch_bounds__(256) k_copy(const uint4* __restrict__ src, uint4* __restrict__ dst, uint64_t n) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
+}  // namespace
+
+extern "C" {
+
+int ouro_bench_atomics(uint32_t threads, uint32_t iters, int pattern, double* ops_per_s) {
+  if (!ops_per_s || threads == 0 || threads % 256 || iters == 0 || pattern < 0 || pattern > 2) return OURO_E_INVALID;
+  DevBuf buf, sink;
+  const size_t words = pattern == 0 ? (size_t)threads * 32 : (pattern == 1 ? (size_t)(threads / 64) * 32 : 32);
+  OURO_TRY(buf.alloc(words * 4));
+  OURO_TRY(sink.alloc(4));
+  OURO_TRY(hipMemset(buf.p, 0, words * 4));
+  hipEvent_t a, b;
+  OURO_TRY(hipEventCreate(&a));
+  OURO_TRY(hipEventCreate(&b));
+  hipLaunchKernelGGL(k_atomics, dim3(threads / 256), dim3(256), 0, 0, buf.as<uint32_t>(), 1u, pattern, sink.as<uint32_t>());
+  OURO_TRY(hipEventRecord(a, 0));
+  hipLaunchKernelGGL(k_atomics, dim3(threads / 256), dim3(256), 0, 0, buf.as<uint32_t>(), iters, pattern, sink.as<uint32_t>());
+  OURO_TRY(hipEventRecord(b, 0));
+  OURO_TRY(hipEventSynchronize(b));
+  float ms = 0;
+  OURO_TRY(hipEventElapsedTime(&ms, a, b));
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  *ops_per_s = (double)threads * iters / (ms * 1e-3);
+  return OURO_OK;
+}
+
+int ouro_bench_copy(uint64_t bytes, uint32_t reps, double* gb_per_s) {
+  if (!gb_per_s || bytes < 16 || reps == 0) return OURO_E_INVALID;
+  const uint64_t n = bytes / 16;
+  DevBuf src, dst;
+  OURO_TRY(src.alloc(n * 16));
+  OURO_TRY(dst.alloc(n * 16));
+  OURO_TRY(hipMemset(src.p, 1, n * 16));
+  hipEvent_t a, b;
+  OURO_TRY(hipEventCreate(&a));
+  OURO_TRY(hipEventCreate(&b));
+  const uint32_t grid = 256 * 32;
+  hipLaunchKernelGGL(k_copy, dim3(grid), dim3(256), 0, 0, src.as<uint4>(), dst.as<uint4>(), n);
+  OURO_TRY(hipEventRecord(a, 0));
+  for (uint32_t r = 0; r < reps; ++r)
+    hipLaunchKernelGGL(k_copy, dim3(grid), dim3(256), 0, 0, src.as<uint4>(), dst.as<uint4>(), n);
+  OURO_TRY(hipEventRecord(b, 0));
+  OURO_TRY(hipEventSynchronize(b));
+  float ms = 0;
+  OURO_TRY(hipEventElapsedTime(&ms, a, b));
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  *gb_per_s = 2.0 * (double)(n * 16) * reps / (ms * 1e-3) / 1e9;
+  return OURO_OK;
+}
+
+
+int ouro_create(int variant, uint64_t pool_bytes, ouro_handle* out) {
+  if (!out) return OURO_E_INVALID;
+  *out = nullptr;
+  IManager* m = make_variant(variant);
+  if (!m) return OURO_E_INVALID;
+  std::unique_ptr<ouro_manager> h(new (std::nothrow) ouro_manager);
+  if (!h) {
+    delete m;
+    return OURO_E_NOMEM;
+  }
+  h->impl.reset(m);
+  const hipError_t e = m->init(pool_bytes);
+  if (e == hipErrorOutOfMemory) return OURO_E_NOMEM;
+  if (e == hipErrorInvalidValue) return OURO_E_INVALID;
+  if (e != hipSuccess) return OURO_E_HIP;
+  *out = h.release();
+  return OURO_OK;
+}
+
+int ouro_destroy(ouro_handle h) {
+  delete h;
+  return OURO_OK;
+}
+
+void* ouro_device_manager(ouro_handle h) { return h ? h->impl->device_manager() : nullptr; }
+void* ouro_data_base(ouro_handle h) { return h ? h->impl->data_base() : nullptr; }
+
+int ouro_layout(int variant, uint64_t pool_bytes, uint64_t* max_chunks, uint64_t* allocation_size) {
+  ouro::InstanceGeometry g[2];
+  const ouro::Kind k = (ouro::Kind)variant;
+  if (variant < 0 || variant > 5) return OURO_E_INVALID;
+  const uint64_t entries = ouro::is_std(k) ? ouro::kStdEntries : (ouro::is_va(k) ? ouro::kVAEntries : 0);
+  g[0] = {(uint32_t)variant, 10, entries};
+  g[1] = {(uint32_t)variant, 2, entries};
+  const ouro::PoolLayout L = ouro::compute_layout(pool_bytes, g, 2);
+  if (max_chunks) *max_chunks = L.max_chunks;
+  if (allocation_size) *allocation_size = L.allocation_size;
+  return OURO_OK;
+}
+
+int ouro_malloc_batch(ouro_handle h, const uint32_t* d_sizes, void** d_out, uint32_t n, int agg, void* stream) {
+  if (!h || (n && (!d_sizes || !d_out))) return OURO_E_INVALID;
+  if (agg != 1 && agg != 16 && agg != 64) return OURO_E_INVALID;
+  return h->impl->malloc_batch(d_sizes, d_out, n, agg, (hipStream_t)stream, nullptr, 256) == hipSuccess ? OURO_OK
+                                                                                                     : OURO_E_HIP;
+}
+
+int ouro_free_batch(ouro_handle h, void* const* d_ptrs, uint32_t n, int agg, void* stream) {
+  if (!h || (n && !d_ptrs)) return OURO_E_INVALID;
+  if (agg != 1 && agg != 16 && agg != 64) return OURO_E_INVALID;
+  return h->impl->free_batch(d_ptrs, n, agg, (hipStream_t)stream, 256) == hipSuccess ? OURO_OK : OURO_E_HIP;
+}
+
+int ouro_query(ouro_handle h, ouro_state* out) {
+  if (!h || !out) return OURO_E_INVALID;
+  return h->impl->query(out) == hipSuccess ? OURO_OK : OURO_E_HIP;
+}
+
+int ouro_snapshot(ouro_handle h, void* dst) {
+  if (!h || !dst) return OURO_E_INVALID;
+  return hipMemcpy(dst, h->impl->device_manager(), h->impl->allocation_size(), hipMemcpyDefault) == hipSuccess
+             ? OURO_OK
+             : OURO_E_HIP;
+}
+
+int ouro_gen_sizes(uint64_t seed, uint32_t n, int dist, double param, uint32_t classes, uint32_t* out) {
+  if (!out || classes == 0 || classes > 16) return OURO_E_INVALID;
+  if (dist == 0) {
+    for (uint32_t i = 0; i < n; ++i) out[i] = (uint32_t)param;
+    return OURO_OK;
+  }
+  if (dist == 1) {
+    for (uint32_t i = 0; i < n; ++i) out[i] = 16u << (uint32_t)(keyed(seed, i) % classes);
+    return OURO_OK;
+  }
+  if (dist == 2) {
+    // Zipf(param) over class ranks: P(q) proportional to (q + 1)^-param
+    std::vector<double> cdf(classes);
+    double acc = 0;
+    for (uint32_t q = 0; q < classes; ++q) cdf[q] = (acc += std::pow((double)(q + 1), -param));
+    for (uint32_t q = 0; q < classes; ++q) cdf[q] /= acc;
+    for (uint32_t i = 0; i < n; ++i) {
+      const double u = unit(keyed(seed, i));
+      uint32_t q = 0;
+      while (q + 1 < classes && u >= cdf[q]) ++q;
+      out[i] = 16u << q;
+    }
+    return OURO_OK;
+  }
+  return OURO_E_INVALID;
+}
+
+int ouro_run_trace(ouro_handle h, const ouro_trace_config* cfg, ouro_trace_result* out) {
+  if (!h || !cfg || !out) return OURO_E_INVALID;
+  if (cfg->agg != 1 && cfg->agg != 16 && cfg->agg != 64) return OURO_E_INVALID;
+  if (!cfg->sizes && !cfg->round_sizes) return OURO_E_INVALID;
+  IManager& m = *h->impl;
+  const uint32_t n = cfg->threads, R = cfg->rounds;
+  const uint32_t block = cfg->block ? cfg->block : 256;
+  if (block % 64 != 0 || block > 1024) return OURO_E_INVALID;
+  memset(out, 0, sizeof(*out));
+  if (n == 0 || R == 0) return OURO_OK;
+
+  DevBuf d_sizes[2], d_ptrs[2], d_skip_cur, d_skip_prev, d_errs, d_lat;
+  for (int i = 0; i < 2; ++i) {
+    OURO_TRY(d_sizes[i].alloc((size_t)n * 4));
+    OURO_TRY(d_ptrs[i].alloc((size_t)n * 8));
+    OURO_TRY(hipMemset(d_ptrs[i].p, 0, (size_t)n * 8));
+  }
+  OURO_TRY(d_skip_cur.alloc(n));
+  OURO_TRY(d_skip_prev.alloc(n));
+  OURO_TRY(d_errs.alloc(16));
+  OURO_TRY(hipMemset(d_errs.p, 0, 16));
+  const uint32_t nlat = (n + 63) / 64;
+  if (cfg->latency) OURO_TRY(d_lat.alloc((size_t)nlat * 8));
+
+  hipStream_t s;
+  OURO_TRY(hipStreamCreate(&s));
+  hipEvent_t ev[6];
+  for (auto& e : ev) OURO_TRY(hipEventCreate(&e));
+
+  const uint8_t* lo = reinterpret_cast<const uint8_t*>(m.data_base());
+  const uint8_t* hi = lo + m.max_chunks() * 8320ull;
+  const uint32_t grid = (n + block - 1) / block;
+  std::vector<uint32_t> hsizes(n), psizes(n);
+  std::vector<uint8_t> hold(n, 0), phold(n, 0), inv(n, 0);
+  std::vector<uint64_t> offs(n), poffs(n, UINT64_MAX);
+  std::vector<void*> hptrs(n);
+  std::vector<uint64_t> lat_all;
+  int wallclock_khz = 0;
+  (void)hipDeviceGetAttribute(&wallclock_khz, hipDeviceAttributeWallClockRate, 0);
+  const bool held = cfg->hold_permille != 0;
+
+  for (uint32_t r = 0; r < R; ++r) {
+    const int cur = r & 1, prv = cur ^ 1;
+    const uint32_t* src = cfg->round_sizes ? cfg->round_sizes + (size_t)r * n : cfg->sizes;
+    memcpy(hsizes.data(), src, (size_t)n * 4);
+    OURO_TRY(hipMemcpyAsync(d_sizes[cur].p, hsizes.data(), (size_t)n * 4, hipMemcpyHostToDevice, s));
+    if (held) {
+      // a held block of round r is freed in round r + 1 (or after the last round)
+      for (uint32_t i = 0; i < n; ++i)
+        hold[i] = (keyed(cfg->hold_seed ^ (0x9E3779B9ull * (r + 1)), i) % 1000) < cfg->hold_permille;
+      for (uint32_t i = 0; i < n; ++i) inv[i] = !phold[i];
+      OURO_TRY(hipMemcpyAsync(d_skip_cur.p, hold.data(), n, hipMemcpyHostToDevice, s));
+      OURO_TRY(hipMemcpyAsync(d_skip_prev.p, inv.data(), n, hipMemcpyHostToDevice, s));
+      OURO_TRY(hipStreamSynchronize(s));
+    }
+    OURO_TRY(hipEventRecord(ev[0], s));
+    OURO_TRY(m.malloc_batch(d_sizes[cur].as<uint32_t>(), d_ptrs[cur].as<void*>(), n, cfg->agg, s,
+                            cfg->latency ? d_lat.as<uint64_t>() : nullptr, block));
+    OURO_TRY(hipEventRecord(ev[1], s));
+    hipLaunchKernelGGL(k_touch, dim3(grid), dim3(block), 0, s, d_ptrs[cur].as<void*>(), d_sizes[cur].as<uint32_t>(), n,
+                       lo, hi, 0, d_errs.as<unsigned long long>());
+    OURO_TRY(hipEventRecord(ev[2], s));
+    hipLaunchKernelGGL(k_touch, dim3(grid), dim3(block), 0, s, d_ptrs[cur].as<void*>(), d_sizes[cur].as<uint32_t>(), n,
+                       lo, hi, 1, d_errs.as<unsigned long long>());
+    OURO_TRY(hipEventRecord(ev[3], s));
+    OURO_TRY(hipStreamSynchronize(s));
+    {
+      OURO_TRY(hipMemcpy(hptrs.data(), d_ptrs[cur].p, (size_t)n * 8, hipMemcpyDeviceToHost));
+      for (uint32_t i = 0; i < n; ++i) {
+        offs[i] = hptrs[i] ? (uint64_t)(reinterpret_cast<const uint8_t*>(hptrs[i]) - lo) : UINT64_MAX;
+        out->null_returns += hptrs[i] == nullptr;
+      }
+      if (cfg->offsets_out) memcpy(cfg->offsets_out + (size_t)r * n, offs.data(), (size_t)n * 8);
+    }
+    if (cfg->check_rounds) {
+      // live set = this round's blocks + the blocks held back from the previous round
+      std::vector<uint64_t> live = offs;
+      std::vector<uint32_t> lsz = hsizes;
+      if (held)
+        for (uint32_t i = 0; i < n; ++i)
+          if (phold[i] && poffs[i] != UINT64_MAX) {
+            live.push_back(poffs[i]);
+            lsz.push_back(psizes[i]);
+          }
+      check_live(live, lsz, out);
+    }
+    OURO_TRY(hipEventRecord(ev[4], s));
+    if (held) {
+      if (r > 0) OURO_TRY(m.free_masked(d_ptrs[prv].as<void*>(), d_skip_prev.as<uint8_t>(), n, cfg->agg, s, block));
+      OURO_TRY(m.free_masked(d_ptrs[cur].as<void*>(), d_skip_cur.as<uint8_t>(), n, cfg->agg, s, block));
+    } else {
+      OURO_TRY(m.free_batch(d_ptrs[cur].as<void*>(), n, cfg->agg, s, block));
+    }
+    OURO_TRY(hipEventRecord(ev[5], s));
+    OURO_TRY(hipEventSynchronize(ev[5]));
+    float t = 0;
+    OURO_TRY(hipEventElapsedTime(&t, ev[0], ev[1]));
+    out->alloc_ms += t;
+    OURO_TRY(hipEventElapsedTime(&t, ev[1], ev[2]));
+    out->write_ms += t;
+    OURO_TRY(hipEventElapsedTime(&t, ev[2], ev[3]));
+    out->verify_ms += t;
+    OURO_TRY(hipEventElapsedTime(&t, ev[4], ev[5]));
+    out->free_ms += t;
+    out->mallocs += n;
+    if (cfg->latency) {
+      std::vector<uint64_t> l(nlat);
+      OURO_TRY(hipMemcpy(l.data(), d_lat.p, (size_t)nlat * 8, hipMemcpyDeviceToHost));
+      lat_all.insert(lat_all.end(), l.begin(), l.end());
+    }
+    for (uint32_t i = 0; i < n; ++i) out->touch_bytes += hptrs[i] ? 2ull * (hsizes[i] & ~3u) : 0ull;
+    phold.swap(hold);
+    psizes.swap(hsizes);
+    poffs.swap(offs);
+  }
+  if (held) {
+    // the last round's held blocks
+    const int last = (R - 1) & 1;
+    for (uint32_t i = 0; i < n; ++i) inv[i] = !phold[i];
+    OURO_TRY(hipMemcpy(d_skip_prev.p, inv.data(), n, hipMemcpyHostToDevice));
+    OURO_TRY(hipEventRecord(ev[4], s));
+    OURO_TRY(m.free_masked(d_ptrs[last].as<void*>(), d_skip_prev.as<uint8_t>(), n, cfg->agg, s, block));
+    OURO_TRY(hipEventRecord(ev[5], s));
+    OURO_TRY(hipEventSynchronize(ev[5]));
+    float t = 0;
+    OURO_TRY(hipEventElapsedTime(&t, ev[4], ev[5]));
+    out->free_ms += t;
+  }
+  OURO_TRY(hipStreamSynchronize(s));
+  unsigned long long errs[2];
+  OURO_TRY(hipMemcpy(errs, d_errs.p, 16, hipMemcpyDeviceToHost));
+  out->verify_errors = errs[0];
+  out->bounds_errors = errs[1];
+  ouro_state st;
+  if (m.query(&st) != hipSuccess) return OURO_E_HIP;
+  out->error_word = st.error;
+  out->peak_chunks = st.next_free_chunk;
+  out->frees = out->mallocs - out->null_returns;
+  if (!lat_all.empty()) {
+    std::sort(lat_all.begin(), lat_all.end());
+    const double ns_per_tick = wallclock_khz > 0 ? 1e6 / (double)wallclock_khz : 10.0;
+    auto pct = [&](double p) {
+      size_t i = (size_t)std::min<double>((double)lat_all.size() - 1, std::floor(p * (double)(lat_all.size() - 1) + 0.5));
+      return (double)lat_all[i] * ns_per_tick;
+    };
+    out->lat_samples = lat_all.size();
+    out->lat_p50_ns = pct(0.50);
+    out->lat_p99_ns = pct(0.99);
+    out->lat_max_ns = (double)lat_all.back() * ns_per_tick;
+  }
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  (void)hipStreamDestroy(s);
+  return OURO_OK;
+}
+
+// ------------------------------------------------------------------ quiescent invariant check
+int ouro_check_quiescent(ouro_handle h, ouro_check_result* out) {
+  if (!h || !out) return OURO_E_INVALID;
+  memset(out, 0, sizeof(*out));
+  out->first_class = -1;
+  out->first_chunk = -1;
+  IManager& m = *h->impl;
+  ouro_state st;
+  if (m.query(&st) != hipSuccess) return OURO_E_HIP;
+  auto fail = [&](int cls, int64_t chunk, const char* msg, long long a, long long b) {
+    if (out->violations == 0) {
+      out->first_class = cls;
+      out->first_chunk = (int32_t)chunk;
+      snprintf(out->message, sizeof(out->message), "%s (%lld vs %lld)", msg, a, b);
+    }
+    out->violations++;
+  };
+  const uint8_t* d_data = reinterpret_cast<const uint8_t*>(m.data_base());
+  const uint8_t* d_mem = reinterpret_cast<const uint8_t*>(m.device_manager());
+  const uint32_t nfc = std::min<uint32_t>(st.next_free_chunk, (uint32_t)st.max_chunks);
+  // chunk headers and the run-start locator
+  std::vector<uint8_t> hdrs((size_t)nfc * 128 + 128);
+  if (nfc) OURO_TRY(hipMemcpy2D(hdrs.data(), 128, d_data, 8320, 128, nfc, hipMemcpyDeviceToHost));
+  const uint64_t loc_bytes = ((st.max_chunks + 31) / 32) * 4;
+  std::vector<uint32_t> loc((loc_bytes + 3) / 4);
+  {
+    const uint64_t loc_region = ((loc_bytes + 8319) / 8320) * 8320;
+    OURO_TRY(hipMemcpy(loc.data(), d_mem + st.allocation_size - loc_region, loc_bytes, hipMemcpyDeviceToHost));
+  }
+  const bool chunk_kind = st.kind[0] == OURO_CQ || st.kind[0] == OURO_VACQ || st.kind[0] == OURO_VLCQ;
+  std::vector<int64_t> class_pages(st.num_classes, 0), class_chunks(st.num_classes, 0);
+  std::vector<std::vector<uint32_t>> members(st.num_classes);
+  for (uint32_t c = 0; c < nfc; ++c) {
+    if (!((loc[c >> 5] >> (c & 31)) & 1u)) continue;
+    const uint32_t* w = reinterpret_cast<const uint32_t*>(hdrs.data() + (size_t)c * 128);
+    const uint32_t ps = w[0];
+    if (ps == 0) continue;  // queue chunk or recycled queue chunk
+    int k = -1;
+    for (uint32_t q = 0; q < st.num_classes; ++q)
+      if (st.page_size[q] == ps) k = (int)q;
+    if (k < 0) {
+      fail(-1, c, "chunk header holds an unknown page size", ps, 0);
+      continue;
+    }
+    out->chunks_checked++;
+    const uint32_t ppc = st.pages_per_chunk[k];
+    class_chunks[k]++;
+    members[k].push_back(c);
+    if (chunk_kind) {
+      const ouro::ChunkHdr* ch = reinterpret_cast<const ouro::ChunkHdr*>(w);
+      if (ch->size != ppc) fail(k, c, "chunk size field differs from pages per chunk", ch->size, ppc);
+      if (ch->count != (int32_t)ppc) fail(k, c, "chunk free count is not full", ch->count, ppc);
+      for (uint32_t wi = 0; wi < ouro::kMaskWords; ++wi) {
+        const uint32_t lo = wi * 64;
+        uint64_t full = 0;
+        if (ppc > lo) full = (ppc - lo >= 64) ? ~0ull : ((1ull << (ppc - lo)) - 1ull);
+        if (ch->mask[wi] != full) fail(k, c, "chunk availability mask is not full", (long long)wi, 0);
+      }
+      class_pages[k] += ch->count;
+    } else {
+      const ouro::PageChunkHdr* ph = reinterpret_cast<const ouro::PageChunkHdr*>(w);
+      if (ph->number_pages != ppc) fail(k, c, "page chunk page count differs", ph->number_pages, ppc);
+      class_pages[k] += ppc;
+    }
+  }
+  for (uint32_t k = 0; k < st.num_classes; ++k) {
+    out->pages_free += class_pages[k];
+    if (st.sem_count[k] != class_pages[k]) fail(k, -1, "semaphore count != free pages of the class", st.sem_count[k], class_pages[k]);
+    if (st.sem_expected[k] != 0) fail(k, -1, "semaphore expected != 0", st.sem_expected[k], 0);
+    if (st.sem_reserved[k] != 0) fail(k, -1, "semaphore reserved != 0", st.sem_reserved[k], 0);
+    const uint32_t kind = st.kind[k];
+    const bool ck = kind == OURO_CQ || kind == OURO_VACQ || kind == OURO_VLCQ;
+    if (ck) {
+      if ((int64_t)st.carved[k] != class_chunks[k]) fail(k, -1, "carved chunks != chunks of the class", st.carved[k], class_chunks[k]);
+      if ((int64_t)st.queue_count[k] != class_chunks[k]) fail(k, -1, "queue count != chunks of the class", st.queue_count[k], class_chunks[k]);
+    } else {
+      if ((int64_t)st.carved[k] != class_pages[k]) fail(k, -1, "carved pages != pages of the class", st.carved[k], class_pages[k]);
+      if ((int64_t)(uint32_t)(st.back[k] - st.front[k]) != class_pages[k])
+        fail(k, -1, "queue back - front != free pages", (long long)(uint32_t)(st.back[k] - st.front[k]), class_pages[k]);
+    }
+    // standard array queues: the live ring window must hold exactly the class's pages / chunks
+    uint32_t* ring = nullptr;
+    uint32_t rsize = 0;
+    m.class_ring(k, &ring, &rsize);
+    if ((kind == OURO_PQ || kind == OURO_CQ) && ring && rsize) {
+      const uint32_t len = st.back[k] - st.front[k];
+      if (len > rsize) {
+        fail(k, -1, "queue window exceeds ring", len, rsize);
+        continue;
+      }
+      std::vector<uint32_t> win(len);
+      const uint32_t f = st.front[k] & (rsize - 1);
+      const uint32_t first = std::min(len, rsize - f);
+      if (first) OURO_TRY(hipMemcpy(win.data(), ring + f, (size_t)first * 4, hipMemcpyDeviceToHost));
+      if (len > first) OURO_TRY(hipMemcpy(win.data() + first, ring, (size_t)(len - first) * 4, hipMemcpyDeviceToHost));
+      std::unordered_set<uint64_t> seen;
+      std::unordered_set<uint32_t> mem(members[k].begin(), members[k].end());
+      for (uint32_t i = 0; i < len; ++i) {
+        const uint32_t v = win[i];
+        if (v == 0xFFFFFFFFu) {
+          if (kind == OURO_PQ) fail(k, -1, "hole inside the page queue window", i, len);
+          continue;
+        }
+        const uint32_t c = kind == OURO_PQ ? (v >> 9) : v;
+        const uint32_t pg = kind == OURO_PQ ? (v & 511u) : 0u;
+        if (!mem.count(c)) fail(k, c, "queue entry names a chunk outside the class", c, 0);
+        if (kind == OURO_PQ && pg >= st.pages_per_chunk[k]) fail(k, c, "queue entry page out of range", pg, st.pages_per_chunk[k]);
+        if (!seen.insert(v).second) fail(k, c, "duplicate queue entry", v, 0);
+      }
+      if (kind == OURO_CQ && (int64_t)seen.size() != class_chunks[k])
+        fail(k, -1, "chunk queue entries != chunks of the class", (long long)seen.size(), class_chunks[k]);
+    }
+  }
+  if (st.error) fail(-1, -1, "device error word is set", st.error, 0);
+  return OURO_OK;
+}
+
+}  // extern "C"

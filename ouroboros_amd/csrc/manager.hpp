@@ -1,0 +1,195 @@
+// Type-erased wrapper around one Ouroboros<...> instantiation, used by the C ABI.
+// Each variant is compiled in its own translation unit (variant.hip) so the six instantiations
+// build in parallel.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string.h>
+#include <vector>
+
+#include "ouro/instances.hpp"
+#include "ouro_capi.h"
+
+namespace ouro_lib {
+
+struct IManager {
+  virtual ~IManager() {}
+  virtual hipError_t init(uint64_t pool_bytes) = 0;
+  virtual void* device_manager() = 0;
+  virtual void* data_base() = 0;
+  virtual uint64_t allocation_size() = 0;
+  virtual uint64_t max_chunks() = 0;
+  virtual int variant() = 0;
+  virtual hipError_t malloc_batch(const uint32_t* d_sizes, void** d_out, uint32_t n, int agg, hipStream_t s,
+                                  uint64_t* d_lat, uint32_t block) = 0;
+  virtual hipError_t free_batch(void* const* d_ptrs, uint32_t n, int agg, hipStream_t s, uint32_t block) = 0;
+  virtual hipError_t free_masked(void* const* d_ptrs, const uint8_t* d_skip, uint32_t n, int agg, hipStream_t s,
+                                 uint32_t block) = 0;
+  virtual hipError_t query(ouro_state* out) = 0;
+  // immutable queue placement of class k (chain order)
+  virtual void class_ring(uint32_t k, uint32_t** ring, uint32_t* ring_size) = 0;
+};
+
+// Factories, one per variant translation unit.
+IManager* make_pq();
+IManager* make_cq();
+IManager* make_vapq();
+IManager* make_vacq();
+IManager* make_vlpq();
+IManager* make_vlcq();
+
+// ------------------------------------------------------------------ harness kernels
+template <class M, int W>
+__global__ void __launch_bounds__(256) k_malloc(M* mm, const uint32_t* __restrict__ sizes, void** __restrict__ out,
+                                                uint32_t n) {
+  const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= n) return;
+  out[tid] = mm->template malloc<W>(sizes[tid]);
+}
+
+// Latency sample: lane 0 of every wave records wall_clock64() ticks around its malloc.
+template <class M, int W>
+__global__ void __launch_bounds__(256) k_malloc_lat(M* mm, const uint32_t* __restrict__ sizes, void** __restrict__ out,
+                                                    uint32_t n, uint64_t* __restrict__ lat) {
+  const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= n) return;
+  const uint64_t t0 = wall_clock64();
+  void* p = mm->template malloc<W>(sizes[tid]);
+  const uint64_t t1 = wall_clock64();
+  out[tid] = p;
+  if ((tid & 63u) == 0) lat[tid >> 6] = t1 - t0;
+}
+
+template <class M, int W>
+__global__ void __launch_bounds__(256) k_free(M* mm, void* const* __restrict__ ptrs, uint32_t n) {
+  const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= n) return;
+  mm->template free<W>(ptrs[tid]);
+}
+
+template <class M, int W>
+__global__ void __launch_bounds__(256) k_free_masked(M* mm, void* const* __restrict__ ptrs,
+                                                     const uint8_t* __restrict__ skip, uint32_t n) {
+  const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= n) return;
+  mm->template free<W>(skip[tid] ? nullptr : ptrs[tid]);
+}
+
+inline uint32_t grid_for(uint32_t n, uint32_t block) { return (n + block - 1) / block; }
+
+template <class M>
+struct Manager final : IManager {
+  M mm;
+  int var;
+  explicit Manager(int v) : var(v) {}
+  hipError_t init(uint64_t pool_bytes) override { return mm.initialize(pool_bytes); }
+  void* device_manager() override { return mm.getDeviceMemoryManager(); }
+  void* data_base() override { return mm.memory.d_data; }
+  uint64_t allocation_size() override { return mm.memory.allocation_size; }
+  uint64_t max_chunks() override { return mm.memory.max_chunks; }
+  int variant() override { return var; }
+
+  template <int W>
+  hipError_t launch_malloc(const uint32_t* d_sizes, void** d_out, uint32_t n, hipStream_t s, uint64_t* d_lat,
+                           uint32_t block) {
+    M* dm = mm.getDeviceMemoryManager();
+    if (d_lat)
+      hipLaunchKernelGGL((k_malloc_lat<M, W>), dim3(grid_for(n, block)), dim3(block), 0, s, dm, d_sizes, d_out, n,
+                         d_lat);
+    else
+      hipLaunchKernelGGL((k_malloc<M, W>), dim3(grid_for(n, block)), dim3(block), 0, s, dm, d_sizes, d_out, n);
+    return hipGetLastError();
+  }
+  hipError_t malloc_batch(const uint32_t* d_sizes, void** d_out, uint32_t n, int agg, hipStream_t s, uint64_t* d_lat,
+                          uint32_t block) override {
+    if (n == 0) return hipSuccess;
+    switch (agg) {
+      case 1: return launch_malloc<1>(d_sizes, d_out, n, s, d_lat, block);
+      case 16: return launch_malloc<16>(d_sizes, d_out, n, s, d_lat, block);
+      case 64: return launch_malloc<64>(d_sizes, d_out, n, s, d_lat, block);
+      default: return hipErrorInvalidValue;
+    }
+  }
+  template <int W>
+  hipError_t launch_free(void* const* d_ptrs, const uint8_t* d_skip, uint32_t n, hipStream_t s, uint32_t block) {
+    M* dm = mm.getDeviceMemoryManager();
+    if (d_skip)
+      hipLaunchKernelGGL((k_free_masked<M, W>), dim3(grid_for(n, block)), dim3(block), 0, s, dm, d_ptrs, d_skip, n);
+    else
+      hipLaunchKernelGGL((k_free<M, W>), dim3(grid_for(n, block)), dim3(block), 0, s, dm, d_ptrs, n);
+    return hipGetLastError();
+  }
+  hipError_t free_batch(void* const* d_ptrs, uint32_t n, int agg, hipStream_t s, uint32_t block) override {
+    return free_masked(d_ptrs, nullptr, n, agg, s, block);
+  }
+  hipError_t free_masked(void* const* d_ptrs, const uint8_t* d_skip, uint32_t n, int agg, hipStream_t s,
+                         uint32_t block) override {
+    if (n == 0) return hipSuccess;
+    switch (agg) {
+      case 1: return launch_free<1>(d_ptrs, d_skip, n, s, block);
+      case 16: return launch_free<16>(d_ptrs, d_skip, n, s, block);
+      case 64: return launch_free<64>(d_ptrs, d_skip, n, s, block);
+      default: return hipErrorInvalidValue;
+    }
+  }
+
+  template <int N, class... Is>
+  static void collect(ouro::Chain<>&, ouro_state*, uint32_t&) {}
+  template <int N, class I, class... Rest>
+  static void collect(ouro::Chain<I, Rest...>& ch, ouro_state* out, uint32_t& k) {
+    if (N < 2) out->reuse_count[N] = ch.inst.reuse.count;
+    for (uint32_t i = 0; i < I::kNQ && k < OURO_MAX_CLASSES; ++i, ++k) {
+      const ouro::ClassQueue& q = ch.inst.q[i];
+      out->page_size[k] = q.page_size;
+      out->pages_per_chunk[k] = q.ppc;
+      out->sem_count[k] = ouro::Sem::count(q.sem);
+      out->sem_expected[k] = (uint32_t)ouro::Sem::expected(q.sem);
+      out->sem_reserved[k] = (uint32_t)ouro::Sem::reserved(q.sem);
+      out->front[k] = q.front;
+      out->back[k] = q.back;
+      out->queue_count[k] = q.count;
+      out->carved[k] = q.carved;
+      out->front_qc[k] = q.front_qc;
+      out->kind[k] = (uint32_t)I::kKind;
+    }
+    collect<N + 1, Rest...>(ch.next, out, k);
+  }
+
+  template <int N, class... Is>
+  static void ring_of(ouro::Chain<>&, uint32_t, uint32_t**, uint32_t*) {}
+  template <int N, class I, class... Rest>
+  static void ring_of(ouro::Chain<I, Rest...>& ch, uint32_t k, uint32_t** ring, uint32_t* size) {
+    if (k < I::kNQ) {
+      *ring = ch.inst.q[k].ring;
+      *size = ch.inst.q[k].ring_size;
+      return;
+    }
+    ring_of<N + 1, Rest...>(ch.next, k - I::kNQ, ring, size);
+  }
+  void class_ring(uint32_t k, uint32_t** ring, uint32_t* ring_size) override {
+    *ring = nullptr;
+    *ring_size = 0;
+    ring_of<0>(mm.chain, k, ring, ring_size);
+  }
+
+  hipError_t query(ouro_state* out) override {
+    // the manager state is ~8 KiB; stage it in heap memory, it must not be constructed
+    std::vector<uint8_t> buf(sizeof(M));
+    hipError_t e = hipMemcpy(buf.data(), mm.memory.d_memory, sizeof(M), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return e;
+    M* h = reinterpret_cast<M*>(buf.data());
+    memset(out, 0, sizeof(*out));
+    out->allocation_size = h->memory.allocation_size;
+    out->max_chunks = h->memory.max_chunks;
+    out->data_offset = (uint64_t)(h->memory.d_data - h->memory.d_memory);
+    out->next_free_chunk = h->memory.next_free_chunk;
+    out->error = h->memory.error;
+    out->variant = (uint32_t)var;
+    uint32_t k = 0;
+    collect<0>(h->chain, out, k);
+    out->num_classes = k;
+    return hipSuccess;
+  }
+};
+
+}  // namespace ouro_lib

@@ -1,0 +1,57 @@
+"""CPU tests of the C ABI library: it loads, exports every entry point include/*.h declares, and its
+pure-host functions (pool arithmetic, trace generator) agree with the oracle and with statistics."""
+import re
+from pathlib import Path
+
+import numpy as np
+
+import oracle
+import ouroboros_amd as ob
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+def declared_symbols():
+    names = set()
+    for h in (ROOT / "include").glob("*.h"):
+        text = h.read_text()
+        names |= set(re.findall(r"^\s*(?:int|void\*|void)\s+(ouro_\w+)\s*\(", text, re.M))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ob.lib()
+    names = declared_symbols()
+    assert len(names) >= 12
+    for n in names:
+        assert hasattr(lib, n), n
+    assert names <= set(ob.SIGNATURES), names - set(ob.SIGNATURES)
+
+
+def test_layout_agrees_with_oracle():
+    for v in ob.VARIANTS:
+        for pool in [400 << 20, 512 << 20, 4 << 30, 64 << 30]:
+            assert ob.layout(v, pool) == oracle.layout(v, pool)
+
+
+def test_trace_generator_is_deterministic_and_uniform():
+    a = ob.gen_sizes(42, 1 << 20, "uniform")
+    b = ob.gen_sizes(42, 1 << 20, "uniform")
+    assert np.array_equal(a, b)
+    assert set(np.unique(a)) == {16 << q for q in range(10)}
+    counts = np.bincount(np.log2(a).astype(int) - 4, minlength=10)
+    expected = len(a) / 10
+    chi2 = ((counts - expected) ** 2 / expected).sum()
+    assert chi2 < 40  # 9 dof; p ~ 1e-5
+    # mean 1636.8 B (uniform over the ten power-of-two classes)
+    assert abs(a.mean() - 1636.8) / 1636.8 < 0.01
+
+
+def test_zipf_generator_matches_rank_probabilities():
+    s = 1.2
+    a = ob.gen_sizes(9, 1 << 20, "zipf", s)
+    p = np.array([(q + 1) ** -s for q in range(10)])
+    p /= p.sum()
+    got = np.bincount(np.log2(a).astype(int) - 4, minlength=10) / len(a)
+    assert np.allclose(got, p, atol=3e-3)
+    assert abs(p[0] - 0.405) < 1e-3  # rank-1 probability quoted for config 3

@@ -1,0 +1,89 @@
+"""Device parity tests (MI355X): the HIP allocator through the C ABI (libouro.so) against the oracle.
+
+Single-wavefront traces must equal the oracle's sequential replay bit for bit; concurrent runs must
+pass the invariant checkers (no overlap, page-grid alignment, every written id read back, and a
+quiescent state that holds exactly the carved pages)."""
+import numpy as np
+import pytest
+
+import oracle
+import ouroboros_amd as ob
+
+pytestmark = pytest.mark.gpu
+
+PAGE_KINDS = ["PQ", "VAPQ", "VLPQ"]
+CHUNK_KINDS = ["CQ", "VACQ", "VLCQ"]
+ALL = PAGE_KINDS + CHUNK_KINDS
+
+
+def run_single_wave(kind, sizes, rounds, agg, pool=512 << 20):
+    print(f"single-wave {kind} agg={agg} lanes={sizes.shape[-1]} rounds={rounds}", flush=True)
+    with ob.MemoryManager(kind, pool) as mm:
+        out = mm.run_trace(sizes, rounds, agg=agg, keep_offsets=True, check_rounds=True)
+        st = mm.query()
+    return out, st
+
+
+def assert_clean(r):
+    assert r["verify_errors"] == 0
+    assert r["bounds_errors"] == 0
+    assert r["overlap_errors"] == 0
+    assert r["align_errors"] == 0
+    assert r["error_word"] == 0, r["errors"]
+
+
+CASES = [
+    ("config1-16B", lambda: np.full(64, 16, np.uint32), 10),
+    ("uniform-seed42", lambda: ob.gen_sizes(42, 64, "uniform"), 10),
+    ("class0-40rounds", lambda: np.full(64, 16, np.uint32), 40),
+    ("ragged-37-lanes", lambda: ob.gen_sizes(7, 37, "uniform"), 6),
+    ("with-instance2", lambda: (16 << (np.arange(64) % 12)).astype(np.uint32), 8),
+]
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+@pytest.mark.parametrize("kind,agg", [(k, 64) for k in ALL] + [(k, 1) for k in PAGE_KINDS])
+def test_single_wave_bit_exact(kind, agg, case):
+    name, mk, rounds = case
+    sizes = mk()
+    out, st = run_single_wave(kind, sizes, rounds, agg)
+    assert_clean(out.result)
+    sim = oracle.Sim(kind, 512 << 20)
+    want = sim.trace(sizes, rounds)
+    assert sim.state().error == 0
+    got = out.offsets
+    mism = np.argwhere(got != want)
+    assert len(mism) == 0, f"first mismatch at (round, lane) {mism[0]}: device {got[tuple(mism[0])]} oracle {want[tuple(mism[0])]}"
+    assert st.next_free_chunk == sim.state().next_free_chunk
+
+
+def test_config1_closed_form_on_device():
+    # 8,192 threads x 16 B would interleave 128 waves; the single-wave sub-trace is threads 0..63
+    sizes = np.full(64, 16, np.uint32)
+    out, _ = run_single_wave("PQ", sizes, 10, 1)
+    want = np.array([[128 + 16 * ((64 * r + l) % 512) for l in range(64)] for r in range(10)], dtype=np.uint64)
+    assert (out.offsets == want).all()
+
+
+@pytest.mark.parametrize("kind", ALL)
+@pytest.mark.parametrize("agg", [1, 16, 64])
+def test_concurrent_mixed_invariants(kind, agg):
+    print(f"concurrent {kind} agg={agg}", flush=True)
+    sizes = ob.gen_sizes(42, 65536, "uniform")
+    with ob.MemoryManager(kind, 1 << 30) as mm:
+        out = mm.run_trace(sizes, 4, agg=agg, check_rounds=True)
+        r = out.result
+        assert_clean(r)
+        assert r["null_returns"] == 0
+        chk = mm.check_quiescent()
+        assert chk.violations == 0, chk.message.decode()
+
+
+def test_config1_full_concurrent():
+    # config 1: MultiOuroPQ, 512 MiB, 8,192 threads x 16 B, 10 rounds, plain path
+    with ob.MemoryManager("PQ", 512 << 20) as mm:
+        out = mm.run_trace(np.full(8192, 16, np.uint32), 10, agg=1, check_rounds=True)
+        assert_clean(out.result)
+        assert out.result["peak_chunks"] == 16  # 8,192 x 16 B = 16 chunks of 512 pages
+        chk = mm.check_quiescent()
+        assert chk.violations == 0, chk.message.decode()

@@ -1,0 +1,178 @@
+"""CPU tests that pin the oracle (oracle/ouro_sim.c) against facts it does not compute itself:
+the survey's pool-partition table, closed forms, brute force and cross-variant invariances."""
+import numpy as np
+import pytest
+
+import oracle
+
+KINDS = ["PQ", "CQ", "VAPQ", "VACQ", "VLPQ", "VLCQ"]
+NONE = oracle.NONE
+STRIDE = 8320
+
+
+def test_layout_matches_survey_table():
+    # chunk counts per pool size, survey table restated from MemoryInitialization.cuh:156-166
+    table = {
+        "PQ": [16068, 467763, 2016429, 4081317, 8211093],
+        "VAPQ": [64369, 516064, 2064730, 4129618, 8259394],
+        "VLPQ": [64464, 516159, 2064825, 4129713, 8259489],
+    }
+    pools = [512 << 20, 4 << 30, 16 << 30, 32 << 30, 64 << 30]
+    for v, want in table.items():
+        assert [oracle.layout(v, p)[0] for p in pools] == want
+    # cudaMalloc asks for slightly more than 4 GiB: the locator is added on top
+    assert oracle.layout("PQ", 4 << 30)[1] == 4_295_041_920
+    # chunk kinds share the page kinds' geometry
+    for a, b in [("CQ", "PQ"), ("VACQ", "VAPQ"), ("VLCQ", "VLPQ")]:
+        assert oracle.layout(a, 4 << 30) == oracle.layout(b, 4 << 30)
+
+
+def test_size_class_closed_form_exhaustive():
+    # QueueIndex<16,8192>::getQueueIndex = max(ceil(log2 size) - 4, 0), independent restatement
+    for size in range(1, 32769):
+        want = max((size - 1).bit_length() - 4, 0)
+        assert oracle.class_of(size, 16) == want
+    # instance 2 (16 KiB smallest page)
+    assert oracle.class_of(16384, 16384) == 0 and oracle.class_of(16385, 16384) == 1
+    assert oracle.class_of(32768, 16384) == 1
+
+
+def test_semaphore_word_round_trip_and_identities():
+    rng = np.random.default_rng(1)
+    for _ in range(2000):
+        c = int(rng.integers(-(1 << 24), 1 << 24))
+        e = int(rng.integers(0, 1 << 19))
+        r = int(rng.integers(0, 1 << 19))
+        v = oracle.sem_make(c, e, r)
+        assert oracle.sem_fields(v) == (c, e, r)
+        # signal(N): a plain 64-bit add of N moves only the count
+        n = int(rng.integers(0, 1 << 10))
+        assert oracle.sem_fields((v + n) % (1 << 64)) == (c + n, e, r)
+        # signalExpected(N) (BulkSemaphore_impl.cuh:305-308): count += N, expected -= N
+        if e >= n:
+            v2 = (v + n - (n << 26)) % (1 << 64)
+            assert oracle.sem_fields(v2) == (c + n, e - n, r)
+
+
+def test_locator_brute_force():
+    rng = np.random.default_rng(7)
+    for trial in range(200):
+        nwords = int(rng.integers(1, 6))
+        bits = rng.integers(0, 2**32, size=nwords, dtype=np.uint64).astype(np.uint32)
+        bits &= rng.integers(0, 2**32, size=nwords, dtype=np.uint64).astype(np.uint32)  # sparser
+        if trial % 3 == 0:
+            bits[:] = 0
+            bits[0] = 1
+        for idx in range(nwords * 32):
+            want = max([i for i in range(idx + 1) if (int(bits[i >> 5]) >> (i & 31)) & 1], default=0xFFFFFFFF)
+            assert oracle.locate(bits, idx) == want
+
+
+def closed_form_pq(rounds, lanes, chunk):
+    # PageQueue FIFO: round r, lane l gets page (64 r + l) mod 512 of the single carved chunk
+    return np.array([[chunk * STRIDE + 128 + 16 * ((lanes * r + l) % 512) for l in range(lanes)]
+                     for r in range(rounds)], dtype=np.uint64)
+
+
+@pytest.mark.parametrize("kind,first_chunk", [("PQ", 0), ("VAPQ", 12), ("VLPQ", 12)])
+def test_config1_single_wave_closed_form(kind, first_chunk):
+    # 40 rounds push the virtualized queues across queue-chunk boundaries and through reclamation;
+    # the FIFO page order must not depend on how the queue is stored
+    s = oracle.Sim(kind, 512 << 20)
+    t = s.trace(np.full(64, 16), 40)
+    assert (t == closed_form_pq(40, 64, first_chunk)).all()
+    st = s.state()
+    assert st.error == 0
+    assert st.sem_count[0] == 512 and st.carved[0] == 512
+
+
+@pytest.mark.parametrize("kind", ["CQ", "VACQ", "VLCQ"])
+def test_chunk_kinds_single_wave_closed_form(kind):
+    # group claim takes the lowest free bits from word (size - count) / 64: with every page back
+    # before the next round, lane l always receives page l of the first data chunk
+    first = 0 if kind == "CQ" else 12
+    s = oracle.Sim(kind, 512 << 20)
+    t = s.trace(np.full(64, 16), 12)
+    want = np.array([[first * STRIDE + 128 + 16 * l for l in range(64)]] * 12, dtype=np.uint64)
+    assert (t == want).all()
+    t_, ps, cnt, mask = s.chunk(first)
+    assert (t_, ps, cnt) == (2, 16, 512) and (mask == np.uint64(0xFFFFFFFFFFFFFFFF)).all()
+
+
+def test_va_first_carve_preallocates_queue_chunk_13():
+    # PageQueueVA init takes queue chunks 0..11; the first carve (data chunk 12) is enqueued at
+    # virtual position 0 and pre-allocates queue chunk 13 (PageQueueVA_impl.cuh:43-53)
+    s = oracle.Sim("VAPQ", 512 << 20)
+    off = s.wave_malloc(np.full(1, 16))
+    assert off[0] == 12 * STRIDE + 128
+    assert s.state().next_free_chunk == 14
+    t, ps, _, _ = s.chunk(13)
+    assert t == 3 and ps == 0
+
+
+def _offset_checks(offs, sizes):
+    """alignment to the page grid and no overlap among live blocks"""
+    iv = []
+    for o, sz in zip(offs, sizes):
+        if o == NONE:
+            continue
+        ps = 16
+        while ps < sz:
+            ps <<= 1
+        o = int(o)
+        if ps <= 8192:
+            c, inside = divmod(o, STRIDE)
+            assert inside >= 128 and (inside - 128) % ps == 0 and inside - 128 + ps <= 8192
+        iv.append((o, o + ps))
+    iv.sort()
+    for a, b in zip(iv, iv[1:]):
+        assert b[0] >= a[1], "overlapping live blocks"
+
+
+@pytest.mark.parametrize("kind", KINDS)
+def test_sequential_waves_invariants(kind):
+    # many waves replayed one after another, mixed sizes (incl. instance 2), live blocks held across
+    # waves: no overlap, page-grid alignment, and a quiescent state that holds exactly the carved pages
+    rng = np.random.default_rng(3)
+    s = oracle.Sim(kind, 512 << 20)
+    live = []
+    for step in range(300):
+        n = int(rng.integers(1, 65))
+        sizes = (16 << rng.integers(0, 12, size=n)).astype(np.uint32)
+        offs = s.wave_malloc(sizes)
+        live.append((offs, sizes))
+        all_offs = np.concatenate([o for o, _ in live])
+        all_sizes = np.concatenate([z for _, z in live])
+        _offset_checks(all_offs, all_sizes)
+        if len(live) > 4 or rng.random() < 0.3:
+            o, _ = live.pop(int(rng.integers(0, len(live))))
+            s.wave_free(o)
+    for o, _ in live:
+        s.wave_free(o)
+    st = s.state()
+    assert st.error == 0
+    for k in range(st.num_classes):
+        assert st.sem_expected[k] == 0 and st.sem_reserved[k] == 0
+        if kind in ("PQ", "VAPQ", "VLPQ"):
+            assert st.back[k] - st.front[k] == st.sem_count[k] == st.carved[k]
+        else:
+            ppc = (512 >> k) if k < 10 else (2 >> (k - 10))
+            assert st.sem_count[k] == st.queue_count[k] * ppc == st.carved[k] * ppc
+
+
+def test_out_of_memory_returns_nullptr_and_sets_error():
+    # a small pool, 8 KiB pages (one per chunk): 64-lane groups are served whole or not at all;
+    # chunks carved by a group that could not be served stay available to later single requests.
+    # In total exactly maxChunks - 1 pages are handed out, the reference's bound chunk + 1 < maxChunks
+    # (Ouroboros.cuh:133), and the OOM bit is set instead of using an out-of-range chunk.
+    mc, _ = oracle.layout("PQ", 400 << 20)  # standard queues take 384.5 MiB of the pool
+    s = oracle.Sim("PQ", 400 << 20)
+    got = 0
+    for _ in range(mc // 64 + 2):
+        offs = s.wave_malloc(np.full(64, 8192))
+        assert (offs != NONE).all() or (offs == NONE).all()
+        got += int((offs != NONE).sum())
+    for _ in range(128):
+        got += int((s.wave_malloc(np.full(1, 8192)) != NONE).sum())
+    assert got == mc - 1
+    assert s.state().error & 1
