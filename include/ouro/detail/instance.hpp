@@ -473,14 +473,18 @@ __device__ inline bool sem_acquire(Ctx<I>& x, ClassQueue* cq, uint64_t grp, bool
   const int L = leader_of(grp);
   const bool leader = want && (int)lane() == L;
   const int64_t num = (int64_t)popc(grp);
-  bool pending = leader, ok = false;
+  const int64_t max_k = (int64_t)popc(active());
+  bool pending = leader, ok = false, first_pass = true;
   Spin it;
   while (ballot(pending)) {
     uint32_t carve_k = 0;
     bool reserve = false;
     if (pending) {
       uint64_t* s = &cq->sem;
-      uint64_t v = ld(s);
+      // First pass: take speculatively (one access to the semaphore line in the common case);
+      // later passes look before they take, as BulkSemaphore_impl.cuh:216 always does.
+      uint64_t v = first_pass ? Sem::make(num, 0, 0) : ld(s);
+      first_pass = false;
       if (Sem::count(v) >= num) {
         const uint64_t old = add(s, (uint64_t)(-num));
         if (Sem::count(old) >= num) {
@@ -499,7 +503,13 @@ __device__ inline bool sem_acquire(Ctx<I>& x, ClassQueue* cq, uint64_t grp, bool
           uint32_t k = 0;
           bool take = false, res = false;
           if (c + e - r < num) {
-            k = (uint32_t)((num - (c + e - r) + ppc - 1) / ppc);
+            // Carve for the whole deficit, i.e. this group plus the reservations already waiting
+            // (one chunk per loop pass in the reference leaves every waiter to wake once per carve).
+            // A transiently negative count (optimistic takes being undone) counts as zero, and a
+            // carve never exceeds one chunk per active lane.
+            const int64_t c0 = c > 0 ? c : 0;
+            const int64_t want_k = (num - (c0 + e - r) + ppc - 1) / ppc;
+            k = (uint32_t)(want_k < max_k ? want_k : max_k);
             nv = Sem::make(c, (uint64_t)(e + (int64_t)k * ppc), (uint64_t)r);
           } else if (c >= num) {
             nv = Sem::make(c - num, (uint64_t)e, (uint64_t)r);
@@ -561,16 +571,6 @@ __device__ __forceinline__ void sem_signal_expected(ClassQueue* cq, uint32_t n, 
   add(&cq->sem, (uint64_t)n - ((uint64_t)expected_removed << Sem::kExpShift));
 }
 __device__ __forceinline__ void sem_signal(ClassQueue* cq, uint32_t n) { add(&cq->sem, (uint64_t)n); }
-// tryReduce (BulkSemaphore_impl.cuh:7-17)
-__device__ __forceinline__ bool sem_try_reduce(ClassQueue* cq, uint32_t n) {
-  const uint64_t old = add(&cq->sem, (uint64_t)0 - (uint64_t)n);
-  if (Sem::count(old) < (int64_t)n) {
-    add(&cq->sem, (uint64_t)n);
-    return false;
-  }
-  return true;
-}
-
 // Positions for a queue operation: lanes grouped by queue (match<G>), one fetch-add per group,
 // rank order inside the group (atomicAggInc, Utility.cuh:76-95, when G = 64).
 template <int G>
@@ -757,13 +757,12 @@ __device__ inline uint32_t peek_slot(Ctx<I>& x, ClassQueue* cq, uint32_t v, uint
   }
 }
 
-// Remove the chunk at virtual position v from the queue (DEQUEUE_CHUNK branches:
-// ChunkQueue_impl.cuh:429-437, ChunkQueueVA_impl.cuh:209-243, ChunkQueueVL_impl.cuh:176-187).
-// Queue-chunk reclamation is returned through `reclaim` (VA) or done inline (VL).
+// Clear the slot at virtual position v (ChunkQueue_impl.cuh:434, QueueChunk::deleteElement
+// QueueChunk_impl.cuh:349-359). A queue chunk that becomes empty is handed back through `reclaim`
+// (VA: its index; VL: 0x80000000 | the number of front-pointer moves).
 template <class I>
-__device__ inline void erase_slot(Ctx<I>& x, ClassQueue* cq, uint32_t v, uint32_t& vl_cursor, uint32_t& reclaim) {
+__device__ inline void clear_slot(Ctx<I>& x, ClassQueue* cq, uint32_t v, uint32_t& vl_cursor, uint32_t& reclaim) {
   constexpr Kind K = I::kKind;
-  max_(&cq->front, v + 1);
   if constexpr (is_std(K)) {
     xchg(&cq->ring[v & (cq->ring_size - 1)], kMarker);
   } else if constexpr (is_va(K)) {
@@ -786,10 +785,9 @@ __device__ inline void erase_slot(Ctx<I>& x, ClassQueue* cq, uint32_t v, uint32_
       if (qc_empty_after_dequeue(old)) reclaim = vl_set_front(x, cq, c) | 0x80000000u;
     }
   }
-  sub(&cq->count, 1);
 }
 
-// Finish deferred queue-chunk reclamation in lane order.
+// Finish deferred queue-chunk reclamation (lane order across the wave).
 template <class I>
 __device__ inline void finish_reclaim(Ctx<I>& x, ClassQueue* cq, bool want, uint32_t reclaim) {
   constexpr Kind K = I::kKind;
@@ -799,6 +797,41 @@ __device__ inline void finish_reclaim(Ctx<I>& x, ClassQueue* cq, bool want, uint
     const bool w = want && reclaim != kMarker;
     vl_set_old(x, cq, w, w ? (reclaim & 0x7FFFFFFFu) : 0u);
   }
+}
+
+// queue_pos while a (re-)enqueue of the chunk is in flight.
+constexpr uint32_t kPending = 0xFFFFFFFEu;
+
+// Lanes that moved a chunk's free count from <= 0 to > 0 make sure the chunk is in its class queue
+// (FIRST_FREE, ChunkQueue_impl.cuh:166-170, and RE_ENQUEUE_CHUNK, :423-428). The queue_pos
+// MARKER -> PENDING CAS admits exactly one enqueue however many lanes or removers race.
+template <int G, class I>
+__device__ inline void ensure_in_queue(Ctx<I>& x, ClassQueue* cq, bool want, uint32_t chunk) {
+  bool won = false;
+  if (want) won = cas(&hdr<ChunkHdr>(x.m, chunk)->queue_pos, kMarker, kPending) == kMarker;
+  if (ballot(won)) enqueue_chunks<G>(x, cq, won, chunk);
+}
+
+// Remove the exhausted chunk at position v, only if v is the queue front. front_ therefore moves
+// past removed entries only and never strands a chunk that still has free pages; the reference's
+// atomicMax(front_, v + 1) (ChunkQueue_impl.cuh:432) can, and then finds it again only by scanning
+// around its whole ring. Returns true if removed; `again` reports a free that raced in.
+template <class I>
+__device__ inline bool try_remove_front(Ctx<I>& x, ClassQueue* cq, uint32_t v, uint32_t chunk, uint32_t& cursor,
+                                        uint32_t& reclaim, bool& again) {
+  again = false;
+  if (ld(&cq->front) != v || cas(&cq->front, v, v + 1) != v) return false;
+  ChunkHdr* h = hdr<ChunkHdr>(x.m, chunk);
+  if (cas(&h->queue_pos, v, kMarker) != v) set_error(x.m, ERR_CORRUPT);
+  uint32_t rc = kMarker;
+  clear_slot(x, cq, v, cursor, rc);
+  if (rc != kMarker) {
+    if (reclaim != kMarker) finish_reclaim(x, cq, true, reclaim);
+    reclaim = rc;
+  }
+  sub(&cq->count, 1);
+  again = ld(&h->count) > 0;
+  return true;
 }
 
 // ChunkAccess::allocPage (ChunkAccess_impl.cuh:38-135), per lane, with 64-bit mask words.
@@ -836,6 +869,16 @@ __device__ inline AccessMode access_alloc_lane(Ctx<I>& x, ChunkHdr* h, uint32_t&
   }
 }
 
+// Wrap a scan position that ran past back_ to the front (no live chunk is outside [front, back)).
+template <class I>
+__device__ __forceinline__ void scan_advance(ClassQueue* cq, uint32_t& v, uint32_t& cursor) {
+  ++v;
+  if (v - ld(&cq->back) < 0x80000000u) {
+    v = ld(&cq->front);
+    cursor = kMarker;
+  }
+}
+
 template <int W, class I>
 __device__ inline void* alloc_chunk_pages(Ctx<I>& x, uint32_t qi, bool want) {
   constexpr int GS = W == 1 ? 64 : W;
@@ -852,48 +895,50 @@ __device__ inline void* alloc_chunk_pages(Ctx<I>& x, uint32_t qi, bool want) {
   uint32_t reclaim = kMarker;
   bool got = false;
   if constexpr (W == 1) {
-    // reference protocol, every lane on its own
-    bool reenq = false;
+    // the reference's per-thread protocol: every lane scans from front_ on its own
     if (ok) {
       uint32_t v = ld(&cq->front);
       uint32_t cursor = kMarker;
       Spin it;
       while (true) {
         const uint32_t c = peek_slot(x, cq, v, cursor);
-        if (c != kMarker && (uint64_t)c < x.m.max_chunks) {
+        if (chunk_ok(x.m, c)) {
           ChunkHdr* h = hdr<ChunkHdr>(x.m, c);
           const AccessMode md = access_alloc_lane(x, h, page);
           if (md == AccessMode::ERROR) break;
-          if (md != AccessMode::CONTINUE) {
+          bool again = false;
+          if (md == AccessMode::CONTINUE) {
+            if (try_remove_front(x, cq, v, c, cursor, reclaim, again) && again) ensure_in_queue<1>(x, cq, true, c);
+          } else {
             chunk = c;
             got = true;
-            if (md == AccessMode::RE_ENQUEUE) reenq = true;
-            if (md == AccessMode::DEQUEUE) {
-              st(&h->queue_pos, kMarker);
-              erase_slot(x, cq, v, cursor, reclaim);
-            }
+            if (md == AccessMode::RE_ENQUEUE) ensure_in_queue<1>(x, cq, true, c);
+            if (md == AccessMode::DEQUEUE && try_remove_front(x, cq, v, c, cursor, reclaim, again) && again)
+              ensure_in_queue<1>(x, cq, true, c);
             break;
           }
         }
-        ++v;
-        if (v - ld(&cq->back) < 0x80000000u) {  // ran past back_: restart from front_
-          v = ld(&cq->front);
-          cursor = kMarker;
-        }
+        scan_advance<I>(cq, v, cursor);
         if (it.expired()) { set_error(x.m, ERR_TIMEOUT); break; }
         backoff(it.it > 64 ? it.it : 0);
       }
     }
-    // RE_ENQUEUE_CHUNK: put the chunk back at the end (ChunkQueue_impl.cuh:423-428)
-    enqueue_chunks<1>(x, cq, reenq, chunk);
   } else {
     // group protocol: the leader reserves pages chunk by chunk with a CAS on the chunk count and
-    // claims them from the chunk's 64-bit mask words; lanes take them in rank order.
+    // claims them from the chunk's 64-bit mask words; lanes take them in rank order. Leaders of one
+    // class start at spread positions of the live queue window (a per-class scan cursor) instead
+    // of all at front_.
     const int L = first(grp);
     const bool leader = ok && (int)lane() == L;
     const uint32_t num = popc(grp);
-    uint32_t v = leader ? ld(&cq->front) : 0u;
-    uint32_t cursor = kMarker;
+    uint32_t v = 0, cursor = kMarker;
+    if (leader) {
+      const uint32_t ppc = cq->ppc;
+      const uint32_t spread = add(&cq->scan, (num + ppc - 1) / ppc);
+      const uint32_t f = ld(&cq->front), b = ld(&cq->back);
+      const uint32_t len = b - f;
+      v = (len == 0 || len >= 0x80000000u) ? f : f + spread % len;
+    }
     uint32_t need = leader ? num : 0u, assigned = 0, t_rem = 0, w = 0, c = kMarker;
     Spin it;
     bool done = !ok, gave_up = false;
@@ -904,7 +949,7 @@ __device__ inline void* alloc_chunk_pages(Ctx<I>& x, uint32_t qi, bool want) {
         if (t_rem == 0) {
           const uint32_t s = peek_slot(x, cq, v, cursor);
           bool advance = true;
-          if (s != kMarker && (uint64_t)s < x.m.max_chunks) {
+          if (chunk_ok(x.m, s)) {
             ChunkHdr* h = hdr<ChunkHdr>(x.m, s);
             int32_t cnt = ld(&h->count);
             uint32_t t = 0;
@@ -914,6 +959,7 @@ __device__ inline void* alloc_chunk_pages(Ctx<I>& x, uint32_t qi, bool want) {
               if (seen == cnt) break;
               cnt = seen;
             }
+            bool again = false;
             if (cnt > 0) {
               c = s;
               t_rem = t;
@@ -921,26 +967,17 @@ __device__ inline void* alloc_chunk_pages(Ctx<I>& x, uint32_t qi, bool want) {
               const uint32_t size = ld(&h->size);
               w = ((size - (uint32_t)cnt) / 64u) % mask_words(size);
               if (cnt == (int32_t)t) {
-                st(&h->queue_pos, kMarker);
-                uint32_t rc = kMarker;
-                erase_slot(x, cq, v, cursor, rc);
-                if (rc != kMarker) {
-                  if (reclaim != kMarker) finish_reclaim(x, cq, true, reclaim);
-                  reclaim = rc;
-                }
+                if (try_remove_front(x, cq, v, s, cursor, reclaim, again) && again) ensure_in_queue<1>(x, cq, true, s);
               } else {
                 advance = false;
               }
+            } else if (try_remove_front(x, cq, v, s, cursor, reclaim, again) && again) {
+              ensure_in_queue<1>(x, cq, true, s);
             }
           }
-          if (advance && t_rem == 0) {
-            ++v;
-            if (v - ld(&cq->back) < 0x80000000u) {
-              v = ld(&cq->front);
-              cursor = kMarker;
-            }
-          } else if (advance) {
-            ++v;
+          if (advance) {
+            if (t_rem == 0) scan_advance<I>(cq, v, cursor);
+            else ++v;
           }
         }
         if (t_rem > 0) {
@@ -995,47 +1032,18 @@ __device__ inline void free_chunk_pages(Ctx<I>& x, uint32_t qi, bool want, uint3
   // ChunkAccess::freePage (ChunkAccess_impl.cuh:7-25): set the bit, then count
   if (want) or_(&h->mask[page / 64u], 1ull << (page % 64u));
   order();
-  constexpr int G = W;  // W = 1: per lane
   const uint64_t key = want ? (uint64_t)chunk : ~0ull;
-  const uint64_t grp = match<G>(key);
+  const uint64_t grp = match<W>(key);  // W = 1: every lane on its own
   const int L = first(grp);
   const uint32_t n = popc(grp);
-  bool first_free = false;
-  uint32_t sig = 0;
+  bool crossing = false;
   if (want && (int)lane() == L) {
     const int32_t old = add(&h->count, (int32_t)n);
-    sig = n;
-    const uint32_t size = ld(&h->size);
-    if (old == 0) {
-      first_free = true;
-    } else if ((uint32_t)(old + (int32_t)n) == size && I::kKind == Kind::CQ &&
-               ld(&cq->count) > (int32_t)x.m.lower_fill_level) {
-      // flash the chunk back to the reuse ring (ChunkQueue_impl.cuh:163-199)
-      if (sem_try_reduce(cq, size - n)) {
-        if (cas(&h->count, (int32_t)size, 0) == (int32_t)size) {
-          uint32_t qp;
-          Spin spin;
-          while ((qp = ld(&h->queue_pos)) == kMarker) {
-            if (spin.expired()) { set_error(x.m, ERR_TIMEOUT); break; }
-            backoff(spin.it);
-          }
-          if (qp != kMarker) {
-            xchg(&cq->ring[qp & (cq->ring_size - 1)], kMarker);
-            sub(&cq->count, 1);
-            sub(&cq->carved, size);
-          }
-          sig = 0;
-        } else {
-          sig = size;
-        }
-      }
-    }
+    crossing = old <= 0 && old + (int32_t)n > 0;
   }
-  const bool flashed = want && (int)lane() == L && sig == 0;
-  // FIRST_FREE: back into the queue before the pages are signalled
-  enqueue_chunks<W>(x, cq, first_free, chunk);
-  ring_put(x.m, x.inst.reuse, flashed, chunk);
-  if (want && (int)lane() == L && sig) sem_signal(cq, sig);
+  // back into the queue before the pages are signalled
+  ensure_in_queue<W>(x, cq, crossing, chunk);
+  if (want && (int)lane() == L) sem_signal(cq, n);
 }
 
 }  // namespace detail

@@ -114,7 +114,7 @@ struct alignas(128) Memory {
   uint32_t max_chunks;
   uint32_t variant;
   uint32_t initialized;
-  uint32_t lower_fill_level;  // ChunkQueue flash threshold (ChunkQueue.cuh:27)
+  uint32_t reserved0;
   alignas(128) uint32_t next_free_chunk;
   alignas(128) uint32_t error;
   alignas(128) uint32_t stats[4];  // pageAlloc, pageFree, chunkAlloc, chunkReuse (Statistics.h:5-17)
@@ -143,7 +143,8 @@ struct alignas(128) ClassQueue {
   alignas(128) uint32_t back;
   uint32_t back_qc;      // VL back_ptr_
   alignas(128) int32_t count;   // chunk queues: chunks in the queue (count_)
-  uint32_t carved;       // pages ever carved into this class (capacity guard)
+  uint32_t scan;         // chunk queues: cursor that spreads group leaders over the queue window
+  uint32_t carved;       // pages (page queues) or chunks (chunk queues) carved into this class
   uint32_t old_qc;       // VL old_ptr_
   uint32_t old_count;    // VL old_count_
 };

@@ -229,8 +229,7 @@ struct Ouroboros {
 
   // Ouroboros::initialize (MemoryInitialization.cuh:123-219). Idempotent; returns hipSuccess or the
   // first HIP error instead of exiting.
-  hipError_t initialize(size_t instantiation_size, size_t additionalSizeBeginning = 0, size_t additionalSizeEnd = 0,
-                        uint32_t lower_fill_level = (uint32_t)(kStdEntries * 0.1f)) {
+  hipError_t initialize(size_t instantiation_size, size_t additionalSizeBeginning = 0, size_t additionalSizeEnd = 0) {
     if (memory.initialized) return hipSuccess;
     static_assert(sizeof(MyType) <= kStateBytes, "manager state exceeds its 8 KiB slot");
     const PoolLayout L = layout(instantiation_size, additionalSizeBeginning, additionalSizeEnd);
@@ -244,7 +243,6 @@ struct Ouroboros {
     memory.max_chunks = (uint32_t)L.max_chunks;
     memory.locator_bytes = L.locator_bytes;
     memory.locator = reinterpret_cast<uint32_t*>(d + L.allocation_size - L.locator_bytes);
-    memory.lower_fill_level = lower_fill_level;
     memory.next_free_chunk = 0;
     memory.error = 0;
     // queue region: rings and class queues are placed downward from the locator (initNew, :42-58)
@@ -325,6 +323,7 @@ struct Ouroboros {
       q.sem = Sem::kInit;
       q.front = q.back = 0;
       q.count = 0;
+      q.scan = 0;
       q.carved = 0;
       q.old_count = 0;
       q.front_qc = q.back_qc = q.old_qc = kMarker;

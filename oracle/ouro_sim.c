@@ -23,6 +23,7 @@
 #define OLD_HOLD 10u          /* LARGEST_OLD_COUNT_VALUE (Parameters.h:13) */
 #define STATE_BYTES 8192u
 #define BLOCK 4096u
+#define PENDING 0xFFFFFFFEu
 
 enum { ERR_OOM = 1, ERR_QUEUE_FULL = 2, ERR_VSTART = 4, ERR_TIMEOUT = 8, ERR_BAD_FREE = 16, ERR_TOO_LARGE = 32,
        ERR_REUSE_FULL = 64, ERR_CORRUPT = 256 };
@@ -47,7 +48,7 @@ typedef struct {
 typedef struct {
   int64_t cnt;
   int64_t exp, res;
-  uint32_t front, back, carved, old_count, front_qc, back_qc, old_qc;
+  uint32_t front, back, carved, old_count, front_qc, back_qc, old_qc, scan;
   int32_t count;
   uint32_t page_size, ppc, ring_size;
   uint32_t** blocks; /* std ring, allocated in 4096-entry blocks */
@@ -70,7 +71,7 @@ typedef struct {
 struct ouro_sim {
   int variant;
   uint64_t max_chunks, alloc_size;
-  uint32_t next_free, error, lower_fill;
+  uint32_t next_free, error;
   uint32_t* locator;
   chunk_t* chunks;
   inst_t inst[2];
@@ -416,10 +417,9 @@ static uint32_t peek(ouro_sim* s, inst_t* I, cls_t* q, uint32_t v, uint32_t* cur
   return ch(s, c)->slots[v % QC_SPOTS];
 }
 
-/* DEQUEUE_CHUNK erase (ChunkQueue_impl.cuh:429-437, ChunkQueueVA_impl.cuh:209-243,
- * ChunkQueueVL_impl.cuh:176-187). A queue chunk emptied here is reclaimed after the scan. */
-static void erase(ouro_sim* s, inst_t* I, cls_t* q, uint32_t v, uint32_t* cursor, uint32_t* reclaim) {
-  if (q->front < v + 1) q->front = v + 1;
+/* Clear the slot at v (ChunkQueue_impl.cuh:434, QueueChunk::deleteElement QueueChunk_impl.cuh:349-359).
+ * A queue chunk emptied here is reclaimed after the scan. */
+static void clear_slot(ouro_sim* s, inst_t* I, cls_t* q, uint32_t v, uint32_t* cursor, uint32_t* reclaim) {
   if (is_std(I->kind)) {
     *std_slot(q, v) = MARKER;
   } else if (is_va(I->kind)) {
@@ -446,7 +446,27 @@ static void erase(ouro_sim* s, inst_t* I, cls_t* q, uint32_t v, uint32_t* cursor
       if (old - 0x10000u == QC_SPOTS) *reclaim = vl_set_front(s, q, c) | 0x80000000u;
     }
   }
+}
+
+static void finish_reclaim(ouro_sim* s, inst_t* I, cls_t* q, uint32_t reclaim);
+
+/* Remove the exhausted chunk at v only if v is the queue front (the device's try_remove_front):
+ * front_ moves past removed entries only. Returns 1 when removed. */
+static int remove_front(ouro_sim* s, inst_t* I, cls_t* q, uint32_t v, uint32_t chunk, uint32_t* cursor,
+                        uint32_t* reclaim) {
+  if (q->front != v) return 0;
+  q->front = v + 1;
+  chunk_t* x = ch(s, chunk);
+  if (x->queue_pos != v) s->error |= ERR_CORRUPT;
+  x->queue_pos = MARKER;
+  uint32_t rc = MARKER;
+  clear_slot(s, I, q, v, cursor, &rc);
+  if (rc != MARKER) {
+    if (*reclaim != MARKER) finish_reclaim(s, I, q, *reclaim);
+    *reclaim = rc;
+  }
   q->count--;
+  return 1;
 }
 
 /* ------------------------------------------------------------------ carves */
@@ -551,7 +571,7 @@ static int make_groups(const int* lanes, int n, const uint32_t* cls, group_t* g)
 
 /* BulkSemaphore::wait (BulkSemaphore_impl.cuh:199-300) for every group leader of one wave: each
  * pass decides for all pending leaders, then carves run in leader order. */
-static void acquire(ouro_sim* s, inst_t* I, group_t* g, int ng, int* ok) {
+static void acquire(ouro_sim* s, inst_t* I, group_t* g, int ng, int* ok, int nact) {
   int pending[64];
   for (int j = 0; j < ng; ++j) {
     pending[j] = 1;
@@ -573,7 +593,9 @@ static void acquire(ouro_sim* s, inst_t* I, group_t* g, int ng, int* ok) {
         continue;
       }
       if (q->cnt + q->exp - q->res < num) {
-        const int64_t k = (num - (q->cnt + q->exp - q->res) + q->ppc - 1) / q->ppc;
+        const int64_t c0 = q->cnt > 0 ? q->cnt : 0;
+        int64_t k = (num - (c0 + q->exp - q->res) + q->ppc - 1) / q->ppc;
+        if (k > nact) k = nact;
         q->exp += k * q->ppc;
         carve_k[j] = (uint32_t)k;
       } else if (q->cnt >= num) {
@@ -605,13 +627,18 @@ static uint32_t mask_words(uint32_t ppc) {
   return w == 0 ? 1 : (w > 8 ? 8 : w);
 }
 
-/* Group claim for chunk-based queues: reserve pages chunk by chunk on the chunk count, take them
- * from the 64-bit mask words, lowest bits first, starting at word (size - count) / 64.
- * (ChunkQueue::allocPage ChunkQueue_impl.cuh:88-151 and ChunkAccess::allocPage
- * ChunkAccess_impl.cuh:38-135, as the device's W = 64 rule.) */
+/* Group claim for chunk-based queues: the leader starts at a spread position of the live queue
+ * window (per-class scan cursor), reserves pages chunk by chunk on the chunk count and takes them from
+ * the 64-bit mask words, lowest bits first, starting at word (size - count) / 64. Exhausted chunks
+ * leave the queue only from its front. (ChunkQueue::allocPage ChunkQueue_impl.cuh:88-151 and
+ * ChunkAccess::allocPage ChunkAccess_impl.cuh:38-135, as the device's W = 64 rule.) */
 static void claim_group(ouro_sim* s, inst_t* I, group_t* g, uint64_t* out, uint32_t* reclaim) {
   cls_t* q = &I->q[g->qi];
-  uint32_t v = q->front, cursor = MARKER, need = (uint32_t)g->n, assigned = 0;
+  const uint32_t spread = q->scan;
+  q->scan += ((uint32_t)g->n + q->ppc - 1) / q->ppc;
+  const uint32_t len = q->back - q->front;
+  uint32_t v = (len == 0 || len >= 0x80000000u) ? q->front : q->front + spread % len;
+  uint32_t cursor = MARKER, need = (uint32_t)g->n, assigned = 0;
   int guard = 0;
   *reclaim = MARKER;
   while (assigned < (uint32_t)g->n) {
@@ -620,31 +647,19 @@ static void claim_group(ouro_sim* s, inst_t* I, group_t* g, uint64_t* out, uint3
       return;
     }
     const uint32_t sl = peek(s, I, q, v, &cursor);
-    int advance = 1;
+    int advance = 1, claimed = 0;
     if (sl != MARKER && chunk_ok(s, sl)) {
       chunk_t* x = ch(s, sl);
       const int32_t cnt = x->count;
       if (cnt > 0) {
+        claimed = 1;
         const uint32_t t = (uint32_t)cnt < need ? (uint32_t)cnt : need;
         x->count -= (int32_t)t;
         need -= t;
         const uint32_t nw = mask_words(x->size);
         uint32_t w = ((x->size - (uint32_t)cnt) / 64u) % nw;
-        if (cnt == (int32_t)t) {
-          x->queue_pos = MARKER;
-          uint32_t rc = MARKER;
-          erase(s, I, q, v, &cursor, &rc);
-          if (rc != MARKER) {
-            if (*reclaim != MARKER) {
-              /* a second reclaim in one scan is issued immediately */
-              if (is_va(I->kind)) ring_put(s, &s->inst[0].reuse, *reclaim);
-              else vl_set_old(s, q, *reclaim & 0x7FFFFFFFu);
-            }
-            *reclaim = rc;
-          }
-        } else {
-          advance = 0;
-        }
+        if (cnt == (int32_t)t) remove_front(s, I, q, v, sl, &cursor, reclaim);
+        else advance = 0;
         uint32_t t_rem = t;
         int wguard = 0;
         while (t_rem > 0 && wguard++ < 64) {
@@ -670,11 +685,13 @@ static void claim_group(ouro_sim* s, inst_t* I, group_t* g, uint64_t* out, uint3
           if (t_rem > 0) w = (w + 1) % nw;
         }
         if (t_rem) s->error |= ERR_TIMEOUT;
+      } else {
+        remove_front(s, I, q, v, sl, &cursor, reclaim);
       }
     }
     if (advance) {
       ++v;
-      if (need > 0 && v - q->back < 0x80000000u) {
+      if (!claimed && v - q->back < 0x80000000u) {
         v = q->front;
         cursor = MARKER;
       }
@@ -688,13 +705,15 @@ static void finish_reclaim(ouro_sim* s, inst_t* I, cls_t* q, uint32_t reclaim) {
   else if (is_vl(I->kind)) vl_set_old(s, q, reclaim & 0x7FFFFFFFu);
 }
 
-static void inst_malloc(ouro_sim* s, inst_t* I, const int* lanes, int n, const uint32_t* sizes, uint64_t* out) {
+/* nact: active lanes of the whole wave (the device enters each instance's block with all of them). */
+static void inst_malloc(ouro_sim* s, inst_t* I, const int* lanes, int n, const uint32_t* sizes, uint64_t* out,
+                        int nact) {
   uint32_t cls[64];
   for (int i = 0; i < n; ++i) cls[i] = (uint32_t)sim_class_of(sizes[lanes[i]], I->smallest);
   group_t g[64];
   const int ng = make_groups(lanes, n, cls, g);
   int ok[64];
-  acquire(s, I, g, ng, ok);
+  acquire(s, I, g, ng, ok, nact);
   if (!is_chunk_kind(I->kind)) {
     /* dequeue positions: front fetch-add per group, rank order (atomicAggInc) */
     cls_t* qs[64];
@@ -746,6 +765,7 @@ void sim_wave_malloc(ouro_sim* s, int n, const uint32_t* sizes, uint64_t* out) {
     out[i] = UINT64_MAX;
     if (sizes[i]) rest[nr++] = i;
   }
+  const int nact = nr;
   for (int k = 0; k < s->ninst && nr; ++k) {
     inst_t* I = &s->inst[k];
     int here[64], nh = 0, later[64], nl = 0;
@@ -753,7 +773,7 @@ void sim_wave_malloc(ouro_sim* s, int n, const uint32_t* sizes, uint64_t* out) {
       if (sizes[rest[i]] <= I->largest) here[nh++] = rest[i];
       else later[nl++] = rest[i];
     }
-    if (nh) inst_malloc(s, I, here, nh, sizes, out);
+    if (nh) inst_malloc(s, I, here, nh, sizes, out, nact);
     memcpy(rest, later, sizeof(int) * (size_t)nl);
     nr = nl;
   }
@@ -796,9 +816,6 @@ static void inst_free(ouro_sim* s, inst_t* I, const int* lanes, int n, const uin
   uint32_t ff_chunk[64];
   cls_t* ff_q[64];
   int nff = 0;
-  uint32_t flashed[64];
-  int nfl = 0;
-  int64_t sig[64];
   for (int j = 0; j < ng; ++j) {
     const uint32_t c = g[j].qi;
     int idx = 0;
@@ -808,32 +825,19 @@ static void inst_free(ouro_sim* s, inst_t* I, const int* lanes, int n, const uin
     chunk_t* x = ch(s, c);
     const int32_t old = x->count;
     x->count += g[j].n;
-    sig[j] = g[j].n;
-    if (old == 0) {
+    /* count crossed from <= 0 to > 0: the chunk must be in the queue; queue_pos MARKER means it left */
+    if (old <= 0 && old + g[j].n > 0 && x->queue_pos == MARKER) {
+      x->queue_pos = PENDING;
       ff_chunk[nff] = c;
       ff_q[nff++] = q;
-    } else if ((uint32_t)(old + g[j].n) == x->size && I->kind == K_CQ && q->count > (int32_t)s->lower_fill) {
-      /* flash (ChunkQueue_impl.cuh:163-199) */
-      const int64_t red = (int64_t)x->size - g[j].n;
-      if (q->cnt >= red) {
-        q->cnt -= red;
-        x->count = 0;
-        *std_slot(q, x->queue_pos) = MARKER;
-        q->count--;
-        q->carved -= x->size;
-        flashed[nfl++] = c;
-        sig[j] = 0;
-      }
     }
   }
-  /* FIRST_FREE: back into the queue (positions per queue in lane order), before signalling */
   if (nff) enqueue_chunks(s, I, ff_q, nff, ff_chunk);
-  for (int i = 0; i < nfl; ++i) ring_put(s, &I->reuse, flashed[i]);
   for (int j = 0; j < ng; ++j) {
     int idx = 0;
     for (int i = 0; i < n; ++i)
       if (lanes[i] == g[j].lanes[0]) idx = i;
-    I->q[qi[idx]].cnt += sig[j];
+    I->q[qi[idx]].cnt += g[j].n;
   }
 }
 
@@ -889,7 +893,6 @@ ouro_sim* sim_create(int variant, uint64_t pool_bytes) {
   sim_layout(variant, pool_bytes, &s->max_chunks, &s->alloc_size);
   s->chunks = (chunk_t*)calloc(s->max_chunks + 4, sizeof(chunk_t));
   s->locator = (uint32_t*)calloc((s->max_chunks + 31) / 32 + 1, 4);
-  s->lower_fill = (uint32_t)(STD_ENTRIES * 0.1f);
   s->ninst = 2;
   uint32_t qc_next = 0;
   for (int k = 0; k < 2; ++k) {

@@ -176,3 +176,23 @@ def test_out_of_memory_returns_nullptr_and_sets_error():
         got += int((s.wave_malloc(np.full(1, 8192)) != NONE).sum())
     assert got == mc - 1
     assert s.state().error & 1
+
+
+GOLDEN = np.load(__import__("pathlib").Path(__file__).with_name("golden") / "single_wave.npz")
+
+
+@pytest.mark.parametrize("kind", KINDS)
+def test_oracle_reproduces_committed_fixtures(kind):
+    cases = sorted({k.split("__")[0] for k in GOLDEN.files})
+    for case in cases:
+        sizes = GOLDEN[f"{case}__sizes"]
+        want = GOLDEN[f"{case}__{kind}"]
+        got = oracle.Sim(kind, 512 << 20).trace(sizes, want.shape[0])
+        assert np.array_equal(got, want), (case, kind)
+
+
+def test_trace_generator_matches_fixture_restatement():
+    import ouroboros_amd as ob
+
+    assert np.array_equal(GOLDEN["uniform_seed42__sizes"], ob.gen_sizes(42, 64, "uniform"))
+    assert np.array_equal(GOLDEN["ragged_37__sizes"], ob.gen_sizes(7, 37, "uniform"))
