@@ -100,12 +100,19 @@ __device__ __forceinline__ void backoff(uint32_t iter) {
   else __builtin_amdgcn_s_sleep(8);
 }
 
-// A bounded wait: the first 256 iterations are free, after that the wall clock decides.
+// A bounded wait: the first 256 iterations are free, after that the wall clock decides. Once any
+// wait of the kernel has given up (ERR_TIMEOUT in the pool's error word), every other wait gives up
+// at its next clock check, so one stuck wait cannot keep thousands of waves spinning.
 struct Spin {
   uint32_t it = 0;
   uint64_t t0 = 0;
+  const uint32_t* err = nullptr;
+  __device__ __forceinline__ Spin() {}
+  __device__ __forceinline__ explicit Spin(const uint32_t* error_word) : err(error_word) {}
   __device__ __forceinline__ bool expired() {
     if (++it < 256) return false;
+    if ((it & 63u) != 0) return false;
+    if (err && (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & 8u)) return true;
     const uint64_t now = wall_clock64();
     if (t0 == 0) {
       t0 = now;

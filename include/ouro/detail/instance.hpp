@@ -83,7 +83,7 @@ __device__ inline void ring_put(Memory& m, ReuseRing& r, bool want, uint32_t val
     const uint32_t rk = rank_in(mask);
     if (rk < fill) {
       uint32_t* slot = &r.ring[(pos + rk) & (r.size - 1)];
-      Spin it;
+      Spin it(&m.error);
       while (cas(slot, kMarker, value) != kMarker) {
         if (it.expired()) { set_error(m, ERR_TIMEOUT); break; }
         backoff(it.it);
@@ -106,7 +106,7 @@ __device__ inline uint32_t ring_take(Memory& m, ReuseRing& r, uint32_t k, int l,
   pos = readlane(pos, l);
   if (me < t) {
     uint32_t* slot = &r.ring[(pos + me) & (r.size - 1)];
-    Spin it;
+    Spin it(&m.error);
     uint32_t v;
     while ((v = xchg(slot, kMarker)) == kMarker) {
       if (it.expired()) { set_error(m, ERR_TIMEOUT); break; }
@@ -190,7 +190,7 @@ struct Ctx {
 // ---- VL pointer maintenance (QueueChunk_impl.cuh:440-551)
 template <class I>
 __device__ inline void vl_set_back(Ctx<I>& x, ClassQueue* cq, uint32_t c) {
-  Spin it;
+  Spin it(&x.m.error);
   while (true) {
     QueueChunkHdr* h = hdr<QueueChunkHdr>(x.m, c);
     const uint32_t nx = ld(&h->next);
@@ -236,7 +236,7 @@ __device__ inline void vl_set_old(Ctx<I>& x, ClassQueue* cq, bool want, uint32_t
     free_count = 0;
   }
   uint32_t cur = free_count ? ld(&cq->old_qc) : 0u;
-  Spin it;
+  Spin it(&x.m.error);
   while (ballot(free_count > 0)) {
     bool put = false;
     uint32_t victim = 0;
@@ -262,12 +262,15 @@ __device__ inline void vl_set_old(Ctx<I>& x, ClassQueue* cq, bool want, uint32_t
 template <class I>
 __device__ inline uint32_t vl_locate(Ctx<I>& x, uint32_t start, uint32_t v) {
   uint32_t c = start;
-  Spin it;
+  Spin it(&x.m.error);
   if (!chunk_ok(x.m, c)) {
     set_error(x.m, ERR_VSTART);
     return kMarker;
   }
-  while (ld(&hdr<QueueChunkHdr>(x.m, c)->virtual_start) / kQCSpots != v / kQCSpots) {
+  while (true) {
+    const uint32_t vs = ld(&hdr<QueueChunkHdr>(x.m, c)->virtual_start) / kQCSpots;
+    if (vs == v / kQCSpots) break;
+    if (vs - v / kQCSpots < 0x80000000u) return kMarker;  // already past v: its queue chunk is gone
     uint32_t nx;
     while ((nx = ld(&hdr<QueueChunkHdr>(x.m, c)->next)) == kMarker) {
       if (it.expired()) { set_error(x.m, ERR_TIMEOUT); return kMarker; }
@@ -288,7 +291,7 @@ template <class I>
 __device__ inline uint32_t va_wait_qc(Ctx<I>& x, ClassQueue* cq, uint32_t v) {
   const uint32_t id = (v / kQCSpots) & (cq->ring_size - 1);
   uint32_t qc;
-  Spin it;
+  Spin it(&x.m.error);
   while ((qc = ld(&cq->ring[id])) == kMarker) {
     if (it.expired()) { set_error(x.m, ERR_TIMEOUT); return kMarker; }
     backoff(it.it);
@@ -316,7 +319,7 @@ __device__ inline void put_lanes(Ctx<I>& x, ClassQueue* cq, bool want, uint32_t 
   if constexpr (is_std(K)) {
     if (want) {
       uint32_t* slot = &cq->ring[v & (cq->ring_size - 1)];
-      Spin it;
+      Spin it(&x.m.error);
       while (cas(slot, kMarker, val) != kMarker) {
         if (it.expired()) { set_error(x.m, ERR_TIMEOUT); break; }
         backoff(it.it);
@@ -349,7 +352,7 @@ __device__ inline void put_lanes(Ctx<I>& x, ClassQueue* cq, bool want, uint32_t 
         const uint32_t c = va_wait_qc(x, cq, v);
         if (c != kMarker) {
           uint32_t* slot = &qc_slots(x.m, c)[v % kQCSpots];
-          Spin it;
+          Spin it(&x.m.error);
           while (cas(slot, kMarker, val) != kMarker) {
             if (it.expired()) { set_error(x.m, ERR_TIMEOUT); break; }
             backoff(it.it);
@@ -369,7 +372,7 @@ __device__ inline void put_lanes(Ctx<I>& x, ClassQueue* cq, bool want, uint32_t 
       if (want && !todo) set_error(x.m, ERR_VSTART);
       bool old_want = false;
       uint32_t old_free = 0;
-      Spin it;
+      Spin it(&x.m.error);
       while (ballot(todo)) {
         if (todo) {
           QueueChunkHdr* h = hdr<QueueChunkHdr>(x.m, cur);
@@ -414,7 +417,7 @@ __device__ inline uint32_t take_lanes(Ctx<I>& x, ClassQueue* cq, bool want, uint
   if constexpr (is_std(K)) {
     if (want) {
       uint32_t* slot = &cq->ring[v & (cq->ring_size - 1)];
-      Spin it;
+      Spin it(&x.m.error);
       while ((val = xchg(slot, kMarker)) == kMarker) {
         if (it.expired()) { set_error(x.m, ERR_TIMEOUT); break; }
         backoff(it.it);
@@ -427,7 +430,7 @@ __device__ inline uint32_t take_lanes(Ctx<I>& x, ClassQueue* cq, bool want, uint
       const uint32_t c = va_wait_qc(x, cq, v);
       if (c != kMarker) {
         uint32_t* slot = &qc_slots(x.m, c)[v % kQCSpots];
-        Spin it;
+        Spin it(&x.m.error);
         while ((val = xchg(slot, kMarker)) == kMarker) {
           if (it.expired()) { set_error(x.m, ERR_TIMEOUT); break; }
           backoff(it.it);
@@ -447,7 +450,7 @@ __device__ inline uint32_t take_lanes(Ctx<I>& x, ClassQueue* cq, bool want, uint
       const uint32_t c = vl_locate(x, ld(&cq->front_qc), v);
       if (c != kMarker) {
         uint32_t* slot = &qc_slots(x.m, c)[v % kQCSpots];
-        Spin it;
+        Spin it(&x.m.error);
         while ((val = xchg(slot, kMarker)) == kMarker) {
           if (it.expired()) { set_error(x.m, ERR_TIMEOUT); break; }
           backoff(it.it);
@@ -475,7 +478,7 @@ __device__ inline bool sem_acquire(Ctx<I>& x, ClassQueue* cq, uint64_t grp, bool
   const int64_t num = (int64_t)popc(grp);
   const int64_t max_k = (int64_t)popc(active());
   bool pending = leader, ok = false, first_pass = true;
-  Spin it;
+  Spin it(&x.m.error);
   while (ballot(pending)) {
     uint32_t carve_k = 0;
     bool reserve = false;
@@ -498,26 +501,14 @@ __device__ inline bool sem_acquire(Ctx<I>& x, ClassQueue* cq, uint64_t grp, bool
         v = ld(s);
         const int64_t ppc = cq->ppc;
         while (true) {
-          const int64_t c = Sem::count(v), e = (int64_t)Sem::expected(v), r = (int64_t)Sem::reserved(v);
-          uint64_t nv;
-          uint32_t k = 0;
-          bool take = false, res = false;
-          if (c + e - r < num) {
-            // Carve for the whole deficit, i.e. this group plus the reservations already waiting
-            // (one chunk per loop pass in the reference leaves every waiter to wake once per carve).
-            // A transiently negative count (optimistic takes being undone) counts as zero, and a
-            // carve never exceeds one chunk per active lane.
-            const int64_t c0 = c > 0 ? c : 0;
-            const int64_t want_k = (num - (c0 + e - r) + ppc - 1) / ppc;
-            k = (uint32_t)(want_k < max_k ? want_k : max_k);
-            nv = Sem::make(c, (uint64_t)(e + (int64_t)k * ppc), (uint64_t)r);
-          } else if (c >= num) {
-            nv = Sem::make(c - num, (uint64_t)e, (uint64_t)r);
-            take = true;
-          } else {
-            nv = Sem::make(c, (uint64_t)e, (uint64_t)(r + num));
-            res = true;
+          const SemDecision d = sem_decide(v, num, ppc, max_k);
+          if (d.mode == SemMode::HOLD) {
+            backoff(64);
+            break;
           }
+          const uint64_t nv = d.next;
+          const uint32_t k = d.mode == SemMode::CARVE ? d.k : 0u;
+          const bool take = d.mode == SemMode::TAKE, res = d.mode == SemMode::RESERVE;
           const uint64_t seen = cas(s, v, nv);
           if (seen == v) {
             carve_k = k;
@@ -544,7 +535,7 @@ __device__ inline bool sem_acquire(Ctx<I>& x, ClassQueue* cq, uint64_t grp, bool
     }
     if (reserve) {
       uint64_t* s = &cq->sem;
-      Spin spin;
+      Spin spin(&x.m.error);
       while (true) {
         const uint64_t v = ld(s);
         const int64_t c = Sem::count(v);
@@ -841,7 +832,7 @@ template <class I>
 __device__ inline AccessMode access_alloc_lane(Ctx<I>& x, ChunkHdr* h, uint32_t& page) {
   int32_t cur = 0;
   AccessMode mode = AccessMode::SUCCESS;
-  Spin it;
+  Spin it(&x.m.error);
   while ((cur = sub(&h->count, 1)) <= 0) {
     if ((cur = add(&h->count, 1)) < 0) return AccessMode::CONTINUE;
     if (cur == 0) mode = AccessMode::RE_ENQUEUE;
@@ -873,8 +864,9 @@ __device__ inline AccessMode access_alloc_lane(Ctx<I>& x, ChunkHdr* h, uint32_t&
 template <class I>
 __device__ __forceinline__ void scan_advance(ClassQueue* cq, uint32_t& v, uint32_t& cursor) {
   ++v;
-  if (v - ld(&cq->back) < 0x80000000u) {
-    v = ld(&cq->front);
+  const uint32_t f = ld(&cq->front);
+  if (v - ld(&cq->back) < 0x80000000u || v - f >= 0x80000000u) {  // past back_, or behind front_
+    v = f;
     cursor = kMarker;
   }
 }
@@ -899,7 +891,7 @@ __device__ inline void* alloc_chunk_pages(Ctx<I>& x, uint32_t qi, bool want) {
     if (ok) {
       uint32_t v = ld(&cq->front);
       uint32_t cursor = kMarker;
-      Spin it;
+      Spin it(&x.m.error);
       while (true) {
         const uint32_t c = peek_slot(x, cq, v, cursor);
         if (chunk_ok(x.m, c)) {
@@ -940,7 +932,7 @@ __device__ inline void* alloc_chunk_pages(Ctx<I>& x, uint32_t qi, bool want) {
       v = (len == 0 || len >= 0x80000000u) ? f : f + spread % len;
     }
     uint32_t need = leader ? num : 0u, assigned = 0, t_rem = 0, w = 0, c = kMarker;
-    Spin it;
+    Spin it(&x.m.error);
     bool done = !ok, gave_up = false;
     while (ballot(!done)) {
       uint32_t bc = kMarker, r0 = 0;

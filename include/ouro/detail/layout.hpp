@@ -77,6 +77,38 @@ struct Sem {
   static constexpr uint64_t kInit = kBias;
 };
 
+// One pass of BulkSemaphore::wait's decision (BulkSemaphore_impl.cuh:236-263) for a group of `num`
+// pages on a class with `ppc` pages per chunk, given the semaphore word v:
+//   CARVE   -- expected += k * ppc, then the caller carves k chunks (1 <= k <= max_k);
+//   TAKE    -- count -= num;
+//   RESERVE -- reserved += num, then the caller waits;
+//   HOLD    -- expected and reserved are both saturated: change nothing, retry later.
+// A transiently negative count (optimistic takes about to be undone) counts as zero. A carve covers
+// the whole deficit (this group plus the waiting reservations) instead of one chunk per pass.
+enum class SemMode : int { CARVE = 0, TAKE = 1, RESERVE = 2, HOLD = 3 };
+struct SemDecision {
+  SemMode mode;
+  uint32_t k;
+  uint64_t next;
+};
+OURO_HD inline SemDecision sem_decide(uint64_t v, int64_t num, int64_t ppc, int64_t max_k) {
+  const int64_t c = Sem::count(v), e = (int64_t)Sem::expected(v), r = (int64_t)Sem::reserved(v);
+  const int64_t c0 = c > 0 ? c : 0;
+  const int64_t avail = c0 + e - r;
+  const int64_t field = (int64_t)Sem::kFieldMask;
+  if (avail < num && e + ppc <= field && max_k >= 1) {
+    int64_t k = (num - avail + ppc - 1) / ppc;
+    const int64_t room = (field - e) / ppc;
+    if (k > max_k) k = max_k;
+    if (k > room) k = room;
+    if (k < 1) k = 1;
+    return {SemMode::CARVE, (uint32_t)k, Sem::make(c, (uint64_t)(e + k * ppc), (uint64_t)r)};
+  }
+  if (c >= num) return {SemMode::TAKE, 0u, Sem::make(c - num, (uint64_t)e, (uint64_t)r)};
+  if (r + num <= field) return {SemMode::RESERVE, 0u, Sem::make(c, (uint64_t)e, (uint64_t)(r + num))};
+  return {SemMode::HOLD, 0u, v};
+}
+
 // ------------------------------------------------------------------ chunk headers (128 B)
 // PageChunk (PageIndexChunk.cuh:5-20): {page_size, number_pages}.
 struct PageChunkHdr {

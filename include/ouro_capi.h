@@ -159,6 +159,11 @@ int ouro_run_trace(ouro_handle h, const ouro_trace_config* cfg, ouro_trace_resul
  * sizes 16 << q (q < classes), 2 = Zipf(param) over class ranks. */
 int ouro_gen_sizes(uint64_t seed, uint32_t n, int dist, double param, uint32_t classes, uint32_t* out);
 
+/* ---- test hook (no reference counterpart) --------------------------------------------------- */
+/* The device semaphore decision (BulkSemaphore_impl.cuh:236-263 as changed here) evaluated on the
+ * host: mode 0 carve k chunks, 1 take, 2 reserve, 3 hold; *next = the word the CAS would install. */
+int ouro_sem_decide(uint64_t word, int64_t num, int64_t ppc, int64_t max_k, int* mode, uint32_t* k, uint64_t* next);
+
 /* ---- measurement helpers (no reference counterpart) ---------------------------------------- */
 /* Peak device-scope atomic throughput: `threads` threads each issue `iters` u32 fetch-adds with
  * return. pattern 0 = every lane its own address (distinct cache lines), 1 = one address per wave,

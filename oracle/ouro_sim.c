@@ -592,10 +592,11 @@ static void acquire(ouro_sim* s, inst_t* I, group_t* g, int ng, int* ok, int nac
         ok[j] = 1;
         continue;
       }
-      if (q->cnt + q->exp - q->res < num) {
-        const int64_t c0 = q->cnt > 0 ? q->cnt : 0;
+      const int64_t c0 = q->cnt > 0 ? q->cnt : 0;
+      if (c0 + q->exp - q->res < num) {
         int64_t k = (num - (c0 + q->exp - q->res) + q->ppc - 1) / q->ppc;
         if (k > nact) k = nact;
+        if (k < 1) k = 1;
         q->exp += k * q->ppc;
         carve_k[j] = (uint32_t)k;
       } else if (q->cnt >= num) {

@@ -172,6 +172,15 @@ __global__ void __launch_bounds__(256) k_copy(const uint4* __restrict__ src, uin
 
 extern "C" {
 
+int ouro_sem_decide(uint64_t word, int64_t num, int64_t ppc, int64_t max_k, int* mode, uint32_t* k, uint64_t* next) {
+  if (!mode || !k || !next || num < 1 || ppc < 1) return OURO_E_INVALID;
+  const ouro::SemDecision d = ouro::sem_decide(word, num, ppc, max_k);
+  *mode = (int)d.mode;
+  *k = d.k;
+  *next = d.next;
+  return OURO_OK;
+}
+
 int ouro_bench_atomics(uint32_t threads, uint32_t iters, int pattern, double* ops_per_s) {
   if (!ops_per_s || threads == 0 || threads % 256 || iters == 0 || pattern < 0 || pattern > 2) return OURO_E_INVALID;
   DevBuf buf, sink;
