@@ -34,10 +34,19 @@ enum : uint32_t {
 };
 
 constexpr uint32_t kMarker = 0xFFFFFFFFu;  // empty slot, DeletionMarker<index_t> (Definitions.h:31-35)
+constexpr uint32_t kTomb = 0xFFFFFFFDu;    // chunk-queue slot whose chunk was removed (front_ may pass it)
 constexpr uint32_t kWave = 64;             // CDNA wavefront width (WARP_SIZE=32 in Definitions.h:16)
 // Spin budget: a wait that lasts longer than this (wall-clock ticks of s_memrealtime, 100 MHz on
 // MI355X, so about 2 s) gives up and records ERR_TIMEOUT; nested waits are bounded by depth x budget.
 constexpr uint64_t kSpinTicks = 200000000ull;
+
+#ifdef OURO_DIAG
+// Diagnostic event counters (diagnostic builds only).
+static __device__ unsigned long long g_diag[16];
+#define OURO_DIAG_ADD(i, n) atomicAdd(&::ouro::g_diag[i], (unsigned long long)(n))
+#else
+#define OURO_DIAG_ADD(i, n) ((void)0)
+#endif
 
 namespace dev {
 

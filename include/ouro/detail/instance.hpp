@@ -156,6 +156,7 @@ __device__ inline void qc_init_lanes(Memory& m, bool mine, uint32_t qc, uint32_t
   const uint64_t act = active();
   const uint32_t me = rank_in(act), nact = popc(act);
   uint64_t todo = ballot(mine);
+  if (mine) OURO_DIAG_ADD(11, 1);
   while (todo) {
     const int l = first(todo);
     todo &= todo - 1;
@@ -281,10 +282,18 @@ __device__ inline uint32_t vl_locate(Ctx<I>& x, uint32_t start, uint32_t v) {
       return kMarker;
     }
     c = nx;
+    OURO_DIAG_ADD(10, 1);
     if (it.expired()) { set_error(x.m, ERR_TIMEOUT); return kMarker; }
   }
   return c;
 }
+
+// VA pointer-ring entry left by a reclaimed queue chunk: every position of virtual queue chunk vqc
+// was enqueued and dequeued. Chunk indices stay below 2^23 and vqc below 2^21, so a tag is never
+// mistaken for an index or a marker. (The reference writes DeletionMarker here and cannot tell a
+// consumed queue chunk from one not yet published.)
+__device__ __forceinline__ uint32_t qc_tag(uint32_t vqc) { return 0x80000000u | (vqc & 0x1FFFFFu); }
+__device__ __forceinline__ bool is_qc_tag(uint32_t e) { return (e >> 21) == 0x400u; }
 
 // VA: queue chunk that holds virtual position v (accessQueueElement, PageQueueVA_impl.cuh:231-248).
 template <class I>
@@ -292,7 +301,7 @@ __device__ inline uint32_t va_wait_qc(Ctx<I>& x, ClassQueue* cq, uint32_t v) {
   const uint32_t id = (v / kQCSpots) & (cq->ring_size - 1);
   uint32_t qc;
   Spin it(&x.m.error);
-  while ((qc = ld(&cq->ring[id])) == kMarker) {
+  while ((qc = ld(&cq->ring[id])) == kMarker || is_qc_tag(qc)) {
     if (it.expired()) { set_error(x.m, ERR_TIMEOUT); return kMarker; }
     backoff(it.it);
   }
@@ -301,6 +310,50 @@ __device__ inline uint32_t va_wait_qc(Ctx<I>& x, ClassQueue* cq, uint32_t v) {
     return kMarker;
   }
   return qc;
+}
+
+// ------------------------------------------------------------------ chunk-queue front
+// Chunk queues remove exhausted chunks wherever they sit (a tombstone in the slot); front_ then
+// trails the oldest live entry. compact() moves it over dead positions: tombstones, a VA queue
+// chunk whose pointer-ring entry holds its reclaimed tag, and for VL every queue chunk that
+// front_ptr_ has already left (those are empty, all 2048 positions dequeued). Any lane may help;
+// each call does bounded work and stops at the first position it cannot prove dead.
+constexpr uint32_t kCompactSteps = 256;
+
+template <class I>
+__device__ inline void compact(Ctx<I>& x, ClassQueue* cq) {
+  constexpr Kind K = I::kKind;
+  static_assert(is_chunk_kind(K), "page queues dequeue strictly at front_");
+  uint32_t f = ld(&cq->front);
+  const uint32_t b = ld(&cq->back);
+  for (uint32_t i = 0; i < kCompactSteps && f != b && b - f < 0x80000000u; ++i) {
+    uint32_t nf = f + 1u;
+    if constexpr (is_std(K)) {
+      if (ld(&cq->ring[f & (cq->ring_size - 1)]) != kTomb) return;
+    } else if constexpr (is_va(K)) {
+      const uint32_t e = ld(&cq->ring[(f / kQCSpots) & (cq->ring_size - 1)]);
+      if (e == qc_tag(f / kQCSpots)) {
+        nf = (f / kQCSpots + 1u) * kQCSpots;
+      } else if (!chunk_ok(x.m, e) || ld(&hdr<QueueChunkHdr>(x.m, e)->virtual_start) / kQCSpots != f / kQCSpots ||
+                 ld(&qc_slots(x.m, e)[f % kQCSpots]) != kTomb) {
+        return;
+      }
+    } else {
+      const uint32_t c = ld(&cq->front_qc);
+      if (!chunk_ok(x.m, c)) return;
+      const uint32_t vs = ld(&hdr<QueueChunkHdr>(x.m, c)->virtual_start) / kQCSpots, fq = f / kQCSpots;
+      if (vs != fq) {
+        if (vs - fq >= 0x80000000u) return;  // front_ptr_ still behind f: its move is in flight
+        nf = vs * kQCSpots;
+      } else if (ld(&qc_slots(x.m, c)[f % kQCSpots]) != kTomb) {
+        return;
+      }
+    }
+    if (cas(&cq->front, f, nf) != f) return;
+    // the ring slot becomes writable again for position f + ring_size
+    if constexpr (is_std(K)) cas(&cq->ring[f & (cq->ring_size - 1)], kTomb, kMarker);
+    f = nf;
+  }
 }
 
 // ------------------------------------------------------------------ slot sequences
@@ -321,6 +374,8 @@ __device__ inline void put_lanes(Ctx<I>& x, ClassQueue* cq, bool want, uint32_t 
       uint32_t* slot = &cq->ring[v & (cq->ring_size - 1)];
       Spin it(&x.m.error);
       while (cas(slot, kMarker, val) != kMarker) {
+        // a chunk queue's slot may still hold the tombstone of position v - ring_size
+        if constexpr (is_chunk_kind(K)) compact(x, cq);
         if (it.expired()) { set_error(x.m, ERR_TIMEOUT); break; }
         backoff(it.it);
       }
@@ -342,7 +397,19 @@ __device__ inline void put_lanes(Ctx<I>& x, ClassQueue* cq, bool want, uint32_t 
       const bool have = need && qc != kMarker;
       qc_init_lanes(x.m, have, qc, v + kQCSpots);
       if constexpr (is_va(K)) {
-        if (have) xchg(&cq->ring[((v / kQCSpots) + 1) & (cq->ring_size - 1)], qc);
+        if (have) {
+          // the entry still names the queue chunk ring_size positions back: publish only over a
+          // marker or a reclaimed tag, never over a queue chunk with live entries
+          uint32_t* e = &cq->ring[((v / kQCSpots) + 1) & (cq->ring_size - 1)];
+          uint32_t cur = ld(e);
+          bool placed = false;
+          while (!placed && (cur == kMarker || is_qc_tag(cur))) {
+            const uint32_t seen = cas(e, cur, qc);
+            placed = seen == cur;
+            cur = seen;
+          }
+          if (!placed) set_error(x.m, ERR_QUEUE_FULL);
+        }
       }
     }
     if constexpr (is_va(K)) {
@@ -359,8 +426,8 @@ __device__ inline void put_lanes(Ctx<I>& x, ClassQueue* cq, bool want, uint32_t 
           }
           const uint32_t old = add(&hdr<QueueChunkHdr>(x.m, c)->count, 0x10001u);
           if (qc_empty_after_enqueue(old, 0x10001u)) {
-            victim = xchg(&cq->ring[(v / kQCSpots) & (cq->ring_size - 1)], kMarker);
-            reclaim = victim != kMarker;
+            victim = xchg(&cq->ring[(v / kQCSpots) & (cq->ring_size - 1)], qc_tag(v / kQCSpots));
+            reclaim = chunk_ok(x.m, victim);
           }
         }
       }
@@ -375,6 +442,7 @@ __device__ inline void put_lanes(Ctx<I>& x, ClassQueue* cq, bool want, uint32_t 
       Spin it(&x.m.error);
       while (ballot(todo)) {
         if (todo) {
+          OURO_DIAG_ADD(9, 1);
           QueueChunkHdr* h = hdr<QueueChunkHdr>(x.m, cur);
           if (ld(&h->virtual_start) / kQCSpots == v / kQCSpots) {
             if (v % kQCSpots == 0 && qc != kMarker) xchg(&h->next, qc);
@@ -437,8 +505,8 @@ __device__ inline uint32_t take_lanes(Ctx<I>& x, ClassQueue* cq, bool want, uint
         }
         const uint32_t old = sub(&hdr<QueueChunkHdr>(x.m, c)->count, 1u << 16);
         if (qc_empty_after_dequeue(old)) {
-          victim = xchg(&cq->ring[(v / kQCSpots) & (cq->ring_size - 1)], kMarker);
-          reclaim = victim != kMarker;
+          victim = xchg(&cq->ring[(v / kQCSpots) & (cq->ring_size - 1)], qc_tag(v / kQCSpots));
+          reclaim = chunk_ok(x.m, victim);
         }
       }
     }
@@ -483,6 +551,7 @@ __device__ inline bool sem_acquire(Ctx<I>& x, ClassQueue* cq, uint64_t grp, bool
     uint32_t carve_k = 0;
     bool reserve = false;
     if (pending) {
+      OURO_DIAG_ADD(12, 1);
       uint64_t* s = &cq->sem;
       // First pass: take speculatively (one access to the semaphore line in the common case);
       // later passes look before they take, as BulkSemaphore_impl.cuh:216 always does.
@@ -510,7 +579,9 @@ __device__ inline bool sem_acquire(Ctx<I>& x, ClassQueue* cq, uint64_t grp, bool
           const uint32_t k = d.mode == SemMode::CARVE ? d.k : 0u;
           const bool take = d.mode == SemMode::TAKE, res = d.mode == SemMode::RESERVE;
           const uint64_t seen = cas(s, v, nv);
+          if (seen != v) OURO_DIAG_ADD(15, 1);
           if (seen == v) {
+            if (k) OURO_DIAG_ADD(13, k);
             carve_k = k;
             reserve = res;
             if (take) {
@@ -540,6 +611,7 @@ __device__ inline bool sem_acquire(Ctx<I>& x, ClassQueue* cq, uint64_t grp, bool
         const uint64_t v = ld(s);
         const int64_t c = Sem::count(v);
         if (!(c < num && (int64_t)Sem::reserved(v) < c + (int64_t)Sem::expected(v))) break;
+        OURO_DIAG_ADD(14, 1);
         if (spin.expired()) {
           set_error(x.m, ERR_TIMEOUT);
           pending = false;
@@ -727,54 +799,110 @@ __device__ inline uint32_t carve_chunks(Ctx<I>& x, ClassQueue* cq, uint32_t k) {
   return k_ok;
 }
 
-// Read the chunk index at virtual position v without removing it (ChunkQueue scan,
-// ChunkQueue_impl.cuh:105-145; QueueChunk::access, QueueChunk.cuh:111-114; accessLinked).
+// ---- chunk-queue scanning
+// Peeks load kPeek consecutive slots (and then their chunk headers) at once: one round trip per
+// batch instead of one per slot.
+constexpr uint32_t kPeek = 8;
+
+// Load the slots of virtual positions [v, v + n), n <= kPeek, which lie in one queue chunk for
+// VA / VL. Positions of a reclaimed or bypassed queue chunk read as tombstones, of an unpublished
+// one as markers; lanes past n read markers. (ChunkQueue scan, ChunkQueue_impl.cuh:105-145;
+// QueueChunk::access, QueueChunk.cuh:111-114, and the linked walk of accessLinked.)
 template <class I>
-__device__ inline uint32_t peek_slot(Ctx<I>& x, ClassQueue* cq, uint32_t v, uint32_t& vl_cursor) {
+__device__ inline void peek_slots(Ctx<I>& x, ClassQueue* cq, uint32_t v, uint32_t n, uint32_t& cursor,
+                                  uint32_t (&s)[kPeek]) {
   constexpr Kind K = I::kKind;
   if constexpr (is_std(K)) {
-    return ld(&cq->ring[v & (cq->ring_size - 1)]);
-  } else if constexpr (is_va(K)) {
-    const uint32_t qc = ld(&cq->ring[(v / kQCSpots) & (cq->ring_size - 1)]);
-    if (!chunk_ok(x.m, qc)) return kMarker;
-    if (ld(&hdr<QueueChunkHdr>(x.m, qc)->virtual_start) / kQCSpots != v / kQCSpots) return kMarker;
-    return ld(&qc_slots(x.m, qc)[v % kQCSpots]);
+#pragma unroll
+    for (uint32_t i = 0; i < kPeek; ++i) s[i] = i < n ? ld(&cq->ring[(v + i) & (cq->ring_size - 1)]) : kMarker;
   } else {
-    if (vl_cursor == kMarker) vl_cursor = ld(&cq->front_qc);
-    const uint32_t c = vl_locate(x, vl_cursor, v);
-    if (c == kMarker) return kMarker;
-    vl_cursor = c;
-    return ld(&qc_slots(x.m, c)[v % kQCSpots]);
+    const uint32_t* sl = nullptr;
+    uint32_t fill = kMarker;
+    if constexpr (is_va(K)) {
+      const uint32_t e = ld(&cq->ring[(v / kQCSpots) & (cq->ring_size - 1)]);
+      if (e == qc_tag(v / kQCSpots)) fill = kTomb;
+      else if (chunk_ok(x.m, e) && ld(&hdr<QueueChunkHdr>(x.m, e)->virtual_start) / kQCSpots == v / kQCSpots)
+        sl = qc_slots(x.m, e) + v % kQCSpots;
+    } else {
+      if (cursor == kMarker) cursor = ld(&cq->front_qc);
+      uint32_t c = vl_locate(x, cursor, v);
+      if (c == kMarker) {  // the cursor's queue chunk is already past v: retry from front_ptr_
+        cursor = ld(&cq->front_qc);
+        c = vl_locate(x, cursor, v);
+      }
+      if (c != kMarker) {
+        cursor = c;
+        sl = qc_slots(x.m, c) + v % kQCSpots;
+      } else {
+        fill = kTomb;
+      }
+    }
+#pragma unroll
+    for (uint32_t i = 0; i < kPeek; ++i) s[i] = (i < n && sl) ? ld(&sl[i]) : fill;
   }
 }
 
-// Clear the slot at virtual position v (ChunkQueue_impl.cuh:434, QueueChunk::deleteElement
-// QueueChunk_impl.cuh:349-359). A queue chunk that becomes empty is handed back through `reclaim`
-// (VA: its index; VL: 0x80000000 | the number of front-pointer moves).
+// Offset of the first position in [v, v + n) that holds a chunk of this class queued exactly there
+// (queue_pos == position) with free pages; n when there is none. The queue_pos test also rejects a
+// stale slot read out of a queue chunk that was recycled meanwhile.
 template <class I>
-__device__ inline void clear_slot(Ctx<I>& x, ClassQueue* cq, uint32_t v, uint32_t& vl_cursor, uint32_t& reclaim) {
+__device__ inline uint32_t find_chunk(Ctx<I>& x, ClassQueue* cq, uint32_t v, uint32_t n, uint32_t& cursor,
+                                      uint32_t& chunk, int32_t& cnt) {
+  uint32_t s[kPeek];
+  peek_slots(x, cq, v, n, cursor, s);
+  int32_t c_[kPeek];
+  uint32_t qp[kPeek], ps[kPeek];
+#pragma unroll
+  for (uint32_t i = 0; i < kPeek; ++i) {
+    const ChunkHdr* h = hdr<ChunkHdr>(x.m, chunk_ok(x.m, s[i]) ? s[i] : 0u);
+    c_[i] = ld(&h->count);
+    qp[i] = ld(&h->queue_pos);
+    ps[i] = ld(&h->page_size);
+  }
+  const uint32_t psize = cq->page_size;
+#pragma unroll
+  for (uint32_t i = 0; i < kPeek; ++i) {
+    if (i < n && chunk_ok(x.m, s[i]) && c_[i] > 0 && qp[i] == v + i && ps[i] == psize) {
+      chunk = s[i];
+      cnt = c_[i];
+      return i;
+    }
+  }
+  return n;
+}
+
+// Tombstone the slot of a removed chunk at virtual position p (ChunkQueue_impl.cuh:434 and
+// QueueChunk::deleteElement, QueueChunk_impl.cuh:349-359, write DeletionMarker instead). A queue
+// chunk whose 2048 positions are now all dequeued is handed back through `reclaim` (VA: its index;
+// VL: 0x80000000 | the number of front-pointer moves).
+template <class I>
+__device__ inline void tomb_slot(Ctx<I>& x, ClassQueue* cq, uint32_t p, uint32_t& cursor, uint32_t& reclaim) {
   constexpr Kind K = I::kKind;
   if constexpr (is_std(K)) {
-    xchg(&cq->ring[v & (cq->ring_size - 1)], kMarker);
+    xchg(&cq->ring[p & (cq->ring_size - 1)], kTomb);
   } else if constexpr (is_va(K)) {
-    const uint32_t id = (v / kQCSpots) & (cq->ring_size - 1);
+    const uint32_t id = (p / kQCSpots) & (cq->ring_size - 1);
     const uint32_t qc = ld(&cq->ring[id]);
-    if (chunk_ok(x.m, qc)) {
-      xchg(&qc_slots(x.m, qc)[v % kQCSpots], kMarker);
-      const uint32_t old = sub(&hdr<QueueChunkHdr>(x.m, qc)->count, 1u << 16);
-      if (qc_empty_after_dequeue(old)) {
-        const uint32_t victim = xchg(&cq->ring[id], kMarker);
-        if (victim != kMarker) reclaim = victim;
-      }
+    if (!chunk_ok(x.m, qc) || ld(&hdr<QueueChunkHdr>(x.m, qc)->virtual_start) / kQCSpots != p / kQCSpots) {
+      set_error(x.m, ERR_CORRUPT);  // a queued chunk's queue chunk is always published
+      return;
+    }
+    xchg(&qc_slots(x.m, qc)[p % kQCSpots], kTomb);
+    const uint32_t old = sub(&hdr<QueueChunkHdr>(x.m, qc)->count, 1u << 16);
+    if (qc_empty_after_dequeue(old)) {
+      const uint32_t victim = xchg(&cq->ring[id], qc_tag(p / kQCSpots));
+      if (chunk_ok(x.m, victim)) reclaim = victim;
     }
   } else {
-    if (vl_cursor == kMarker) vl_cursor = ld(&cq->front_qc);
-    const uint32_t c = vl_locate(x, vl_cursor, v);
-    if (c != kMarker) {
-      xchg(&qc_slots(x.m, c)[v % kQCSpots], kMarker);
-      const uint32_t old = sub(&hdr<QueueChunkHdr>(x.m, c)->count, 1u << 16);
-      if (qc_empty_after_dequeue(old)) reclaim = vl_set_front(x, cq, c) | 0x80000000u;
+    uint32_t c = cursor != kMarker ? vl_locate(x, cursor, p) : kMarker;
+    if (c == kMarker) c = vl_locate(x, ld(&cq->front_qc), p);
+    if (c == kMarker) {
+      set_error(x.m, ERR_CORRUPT);
+      return;
     }
+    xchg(&qc_slots(x.m, c)[p % kQCSpots], kTomb);
+    const uint32_t old = sub(&hdr<QueueChunkHdr>(x.m, c)->count, 1u << 16);
+    if (qc_empty_after_dequeue(old)) reclaim = vl_set_front(x, cq, c) | 0x80000000u;
   }
 }
 
@@ -803,25 +931,31 @@ __device__ inline void ensure_in_queue(Ctx<I>& x, ClassQueue* cq, bool want, uin
   if (ballot(won)) enqueue_chunks<G>(x, cq, won, chunk);
 }
 
-// Remove the exhausted chunk at position v, only if v is the queue front. front_ therefore moves
-// past removed entries only and never strands a chunk that still has free pages; the reference's
-// atomicMax(front_, v + 1) (ChunkQueue_impl.cuh:432) can, and then finds it again only by scanning
-// around its whole ring. Returns true if removed; `again` reports a free that raced in.
+// Remove an exhausted chunk from its class queue, wherever it sits (the reference removes with
+// atomicMax(front_, v + 1), ChunkQueue_impl.cuh:432, which can strand live chunks in front of v).
+// The chunk's own queue_pos names its position: the CAS queue_pos -> MARKER admits one remover
+// and ties the removal to that queue entry, whatever slot the caller happened to read. Then the
+// slot is tombstoned and front_ compacted. Returns true if removed; `again` reports a free that
+// raced in (the caller re-enqueues: the free saw queue_pos != MARKER and left it to us).
 template <class I>
-__device__ inline bool try_remove_front(Ctx<I>& x, ClassQueue* cq, uint32_t v, uint32_t chunk, uint32_t& cursor,
-                                        uint32_t& reclaim, bool& again) {
+__device__ inline bool cq_remove(Ctx<I>& x, ClassQueue* cq, uint32_t chunk, uint32_t& cursor, uint32_t& reclaim,
+                                 bool& again) {
   again = false;
-  if (ld(&cq->front) != v || cas(&cq->front, v, v + 1) != v) return false;
+  OURO_DIAG_ADD(7, 1);
   ChunkHdr* h = hdr<ChunkHdr>(x.m, chunk);
-  if (cas(&h->queue_pos, v, kMarker) != v) set_error(x.m, ERR_CORRUPT);
+  const uint32_t p = ld(&h->queue_pos);
+  if (p == kMarker || p == kPending || cas(&h->queue_pos, p, kMarker) != p) return false;
+  OURO_DIAG_ADD(8, 1);
   uint32_t rc = kMarker;
-  clear_slot(x, cq, v, cursor, rc);
+  tomb_slot(x, cq, p, cursor, rc);
   if (rc != kMarker) {
     if (reclaim != kMarker) finish_reclaim(x, cq, true, reclaim);
     reclaim = rc;
   }
   sub(&cq->count, 1);
+  order();
   again = ld(&h->count) > 0;
+  compact(x, cq);
   return true;
 }
 
@@ -860,15 +994,29 @@ __device__ inline AccessMode access_alloc_lane(Ctx<I>& x, ChunkHdr* h, uint32_t&
   }
 }
 
-// Wrap a scan position that ran past back_ to the front (no live chunk is outside [front, back)).
+// Scan positions run over [front_, back_): a position at or past back_, or behind front_, restarts
+// at front_.
 template <class I>
-__device__ __forceinline__ void scan_advance(ClassQueue* cq, uint32_t& v, uint32_t& cursor) {
-  ++v;
+__device__ __forceinline__ void scan_wrap(ClassQueue* cq, uint32_t& v, uint32_t& cursor, uint32_t b) {
   const uint32_t f = ld(&cq->front);
-  if (v - ld(&cq->back) < 0x80000000u || v - f >= 0x80000000u) {  // past back_, or behind front_
+  if (v - b < 0x80000000u || v - f >= 0x80000000u) {
+    OURO_DIAG_ADD(6, 1);
     v = f;
     cursor = kMarker;
   }
+}
+
+// Batch length at v: at most kPeek, not past back_, and inside one queue chunk for VA / VL.
+template <class I>
+__device__ __forceinline__ uint32_t batch_at(uint32_t v, uint32_t b) {
+  uint32_t n = b - v;
+  if (n >= 0x80000000u) n = 0;
+  if (n > kPeek) n = kPeek;
+  if constexpr (!is_std(I::kKind)) {
+    const uint32_t room = kQCSpots - v % kQCSpots;
+    if (n > room) n = room;
+  }
+  return n;
 }
 
 template <int W, class I>
@@ -887,39 +1035,45 @@ __device__ inline void* alloc_chunk_pages(Ctx<I>& x, uint32_t qi, bool want) {
   uint32_t reclaim = kMarker;
   bool got = false;
   if constexpr (W == 1) {
-    // the reference's per-thread protocol: every lane scans from front_ on its own
+    // the reference's per-thread protocol: every lane walks the queue from front_ on its own and
+    // probes each chunk's count with a decrement
     if (ok) {
       uint32_t v = ld(&cq->front);
       uint32_t cursor = kMarker;
       Spin it(&x.m.error);
       while (true) {
-        const uint32_t c = peek_slot(x, cq, v, cursor);
-        if (chunk_ok(x.m, c)) {
+        OURO_DIAG_ADD(0, 1);
+        const uint32_t b = ld(&cq->back);
+        scan_wrap<I>(cq, v, cursor, b);
+        uint32_t s[kPeek];
+        const uint32_t n = batch_at<I>(v, b) ? 1u : 0u;
+        s[0] = kMarker;
+        if (n) peek_slots(x, cq, v, n, cursor, s);  // never walk towards a position at back_
+        const uint32_t c = s[0];
+        if (n && chunk_ok(x.m, c) && ld(&hdr<ChunkHdr>(x.m, c)->queue_pos) == v) {
           ChunkHdr* h = hdr<ChunkHdr>(x.m, c);
           const AccessMode md = access_alloc_lane(x, h, page);
           if (md == AccessMode::ERROR) break;
-          bool again = false;
-          if (md == AccessMode::CONTINUE) {
-            if (try_remove_front(x, cq, v, c, cursor, reclaim, again) && again) ensure_in_queue<1>(x, cq, true, c);
-          } else {
+          if (md != AccessMode::CONTINUE) {
             chunk = c;
             got = true;
+            bool again = false;
             if (md == AccessMode::RE_ENQUEUE) ensure_in_queue<1>(x, cq, true, c);
-            if (md == AccessMode::DEQUEUE && try_remove_front(x, cq, v, c, cursor, reclaim, again) && again)
+            if (md == AccessMode::DEQUEUE && cq_remove(x, cq, c, cursor, reclaim, again) && again)
               ensure_in_queue<1>(x, cq, true, c);
             break;
           }
         }
-        scan_advance<I>(cq, v, cursor);
+        v += n;
         if (it.expired()) { set_error(x.m, ERR_TIMEOUT); break; }
         backoff(it.it > 64 ? it.it : 0);
       }
     }
   } else {
-    // group protocol: the leader reserves pages chunk by chunk with a CAS on the chunk count and
-    // claims them from the chunk's 64-bit mask words; lanes take them in rank order. Leaders of one
-    // class start at spread positions of the live queue window (a per-class scan cursor) instead
-    // of all at front_.
+    // group protocol: the leader finds chunks kPeek slots at a time, reserves pages on the chunk
+    // count with a CAS and claims them from the chunk's 64-bit mask words; lanes take them in rank
+    // order. Leaders of one class start at spread positions of the live window [front_, back_) (a
+    // per-class scan cursor) instead of all at front_.
     const int L = first(grp);
     const bool leader = ok && (int)lane() == L;
     const uint32_t num = popc(grp);
@@ -938,38 +1092,43 @@ __device__ inline void* alloc_chunk_pages(Ctx<I>& x, uint32_t qi, bool want) {
       uint32_t bc = kMarker, r0 = 0;
       uint64_t bits = 0;
       if (leader && assigned < num && !gave_up) {
+        OURO_DIAG_ADD(0, 1);
         if (t_rem == 0) {
-          const uint32_t s = peek_slot(x, cq, v, cursor);
-          bool advance = true;
-          if (chunk_ok(x.m, s)) {
+          OURO_DIAG_ADD(1, 1);
+          const uint32_t b = ld(&cq->back);
+          scan_wrap<I>(cq, v, cursor, b);
+          const uint32_t n = batch_at<I>(v, b);
+          uint32_t s = kMarker;
+          int32_t cnt = 0;
+          const uint32_t i = n ? find_chunk(x, cq, v, n, cursor, s, cnt) : 0u;
+          if (i < n) {
             ChunkHdr* h = hdr<ChunkHdr>(x.m, s);
-            int32_t cnt = ld(&h->count);
             uint32_t t = 0;
             while (cnt > 0) {
               t = (uint32_t)cnt < need ? (uint32_t)cnt : need;
               const int32_t seen = cas(&h->count, cnt, cnt - (int32_t)t);
               if (seen == cnt) break;
+              OURO_DIAG_ADD(3, 1);
               cnt = seen;
             }
-            bool again = false;
             if (cnt > 0) {
               c = s;
               t_rem = t;
               need -= t;
               const uint32_t size = ld(&h->size);
               w = ((size - (uint32_t)cnt) / 64u) % mask_words(size);
-              if (cnt == (int32_t)t) {
-                if (try_remove_front(x, cq, v, s, cursor, reclaim, again) && again) ensure_in_queue<1>(x, cq, true, s);
-              } else {
-                advance = false;
-              }
-            } else if (try_remove_front(x, cq, v, s, cursor, reclaim, again) && again) {
-              ensure_in_queue<1>(x, cq, true, s);
+              bool again = false;
+              if (cnt == (int32_t)t && cq_remove(x, cq, s, cursor, reclaim, again) && again)
+                ensure_in_queue<1>(x, cq, true, s);
+              v += cnt == (int32_t)t ? i + 1u : i;
+            } else {
+              OURO_DIAG_ADD(4, 1);
+              v += i + 1u;  // drained meanwhile; its last taker removes it
             }
-          }
-          if (advance) {
-            if (t_rem == 0) scan_advance<I>(cq, v, cursor);
-            else ++v;
+          } else {
+            OURO_DIAG_ADD(2, n);
+            v += n;
+            if (n == 0) backoff(it.it);
           }
         }
         if (t_rem > 0) {
@@ -982,6 +1141,7 @@ __device__ inline void* alloc_chunk_pages(Ctx<I>& x, uint32_t qi, bool want) {
           } else {
             const uint64_t old = and_(&h->mask[w], ~take);
             const uint64_t gotb = old & take;
+            if (gotb != take) OURO_DIAG_ADD(5, 1);
             bits = gotb;
             bc = c | (w << 29);  // word index rides in the top bits (chunk < 2^23)
             r0 = assigned;

@@ -63,6 +63,10 @@ def _load_sim() -> C.CDLL:
         lib.sim_chunk_info.restype = C.c_int
         lib.sim_chunk_info.argtypes = [C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32), C.POINTER(C.c_int32), u64p]
         lib.sim_class_of.restype = C.c_int
+        lib.sim_chunk_queue_pos.argtypes = [C.c_void_p, C.c_uint32]
+        lib.sim_chunk_queue_pos.restype = C.c_uint32
+        lib.sim_queue_window.argtypes = [C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32), C.c_uint32]
+        lib.sim_queue_window.restype = C.c_uint32
         lib.sim_class_of.argtypes = [C.c_uint32, C.c_uint32]
         lib.sim_sem_make.restype = C.c_uint64
         lib.sim_sem_make.argtypes = [C.c_int64, C.c_uint64, C.c_uint64]
@@ -142,6 +146,16 @@ class Sim:
         mask = np.zeros(8, dtype=np.uint64)
         t = self._lib.sim_chunk_info(self._h, c, C.byref(ps), C.byref(cnt), mask.ctypes.data_as(C.POINTER(C.c_uint64)))
         return t, ps.value, cnt.value, mask
+
+    def queue_pos(self, c: int) -> int:
+        return self._lib.sim_chunk_queue_pos(self._h, c)
+
+    def window(self, k: int) -> np.ndarray:
+        """Slot values of class k's queue window [front, back)."""
+        n = self._lib.sim_queue_window(self._h, k, None, 0)
+        out = np.empty(max(n, 1), dtype=np.uint32)
+        self._lib.sim_queue_window(self._h, k, out.ctypes.data_as(C.POINTER(C.c_uint32)), n)
+        return out[:n]
 
     def trace(self, sizes, rounds: int) -> np.ndarray:
         """Single-wave trace: `rounds` x (wave malloc, wave free) over <= 64 lanes.

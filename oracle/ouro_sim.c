@@ -24,6 +24,9 @@
 #define STATE_BYTES 8192u
 #define BLOCK 4096u
 #define PENDING 0xFFFFFFFEu
+#define TOMB 0xFFFFFFFDu      /* chunk-queue slot whose chunk was removed */
+#define PEEK 8u               /* slots looked at per scan step */
+#define COMPACT_STEPS 256u    /* front moves per compaction call */
 
 enum { ERR_OOM = 1, ERR_QUEUE_FULL = 2, ERR_VSTART = 4, ERR_TIMEOUT = 8, ERR_BAD_FREE = 16, ERR_TOO_LARGE = 32,
        ERR_REUSE_FULL = 64, ERR_CORRUPT = 256 };
@@ -33,6 +36,9 @@ static int is_chunk_kind(int k) { return k == K_CQ || k == K_VACQ || k == K_VLCQ
 static int is_va(int k) { return k == K_VAPQ || k == K_VACQ; }
 static int is_vl(int k) { return k == K_VLPQ || k == K_VLCQ; }
 static int is_std(int k) { return k == K_PQ || k == K_CQ; }
+/* VA pointer-ring entry of a reclaimed queue chunk: all positions of virtual queue chunk vqc dequeued */
+static uint32_t qtag(uint32_t vqc) { return 0x80000000u | (vqc & 0x1FFFFFu); }
+static int is_qtag(uint32_t e) { return (e >> 21) == 0x400u; }
 
 typedef struct {
   int type; /* 0 unused, 1 page chunk, 2 chunk-queue chunk, 3 queue chunk */
@@ -253,14 +259,22 @@ static void vl_set_old(ouro_sim* s, cls_t* q, uint32_t free_count) {
   }
 }
 
+/* QueueChunk::locateQueueChunkForPosition (QueueChunk_impl.cuh:386-414); MARKER when the walk
+ * starts past v's queue chunk (that one is gone). */
 static uint32_t vl_locate(ouro_sim* s, uint32_t c, uint32_t v) {
-  while (chunk_ok(s, c) && ch(s, c)->vstart / QC_SPOTS != v / QC_SPOTS) c = ch(s, c)->next;
-  if (!chunk_ok(s, c)) {
-    s->error |= ERR_VSTART;
-    return MARKER;
+  for (;;) {
+    if (!chunk_ok(s, c)) {
+      s->error |= ERR_VSTART;
+      return MARKER;
+    }
+    const uint32_t vs = ch(s, c)->vstart / QC_SPOTS;
+    if (vs == v / QC_SPOTS) return c;
+    if (vs - v / QC_SPOTS < 0x80000000u) return MARKER;
+    c = ch(s, c)->next;
   }
-  return c;
 }
+
+static void compact(ouro_sim* s, inst_t* I, cls_t* q);
 
 /* ------------------------------------------------------------------ slot sequences */
 /* put for a set of lanes already sorted in lane order: positions v[i], values val[i].
@@ -271,6 +285,7 @@ static void put_lanes(ouro_sim* s, inst_t* I, cls_t** qs, int n, const uint32_t*
   if (is_std(K)) {
     for (int i = 0; i < n; ++i) {
       uint32_t* slot = std_slot(qs[i], v[i]);
+      if (*slot == TOMB && is_chunk_kind(K)) compact(s, I, qs[i]);
       if (*slot != MARKER) s->error |= ERR_TIMEOUT;
       else *slot = val[i];
     }
@@ -289,7 +304,12 @@ static void put_lanes(ouro_sim* s, inst_t* I, cls_t** qs, int n, const uint32_t*
       if ((uint32_t)j < ok) {
         qc[need[j]] = got[j];
         qc_init(s, got[j], v[need[j]] + QC_SPOTS);
-        if (is_va(K)) qs[need[j]]->va[((v[need[j]] / QC_SPOTS) + 1) & (qs[need[j]]->ring_size - 1)] = got[j];
+        if (is_va(K)) {
+          /* publish over a marker or a reclaimed tag only */
+          uint32_t* e = &qs[need[j]]->va[((v[need[j]] / QC_SPOTS) + 1) & (qs[need[j]]->ring_size - 1)];
+          if (*e == MARKER || is_qtag(*e)) *e = got[j];
+          else s->error |= ERR_QUEUE_FULL;
+        }
       }
   }
   if (is_va(K)) {
@@ -310,7 +330,7 @@ static void put_lanes(ouro_sim* s, inst_t* I, cls_t** qs, int n, const uint32_t*
       x->qc_count += 0x10001u;
       if (old + 0x10001u == QC_SPOTS) {
         victims[nv++] = q->va[id];
-        q->va[id] = MARKER;
+        q->va[id] = qtag(v[i] / QC_SPOTS);
       }
     }
     for (int i = 0; i < nv; ++i) ring_put(s, &s->inst[0].reuse, victims[i]);
@@ -373,7 +393,7 @@ static void take_lanes(ouro_sim* s, inst_t* I, cls_t** qs, int n, const uint32_t
       x->qc_count -= 0x10000u;
       if (old - 0x10000u == QC_SPOTS) {
         victims[nv++] = q->va[id];
-        q->va[id] = MARKER;
+        q->va[id] = qtag(v[i] / QC_SPOTS);
       }
     }
     for (int i = 0; i < nv; ++i) ring_put(s, &s->inst[0].reuse, victims[i]);
@@ -403,69 +423,145 @@ static void take_lanes(ouro_sim* s, inst_t* I, cls_t** qs, int n, const uint32_t
     if (oldw[i]) vl_set_old(s, qs[i], oldf[i]);
 }
 
-static uint32_t peek(ouro_sim* s, inst_t* I, cls_t* q, uint32_t v, uint32_t* cursor) {
-  if (is_std(I->kind)) return *std_slot(q, v);
-  if (is_va(I->kind)) {
-    const uint32_t c = q->va[(v / QC_SPOTS) & (q->ring_size - 1)];
-    if (!chunk_ok(s, c) || ch(s, c)->vstart / QC_SPOTS != v / QC_SPOTS) return MARKER;
-    return ch(s, c)->slots[v % QC_SPOTS];
-  }
-  if (*cursor == MARKER) *cursor = q->front_qc;
-  const uint32_t c = vl_locate(s, *cursor, v);
-  if (c == MARKER) return MARKER;
-  *cursor = c;
-  return ch(s, c)->slots[v % QC_SPOTS];
-}
-
-/* Clear the slot at v (ChunkQueue_impl.cuh:434, QueueChunk::deleteElement QueueChunk_impl.cuh:349-359).
- * A queue chunk emptied here is reclaimed after the scan. */
-static void clear_slot(ouro_sim* s, inst_t* I, cls_t* q, uint32_t v, uint32_t* cursor, uint32_t* reclaim) {
+/* ------------------------------------------------------------------ chunk-queue scanning */
+/* Slots of positions [v, v + n), n <= PEEK, inside one queue chunk for VA / VL: a reclaimed or
+ * bypassed queue chunk reads as tombstones, an unpublished one as markers (the device's peek_slots;
+ * ChunkQueue_impl.cuh:105-145, QueueChunk.cuh:111-114). */
+static void peek_slots(ouro_sim* s, inst_t* I, cls_t* q, uint32_t v, uint32_t n, uint32_t* cursor,
+                       uint32_t* out) {
   if (is_std(I->kind)) {
-    *std_slot(q, v) = MARKER;
-  } else if (is_va(I->kind)) {
-    const uint32_t id = (v / QC_SPOTS) & (q->ring_size - 1);
-    const uint32_t c = q->va[id];
-    if (chunk_ok(s, c)) {
-      chunk_t* x = ch(s, c);
-      x->slots[v % QC_SPOTS] = MARKER;
-      const uint32_t old = x->qc_count;
-      x->qc_count -= 0x10000u;
-      if (old - 0x10000u == QC_SPOTS) {
-        *reclaim = c;
-        q->va[id] = MARKER;
-      }
-    }
+    for (uint32_t i = 0; i < PEEK; ++i) out[i] = i < n ? *std_slot(q, v + i) : MARKER;
+    return;
+  }
+  const uint32_t* sl = NULL;
+  uint32_t fill = MARKER;
+  if (is_va(I->kind)) {
+    const uint32_t e = q->va[(v / QC_SPOTS) & (q->ring_size - 1)];
+    if (e == qtag(v / QC_SPOTS)) fill = TOMB;
+    else if (chunk_ok(s, e) && ch(s, e)->slots && ch(s, e)->vstart / QC_SPOTS == v / QC_SPOTS)
+      sl = ch(s, e)->slots + v % QC_SPOTS;
   } else {
     if (*cursor == MARKER) *cursor = q->front_qc;
-    const uint32_t c = vl_locate(s, *cursor, v);
+    uint32_t c = vl_locate(s, *cursor, v);
+    if (c == MARKER) { /* the cursor's queue chunk is already past v: retry from front_ptr_ */
+      *cursor = q->front_qc;
+      c = vl_locate(s, *cursor, v);
+    }
     if (c != MARKER) {
-      chunk_t* x = ch(s, c);
-      x->slots[v % QC_SPOTS] = MARKER;
-      const uint32_t old = x->qc_count;
-      x->qc_count -= 0x10000u;
-      if (old - 0x10000u == QC_SPOTS) *reclaim = vl_set_front(s, q, c) | 0x80000000u;
+      *cursor = c;
+      sl = ch(s, c)->slots + v % QC_SPOTS;
+    } else {
+      fill = TOMB;
     }
   }
+  for (uint32_t i = 0; i < PEEK; ++i) out[i] = (i < n && sl) ? sl[i] : fill;
+}
+
+/* First offset in [v, v + n) that holds a chunk of this class queued exactly there with free pages;
+ * n when there is none. */
+static uint32_t find_chunk(ouro_sim* s, inst_t* I, cls_t* q, uint32_t v, uint32_t n, uint32_t* cursor,
+                           uint32_t* chunk) {
+  uint32_t sl[PEEK];
+  peek_slots(s, I, q, v, n, cursor, sl);
+  for (uint32_t i = 0; i < n; ++i) {
+    if (!chunk_ok(s, sl[i])) continue;
+    const chunk_t* x = ch(s, sl[i]);
+    if (x->count > 0 && x->queue_pos == v + i && x->page_size == q->page_size) {
+      *chunk = sl[i];
+      return i;
+    }
+  }
+  return n;
+}
+
+/* Advance front over dead positions, at most COMPACT_STEPS moves: tombstones, a VA queue chunk whose
+ * pointer-ring entry holds its reclaimed tag, VL queue chunks front_ptr_ has already left. */
+static void compact(ouro_sim* s, inst_t* I, cls_t* q) {
+  uint32_t f = q->front;
+  const uint32_t b = q->back;
+  for (uint32_t i = 0; i < COMPACT_STEPS && f != b && b - f < 0x80000000u; ++i) {
+    uint32_t nf = f + 1u;
+    if (is_std(I->kind)) {
+      if (*std_slot(q, f) != TOMB) return;
+    } else if (is_va(I->kind)) {
+      const uint32_t e = q->va[(f / QC_SPOTS) & (q->ring_size - 1)];
+      if (e == qtag(f / QC_SPOTS)) {
+        nf = (f / QC_SPOTS + 1u) * QC_SPOTS;
+      } else if (!chunk_ok(s, e) || !ch(s, e)->slots || ch(s, e)->vstart / QC_SPOTS != f / QC_SPOTS ||
+                 ch(s, e)->slots[f % QC_SPOTS] != TOMB) {
+        return;
+      }
+    } else {
+      const uint32_t c = q->front_qc;
+      if (!chunk_ok(s, c)) return;
+      const uint32_t vs = ch(s, c)->vstart / QC_SPOTS, fq = f / QC_SPOTS;
+      if (vs != fq) {
+        if (vs - fq >= 0x80000000u) return;
+        nf = vs * QC_SPOTS;
+      } else if (ch(s, c)->slots[f % QC_SPOTS] != TOMB) {
+        return;
+      }
+    }
+    q->front = nf;
+    if (is_std(I->kind) && *std_slot(q, f) == TOMB) *std_slot(q, f) = MARKER;
+    f = nf;
+  }
+}
+
+/* Tombstone the slot of a removed chunk at p (ChunkQueue_impl.cuh:434, QueueChunk::deleteElement
+ * QueueChunk_impl.cuh:349-359 write DeletionMarker). A queue chunk emptied here is reclaimed later. */
+static void tomb_slot(ouro_sim* s, inst_t* I, cls_t* q, uint32_t p, uint32_t* cursor, uint32_t* reclaim) {
+  if (is_std(I->kind)) {
+    *std_slot(q, p) = TOMB;
+    return;
+  }
+  if (is_va(I->kind)) {
+    const uint32_t id = (p / QC_SPOTS) & (q->ring_size - 1);
+    const uint32_t c = q->va[id];
+    if (!chunk_ok(s, c) || ch(s, c)->vstart / QC_SPOTS != p / QC_SPOTS) {
+      s->error |= ERR_CORRUPT;
+      return;
+    }
+    chunk_t* x = ch(s, c);
+    x->slots[p % QC_SPOTS] = TOMB;
+    const uint32_t old = x->qc_count;
+    x->qc_count -= 0x10000u;
+    if (old - 0x10000u == QC_SPOTS) {
+      q->va[id] = qtag(p / QC_SPOTS);
+      *reclaim = c;
+    }
+    return;
+  }
+  uint32_t c = *cursor != MARKER ? vl_locate(s, *cursor, p) : MARKER;
+  if (c == MARKER) c = vl_locate(s, q->front_qc, p);
+  if (c == MARKER) {
+    s->error |= ERR_CORRUPT;
+    return;
+  }
+  chunk_t* x = ch(s, c);
+  x->slots[p % QC_SPOTS] = TOMB;
+  const uint32_t old = x->qc_count;
+  x->qc_count -= 0x10000u;
+  if (old - 0x10000u == QC_SPOTS) *reclaim = vl_set_front(s, q, c) | 0x80000000u;
 }
 
 static void finish_reclaim(ouro_sim* s, inst_t* I, cls_t* q, uint32_t reclaim);
 
-/* Remove the exhausted chunk at v only if v is the queue front (the device's try_remove_front):
- * front_ moves past removed entries only. Returns 1 when removed. */
-static int remove_front(ouro_sim* s, inst_t* I, cls_t* q, uint32_t v, uint32_t chunk, uint32_t* cursor,
-                        uint32_t* reclaim) {
-  if (q->front != v) return 0;
-  q->front = v + 1;
+/* Remove an exhausted chunk wherever it sits, at the position its queue_pos names (the device's
+ * cq_remove), then compact the front. Returns 1 when removed. */
+static int cq_remove(ouro_sim* s, inst_t* I, cls_t* q, uint32_t chunk, uint32_t* cursor, uint32_t* reclaim) {
   chunk_t* x = ch(s, chunk);
-  if (x->queue_pos != v) s->error |= ERR_CORRUPT;
+  const uint32_t p = x->queue_pos;
+  if (p == MARKER || p == PENDING) return 0;
   x->queue_pos = MARKER;
   uint32_t rc = MARKER;
-  clear_slot(s, I, q, v, cursor, &rc);
+  tomb_slot(s, I, q, p, cursor, &rc);
   if (rc != MARKER) {
     if (*reclaim != MARKER) finish_reclaim(s, I, q, *reclaim);
     *reclaim = rc;
   }
   q->count--;
+  compact(s, I, q);
   return 1;
 }
 
@@ -629,9 +725,10 @@ static uint32_t mask_words(uint32_t ppc) {
 }
 
 /* Group claim for chunk-based queues: the leader starts at a spread position of the live queue
- * window (per-class scan cursor), reserves pages chunk by chunk on the chunk count and takes them from
- * the 64-bit mask words, lowest bits first, starting at word (size - count) / 64. Exhausted chunks
- * leave the queue only from its front. (ChunkQueue::allocPage ChunkQueue_impl.cuh:88-151 and
+ * window (per-class scan cursor) and looks at PEEK slots per step for a chunk queued exactly there
+ * with free pages; it reserves pages on the chunk count and takes them from the 64-bit mask words,
+ * lowest bits first, starting at word (size - count) / 64. The taker of a chunk's last page removes
+ * the chunk wherever it sits. (ChunkQueue::allocPage ChunkQueue_impl.cuh:88-151 and
  * ChunkAccess::allocPage ChunkAccess_impl.cuh:38-135, as the device's W = 64 rule.) */
 static void claim_group(ouro_sim* s, inst_t* I, group_t* g, uint64_t* out, uint32_t* reclaim) {
   cls_t* q = &I->q[g->qi];
@@ -647,56 +744,59 @@ static void claim_group(ouro_sim* s, inst_t* I, group_t* g, uint64_t* out, uint3
       s->error |= ERR_TIMEOUT;
       return;
     }
-    const uint32_t sl = peek(s, I, q, v, &cursor);
-    int advance = 1, claimed = 0;
-    if (sl != MARKER && chunk_ok(s, sl)) {
-      chunk_t* x = ch(s, sl);
-      const int32_t cnt = x->count;
-      if (cnt > 0) {
-        claimed = 1;
-        const uint32_t t = (uint32_t)cnt < need ? (uint32_t)cnt : need;
-        x->count -= (int32_t)t;
-        need -= t;
-        const uint32_t nw = mask_words(x->size);
-        uint32_t w = ((x->size - (uint32_t)cnt) / 64u) % nw;
-        if (cnt == (int32_t)t) remove_front(s, I, q, v, sl, &cursor, reclaim);
-        else advance = 0;
-        uint32_t t_rem = t;
-        int wguard = 0;
-        while (t_rem > 0 && wguard++ < 64) {
-          const uint64_t mw = x->mask[w];
-          uint64_t take = 0;
-          uint32_t c = 0;
-          for (uint32_t b = 0; b < 64 && c < t_rem; ++b)
-            if ((mw >> b) & 1ull) {
-              take |= 1ull << b;
-              ++c;
-            }
-          if (take == 0) {
-            w = (w + 1) % nw;
-            continue;
-          }
-          x->mask[w] &= ~take;
-          for (uint32_t b = 0; b < 64; ++b)
-            if ((take >> b) & 1ull) {
-              const int lane = g->lanes[assigned++];
-              out[lane] = page_offset(s, sl, w * 64 + b, q->page_size);
-            }
-          t_rem -= c;
-          if (t_rem > 0) w = (w + 1) % nw;
+    const uint32_t b = q->back;
+    if (v - b < 0x80000000u || v - q->front >= 0x80000000u) { /* at or past back, or behind front */
+      v = q->front;
+      cursor = MARKER;
+    }
+    uint32_t n = b - v;
+    if (n >= 0x80000000u) n = 0;
+    if (n > PEEK) n = PEEK;
+    if (!is_std(I->kind) && n > QC_SPOTS - v % QC_SPOTS) n = QC_SPOTS - v % QC_SPOTS;
+    if (n == 0) { /* granted pages but an empty window: cannot happen sequentially */
+      s->error |= ERR_TIMEOUT;
+      return;
+    }
+    uint32_t sl = MARKER;
+    const uint32_t i = find_chunk(s, I, q, v, n, &cursor, &sl);
+    if (i == n) {
+      v += n;
+      continue;
+    }
+    chunk_t* x = ch(s, sl);
+    const int32_t cnt = x->count;
+    const uint32_t t = (uint32_t)cnt < need ? (uint32_t)cnt : need;
+    x->count -= (int32_t)t;
+    need -= t;
+    const uint32_t nw = mask_words(x->size);
+    uint32_t w = ((x->size - (uint32_t)cnt) / 64u) % nw;
+    if (cnt == (int32_t)t) cq_remove(s, I, q, sl, &cursor, reclaim);
+    v += cnt == (int32_t)t ? i + 1u : i;
+    uint32_t t_rem = t;
+    int wguard = 0;
+    while (t_rem > 0 && wguard++ < 64) {
+      const uint64_t mw = x->mask[w];
+      uint64_t take = 0;
+      uint32_t c = 0;
+      for (uint32_t bb = 0; bb < 64 && c < t_rem; ++bb)
+        if ((mw >> bb) & 1ull) {
+          take |= 1ull << bb;
+          ++c;
         }
-        if (t_rem) s->error |= ERR_TIMEOUT;
-      } else {
-        remove_front(s, I, q, v, sl, &cursor, reclaim);
+      if (take == 0) {
+        w = (w + 1) % nw;
+        continue;
       }
+      x->mask[w] &= ~take;
+      for (uint32_t bb = 0; bb < 64; ++bb)
+        if ((take >> bb) & 1ull) {
+          const int lane = g->lanes[assigned++];
+          out[lane] = page_offset(s, sl, w * 64 + bb, q->page_size);
+        }
+      t_rem -= c;
+      if (t_rem > 0) w = (w + 1) % nw;
     }
-    if (advance) {
-      ++v;
-      if (!claimed && v - q->back < 0x80000000u) {
-        v = q->front;
-        cursor = MARKER;
-      }
-    }
+    if (t_rem) s->error |= ERR_TIMEOUT;
   }
 }
 
@@ -984,4 +1084,47 @@ int sim_chunk_info(const ouro_sim* s, uint32_t c, uint32_t* page_size, int32_t* 
   if (count) *count = x->count;
   if (mask8) memcpy(mask8, x->mask, sizeof(x->mask));
   return x->type;
+}
+
+uint32_t sim_chunk_queue_pos(const ouro_sim* s, uint32_t c) {
+  return c < s->max_chunks ? s->chunks[c].queue_pos : MARKER;
+}
+
+/* Slot values of class k's queue window [front, back) (k counts the classes of both instances), up to
+ * max entries; returns the window length. Reads only: dead VA / VL queue chunks read as TOMB, positions
+ * whose queue chunk is missing as MARKER. */
+uint32_t sim_queue_window(const ouro_sim* s, uint32_t k, uint32_t* out, uint32_t max) {
+  const inst_t* I = &s->inst[0];
+  if (k >= I->nq) {
+    k -= I->nq;
+    I = &s->inst[1];
+    if (k >= I->nq) return 0;
+  }
+  const cls_t* q = &I->q[k];
+  const uint32_t len = q->back - q->front;
+  for (uint32_t i = 0; i < len && i < max; ++i) {
+    const uint32_t v = q->front + i;
+    uint32_t val = MARKER;
+    if (is_std(I->kind)) {
+      const uint32_t p = v & (q->ring_size - 1);
+      const uint32_t* b = q->blocks[p / BLOCK];
+      val = b ? b[p % BLOCK] : MARKER;
+    } else if (is_va(I->kind)) {
+      const uint32_t e = q->va[(v / QC_SPOTS) & (q->ring_size - 1)];
+      if (e == qtag(v / QC_SPOTS)) val = TOMB;
+      else if (chunk_ok(s, e) && s->chunks[e].slots && s->chunks[e].vstart / QC_SPOTS == v / QC_SPOTS)
+        val = s->chunks[e].slots[v % QC_SPOTS];
+    } else {
+      uint32_t c = q->front_qc;
+      if (chunk_ok(s, c) && s->chunks[c].vstart / QC_SPOTS - v / QC_SPOTS < 0x80000000u &&
+          s->chunks[c].vstart / QC_SPOTS != v / QC_SPOTS) {
+        val = TOMB;
+      } else {
+        while (chunk_ok(s, c) && s->chunks[c].vstart / QC_SPOTS != v / QC_SPOTS) c = s->chunks[c].next;
+        if (chunk_ok(s, c)) val = s->chunks[c].slots[v % QC_SPOTS];
+      }
+    }
+    out[i] = val;
+  }
+  return len;
 }

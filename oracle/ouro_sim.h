@@ -51,6 +51,10 @@ void sim_get_state(const ouro_sim* s, sim_state* out);
 
 /* Chunk-level views for tests: 0 unused, 1 page chunk, 2 chunk-queue chunk, 3 queue chunk. */
 int sim_chunk_info(const ouro_sim* s, uint32_t chunk, uint32_t* page_size, int32_t* count, uint64_t* mask8);
+/* Chunk queues: the position a chunk is queued at (MARKER when not queued). */
+uint32_t sim_chunk_queue_pos(const ouro_sim* s, uint32_t chunk);
+/* Slot values of class k's live window [front, back) (0xFFFFFFFD = removed entry); returns its length. */
+uint32_t sim_queue_window(const ouro_sim* s, uint32_t k, uint32_t* out, uint32_t max);
 
 /* Class index / page size / pages-per-chunk closed forms (Helper.cuh:42-65) for a request size. */
 int sim_class_of(uint32_t size, uint32_t smallest);

@@ -34,20 +34,25 @@ def _run(cmd: list[str]) -> None:
         raise RuntimeError(f"command failed ({r.returncode}): {' '.join(cmd)}\n{r.stderr[-4000:]}")
 
 
-def build(jobs: int | None = None, verbose: bool = False) -> Path:
-    OBJ.mkdir(exist_ok=True)
-    units = [(CSRC / "variant.hip", OBJ / f"variant{v}.o", [f"-DOURO_VARIANT={v}"]) for v in range(6)]
-    units.append((CSRC / "capi.hip", OBJ / "capi.o", []))
-    cmds = [[HIPCC, *FLAGS, *extra, "-c", str(src), "-o", str(obj)] for src, obj, extra in units]
+def build(jobs: int | None = None, verbose: bool = False, extra_flags: list[str] | None = None,
+          out: Path | None = None) -> Path:
+    """extra_flags / out: alternative builds (e.g. -DOURO_DIAG into a separate library file)."""
+    extra_flags = extra_flags or []
+    lib = out or LIB
+    obj_dir = OBJ if not extra_flags else OBJ.with_name("_obj_" + lib.stem)
+    obj_dir.mkdir(exist_ok=True)
+    units = [(CSRC / "variant.hip", obj_dir / f"variant{v}.o", [f"-DOURO_VARIANT={v}"]) for v in range(6)]
+    units.append((CSRC / "capi.hip", obj_dir / "capi.o", []))
+    cmds = [[HIPCC, *FLAGS, *extra_flags, *extra, "-c", str(src), "-o", str(obj)] for src, obj, extra in units]
     jobs = jobs or min(8, os.cpu_count() or 4)
     with ThreadPoolExecutor(max_workers=jobs) as ex:
         list(ex.map(_run, cmds))
-    tmp = LIB.with_suffix(".so.tmp")
+    tmp = lib.with_suffix(".so.tmp")
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *[str(o) for _, o, _ in units]])
-    tmp.replace(LIB)
+    tmp.replace(lib)
     if verbose:
-        print(f"built {LIB}", file=sys.stderr)
-    return LIB
+        print(f"built {lib}", file=sys.stderr)
+    return lib
 
 
 if __name__ == "__main__":

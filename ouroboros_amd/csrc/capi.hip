@@ -172,6 +172,13 @@ __global__ void __launch_bounds__(256) k_copy(const uint4* __restrict__ src, uin
 
 extern "C" {
 
+#ifdef OURO_DIAG
+int ouro_diag_read(ouro_handle h, unsigned long long* out16, int reset) {
+  if (!h || !out16) return OURO_E_INVALID;
+  return h->impl->diag_read(out16, reset) == hipSuccess ? OURO_OK : OURO_E_HIP;
+}
+#endif
+
 int ouro_sem_decide(uint64_t word, int64_t num, int64_t ppc, int64_t max_k, int* mode, uint32_t* k, uint64_t* next) {
   if (!mode || !k || !next || num < 1 || ppc < 1) return OURO_E_INVALID;
   const ouro::SemDecision d = ouro::sem_decide(word, num, ppc, max_k);
@@ -536,6 +543,9 @@ int ouro_check_quiescent(ouro_handle h, ouro_check_result* out) {
         if (ch->mask[wi] != full) fail(k, c, "chunk availability mask is not full", (long long)wi, 0);
       }
       class_pages[k] += ch->count;
+      // every page is free again, so the chunk must sit in its class queue inside [front_, back_)
+      if (ch->queue_pos - st.front[k] >= st.back[k] - st.front[k])
+        fail(k, c, "chunk with free pages is not inside its queue window", ch->queue_pos, st.front[k]);
     } else {
       const ouro::PageChunkHdr* ph = reinterpret_cast<const ouro::PageChunkHdr*>(w);
       if (ph->number_pages != ppc) fail(k, c, "page chunk page count differs", ph->number_pages, ppc);
@@ -576,13 +586,18 @@ int ouro_check_quiescent(ouro_handle h, ouro_check_result* out) {
       std::unordered_set<uint32_t> mem(members[k].begin(), members[k].end());
       for (uint32_t i = 0; i < len; ++i) {
         const uint32_t v = win[i];
-        if (v == 0xFFFFFFFFu) {
-          if (kind == OURO_PQ) fail(k, -1, "hole inside the page queue window", i, len);
+        if (v == ouro::kMarker) {
+          fail(k, -1, "unwritten slot inside the queue window", i, len);
           continue;
         }
+        if (kind == OURO_CQ && v == ouro::kTomb) continue;  // removed chunk
         const uint32_t c = kind == OURO_PQ ? (v >> 9) : v;
         const uint32_t pg = kind == OURO_PQ ? (v & 511u) : 0u;
         if (!mem.count(c)) fail(k, c, "queue entry names a chunk outside the class", c, 0);
+        if (kind == OURO_CQ && c < nfc) {
+          const ouro::ChunkHdr* ch = reinterpret_cast<const ouro::ChunkHdr*>(hdrs.data() + (size_t)c * 128);
+          if (ch->queue_pos != st.front[k] + i) fail(k, c, "chunk queue_pos differs from its queue position", ch->queue_pos, st.front[k] + i);
+        }
         if (kind == OURO_PQ && pg >= st.pages_per_chunk[k]) fail(k, c, "queue entry page out of range", pg, st.pages_per_chunk[k]);
         if (!seen.insert(v).second) fail(k, c, "duplicate queue entry", v, 0);
       }

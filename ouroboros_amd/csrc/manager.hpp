@@ -28,6 +28,9 @@ struct IManager {
   virtual hipError_t query(ouro_state* out) = 0;
   // immutable queue placement of class k (chain order)
   virtual void class_ring(uint32_t k, uint32_t** ring, uint32_t* ring_size) = 0;
+#ifdef OURO_DIAG
+  virtual hipError_t diag_read(unsigned long long* out16, int reset) = 0;
+#endif
 };
 
 // Factories, one per variant translation unit.
@@ -171,6 +174,18 @@ struct Manager final : IManager {
     *ring_size = 0;
     ring_of<0>(mm.chain, k, ring, ring_size);
   }
+
+#ifdef OURO_DIAG
+  // each variant translation unit holds its own copy of the diagnostic counters
+  hipError_t diag_read(unsigned long long* out16, int reset) override {
+    hipError_t e = hipMemcpyFromSymbol(out16, HIP_SYMBOL(ouro::g_diag), sizeof(unsigned long long) * 16);
+    if (e == hipSuccess && reset) {
+      unsigned long long z[16] = {0};
+      e = hipMemcpyToSymbol(HIP_SYMBOL(ouro::g_diag), z, sizeof(z));
+    }
+    return e;
+  }
+#endif
 
   hipError_t query(ouro_state* out) override {
     // the manager state is ~8 KiB; stage it in heap memory, it must not be constructed

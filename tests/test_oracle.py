@@ -160,6 +160,58 @@ def test_sequential_waves_invariants(kind):
             assert st.sem_count[k] == st.queue_count[k] * ppc == st.carved[k] * ppc
 
 
+TOMB = 0xFFFFFFFD
+
+
+def _check_chunk_queues(s, st):
+    """Every class chunk with free pages is queued exactly once, at the position its queue_pos names;
+    exhausted chunks are not queued; removed entries are tombstones; front_ is compacted."""
+    class_ps = [16 << k for k in range(10)] + [16384, 32768]
+    for k in range(st.num_classes):
+        w = s.window(k)
+        live = {}
+        for i, val in enumerate(w):
+            assert val != 0xFFFFFFFF, (k, i, "unwritten slot inside the window")
+            if val == TOMB:
+                continue
+            t, ps, _, _ = s.chunk(int(val))
+            assert t == 2 and ps == class_ps[k], (k, i, val)
+            assert s.queue_pos(int(val)) == (st.front[k] + i) & 0xFFFFFFFF, (k, i, val)
+            assert int(val) not in live, (k, val, "queued twice")
+            live[int(val)] = i
+        assert len(w) == 0 or w[0] != TOMB, (k, "front_ not compacted")
+        assert st.queue_count[k] == len(live)
+        for c in range(st.next_free_chunk):
+            t, ps, cnt, _ = s.chunk(c)
+            if t != 2 or ps != class_ps[k]:
+                continue
+            assert (cnt > 0) == (c in live), (k, c, cnt)
+
+
+@pytest.mark.parametrize("kind", ["CQ", "VACQ", "VLCQ"])
+def test_chunk_queue_removal_keeps_window_exact(kind):
+    # blocks held across waves exhaust chunks in the middle of the queue window; each is removed
+    # where it sits and re-enqueued at back_ by the free that gives it pages again
+    rng = np.random.default_rng(11)
+    s = oracle.Sim(kind, 512 << 20)
+    live = []
+    for step in range(400):
+        n = int(rng.integers(1, 65))
+        sizes = (16 << rng.integers(0, 12, size=n)).astype(np.uint32)
+        live.append(s.wave_malloc(sizes))
+        if len(live) > 6 or rng.random() < 0.4:
+            s.wave_free(live.pop(int(rng.integers(0, len(live)))))
+        if step % 25 == 0:
+            st = s.state()
+            assert st.error == 0
+            _check_chunk_queues(s, st)
+    for o in live:
+        s.wave_free(o)
+    st = s.state()
+    assert st.error == 0
+    _check_chunk_queues(s, st)
+
+
 def test_out_of_memory_returns_nullptr_and_sets_error():
     # a small pool, 8 KiB pages (one per chunk): 64-lane groups are served whole or not at all;
     # chunks carved by a group that could not be served stay available to later single requests.
