@@ -35,6 +35,7 @@ enum : uint32_t {
 
 constexpr uint32_t kMarker = 0xFFFFFFFFu;  // empty slot, DeletionMarker<index_t> (Definitions.h:31-35)
 constexpr uint32_t kTomb = 0xFFFFFFFDu;    // chunk-queue slot whose chunk was removed (front_ may pass it)
+constexpr uint32_t kNullPage = 0xFFFFFFFCu;  // page-queue entry standing for a page the pool could not supply
 constexpr uint32_t kWave = 64;             // CDNA wavefront width (WARP_SIZE=32 in Definitions.h:16)
 // Spin budget: a wait that lasts longer than this (wall-clock ticks of s_memrealtime, 100 MHz on
 // MI355X, so about 2 s) gives up and records ERR_TIMEOUT; nested waits are bounded by depth x budget.

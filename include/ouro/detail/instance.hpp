@@ -92,12 +92,14 @@ __device__ inline void ring_put(Memory& m, ReuseRing& r, bool want, uint32_t val
   }
 }
 
-// IndexQueue::dequeue (Queue_impl.cuh:54-67) for up to k entries, issued by lane `l` (uniform).
-// Returns the number obtained; the j-th active lane (j < result) receives entry j in `out`.
-__device__ inline uint32_t ring_take(Memory& m, ReuseRing& r, uint32_t k, int l, uint32_t me, uint32_t& out) {
+// IndexQueue::dequeue (Queue_impl.cuh:54-67) for up to k entries, issued by lane `l` (uniform),
+// leaving at least `keep` entries in the ring. Returns the number obtained; the j-th active lane
+// (j < result) receives entry j in `out`.
+__device__ inline uint32_t ring_take(Memory& m, ReuseRing& r, uint32_t k, int l, uint32_t me, uint32_t& out,
+                                     uint32_t keep = 0) {
   uint32_t t = 0, pos = 0;
   if ((int)lane() == l) {
-    const int32_t old = sub(&r.count, (int32_t)k);
+    const int32_t old = sub(&r.count, (int32_t)k) - (int32_t)keep;
     t = old <= 0 ? 0u : ((uint32_t)old >= k ? k : (uint32_t)old);
     if (t < k) add(&r.count, (int32_t)(k - t));
     pos = t ? add(&r.front, t) : 0u;
@@ -120,25 +122,45 @@ __device__ inline uint32_t ring_take(Memory& m, ReuseRing& r, uint32_t k, int l,
 // allocateChunk<QUEUECHUNK> (Ouroboros.cuh:108-136) for k chunks at once: the reuse ring first
 // (FIFO), then the bump pointer, exactly the order k sequential calls would produce. The j-th active
 // lane (j < k) receives chunk j. Validity follows the reference's predicate chunk + add < maxChunks
-// (:133); unlike the reference, an invalid chunk is never handed out. Returns the valid prefix length.
-__device__ inline uint32_t alloc_chunks(Memory& m, ReuseRing& r, uint32_t add_factor, uint32_t k, uint32_t& chunk) {
+// (:133) below the queue-chunk reserve (the top m.qc_reserve chunks of a virtualized pool); queue
+// chunks (`queue`) that find the bump region spent take from the reserve, so a full pool can still
+// advance its queues. Unlike the reference, an invalid chunk is never handed out. Returns the valid
+// prefix length.
+__device__ inline uint32_t alloc_chunks(Memory& m, ReuseRing& r, uint32_t add_factor, uint32_t k, uint32_t& chunk,
+                                        bool queue = false) {
   const uint64_t act = active();
   const uint32_t me = rank_in(act);
   const int l = first(act);
   uint32_t got = 0;
-  const uint32_t t = ring_take(m, r, k, l, me, got);
+  // data carves leave the last few recycled chunks of a virtualized pool to its queue chunks
+  const uint32_t t = ring_take(m, r, k, l, me, got, queue ? 0u : m.qc_reserve / 2);
   uint32_t base = 0;
   if ((int)lane() == l && k > t) base = add(&m.next_free_chunk, (k - t) * add_factor);
   base = readlane(base, l);
+  const uint64_t limit = (uint64_t)m.max_chunks - m.qc_reserve;
   bool valid = false;
   if (me < t) {
     chunk = got;
     valid = chunk_ok(m, got);
   } else if (me < k) {
     chunk = base + (me - t) * add_factor;
-    valid = (uint64_t)chunk + add_factor < (uint64_t)m.max_chunks;
-    if (valid) or_(&m.locator[chunk >> 5], 1u << (chunk & 31u));  // ChunkLocator::initChunkIndex
+    valid = (uint64_t)chunk + add_factor < limit;
   }
+  if (queue) {
+    // bump lanes that came up empty continue in the reserve, in rank order
+    const uint64_t miss = ballot(me < k && !valid);
+    if (miss) {
+      uint32_t rbase = 0;
+      if ((int)lane() == l) rbase = add(&m.qc_next, popc(miss));
+      rbase = readlane(rbase, l);
+      if (me < k && !valid) {
+        const uint32_t j = rbase + rank_in(miss);
+        chunk = (uint32_t)limit + j;
+        valid = j < m.qc_reserve && chunk < m.max_chunks;
+      }
+    }
+  }
+  if (me >= t && me < k && valid) or_(&m.locator[chunk >> 5], 1u << (chunk & 31u));  // ChunkLocator::initChunkIndex
   const uint64_t okm = ballot(me < k && valid);
   // valid chunks form a prefix in active-rank order
   const uint32_t k_ok = popc(okm);
@@ -148,6 +170,7 @@ __device__ inline uint32_t alloc_chunks(Memory& m, ReuseRing& r, uint32_t add_fa
 
 // Lane id of the j-th active lane.
 __device__ __forceinline__ int lane_of_rank(uint64_t act, uint32_t j) { return (int)nth_set_bit(act, j); }
+
 
 // ------------------------------------------------------------------ queue chunks (VA / VL)
 // QueueChunk::initializeChunk (QueueChunk.cuh:221-225) plus a wave-cooperative clean of its 2048
@@ -389,7 +412,7 @@ __device__ inline void put_lanes(Ctx<I>& x, ClassQueue* cq, bool want, uint32_t 
       // lanes that need one form a sub-wave; alloc_chunks hands them out in lane order
       if (need) {
         const uint32_t k = popc(ballot(true));
-        const uint32_t ok = alloc_chunks(x.m, x.base_ring, 1u, k, got);
+        const uint32_t ok = alloc_chunks(x.m, x.base_ring, 1u, k, got, true);
         (void)ok;
         qc = got;
         if (rank_in(ballot(true)) >= ok) qc = kMarker;
@@ -536,97 +559,41 @@ __device__ inline uint32_t take_lanes(Ctx<I>& x, ClassQueue* cq, bool want, uint
 }
 
 // ------------------------------------------------------------------ semaphore
-// BulkSemaphore::wait (BulkSemaphore_impl.cuh:199-300), one loop per group leader. A carve decision
-// reserves `k` chunks' worth of expected pages at once (the reference loops one chunk per pass);
-// carves then run wave-cooperatively, one leader at a time in lane order. Returns the group's grant.
+// BulkSemaphore::wait (BulkSemaphore_impl.cuh:199-300) for one group, restated on the debt line
+// (sem_share): the group leader claims its pages with one fetch-add and learns from the displaced
+// word how many chunks it must carve; it promises them (expected += k * ppc) and the carves then
+// run wave-cooperatively, one leader at a time in lane order. No claim waits here: a page-queue
+// claim waits on its own queue slot, a chunk-queue claim on its scan. The reference decides with a
+// CAS loop on the word and parks waiters in `reserved`, which a cold class with thousands of
+// concurrent groups turns into a serial chain of failed CASes.
 template <class I, class Carve>
-__device__ inline bool sem_acquire(Ctx<I>& x, ClassQueue* cq, uint64_t grp, bool want, Carve&& carve) {
+__device__ inline void sem_claim(Ctx<I>& x, ClassQueue* cq, uint64_t grp, bool want, Carve&& carve) {
   const int L = leader_of(grp);
-  const bool leader = want && (int)lane() == L;
-  const int64_t num = (int64_t)popc(grp);
-  const int64_t max_k = (int64_t)popc(active());
-  bool pending = leader, ok = false, first_pass = true;
-  Spin it(&x.m.error);
-  while (ballot(pending)) {
-    uint32_t carve_k = 0;
-    bool reserve = false;
-    if (pending) {
-      OURO_DIAG_ADD(12, 1);
-      uint64_t* s = &cq->sem;
-      // First pass: take speculatively (one access to the semaphore line in the common case);
-      // later passes look before they take, as BulkSemaphore_impl.cuh:216 always does.
-      uint64_t v = first_pass ? Sem::make(num, 0, 0) : ld(s);
-      first_pass = false;
-      if (Sem::count(v) >= num) {
-        const uint64_t old = add(s, (uint64_t)(-num));
-        if (Sem::count(old) >= num) {
-          pending = false;
-          ok = true;
-        } else {
-          add(s, (uint64_t)num);
-        }
-      }
-      if (pending) {
-        v = ld(s);
-        const int64_t ppc = cq->ppc;
-        while (true) {
-          const SemDecision d = sem_decide(v, num, ppc, max_k);
-          if (d.mode == SemMode::HOLD) {
-            backoff(64);
-            break;
-          }
-          const uint64_t nv = d.next;
-          const uint32_t k = d.mode == SemMode::CARVE ? d.k : 0u;
-          const bool take = d.mode == SemMode::TAKE, res = d.mode == SemMode::RESERVE;
-          const uint64_t seen = cas(s, v, nv);
-          if (seen != v) OURO_DIAG_ADD(15, 1);
-          if (seen == v) {
-            if (k) OURO_DIAG_ADD(13, k);
-            carve_k = k;
-            reserve = res;
-            if (take) {
-              pending = false;
-              ok = true;
-            }
-            break;
-          }
-          v = seen;
-        }
-      }
-    }
-    // wave-cooperative carves, leaders in lane order
-    uint64_t carvers = ballot(carve_k > 0);
-    while (carvers) {
-      const int l = first(carvers);
-      carvers &= carvers - 1;
-      ClassQueue* lq = reinterpret_cast<ClassQueue*>(readlane64(reinterpret_cast<uint64_t>(cq), l));
-      const uint32_t k = readlane(carve_k, l);
-      const uint32_t k_ok = carve(lq, k);
-      if ((int)lane() == l && k_ok < k) pending = false;  // out of chunks: the group gets nullptr
-    }
-    if (reserve) {
-      uint64_t* s = &cq->sem;
-      Spin spin(&x.m.error);
-      while (true) {
-        const uint64_t v = ld(s);
-        const int64_t c = Sem::count(v);
-        if (!(c < num && (int64_t)Sem::reserved(v) < c + (int64_t)Sem::expected(v))) break;
-        OURO_DIAG_ADD(14, 1);
-        if (spin.expired()) {
-          set_error(x.m, ERR_TIMEOUT);
-          pending = false;
-          break;
-        }
-        backoff(spin.it);
-      }
-      add(s, (uint64_t)0 - ((uint64_t)num << Sem::kResShift));
-    }
-    if (it.expired()) {
-      if (pending) set_error(x.m, ERR_TIMEOUT);
-      pending = false;
+  uint32_t k = 0;
+  if (want && (int)lane() == L) {
+    OURO_DIAG_ADD(12, 1);
+    const int64_t num = (int64_t)popc(grp);
+    const int64_t ppc = cq->ppc;
+    const uint64_t before = add(&cq->sem, (uint64_t)(-num));
+    k = sem_share(before, num, ppc);
+    if (k) {
+      OURO_DIAG_ADD(13, k);
+      // a wrapped expected field would only make later claims over-carve, never under-carve
+      add(&cq->sem, ((uint64_t)k * (uint64_t)ppc) << Sem::kExpShift);
     }
   }
-  return shfl((uint32_t)ok, L) != 0;
+  uint64_t carvers = ballot(k > 0);
+  while (carvers) {
+    const int l = first(carvers);
+    carvers &= carvers - 1;
+    ClassQueue* lq = reinterpret_cast<ClassQueue*>(readlane64(reinterpret_cast<uint64_t>(cq), l));
+    carve(lq, readlane(k, l));
+  }
+}
+
+// Chunk-queue claims that cannot be served because the pool ran out give their pages back.
+__device__ __forceinline__ bool pool_exhausted(const Memory& m) {
+  return (__hip_atomic_load(&m.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & (ERR_OOM | ERR_QUEUE_FULL)) != 0;
 }
 
 // signalExpected (BulkSemaphore_impl.cuh:305-308) for n pages carved out of k_req * ppc expected.
@@ -679,15 +646,17 @@ __device__ inline uint32_t carve_pages(Ctx<I>& x, ClassQueue* cq, uint32_t k) {
     st(&h->number_pages, ppc);
   }
   order();
-  const uint32_t n = k_ok * ppc;
+  // every promised page gets a queue slot: the carved pages, then a null entry for each page of a
+  // chunk the pool could not supply, so claims already counted on them are answered with nullptr
+  const uint32_t n = k * ppc, n_real = k_ok * ppc;
   uint32_t v0 = 0;
   if ((int)lane() == l0 && n) v0 = add(&cq->back, n);
   v0 = readlane(v0, l0);
   for (uint32_t e0 = 0; e0 < n; e0 += nact) {
     const uint32_t e = e0 + me;
-    const uint32_t j = (e < n ? e : n - 1) / ppc;
+    const uint32_t j = (e < n_real ? e : (n_real ? n_real - 1 : 0)) / ppc;
     const uint32_t c = shfl(chunk, lane_of_rank(act, j));
-    put_lanes(x, cq, e < n, v0 + e, make_index(c, e % ppc));
+    put_lanes(x, cq, e < n, v0 + e, e < n_real ? make_index(c, e % ppc) : kNullPage);
   }
   if ((int)lane() == l0) sem_signal_expected(cq, n, k * ppc);
   return k_ok;
@@ -701,16 +670,14 @@ __device__ inline void* alloc_pages(Ctx<I>& x, uint32_t qi, bool want) {
   ClassQueue* cq = &x.inst.q[want ? qi : 0];
   const uint64_t key = want ? (uint64_t)qi : ~0ull;
   const uint64_t grp = match<GS>(key);
-  bool ok = false;
   {
     auto carve = [&](ClassQueue* lq, uint32_t k) { return carve_pages(x, lq, k); };
-    ok = sem_acquire(x, cq, grp, want, carve);
+    sem_claim(x, cq, grp, want, carve);
   }
-  ok = ok && want;
   uint32_t n;
-  const uint32_t v = fetch_positions<GS>(&cq->front, ok, n);
-  const uint32_t idx = take_lanes(x, cq, ok, v);
-  if (!ok || idx == kMarker) return nullptr;
+  const uint32_t v = fetch_positions<GS>(&cq->front, want, n);
+  const uint32_t idx = take_lanes(x, cq, want, v);
+  if (!want || idx == kMarker || idx == kNullPage) return nullptr;
   if (!chunk_ok(x.m, index_chunk(idx)) || index_page(idx) >= cq->ppc) {
     set_error(x.m, ERR_CORRUPT);
     return nullptr;
@@ -997,13 +964,15 @@ __device__ inline AccessMode access_alloc_lane(Ctx<I>& x, ChunkHdr* h, uint32_t&
 // Scan positions run over [front_, back_): a position at or past back_, or behind front_, restarts
 // at front_.
 template <class I>
-__device__ __forceinline__ void scan_wrap(ClassQueue* cq, uint32_t& v, uint32_t& cursor, uint32_t b) {
+__device__ __forceinline__ bool scan_wrap(ClassQueue* cq, uint32_t& v, uint32_t& cursor, uint32_t b) {
   const uint32_t f = ld(&cq->front);
   if (v - b < 0x80000000u || v - f >= 0x80000000u) {
     OURO_DIAG_ADD(6, 1);
     v = f;
     cursor = kMarker;
+    return true;
   }
+  return false;
 }
 
 // Batch length at v: at most kPeek, not past back_, and inside one queue chunk for VA / VL.
@@ -1025,12 +994,11 @@ __device__ inline void* alloc_chunk_pages(Ctx<I>& x, uint32_t qi, bool want) {
   ClassQueue* cq = &x.inst.q[want ? qi : 0];
   const uint64_t key = want ? (uint64_t)qi : ~0ull;
   const uint64_t grp = match<GS>(key);
-  bool ok;
   {
     auto carve = [&](ClassQueue* lq, uint32_t k) { return carve_chunks(x, lq, k); };
-    ok = sem_acquire(x, cq, grp, want, carve);
+    sem_claim(x, cq, grp, want, carve);
   }
-  ok = ok && want;
+  const bool ok = want;
   uint32_t chunk = kMarker, page = 0;
   uint32_t reclaim = kMarker;
   bool got = false;
@@ -1039,14 +1007,18 @@ __device__ inline void* alloc_chunk_pages(Ctx<I>& x, uint32_t qi, bool want) {
     // probes each chunk's count with a decrement
     if (ok) {
       uint32_t v = ld(&cq->front);
-      uint32_t cursor = kMarker;
+      uint32_t cursor = kMarker, wraps = 0;
       Spin it(&x.m.error);
       while (true) {
         OURO_DIAG_ADD(0, 1);
         const uint32_t b = ld(&cq->back);
-        scan_wrap<I>(cq, v, cursor, b);
+        wraps += scan_wrap<I>(cq, v, cursor, b);
         uint32_t s[kPeek];
         const uint32_t n = batch_at<I>(v, b) ? 1u : 0u;
+        if ((wraps >= 2 || n == 0) && pool_exhausted(x.m)) {
+          sem_signal(cq, 1);  // the pool ran out: give the claim back, return nullptr
+          break;
+        }
         s[0] = kMarker;
         if (n) peek_slots(x, cq, v, n, cursor, s);  // never walk towards a position at back_
         const uint32_t c = s[0];
@@ -1072,20 +1044,23 @@ __device__ inline void* alloc_chunk_pages(Ctx<I>& x, uint32_t qi, bool want) {
   } else {
     // group protocol: the leader finds chunks kPeek slots at a time, reserves pages on the chunk
     // count with a CAS and claims them from the chunk's 64-bit mask words; lanes take them in rank
-    // order. Leaders of one class start at spread positions of the live window [front_, back_) (a
-    // per-class scan cursor) instead of all at front_.
+    // order. Where a leader starts comes from a per-class page cursor: it adds its page count and
+    // starts at queue position cursor / ppc, so concurrent leaders begin at disjoint chunks, in
+    // queue order, and the consumed region stays behind the cursor (the reference starts every
+    // thread at front_, ChunkQueue_impl.cuh:101-105). A cursor outside [front_, back_) folds back
+    // into the window.
     const int L = first(grp);
     const bool leader = ok && (int)lane() == L;
     const uint32_t num = popc(grp);
     uint32_t v = 0, cursor = kMarker;
     if (leader) {
-      const uint32_t ppc = cq->ppc;
-      const uint32_t spread = add(&cq->scan, (num + ppc - 1) / ppc);
+      const uint64_t p = add(&cq->scan, (uint64_t)num);
+      const uint32_t h = (uint32_t)(p >> __builtin_ctz(cq->ppc));
       const uint32_t f = ld(&cq->front), b = ld(&cq->back);
       const uint32_t len = b - f;
-      v = (len == 0 || len >= 0x80000000u) ? f : f + spread % len;
+      v = (len == 0 || len >= 0x80000000u) ? f : (h - f < len ? h : f + (h - f) % len);
     }
-    uint32_t need = leader ? num : 0u, assigned = 0, t_rem = 0, w = 0, c = kMarker;
+    uint32_t need = leader ? num : 0u, assigned = 0, t_rem = 0, w = 0, c = kMarker, wraps = 0;
     Spin it(&x.m.error);
     bool done = !ok, gave_up = false;
     while (ballot(!done)) {
@@ -1093,11 +1068,19 @@ __device__ inline void* alloc_chunk_pages(Ctx<I>& x, uint32_t qi, bool want) {
       uint64_t bits = 0;
       if (leader && assigned < num && !gave_up) {
         OURO_DIAG_ADD(0, 1);
+        uint32_t n = 0;
         if (t_rem == 0) {
           OURO_DIAG_ADD(1, 1);
           const uint32_t b = ld(&cq->back);
-          scan_wrap<I>(cq, v, cursor, b);
-          const uint32_t n = batch_at<I>(v, b);
+          wraps += scan_wrap<I>(cq, v, cursor, b);
+          n = batch_at<I>(v, b);
+          if ((wraps >= 2 || n == 0) && pool_exhausted(x.m)) {
+            // the pool ran out and a whole window pass found nothing: unserved lanes get nullptr
+            sem_signal(cq, need);
+            gave_up = true;
+          }
+        }
+        if (t_rem == 0 && !gave_up) {
           uint32_t s = kMarker;
           int32_t cnt = 0;
           const uint32_t i = n ? find_chunk(x, cq, v, n, cursor, s, cnt) : 0u;

@@ -29,6 +29,7 @@ constexpr uint32_t kPageMask = (1u << kPageBits) - 1u;
 constexpr uint32_t kReuseEntries = 65536;      // chunk_queue_size (Parameters.h:10)
 constexpr uint32_t kStdEntries = 1u << 23;     // page_queue_size (Parameters.h:11)
 constexpr uint32_t kVAEntries = 16384;         // virtual_queue_size (Parameters.h:12)
+constexpr uint32_t kQCReserve = 32;            // queue-chunk reserve of a virtualized pool
 constexpr uint32_t kQCSpots = kBaseChunk / 4;  // QueueChunk::num_spots_ (QueueChunk.cuh:27)
 constexpr uint32_t kOldHold = 10;              // LARGEST_OLD_COUNT_VALUE (Parameters.h:13)
 constexpr uint32_t kMaskWords = 8;             // 512 pages / 64-bit words (ChunkAccess.cuh:13 uses 16 x u32)
@@ -57,56 +58,42 @@ OURO_HD inline uint32_t class_of(uint64_t size, uint32_t smallest_log2) {
 }
 
 // ------------------------------------------------------------------ semaphore word
-// BulkSemaphore (BulkSemaphore.cuh:5-84) packs count | expected | reserved into one u64 with a
-// 25/25/14-bit split. The 14-bit reserved field wraps once more than 16383 pages are reserved,
-// which a 64-lane aggregated wave on a chip with 524,288 resident lanes can reach; the split here
-// is 26/19/19 (count biased by 2^25), identical in behaviour whenever the reference's fields fit.
+// BulkSemaphore word (BulkSemaphore.cuh:5-84), one u64 per class, restated as a debt line:
+//   count    (40 bits, bias 2^39): published pages not yet claimed, minus claims not yet covered,
+//                                 so it goes negative while carves are in flight;
+//   expected (24 bits)           : pages of carves promised but not yet published.
+// The reference's third field (reserved, 14 bits) has no counterpart: a claim never waits on the
+// word, it waits on the queue itself.
 struct Sem {
-  static constexpr uint32_t kCountBits = 26;
-  static constexpr uint64_t kBias = 1ull << 25;
-  static constexpr uint32_t kExpShift = 26;
-  static constexpr uint32_t kResShift = 45;
+  static constexpr uint32_t kCountBits = 40;
+  static constexpr uint64_t kBias = 1ull << 39;
+  static constexpr uint32_t kExpShift = 40;
   static constexpr uint64_t kCountMask = (1ull << kCountBits) - 1ull;
-  static constexpr uint64_t kFieldMask = (1ull << 19) - 1ull;
-  OURO_HD static constexpr uint64_t make(int64_t count, uint64_t expected, uint64_t reserved) {
-    return (uint64_t)(count + (int64_t)kBias) | (expected << kExpShift) | (reserved << kResShift);
+  static constexpr uint64_t kExpMask = (1ull << 24) - 1ull;
+  OURO_HD static constexpr uint64_t make(int64_t count, uint64_t expected) {
+    return (uint64_t)(count + (int64_t)kBias) | (expected << kExpShift);
   }
   OURO_HD static constexpr int64_t count(uint64_t v) { return (int64_t)(v & kCountMask) - (int64_t)kBias; }
-  OURO_HD static constexpr uint64_t expected(uint64_t v) { return (v >> kExpShift) & kFieldMask; }
-  OURO_HD static constexpr uint64_t reserved(uint64_t v) { return (v >> kResShift) & kFieldMask; }
+  OURO_HD static constexpr uint64_t expected(uint64_t v) { return (v >> kExpShift) & kExpMask; }
   static constexpr uint64_t kInit = kBias;
 };
 
-// One pass of BulkSemaphore::wait's decision (BulkSemaphore_impl.cuh:236-263) for a group of `num`
-// pages on a class with `ppc` pages per chunk, given the semaphore word v:
-//   CARVE   -- expected += k * ppc, then the caller carves k chunks (1 <= k <= max_k);
-//   TAKE    -- count -= num;
-//   RESERVE -- reserved += num, then the caller waits;
-//   HOLD    -- expected and reserved are both saturated: change nothing, retry later.
-// A transiently negative count (optimistic takes about to be undone) counts as zero. A carve covers
-// the whole deficit (this group plus the waiting reservations) instead of one chunk per pass.
-enum class SemMode : int { CARVE = 0, TAKE = 1, RESERVE = 2, HOLD = 3 };
-struct SemDecision {
-  SemMode mode;
-  uint32_t k;
-  uint64_t next;
-};
-OURO_HD inline SemDecision sem_decide(uint64_t v, int64_t num, int64_t ppc, int64_t max_k) {
-  const int64_t c = Sem::count(v), e = (int64_t)Sem::expected(v), r = (int64_t)Sem::reserved(v);
-  const int64_t c0 = c > 0 ? c : 0;
-  const int64_t avail = c0 + e - r;
-  const int64_t field = (int64_t)Sem::kFieldMask;
-  if (avail < num && e + ppc <= field && max_k >= 1) {
-    int64_t k = (num - avail + ppc - 1) / ppc;
-    const int64_t room = (field - e) / ppc;
-    if (k > max_k) k = max_k;
-    if (k > room) k = room;
-    if (k < 1) k = 1;
-    return {SemMode::CARVE, (uint32_t)k, Sem::make(c, (uint64_t)(e + k * ppc), (uint64_t)r)};
-  }
-  if (c >= num) return {SemMode::TAKE, 0u, Sem::make(c - num, (uint64_t)e, (uint64_t)r)};
-  if (r + num <= field) return {SemMode::RESERVE, 0u, Sem::make(c, (uint64_t)e, (uint64_t)(r + num))};
-  return {SemMode::HOLD, 0u, v};
+// Pages the class owes its claimants: claims minus (published + promised) pages.
+OURO_HD inline int64_t sem_debt(uint64_t v) { return -(Sem::count(v) + (int64_t)Sem::expected(v)); }
+
+// Chunks a claim of `num` pages must carve, given the word it displaced (BulkSemaphore::wait's
+// AllocateChunk decision, BulkSemaphore_impl.cuh:236-263, as a closed form): the claim moves the
+// debt from d to d + num and owns the chunk boundaries it crosses, ceil(d1 / ppc) - ceil(d0 / ppc)
+// with negative debt counted as zero. Claims see disjoint, consecutive stretches of the line
+// (one fetch-add each), promises only lower the debt by whole chunks and frees by single pages,
+// so promised-but-unpublished chunks always cover ceil(debt / ppc): no stretch is carved twice
+// and none is left uncovered. ppc is a power of two.
+OURO_HD inline uint32_t sem_share(uint64_t before, int64_t num, int64_t ppc) {
+  const int sh = __builtin_ctzll((unsigned long long)ppc);
+  const int64_t d0 = sem_debt(before), d1 = d0 + num;
+  const int64_t f0 = d0 > 0 ? (d0 + ppc - 1) >> sh : 0;
+  const int64_t f1 = d1 > 0 ? (d1 + ppc - 1) >> sh : 0;
+  return (uint32_t)(f1 - f0);
 }
 
 // ------------------------------------------------------------------ chunk headers (128 B)
@@ -146,8 +133,9 @@ struct alignas(128) Memory {
   uint32_t max_chunks;
   uint32_t variant;
   uint32_t initialized;
-  uint32_t reserved0;
+  uint32_t qc_reserve;      // chunks at the top of the pool that only queue chunks may take
   alignas(128) uint32_t next_free_chunk;
+  uint32_t qc_next;         // bump pointer inside the queue-chunk reserve
   alignas(128) uint32_t error;
   alignas(128) uint32_t stats[4];  // pageAlloc, pageFree, chunkAlloc, chunkReuse (Statistics.h:5-17)
 };
@@ -175,8 +163,8 @@ struct alignas(128) ClassQueue {
   alignas(128) uint32_t back;
   uint32_t back_qc;      // VL back_ptr_
   alignas(128) int32_t count;   // chunk queues: chunks in the queue (count_)
-  uint32_t scan;         // chunk queues: cursor that spreads group leaders over the queue window
   uint32_t carved;       // pages (page queues) or chunks (chunk queues) carved into this class
+  uint64_t scan;         // chunk queues: page cursor; scan start = queue position scan / ppc
   uint32_t old_qc;       // VL old_ptr_
   uint32_t old_count;    // VL old_count_
 };

@@ -136,6 +136,7 @@ template <class... Is>
 struct Ouroboros {
   using MyType = Ouroboros<Is...>;
   static constexpr int kNumInstances = sizeof...(Is);
+  static constexpr bool kVirtualized = ((!is_std(Is::kKind)) || ...);
 
   Memory memory;
   Chain<Is...> chain;
@@ -244,6 +245,9 @@ struct Ouroboros {
     memory.locator_bytes = L.locator_bytes;
     memory.locator = reinterpret_cast<uint32_t*>(d + L.allocation_size - L.locator_bytes);
     memory.next_free_chunk = 0;
+    memory.qc_next = 0;
+    // a virtualized pool keeps its top chunks for queue chunks (at most a quarter of a tiny pool)
+    memory.qc_reserve = kVirtualized ? (kQCReserve < L.max_chunks / 4 ? kQCReserve : (uint32_t)(L.max_chunks / 4)) : 0u;
     memory.error = 0;
     // queue region: rings and class queues are placed downward from the locator (initNew, :42-58)
     uint8_t* end = d + L.allocation_size - L.locator_bytes;

@@ -160,9 +160,11 @@ int ouro_run_trace(ouro_handle h, const ouro_trace_config* cfg, ouro_trace_resul
 int ouro_gen_sizes(uint64_t seed, uint32_t n, int dist, double param, uint32_t classes, uint32_t* out);
 
 /* ---- test hook (no reference counterpart) --------------------------------------------------- */
-/* The device semaphore decision (BulkSemaphore_impl.cuh:236-263 as changed here) evaluated on the
- * host: mode 0 carve k chunks, 1 take, 2 reserve, 3 hold; *next = the word the CAS would install. */
-int ouro_sem_decide(uint64_t word, int64_t num, int64_t ppc, int64_t max_k, int* mode, uint32_t* k, uint64_t* next);
+/* The device semaphore's carve decision (BulkSemaphore_impl.cuh:236-263 as changed here) evaluated
+ * on the host: *k = chunks a claim of `num` pages must carve, given the word its fetch-add displaced
+ * (`before`); ppc a power of two. sem_make packs (count, expected) the way the device does. */
+int ouro_sem_share(uint64_t before, int64_t num, int64_t ppc, uint32_t* k);
+uint64_t ouro_sem_make(int64_t count, uint64_t expected);
 
 /* ---- measurement helpers (no reference counterpart) ---------------------------------------- */
 /* Peak device-scope atomic throughput: `threads` threads each issue `iters` u32 fetch-adds with

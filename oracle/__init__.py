@@ -69,13 +69,13 @@ def _load_sim() -> C.CDLL:
         lib.sim_queue_window.restype = C.c_uint32
         lib.sim_class_of.argtypes = [C.c_uint32, C.c_uint32]
         lib.sim_sem_make.restype = C.c_uint64
-        lib.sim_sem_make.argtypes = [C.c_int64, C.c_uint64, C.c_uint64]
+        lib.sim_sem_make.argtypes = [C.c_int64, C.c_uint64]
         lib.sim_sem_count.restype = C.c_int64
         lib.sim_sem_count.argtypes = [C.c_uint64]
         lib.sim_sem_expected.restype = C.c_uint64
         lib.sim_sem_expected.argtypes = [C.c_uint64]
-        lib.sim_sem_reserved.restype = C.c_uint64
-        lib.sim_sem_reserved.argtypes = [C.c_uint64]
+        lib.sim_sem_share.restype = C.c_uint32
+        lib.sim_sem_share.argtypes = [C.c_int64, C.c_int64, C.c_int64, C.c_int64]
         lib.sim_locate.restype = C.c_uint32
         lib.sim_locate.argtypes = [C.POINTER(C.c_uint32), C.c_uint32, C.c_uint32]
         _sim = lib
@@ -97,13 +97,18 @@ def locate(bits: np.ndarray, idx: int) -> int:
     return _load_sim().sim_locate(bits.ctypes.data_as(C.POINTER(C.c_uint32)), len(bits), idx)
 
 
-def sem_make(count: int, expected: int, reserved: int) -> int:
-    return _load_sim().sim_sem_make(count, expected, reserved)
+def sem_make(count: int, expected: int) -> int:
+    return _load_sim().sim_sem_make(count, expected)
 
 
-def sem_fields(v: int) -> tuple[int, int, int]:
+def sem_fields(v: int) -> tuple[int, int]:
     lib = _load_sim()
-    return lib.sim_sem_count(v), lib.sim_sem_expected(v), lib.sim_sem_reserved(v)
+    return lib.sim_sem_count(v), lib.sim_sem_expected(v)
+
+
+def sem_share(count: int, expected: int, num: int, ppc: int) -> int:
+    """Chunks a claim of `num` pages carves when the class word holds (count, expected)."""
+    return _load_sim().sim_sem_share(count, expected, num, ppc)
 
 
 class Sim:

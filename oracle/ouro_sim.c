@@ -21,9 +21,11 @@
 #define REUSE_ENTRIES 65536u  /* chunk_queue_size (Parameters.h:10) */
 #define QC_SPOTS 2048u        /* QueueChunk::num_spots_ (QueueChunk.cuh:27) */
 #define OLD_HOLD 10u          /* LARGEST_OLD_COUNT_VALUE (Parameters.h:13) */
+#define QC_RESERVE 32u        /* pool chunks only queue chunks may take */
 #define STATE_BYTES 8192u
 #define BLOCK 4096u
 #define PENDING 0xFFFFFFFEu
+#define NULLPAGE 0xFFFFFFFCu  /* page-queue entry for a page the pool could not supply */
 #define TOMB 0xFFFFFFFDu      /* chunk-queue slot whose chunk was removed */
 #define PEEK 8u               /* slots looked at per scan step */
 #define COMPACT_STEPS 256u    /* front moves per compaction call */
@@ -54,7 +56,8 @@ typedef struct {
 typedef struct {
   int64_t cnt;
   int64_t exp, res;
-  uint32_t front, back, carved, old_count, front_qc, back_qc, old_qc, scan;
+  uint32_t front, back, carved, old_count, front_qc, back_qc, old_qc;
+  uint64_t scan; /* chunk queues: page cursor */
   int32_t count;
   uint32_t page_size, ppc, ring_size;
   uint32_t** blocks; /* std ring, allocated in 4096-entry blocks */
@@ -77,7 +80,7 @@ typedef struct {
 struct ouro_sim {
   int variant;
   uint64_t max_chunks, alloc_size;
-  uint32_t next_free, error;
+  uint32_t next_free, error, qc_reserve, qc_next;
   uint32_t* locator;
   chunk_t* chunks;
   inst_t inst[2];
@@ -94,13 +97,22 @@ int sim_class_of(uint32_t size, uint32_t smallest) {
 }
 static uint32_t make_index(uint32_t c, uint32_t p) { return (c << PAGE_BITS) + p; }
 
-/* BulkSemaphore (BulkSemaphore.cuh:5-84) with the widened 26/19/19 split. */
-uint64_t sim_sem_make(int64_t count, uint64_t expected, uint64_t reserved) {
-  return (uint64_t)(count + (1ll << 25)) | (expected << 26) | (reserved << 45);
+/* BulkSemaphore word (BulkSemaphore.cuh:5-84) as the device packs it: count in the low 40 bits with a
+ * bias of 2^39, expected (pages of promised carves) in the high 24 bits, no reserved field. */
+uint64_t sim_sem_make(int64_t count, uint64_t expected) {
+  return (uint64_t)(count + (1ll << 39)) + (expected << 40);
 }
-int64_t sim_sem_count(uint64_t v) { return (int64_t)(v & ((1ull << 26) - 1)) - (1ll << 25); }
-uint64_t sim_sem_expected(uint64_t v) { return (v >> 26) & ((1ull << 19) - 1); }
-uint64_t sim_sem_reserved(uint64_t v) { return (v >> 45) & ((1ull << 19) - 1); }
+int64_t sim_sem_count(uint64_t v) { return (int64_t)(v % (1ull << 40)) - (1ll << 39); }
+uint64_t sim_sem_expected(uint64_t v) { return v >> 40; }
+
+/* The carve decision of a claim (BulkSemaphore_impl.cuh:236-263 as the device restates it): a claim of
+ * num pages moves the class debt, claims - (published + promised pages), from d to d + num and carves
+ * one chunk per multiple of ppc that the move crosses above zero. */
+static int64_t chunks_for_debt(int64_t d, int64_t ppc) { return d <= 0 ? 0 : (d + ppc - 1) / ppc; }
+uint32_t sim_sem_share(int64_t count, int64_t expected, int64_t num, int64_t ppc) {
+  const int64_t d = -(count + expected);
+  return (uint32_t)(chunks_for_debt(d + num, ppc) - chunks_for_debt(d, ppc));
+}
 
 /* ChunkLocator::getChunkIndex (ChunkLocator.cuh:29-51). */
 uint32_t sim_locate(const uint32_t* bits, uint32_t nwords, uint32_t idx) {
@@ -172,20 +184,31 @@ static uint32_t ring_take(reuse_t* r) {
 }
 
 /* allocateChunk<QUEUECHUNK> (Ouroboros.cuh:108-136) for k chunks: reuse ring first, then bump.
- * Validity follows the reference's predicate chunk + add < maxChunks (:133). Returns the valid count. */
-static uint32_t alloc_chunks(ouro_sim* s, reuse_t* r, uint32_t add, uint32_t k, uint32_t* out) {
+ * Validity follows the reference's predicate chunk + add < maxChunks (:133) below the queue-chunk
+ * reserve (the top qc_reserve chunks of a virtualized pool); queue chunks that find the bump region
+ * spent continue in the reserve. Returns the valid count. */
+static uint32_t alloc_chunks(ouro_sim* s, reuse_t* r, uint32_t add, uint32_t k, uint32_t* out, int queue) {
+  /* data carves leave the last qc_reserve / 2 recycled chunks of a virtualized pool to queue chunks */
+  const int32_t keep = queue ? 0 : (int32_t)(s->qc_reserve / 2);
   uint32_t t = 0;
-  while (t < k && r->count > 0) out[t++] = ring_take(r);
+  while (t < k && r->count > keep) out[t++] = ring_take(r);
   uint32_t ok = 0;
   for (uint32_t j = 0; j < t; ++j)
     if (chunk_ok(s, out[j])) ++ok;
+  const uint64_t limit = s->max_chunks - s->qc_reserve;
   if (k > t) {
     const uint32_t base = s->next_free;
     s->next_free += (k - t) * add;
     for (uint32_t j = t; j < k; ++j) {
-      const uint32_t c = base + (j - t) * add;
+      uint32_t c = base + (j - t) * add;
+      int valid = (uint64_t)c + add < limit;
+      if (!valid && queue) {
+        const uint32_t idx = s->qc_next++;
+        c = (uint32_t)limit + idx;
+        valid = idx < s->qc_reserve && c < s->max_chunks;
+      }
       out[j] = c;
-      if ((uint64_t)c + add < s->max_chunks) {
+      if (valid) {
         s->locator[c >> 5] |= 1u << (c & 31u);
         ++ok;
       }
@@ -299,7 +322,7 @@ static void put_lanes(ouro_sim* s, inst_t* I, cls_t** qs, int n, const uint32_t*
   }
   if (nneed) {
     uint32_t got[64];
-    const uint32_t ok = alloc_chunks(s, &s->inst[0].reuse, 1, (uint32_t)nneed, got);
+    const uint32_t ok = alloc_chunks(s, &s->inst[0].reuse, 1, (uint32_t)nneed, got, 1);
     for (int j = 0; j < nneed; ++j)
       if ((uint32_t)j < ok) {
         qc[need[j]] = got[j];
@@ -586,7 +609,7 @@ static void enqueue_chunks(ouro_sim* s, inst_t* I, cls_t** qs, int n, const uint
 /* PageQueue allocation lambda + enqueueChunk (PageQueue_impl.cuh:108-133, :58-79), k chunks. */
 static uint32_t carve(ouro_sim* s, inst_t* I, cls_t* q, uint32_t k) {
   uint32_t chunks[64];
-  uint32_t ok = alloc_chunks(s, &I->reuse, I->add, k, chunks);
+  uint32_t ok = alloc_chunks(s, &I->reuse, I->add, k, chunks, 0);
   const uint32_t unit = is_chunk_kind(I->kind) ? 1u : q->ppc;
   if (ok) {
     if ((uint64_t)q->carved + (uint64_t)ok * unit > capacity(I->kind, q)) {
@@ -618,7 +641,8 @@ static uint32_t carve(ouro_sim* s, inst_t* I, cls_t* q, uint32_t k) {
     for (uint32_t j = 0; j < ok; ++j) qs[j] = q;
     enqueue_chunks(s, I, qs, (int)ok, chunks);
   } else {
-    const uint32_t n = ok * q->ppc;
+    /* carved pages, then one null entry per page of a chunk the pool could not supply */
+    const uint32_t n = k * q->ppc, n_real = ok * q->ppc;
     const uint32_t v0 = q->back;
     q->back += n;
     /* the device writes in batches of up to 64 positions; queue-chunk allocation order is the
@@ -630,13 +654,14 @@ static uint32_t carve(ouro_sim* s, inst_t* I, cls_t* q, uint32_t k) {
       for (uint32_t e = e0; e < n && e < e0 + 64; ++e, ++m) {
         qs[m] = q;
         v[m] = v0 + e;
-        val[m] = make_index(chunks[e / q->ppc], e % q->ppc);
+        val[m] = e < n_real ? make_index(chunks[e / q->ppc], e % q->ppc) : NULLPAGE;
       }
       put_lanes(s, I, qs, m, v, val);
     }
   }
-  /* signalExpected (BulkSemaphore_impl.cuh:305-308) after publication */
-  q->cnt += (int64_t)ok * (is_chunk_kind(I->kind) ? q->ppc : q->ppc);
+  /* signalExpected (BulkSemaphore_impl.cuh:305-308) after publication; page queues publish every
+     promised page (nulls included), chunk queues only the carved ones */
+  q->cnt += (int64_t)(is_chunk_kind(I->kind) ? ok : k) * q->ppc;
   q->exp -= (int64_t)k * q->ppc;
   return ok;
 }
@@ -667,51 +692,21 @@ static int make_groups(const int* lanes, int n, const uint32_t* cls, group_t* g)
 
 /* BulkSemaphore::wait (BulkSemaphore_impl.cuh:199-300) for every group leader of one wave: each
  * pass decides for all pending leaders, then carves run in leader order. */
+/* Claims in lane order of the group leaders (one fetch-add each on the device, on different words),
+ * then the promised carves, one group at a time in lane order. */
 static void acquire(ouro_sim* s, inst_t* I, group_t* g, int ng, int* ok, int nact) {
-  int pending[64];
+  (void)nact;
+  uint32_t carve_k[64];
   for (int j = 0; j < ng; ++j) {
-    pending[j] = 1;
-    ok[j] = 0;
+    cls_t* q = &I->q[g[j].qi];
+    const int64_t num = g[j].n;
+    carve_k[j] = sim_sem_share(q->cnt, q->exp, num, q->ppc);
+    q->cnt -= num;
+    q->exp += (int64_t)carve_k[j] * q->ppc;
+    ok[j] = 1;
   }
-  for (int pass = 0; pass < 64; ++pass) {
-    uint32_t carve_k[64];
-    int any = 0;
-    for (int j = 0; j < ng; ++j) {
-      carve_k[j] = 0;
-      if (!pending[j]) continue;
-      any = 1;
-      cls_t* q = &I->q[g[j].qi];
-      const int64_t num = g[j].n;
-      if (q->cnt >= num) {
-        q->cnt -= num;
-        pending[j] = 0;
-        ok[j] = 1;
-        continue;
-      }
-      const int64_t c0 = q->cnt > 0 ? q->cnt : 0;
-      if (c0 + q->exp - q->res < num) {
-        int64_t k = (num - (c0 + q->exp - q->res) + q->ppc - 1) / q->ppc;
-        if (k > nact) k = nact;
-        if (k < 1) k = 1;
-        q->exp += k * q->ppc;
-        carve_k[j] = (uint32_t)k;
-      } else if (q->cnt >= num) {
-        q->cnt -= num;
-        pending[j] = 0;
-        ok[j] = 1;
-      } else {
-        /* reserve: impossible inside a single wavefront, as no carve is in flight */
-        s->error |= ERR_TIMEOUT;
-        pending[j] = 0;
-      }
-    }
-    if (!any) break;
-    for (int j = 0; j < ng; ++j)
-      if (carve_k[j]) {
-        const uint32_t got = carve(s, I, &I->q[g[j].qi], carve_k[j]);
-        if (got < carve_k[j]) pending[j] = 0;
-      }
-  }
+  for (int j = 0; j < ng; ++j)
+    if (carve_k[j]) carve(s, I, &I->q[g[j].qi], carve_k[j]);
 }
 
 static uint64_t page_offset(const ouro_sim* s, uint32_t c, uint32_t page, uint32_t page_size) {
@@ -725,18 +720,18 @@ static uint32_t mask_words(uint32_t ppc) {
 }
 
 /* Group claim for chunk-based queues: the leader starts at a spread position of the live queue
- * window (per-class scan cursor) and looks at PEEK slots per step for a chunk queued exactly there
+ * window (per-class page cursor: position = cursor / ppc) and looks at PEEK slots per step for a chunk queued exactly there
  * with free pages; it reserves pages on the chunk count and takes them from the 64-bit mask words,
  * lowest bits first, starting at word (size - count) / 64. The taker of a chunk's last page removes
  * the chunk wherever it sits. (ChunkQueue::allocPage ChunkQueue_impl.cuh:88-151 and
  * ChunkAccess::allocPage ChunkAccess_impl.cuh:38-135, as the device's W = 64 rule.) */
 static void claim_group(ouro_sim* s, inst_t* I, group_t* g, uint64_t* out, uint32_t* reclaim) {
   cls_t* q = &I->q[g->qi];
-  const uint32_t spread = q->scan;
-  q->scan += ((uint32_t)g->n + q->ppc - 1) / q->ppc;
-  const uint32_t len = q->back - q->front;
-  uint32_t v = (len == 0 || len >= 0x80000000u) ? q->front : q->front + spread % len;
-  uint32_t cursor = MARKER, need = (uint32_t)g->n, assigned = 0;
+  const uint32_t h = (uint32_t)(q->scan / q->ppc);
+  q->scan += (uint64_t)g->n;
+  const uint32_t f = q->front, len = q->back - q->front;
+  uint32_t v = (len == 0 || len >= 0x80000000u) ? f : (h - f < len ? h : f + (h - f) % len);
+  uint32_t cursor = MARKER, need = (uint32_t)g->n, assigned = 0, wraps = 0;
   int guard = 0;
   *reclaim = MARKER;
   while (assigned < (uint32_t)g->n) {
@@ -748,12 +743,17 @@ static void claim_group(ouro_sim* s, inst_t* I, group_t* g, uint64_t* out, uint3
     if (v - b < 0x80000000u || v - q->front >= 0x80000000u) { /* at or past back, or behind front */
       v = q->front;
       cursor = MARKER;
+      ++wraps;
     }
     uint32_t n = b - v;
     if (n >= 0x80000000u) n = 0;
     if (n > PEEK) n = PEEK;
     if (!is_std(I->kind) && n > QC_SPOTS - v % QC_SPOTS) n = QC_SPOTS - v % QC_SPOTS;
-    if (n == 0) { /* granted pages but an empty window: cannot happen sequentially */
+    if ((wraps >= 2 || n == 0) && (s->error & (ERR_OOM | ERR_QUEUE_FULL))) {
+      q->cnt += need; /* the pool ran out: the claim goes back, unserved lanes get nullptr */
+      return;
+    }
+    if (n == 0) { /* claimed pages but an empty window: cannot happen sequentially */
       s->error |= ERR_TIMEOUT;
       return;
     }
@@ -843,8 +843,8 @@ static void inst_malloc(ouro_sim* s, inst_t* I, const int* lanes, int n, const u
     take_lanes(s, I, qs, m, v, got);
     for (int i = 0; i < m; ++i) {
       const uint32_t c = got[i] >> PAGE_BITS, p = got[i] & 511u;
-      if (got[i] == MARKER || !chunk_ok(s, c) || p >= qs[i]->ppc) {
-        if (got[i] != MARKER) s->error |= ERR_CORRUPT;
+      if (got[i] == MARKER || got[i] == NULLPAGE || !chunk_ok(s, c) || p >= qs[i]->ppc) {
+        if (got[i] != MARKER && got[i] != NULLPAGE) s->error |= ERR_CORRUPT;
         continue;
       }
       out[owner[i]] = page_offset(s, c, p, qs[i]->page_size);
@@ -992,6 +992,8 @@ ouro_sim* sim_create(int variant, uint64_t pool_bytes) {
   ouro_sim* s = (ouro_sim*)calloc(1, sizeof(ouro_sim));
   s->variant = variant;
   sim_layout(variant, pool_bytes, &s->max_chunks, &s->alloc_size);
+  /* a virtualized pool keeps its top chunks for queue chunks (at most a quarter of a tiny pool) */
+  s->qc_reserve = is_std(variant) ? 0u : (uint32_t)(QC_RESERVE < s->max_chunks / 4 ? QC_RESERVE : s->max_chunks / 4);
   s->chunks = (chunk_t*)calloc(s->max_chunks + 4, sizeof(chunk_t));
   s->locator = (uint32_t*)calloc((s->max_chunks + 31) / 32 + 1, 4);
   s->ninst = 2;

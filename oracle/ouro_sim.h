@@ -59,11 +59,12 @@ uint32_t sim_queue_window(const ouro_sim* s, uint32_t k, uint32_t* out, uint32_t
 /* Class index / page size / pages-per-chunk closed forms (Helper.cuh:42-65) for a request size. */
 int sim_class_of(uint32_t size, uint32_t smallest);
 
-/* BulkSemaphore packed-word arithmetic with the 26/19/19 split used on the device. */
-uint64_t sim_sem_make(int64_t count, uint64_t expected, uint64_t reserved);
+/* BulkSemaphore packed-word arithmetic with the 40/24 split used on the device. */
+uint64_t sim_sem_make(int64_t count, uint64_t expected);
 int64_t sim_sem_count(uint64_t v);
 uint64_t sim_sem_expected(uint64_t v);
-uint64_t sim_sem_reserved(uint64_t v);
+/* Chunks a claim of num pages carves when the class word holds (count, expected). */
+uint32_t sim_sem_share(int64_t count, int64_t expected, int64_t num, int64_t ppc);
 
 /* ChunkLocator::getChunkIndex (ChunkLocator.cuh:29-51) over a caller-supplied bitmap. */
 uint32_t sim_locate(const uint32_t* bits, uint32_t nwords, uint32_t idx);

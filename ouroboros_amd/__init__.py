@@ -134,8 +134,8 @@ SIGNATURES = {
     "ouro_snapshot": (C.c_int, [C.c_void_p, C.c_void_p]),
     "ouro_run_trace": (C.c_int, [C.c_void_p, C.POINTER(TraceConfig), C.POINTER(TraceResult)]),
     "ouro_gen_sizes": (C.c_int, [C.c_uint64, C.c_uint32, C.c_int, C.c_double, C.c_uint32, C.POINTER(C.c_uint32)]),
-    "ouro_sem_decide": (C.c_int, [C.c_uint64, C.c_int64, C.c_int64, C.c_int64, C.POINTER(C.c_int),
-                                  C.POINTER(C.c_uint32), C.POINTER(C.c_uint64)]),
+    "ouro_sem_share": (C.c_int, [C.c_uint64, C.c_int64, C.c_int64, C.POINTER(C.c_uint32)]),
+    "ouro_sem_make": (C.c_uint64, [C.c_int64, C.c_uint64]),
     "ouro_bench_atomics": (C.c_int, [C.c_uint32, C.c_uint32, C.c_int, C.POINTER(C.c_double)]),
     "ouro_bench_copy": (C.c_int, [C.c_uint64, C.c_uint32, C.POINTER(C.c_double)]),
 }
@@ -185,11 +185,16 @@ def gen_sizes(seed: int, n: int, dist: str = "uniform", param: float = 0.0, clas
     return out
 
 
-def sem_decide(word: int, num: int, ppc: int, max_k: int) -> tuple[int, int, int]:
-    """The device semaphore decision on the host: (mode, k, next word)."""
-    m, k, nxt = C.c_int(), C.c_uint32(), C.c_uint64()
-    _check(lib().ouro_sem_decide(word, num, ppc, max_k, C.byref(m), C.byref(k), C.byref(nxt)), "ouro_sem_decide")
-    return m.value, k.value, nxt.value
+def sem_share(before: int, num: int, ppc: int) -> int:
+    """Chunks the device makes a claim of `num` pages carve, given the semaphore word it displaced."""
+    k = C.c_uint32()
+    _check(lib().ouro_sem_share(before, num, ppc, C.byref(k)), "ouro_sem_share")
+    return k.value
+
+
+def sem_make(count: int, expected: int) -> int:
+    """The device's packing of (count, expected) into the semaphore word."""
+    return lib().ouro_sem_make(count, expected)
 
 
 def bench_atomics(threads: int = 1 << 20, iters: int = 64, pattern: str = "distinct") -> float:

@@ -179,14 +179,13 @@ int ouro_diag_read(ouro_handle h, unsigned long long* out16, int reset) {
 }
 #endif
 
-int ouro_sem_decide(uint64_t word, int64_t num, int64_t ppc, int64_t max_k, int* mode, uint32_t* k, uint64_t* next) {
-  if (!mode || !k || !next || num < 1 || ppc < 1) return OURO_E_INVALID;
-  const ouro::SemDecision d = ouro::sem_decide(word, num, ppc, max_k);
-  *mode = (int)d.mode;
-  *k = d.k;
-  *next = d.next;
+int ouro_sem_share(uint64_t before, int64_t num, int64_t ppc, uint32_t* k) {
+  if (!k || num < 0 || ppc < 1 || (ppc & (ppc - 1))) return OURO_E_INVALID;
+  *k = ouro::sem_share(before, num, ppc);
   return OURO_OK;
 }
+
+uint64_t ouro_sem_make(int64_t count, uint64_t expected) { return ouro::Sem::make(count, expected); }
 
 int ouro_bench_atomics(uint32_t threads, uint32_t iters, int pattern, double* ops_per_s) {
   if (!ops_per_s || threads == 0 || threads % 256 || iters == 0 || pattern < 0 || pattern > 2) return OURO_E_INVALID;

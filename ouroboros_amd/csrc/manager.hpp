@@ -147,7 +147,7 @@ struct Manager final : IManager {
       out->pages_per_chunk[k] = q.ppc;
       out->sem_count[k] = ouro::Sem::count(q.sem);
       out->sem_expected[k] = (uint32_t)ouro::Sem::expected(q.sem);
-      out->sem_reserved[k] = (uint32_t)ouro::Sem::reserved(q.sem);
+      out->sem_reserved[k] = 0;  // no reserved field: claims wait on the queue, not the word
       out->front[k] = q.front;
       out->back[k] = q.back;
       out->queue_count[k] = q.count;

@@ -87,3 +87,25 @@ def test_config1_full_concurrent():
         assert out.result["peak_chunks"] == 16  # 8,192 x 16 B = 16 chunks of 512 pages
         chk = mm.check_quiescent()
         assert chk.violations == 0, chk.message.decode()
+
+
+@pytest.mark.parametrize("kind", ALL)
+def test_out_of_memory_concurrent(kind):
+    # 16,384 threads x 8 KiB against a pool of a few thousand chunks: the pool runs out mid-round.
+    # Requests beyond it get nullptr (no fault, no time-out), the OOM bit is set, the blocks handed
+    # out never overlap and read back intact, and once everything is freed the pool is quiescent.
+    print(f"oom {kind}", flush=True)
+    pool = (400 << 20) if kind in ("PQ", "CQ") else (24 << 20)
+    mc, _ = oracle.layout(kind, pool)
+    n = 16384
+    with ob.MemoryManager(kind, pool) as mm:
+        out = mm.run_trace(np.full(n, 8192, np.uint32), 2, agg=64, check_rounds=True)
+        r = out.result
+        assert r["verify_errors"] == 0 and r["bounds_errors"] == 0
+        assert r["overlap_errors"] == 0 and r["align_errors"] == 0
+        assert "OOM" in r["errors"] and "TIMEOUT" not in r["errors"], r["errors"]
+        served = 2 * n - r["null_returns"]
+        assert 0 < served <= 2 * (mc - 1)
+        chk = mm.check_quiescent()
+        # the only violation allowed is the sticky error word itself
+        assert chk.violations == 1 and "error word" in chk.message.decode(), chk.message.decode()
