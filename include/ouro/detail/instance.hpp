@@ -28,9 +28,17 @@ struct Instance {
   static constexpr uint32_t kAdd = kChunkSize / kBaseChunk;         // ChunkAddFactor_ (:62)
   static constexpr bool kIsBase = kAdd == 1;
   static constexpr bool kChunks = is_chunk_kind(K);
+  // entries of each class's queue array (standard: slots; VA: queue-chunk pointers; VL: none)
+  static constexpr uint32_t kQueueEntries = is_std(K) ? kStdEntries : (is_va(K) ? kVAEntries : 0u);
   static_assert(kChunkSize % kBaseChunk == 0, "instance chunk must be a multiple of the base chunk");
   static_assert(kChunkSize / SMALLEST <= 512, "at most 512 pages per chunk (9 page bits)");
+  static_assert(NQ <= 16, "at most 16 classes per instance");
 
+  // Read-only after initialize: the class queue arrays. Kept off the lines of the class counters,
+  // whose atomics slow down when plain loads of the same line keep arriving.
+  struct alignas(128) Table {
+    uint32_t* ring[16];
+  } tab;
   ReuseRing reuse;   // d_chunk_reuse_queue
   ClassQueue q[NQ];  // d_storage_reuse_queue
 };
@@ -211,6 +219,24 @@ struct Ctx {
   ReuseRing& base_ring;  // d_base_chunk_reuse_queue: queue chunks always come from the base ring
 };
 
+// Class geometry from the queue's index in its instance (compile-time sizes, no memory access).
+template <class I>
+__device__ __forceinline__ uint32_t class_index(Ctx<I>& x, const ClassQueue* cq) {
+  return (uint32_t)(cq - x.inst.q);
+}
+template <class I>
+__device__ __forceinline__ uint32_t ppc_of(Ctx<I>& x, const ClassQueue* cq) {
+  return (I::kChunkSize >> I::kSmallLog2) >> class_index(x, cq);
+}
+template <class I>
+__device__ __forceinline__ uint32_t page_size_of(Ctx<I>& x, const ClassQueue* cq) {
+  return I::kSmallest << class_index(x, cq);
+}
+template <class I>
+__device__ __forceinline__ uint32_t* ring_of(Ctx<I>& x, const ClassQueue* cq) {
+  return x.inst.tab.ring[class_index(x, cq)];
+}
+
 // ---- VL pointer maintenance (QueueChunk_impl.cuh:440-551)
 template <class I>
 __device__ inline void vl_set_back(Ctx<I>& x, ClassQueue* cq, uint32_t c) {
@@ -321,10 +347,10 @@ __device__ __forceinline__ bool is_qc_tag(uint32_t e) { return (e >> 21) == 0x40
 // VA: queue chunk that holds virtual position v (accessQueueElement, PageQueueVA_impl.cuh:231-248).
 template <class I>
 __device__ inline uint32_t va_wait_qc(Ctx<I>& x, ClassQueue* cq, uint32_t v) {
-  const uint32_t id = (v / kQCSpots) & (cq->ring_size - 1);
+  const uint32_t id = (v / kQCSpots) & (I::kQueueEntries - 1);
   uint32_t qc;
   Spin it(&x.m.error);
-  while ((qc = ld(&cq->ring[id])) == kMarker || is_qc_tag(qc)) {
+  while ((qc = ld(&ring_of(x, cq)[id])) == kMarker || is_qc_tag(qc)) {
     if (it.expired()) { set_error(x.m, ERR_TIMEOUT); return kMarker; }
     backoff(it.it);
   }
@@ -352,9 +378,9 @@ __device__ inline void compact(Ctx<I>& x, ClassQueue* cq) {
   for (uint32_t i = 0; i < kCompactSteps && f != b && b - f < 0x80000000u; ++i) {
     uint32_t nf = f + 1u;
     if constexpr (is_std(K)) {
-      if (ld(&cq->ring[f & (cq->ring_size - 1)]) != kTomb) return;
+      if (ld(&ring_of(x, cq)[f & (I::kQueueEntries - 1)]) != kTomb) return;
     } else if constexpr (is_va(K)) {
-      const uint32_t e = ld(&cq->ring[(f / kQCSpots) & (cq->ring_size - 1)]);
+      const uint32_t e = ld(&ring_of(x, cq)[(f / kQCSpots) & (I::kQueueEntries - 1)]);
       if (e == qc_tag(f / kQCSpots)) {
         nf = (f / kQCSpots + 1u) * kQCSpots;
       } else if (!chunk_ok(x.m, e) || ld(&hdr<QueueChunkHdr>(x.m, e)->virtual_start) / kQCSpots != f / kQCSpots ||
@@ -374,7 +400,7 @@ __device__ inline void compact(Ctx<I>& x, ClassQueue* cq) {
     }
     if (cas(&cq->front, f, nf) != f) return;
     // the ring slot becomes writable again for position f + ring_size
-    if constexpr (is_std(K)) cas(&cq->ring[f & (cq->ring_size - 1)], kTomb, kMarker);
+    if constexpr (is_std(K)) cas(&ring_of(x, cq)[f & (I::kQueueEntries - 1)], kTomb, kMarker);
     f = nf;
   }
 }
@@ -394,7 +420,7 @@ __device__ inline void put_lanes(Ctx<I>& x, ClassQueue* cq, bool want, uint32_t 
   constexpr Kind K = I::kKind;
   if constexpr (is_std(K)) {
     if (want) {
-      uint32_t* slot = &cq->ring[v & (cq->ring_size - 1)];
+      uint32_t* slot = &ring_of(x, cq)[v & (I::kQueueEntries - 1)];
       Spin it(&x.m.error);
       while (cas(slot, kMarker, val) != kMarker) {
         // a chunk queue's slot may still hold the tombstone of position v - ring_size
@@ -423,7 +449,7 @@ __device__ inline void put_lanes(Ctx<I>& x, ClassQueue* cq, bool want, uint32_t 
         if (have) {
           // the entry still names the queue chunk ring_size positions back: publish only over a
           // marker or a reclaimed tag, never over a queue chunk with live entries
-          uint32_t* e = &cq->ring[((v / kQCSpots) + 1) & (cq->ring_size - 1)];
+          uint32_t* e = &ring_of(x, cq)[((v / kQCSpots) + 1) & (I::kQueueEntries - 1)];
           uint32_t cur = ld(e);
           bool placed = false;
           while (!placed && (cur == kMarker || is_qc_tag(cur))) {
@@ -449,7 +475,7 @@ __device__ inline void put_lanes(Ctx<I>& x, ClassQueue* cq, bool want, uint32_t 
           }
           const uint32_t old = add(&hdr<QueueChunkHdr>(x.m, c)->count, 0x10001u);
           if (qc_empty_after_enqueue(old, 0x10001u)) {
-            victim = xchg(&cq->ring[(v / kQCSpots) & (cq->ring_size - 1)], qc_tag(v / kQCSpots));
+            victim = xchg(&ring_of(x, cq)[(v / kQCSpots) & (I::kQueueEntries - 1)], qc_tag(v / kQCSpots));
             reclaim = chunk_ok(x.m, victim);
           }
         }
@@ -507,7 +533,7 @@ __device__ inline uint32_t take_lanes(Ctx<I>& x, ClassQueue* cq, bool want, uint
   uint32_t val = kMarker;
   if constexpr (is_std(K)) {
     if (want) {
-      uint32_t* slot = &cq->ring[v & (cq->ring_size - 1)];
+      uint32_t* slot = &ring_of(x, cq)[v & (I::kQueueEntries - 1)];
       Spin it(&x.m.error);
       while ((val = xchg(slot, kMarker)) == kMarker) {
         if (it.expired()) { set_error(x.m, ERR_TIMEOUT); break; }
@@ -528,7 +554,7 @@ __device__ inline uint32_t take_lanes(Ctx<I>& x, ClassQueue* cq, bool want, uint
         }
         const uint32_t old = sub(&hdr<QueueChunkHdr>(x.m, c)->count, 1u << 16);
         if (qc_empty_after_dequeue(old)) {
-          victim = xchg(&cq->ring[(v / kQCSpots) & (cq->ring_size - 1)], qc_tag(v / kQCSpots));
+          victim = xchg(&ring_of(x, cq)[(v / kQCSpots) & (I::kQueueEntries - 1)], qc_tag(v / kQCSpots));
           reclaim = chunk_ok(x.m, victim);
         }
       }
@@ -567,28 +593,35 @@ __device__ inline uint32_t take_lanes(Ctx<I>& x, ClassQueue* cq, bool want, uint
 // CAS loop on the word and parks waiters in `reserved`, which a cold class with thousands of
 // concurrent groups turns into a serial chain of failed CASes.
 template <class I, class Carve>
-__device__ inline void sem_claim(Ctx<I>& x, ClassQueue* cq, uint64_t grp, bool want, Carve&& carve) {
-  const int L = leader_of(grp);
+__device__ inline void sem_claim_n(Ctx<I>& x, ClassQueue* cq, bool lead, uint32_t num, Carve&& carve) {
   uint32_t k = 0;
-  if (want && (int)lane() == L) {
+  if (lead) {
     OURO_DIAG_ADD(12, 1);
-    const int64_t num = (int64_t)popc(grp);
-    const int64_t ppc = cq->ppc;
-    const uint64_t before = add(&cq->sem, (uint64_t)(-num));
-    k = sem_share(before, num, ppc);
+    const uint64_t before = add(&cq->sem, (uint64_t)(-(int64_t)num));
+    // pages per chunk only matter in debt; loading it unconditionally puts a plain read of the
+    // semaphore's cache line next to every claim
+    if (sem_debt(before) + (int64_t)num > 0) k = sem_share(before, (int64_t)num, ppc_of(x, cq));
     if (k) {
       OURO_DIAG_ADD(13, k);
       // a wrapped expected field would only make later claims over-carve, never under-carve
-      add(&cq->sem, ((uint64_t)k * (uint64_t)ppc) << Sem::kExpShift);
+      add(&cq->sem, ((uint64_t)k * (uint64_t)ppc_of(x, cq)) << Sem::kExpShift);
     }
   }
   uint64_t carvers = ballot(k > 0);
+  const uint32_t nact = popc(active());
   while (carvers) {
     const int l = first(carvers);
     carvers &= carvers - 1;
     ClassQueue* lq = reinterpret_cast<ClassQueue*>(readlane64(reinterpret_cast<uint64_t>(cq), l));
-    carve(lq, readlane(k, l));
+    const uint32_t kl = readlane(k, l);
+    // a carve hands one chunk to each active lane: larger promises go in batches
+    for (uint32_t d = 0; d < kl; d += nact) carve(lq, kl - d < nact ? kl - d : nact);
   }
+}
+
+template <class I, class Carve>
+__device__ inline void sem_claim(Ctx<I>& x, ClassQueue* cq, uint64_t grp, bool want, Carve&& carve) {
+  sem_claim_n(x, cq, want && (int)lane() == leader_of(grp), popc(grp), carve);
 }
 
 // Chunk-queue claims that cannot be served because the pool ran out give their pages back.
@@ -623,15 +656,15 @@ __device__ inline uint32_t carve_pages(Ctx<I>& x, ClassQueue* cq, uint32_t k) {
   const uint64_t act = active();
   const uint32_t me = rank_in(act), nact = popc(act);
   const int l0 = first(act);
-  const uint32_t ppc = cq->ppc, psize = cq->page_size;
+  const uint32_t ppc = ppc_of(x, cq), psize = page_size_of(x, cq);
   uint32_t chunk = kMarker;
   uint32_t k_ok = alloc_chunks(x.m, x.inst.reuse, I::kAdd, k, chunk);
   // capacity guard: never carve more pages into a class than its queue can hold
   uint32_t cap_ok = 1;
   if ((int)lane() == l0 && k_ok) {
     const uint64_t cap = is_vl(I::kKind) ? 0xFFFFFFFFull
-                         : is_va(I::kKind) ? (uint64_t)cq->ring_size * kQCSpots - 2ull * kQCSpots
-                                            : (uint64_t)cq->ring_size;
+                         : is_va(I::kKind) ? (uint64_t)I::kQueueEntries * kQCSpots - 2ull * kQCSpots
+                                            : (uint64_t)I::kQueueEntries;
     const uint32_t before = add(&cq->carved, k_ok * ppc);
     if ((uint64_t)before + (uint64_t)k_ok * ppc > cap) {
       sub(&cq->carved, k_ok * ppc);
@@ -678,11 +711,11 @@ __device__ inline void* alloc_pages(Ctx<I>& x, uint32_t qi, bool want) {
   const uint32_t v = fetch_positions<GS>(&cq->front, want, n);
   const uint32_t idx = take_lanes(x, cq, want, v);
   if (!want || idx == kMarker || idx == kNullPage) return nullptr;
-  if (!chunk_ok(x.m, index_chunk(idx)) || index_page(idx) >= cq->ppc) {
+  if (!chunk_ok(x.m, index_chunk(idx)) || index_page(idx) >= ppc_of(x, cq)) {
     set_error(x.m, ERR_CORRUPT);
     return nullptr;
   }
-  return chunk_addr(x.m, index_chunk(idx)) + kMeta + (uint64_t)index_page(idx) * cq->page_size;
+  return chunk_addr(x.m, index_chunk(idx)) + kMeta + (uint64_t)index_page(idx) * page_size_of(x, cq);
 }
 
 template <int W, class I>
@@ -729,14 +762,14 @@ __device__ inline uint32_t carve_chunks(Ctx<I>& x, ClassQueue* cq, uint32_t k) {
   const uint64_t act = active();
   const uint32_t me = rank_in(act);
   const int l0 = first(act);
-  const uint32_t ppc = cq->ppc, psize = cq->page_size;
+  const uint32_t ppc = ppc_of(x, cq), psize = page_size_of(x, cq);
   uint32_t chunk = kMarker;
   uint32_t k_ok = alloc_chunks(x.m, x.inst.reuse, I::kAdd, k, chunk);
   uint32_t cap_ok = 1;
   if ((int)lane() == l0 && k_ok) {
     const uint64_t cap = is_vl(I::kKind) ? 0xFFFFFFFFull
-                         : is_va(I::kKind) ? (uint64_t)cq->ring_size * kQCSpots - 2ull * kQCSpots
-                                            : (uint64_t)cq->ring_size;
+                         : is_va(I::kKind) ? (uint64_t)I::kQueueEntries * kQCSpots - 2ull * kQCSpots
+                                            : (uint64_t)I::kQueueEntries;
     const uint32_t before = add(&cq->carved, k_ok);
     if ((uint64_t)before + k_ok > cap) {
       sub(&cq->carved, k_ok);
@@ -781,12 +814,12 @@ __device__ inline void peek_slots(Ctx<I>& x, ClassQueue* cq, uint32_t v, uint32_
   constexpr Kind K = I::kKind;
   if constexpr (is_std(K)) {
 #pragma unroll
-    for (uint32_t i = 0; i < kPeek; ++i) s[i] = i < n ? ld(&cq->ring[(v + i) & (cq->ring_size - 1)]) : kMarker;
+    for (uint32_t i = 0; i < kPeek; ++i) s[i] = i < n ? ld(&ring_of(x, cq)[(v + i) & (I::kQueueEntries - 1)]) : kMarker;
   } else {
     const uint32_t* sl = nullptr;
     uint32_t fill = kMarker;
     if constexpr (is_va(K)) {
-      const uint32_t e = ld(&cq->ring[(v / kQCSpots) & (cq->ring_size - 1)]);
+      const uint32_t e = ld(&ring_of(x, cq)[(v / kQCSpots) & (I::kQueueEntries - 1)]);
       if (e == qc_tag(v / kQCSpots)) fill = kTomb;
       else if (chunk_ok(x.m, e) && ld(&hdr<QueueChunkHdr>(x.m, e)->virtual_start) / kQCSpots == v / kQCSpots)
         sl = qc_slots(x.m, e) + v % kQCSpots;
@@ -826,7 +859,7 @@ __device__ inline uint32_t find_chunk(Ctx<I>& x, ClassQueue* cq, uint32_t v, uin
     qp[i] = ld(&h->queue_pos);
     ps[i] = ld(&h->page_size);
   }
-  const uint32_t psize = cq->page_size;
+  const uint32_t psize = page_size_of(x, cq);
 #pragma unroll
   for (uint32_t i = 0; i < kPeek; ++i) {
     if (i < n && chunk_ok(x.m, s[i]) && c_[i] > 0 && qp[i] == v + i && ps[i] == psize) {
@@ -846,10 +879,10 @@ template <class I>
 __device__ inline void tomb_slot(Ctx<I>& x, ClassQueue* cq, uint32_t p, uint32_t& cursor, uint32_t& reclaim) {
   constexpr Kind K = I::kKind;
   if constexpr (is_std(K)) {
-    xchg(&cq->ring[p & (cq->ring_size - 1)], kTomb);
+    xchg(&ring_of(x, cq)[p & (I::kQueueEntries - 1)], kTomb);
   } else if constexpr (is_va(K)) {
-    const uint32_t id = (p / kQCSpots) & (cq->ring_size - 1);
-    const uint32_t qc = ld(&cq->ring[id]);
+    const uint32_t id = (p / kQCSpots) & (I::kQueueEntries - 1);
+    const uint32_t qc = ld(&ring_of(x, cq)[id]);
     if (!chunk_ok(x.m, qc) || ld(&hdr<QueueChunkHdr>(x.m, qc)->virtual_start) / kQCSpots != p / kQCSpots) {
       set_error(x.m, ERR_CORRUPT);  // a queued chunk's queue chunk is always published
       return;
@@ -857,7 +890,7 @@ __device__ inline void tomb_slot(Ctx<I>& x, ClassQueue* cq, uint32_t p, uint32_t
     xchg(&qc_slots(x.m, qc)[p % kQCSpots], kTomb);
     const uint32_t old = sub(&hdr<QueueChunkHdr>(x.m, qc)->count, 1u << 16);
     if (qc_empty_after_dequeue(old)) {
-      const uint32_t victim = xchg(&cq->ring[id], qc_tag(p / kQCSpots));
+      const uint32_t victim = xchg(&ring_of(x, cq)[id], qc_tag(p / kQCSpots));
       if (chunk_ok(x.m, victim)) reclaim = victim;
     }
   } else {
@@ -988,6 +1021,111 @@ __device__ __forceinline__ uint32_t batch_at(uint32_t v, uint32_t b) {
   return n;
 }
 
+// One leader's claim of `num` pages from a chunk-based class queue, as a state machine that is
+// stepped once per wave iteration: the leader finds chunks kPeek slots at a time, reserves pages on
+// the chunk count with a CAS and claims them from the chunk's 64-bit mask words. Each step that
+// claims bits reports them as (chunk | word << 29, bits, r0): the pages for ranks r0, r0 + 1, ...
+// of the group, lowest bit first. Where a leader starts comes from a per-class page cursor: it adds
+// its page count and starts at queue position cursor / ppc, so concurrent leaders begin at disjoint
+// chunks, in queue order, and the consumed region stays behind the cursor (the reference starts
+// every thread at front_, ChunkQueue_impl.cuh:101-105). A cursor outside [front_, back_) folds back
+// into the window. (ChunkQueue::allocPage, ChunkQueue_impl.cuh:88-151, and ChunkAccess::allocPage,
+// ChunkAccess_impl.cuh:38-135, for a whole group at once.)
+template <class I>
+struct ChunkClaim {
+  uint32_t num = 0, v = 0, cursor = kMarker, need = 0, assigned = 0, t_rem = 0, w = 0, c = kMarker, wraps = 0;
+  uint32_t reclaim = kMarker;
+  bool gave_up = false;
+  Spin it;
+
+  __device__ ChunkClaim(Ctx<I>& x, ClassQueue* cq, bool leader, uint32_t n) : it(&x.m.error) {
+    if (!leader) return;
+    num = need = n;
+    const uint64_t p = add(&cq->scan, (uint64_t)n);
+    const uint32_t h = (uint32_t)(p >> __builtin_ctz(ppc_of(x, cq)));
+    const uint32_t f = ld(&cq->front), b = ld(&cq->back);
+    const uint32_t len = b - f;
+    v = (len == 0 || len >= 0x80000000u) ? f : (h - f < len ? h : f + (h - f) % len);
+  }
+  __device__ bool finished() const { return assigned >= num || gave_up; }
+
+  // One step of an unfinished leader. Returns bc != kMarker when it claimed pages.
+  __device__ void step(Ctx<I>& x, ClassQueue* cq, uint32_t& bc, uint64_t& bits, uint32_t& r0) {
+    bc = kMarker;
+    bits = 0;
+    r0 = 0;
+    OURO_DIAG_ADD(0, 1);
+    uint32_t n = 0;
+    if (t_rem == 0) {
+      OURO_DIAG_ADD(1, 1);
+      const uint32_t b = ld(&cq->back);
+      wraps += scan_wrap<I>(cq, v, cursor, b);
+      n = batch_at<I>(v, b);
+      if ((wraps >= 2 || n == 0) && pool_exhausted(x.m)) {
+        // the pool ran out and a whole window pass found nothing: unserved ranks get nullptr
+        sem_signal(cq, need);
+        gave_up = true;
+        return;
+      }
+      uint32_t s = kMarker;
+      int32_t cnt = 0;
+      const uint32_t i = n ? find_chunk(x, cq, v, n, cursor, s, cnt) : 0u;
+      if (i < n) {
+        ChunkHdr* h = hdr<ChunkHdr>(x.m, s);
+        uint32_t t = 0;
+        while (cnt > 0) {
+          t = (uint32_t)cnt < need ? (uint32_t)cnt : need;
+          const int32_t seen = cas(&h->count, cnt, cnt - (int32_t)t);
+          if (seen == cnt) break;
+          OURO_DIAG_ADD(3, 1);
+          cnt = seen;
+        }
+        if (cnt > 0) {
+          c = s;
+          t_rem = t;
+          need -= t;
+          const uint32_t size = ld(&h->size);
+          w = ((size - (uint32_t)cnt) / 64u) % mask_words(size);
+          bool again = false;
+          if (cnt == (int32_t)t && cq_remove(x, cq, s, cursor, reclaim, again) && again)
+            ensure_in_queue<1>(x, cq, true, s);
+          v += cnt == (int32_t)t ? i + 1u : i;
+        } else {
+          OURO_DIAG_ADD(4, 1);
+          v += i + 1u;  // drained meanwhile; its last taker removes it
+        }
+      } else {
+        OURO_DIAG_ADD(2, n);
+        v += n;
+        if (n == 0) backoff(it.it);
+      }
+    }
+    if (t_rem > 0) {
+      ChunkHdr* h = hdr<ChunkHdr>(x.m, c);
+      const uint32_t nw = mask_words(ld(&h->size));
+      const uint64_t mw = ld(&h->mask[w]);
+      const uint64_t take = lowest_bits(mw, t_rem);
+      if (take == 0) {
+        w = (w + 1) % nw;
+      } else {
+        const uint64_t old = and_(&h->mask[w], ~take);
+        const uint64_t gotb = old & take;
+        if (gotb != take) OURO_DIAG_ADD(5, 1);
+        bits = gotb;
+        bc = c | (w << 29);  // word index rides in the top bits (chunk < 2^23)
+        r0 = assigned;
+        assigned += popc(gotb);
+        t_rem -= popc(gotb);
+        if (gotb == take && t_rem > 0) w = (w + 1) % nw;
+      }
+    }
+    if (it.expired()) {
+      set_error(x.m, ERR_TIMEOUT);
+      gave_up = true;
+    }
+  }
+};
+
 template <int W, class I>
 __device__ inline void* alloc_chunk_pages(Ctx<I>& x, uint32_t qi, bool want) {
   constexpr int GS = W == 1 ? 64 : W;
@@ -1042,106 +1180,20 @@ __device__ inline void* alloc_chunk_pages(Ctx<I>& x, uint32_t qi, bool want) {
       }
     }
   } else {
-    // group protocol: the leader finds chunks kPeek slots at a time, reserves pages on the chunk
-    // count with a CAS and claims them from the chunk's 64-bit mask words; lanes take them in rank
-    // order. Where a leader starts comes from a per-class page cursor: it adds its page count and
-    // starts at queue position cursor / ppc, so concurrent leaders begin at disjoint chunks, in
-    // queue order, and the consumed region stays behind the cursor (the reference starts every
-    // thread at front_, ChunkQueue_impl.cuh:101-105). A cursor outside [front_, back_) folds back
-    // into the window.
+    // group protocol: the group leader claims for the whole group (ChunkClaim); every step's pages
+    // are broadcast and the lanes take them in rank order
     const int L = first(grp);
     const bool leader = ok && (int)lane() == L;
-    const uint32_t num = popc(grp);
-    uint32_t v = 0, cursor = kMarker;
-    if (leader) {
-      const uint64_t p = add(&cq->scan, (uint64_t)num);
-      const uint32_t h = (uint32_t)(p >> __builtin_ctz(cq->ppc));
-      const uint32_t f = ld(&cq->front), b = ld(&cq->back);
-      const uint32_t len = b - f;
-      v = (len == 0 || len >= 0x80000000u) ? f : (h - f < len ? h : f + (h - f) % len);
-    }
-    uint32_t need = leader ? num : 0u, assigned = 0, t_rem = 0, w = 0, c = kMarker, wraps = 0;
-    Spin it(&x.m.error);
-    bool done = !ok, gave_up = false;
+    ChunkClaim<I> cl(x, cq, leader, popc(grp));
+    bool done = !ok;
     while (ballot(!done)) {
       uint32_t bc = kMarker, r0 = 0;
       uint64_t bits = 0;
-      if (leader && assigned < num && !gave_up) {
-        OURO_DIAG_ADD(0, 1);
-        uint32_t n = 0;
-        if (t_rem == 0) {
-          OURO_DIAG_ADD(1, 1);
-          const uint32_t b = ld(&cq->back);
-          wraps += scan_wrap<I>(cq, v, cursor, b);
-          n = batch_at<I>(v, b);
-          if ((wraps >= 2 || n == 0) && pool_exhausted(x.m)) {
-            // the pool ran out and a whole window pass found nothing: unserved lanes get nullptr
-            sem_signal(cq, need);
-            gave_up = true;
-          }
-        }
-        if (t_rem == 0 && !gave_up) {
-          uint32_t s = kMarker;
-          int32_t cnt = 0;
-          const uint32_t i = n ? find_chunk(x, cq, v, n, cursor, s, cnt) : 0u;
-          if (i < n) {
-            ChunkHdr* h = hdr<ChunkHdr>(x.m, s);
-            uint32_t t = 0;
-            while (cnt > 0) {
-              t = (uint32_t)cnt < need ? (uint32_t)cnt : need;
-              const int32_t seen = cas(&h->count, cnt, cnt - (int32_t)t);
-              if (seen == cnt) break;
-              OURO_DIAG_ADD(3, 1);
-              cnt = seen;
-            }
-            if (cnt > 0) {
-              c = s;
-              t_rem = t;
-              need -= t;
-              const uint32_t size = ld(&h->size);
-              w = ((size - (uint32_t)cnt) / 64u) % mask_words(size);
-              bool again = false;
-              if (cnt == (int32_t)t && cq_remove(x, cq, s, cursor, reclaim, again) && again)
-                ensure_in_queue<1>(x, cq, true, s);
-              v += cnt == (int32_t)t ? i + 1u : i;
-            } else {
-              OURO_DIAG_ADD(4, 1);
-              v += i + 1u;  // drained meanwhile; its last taker removes it
-            }
-          } else {
-            OURO_DIAG_ADD(2, n);
-            v += n;
-            if (n == 0) backoff(it.it);
-          }
-        }
-        if (t_rem > 0) {
-          ChunkHdr* h = hdr<ChunkHdr>(x.m, c);
-          const uint32_t nw = mask_words(ld(&h->size));
-          const uint64_t mw = ld(&h->mask[w]);
-          const uint64_t take = lowest_bits(mw, t_rem);
-          if (take == 0) {
-            w = (w + 1) % nw;
-          } else {
-            const uint64_t old = and_(&h->mask[w], ~take);
-            const uint64_t gotb = old & take;
-            if (gotb != take) OURO_DIAG_ADD(5, 1);
-            bits = gotb;
-            bc = c | (w << 29);  // word index rides in the top bits (chunk < 2^23)
-            r0 = assigned;
-            assigned += popc(gotb);
-            t_rem -= popc(gotb);
-            if (gotb == take && t_rem > 0) w = (w + 1) % nw;
-          }
-        }
-        if (it.expired()) {
-          set_error(x.m, ERR_TIMEOUT);
-          gave_up = true;
-        }
-      }
+      if (leader && !cl.finished()) cl.step(x, cq, bc, bits, r0);
       const uint32_t bcl = shfl(bc, L);
       const uint64_t bl = shfl64(bits, L);
       const uint32_t r0l = shfl(r0, L);
-      const uint32_t gul = shfl((uint32_t)gave_up, L);
+      const uint32_t gul = shfl((uint32_t)cl.gave_up, L);
       if (!done) {
         const uint32_t r = rank_in(grp);
         if (bcl != kMarker && r >= r0l && r < r0l + popc(bl)) {
@@ -1154,10 +1206,11 @@ __device__ inline void* alloc_chunk_pages(Ctx<I>& x, uint32_t qi, bool want) {
         }
       }
     }
+    reclaim = cl.reclaim;
   }
   finish_reclaim(x, cq, true, reclaim);
   if (!got) return nullptr;
-  return chunk_addr(x.m, chunk) + kMeta + (uint64_t)page * cq->page_size;
+  return chunk_addr(x.m, chunk) + kMeta + (uint64_t)page * page_size_of(x, cq);
 }
 
 template <int W, class I>

@@ -21,6 +21,7 @@
 #include "detail/device.hpp"
 #include "detail/layout.hpp"
 #include "detail/instance.hpp"
+#include "detail/block.hpp"
 
 #ifndef OURO_DEFAULT_AGG
 #define OURO_DEFAULT_AGG 64
@@ -123,6 +124,106 @@ __device__ inline void chain_free(Memory& m, Chain<I, Rest...>& ch, ReuseRing& b
   }
 }
 
+// validOuroborosPointer (Ouroboros.cuh:350-356), tightened to the chunk region, and the chunk /
+// page-size lookup of Ouroboros::free (Ouroboros_impl.cuh:95-113). Sets ERR_BAD_FREE on a pointer
+// that is not a block of this pool.
+__device__ inline bool route_free(Memory& m, const void* ptr, uint32_t& chunk, uint32_t& page_size, uint64_t& offset) {
+  const uint8_t* p = reinterpret_cast<const uint8_t*>(ptr);
+  if (p == nullptr) return false;
+  bool want = p >= m.d_data && p < m.d_data + (uint64_t)m.max_chunks * kStride;
+  if (want) {
+    const uint64_t diff = (uint64_t)(p - m.d_data);
+    const uint32_t c = locate_run(m, (uint32_t)(diff / kStride));
+    want = c != kMarker;
+    if (want) {
+      chunk = c;
+      page_size = dev::ld(hdr<uint32_t>(m, c));  // CommonChunk::page_size
+      const uint64_t base = (uint64_t)c * kStride + kMeta;
+      want = diff >= base && page_size != 0;
+      offset = diff - base;
+    }
+  }
+  if (!want) set_error(m, ERR_BAD_FREE);
+  return want;
+}
+
+// ---- block-wide calls over the chain: classes are numbered across instances (offset `off`)
+template <class I, class... Rest>
+__device__ inline uint32_t chain_alloc_class(uint64_t size, uint32_t off = 0) {
+  if (size <= I::kLargest) return off + class_of(size, I::kSmallLog2);
+  if constexpr (sizeof...(Rest) > 0) return chain_alloc_class<Rest...>(size, off + I::kNQ);
+  else return kMarker;
+}
+
+// freePageRecursive (Ouroboros_impl.cuh:117-125): the instance whose page sizes include page_size.
+template <class I, class... Rest>
+__device__ inline uint32_t chain_free_class(uint32_t page_size, uint64_t offset, uint32_t& page, uint32_t off = 0) {
+  if (page_size <= I::kLargest) {
+    const bool ok = page_size >= I::kSmallest && (page_size & (page_size - 1)) == 0 &&
+                    (offset & (page_size - 1)) == 0 && (offset >> log2_floor(page_size)) < I::kChunkSize / page_size;
+    if (!ok) return kMarker;
+    page = (uint32_t)(offset >> log2_floor(page_size));
+    return off + log2_floor(page_size) - I::kSmallLog2;
+  }
+  if constexpr (sizeof...(Rest) > 0) return chain_free_class<Rest...>(page_size, offset, page, off + I::kNQ);
+  else return kMarker;
+}
+
+template <class S, class I, class... Rest>
+__device__ inline void chain_block_claim(Memory& m, Chain<I, Rest...>& ch, ReuseRing& base, S& sm, uint32_t nclasses,
+                                         uint32_t off = 0) {
+  if constexpr (I::kChunks) block_claim_chunks(m, ch.inst, base, off, sm, nclasses);
+  else block_claim_pages(m, ch.inst, base, off, sm);
+  if constexpr (sizeof...(Rest) > 0) chain_block_claim(m, ch.next, base, sm, nclasses, off + I::kNQ);
+}
+
+template <class S, class I, class... Rest>
+__device__ inline void* chain_block_take(Memory& m, Chain<I, Rest...>& ch, ReuseRing& base, S& sm, uint32_t gc,
+                                         uint32_t r, uint32_t off = 0) {
+  const bool here = gc != kMarker && gc >= off && gc < off + I::kNQ;
+  void* p = nullptr;
+  if (dev::ballot(here)) {
+    const uint32_t qi = here ? gc - off : 0u;
+    void* q;
+    if constexpr (I::kChunks) q = block_page_of(m, ch.inst, sm, here, here ? gc : 0u, qi, r);
+    else q = block_take_page(m, ch.inst, base, here, qi, here ? sm.base[gc] + r : 0u);
+    if (here) p = q;
+  }
+  if constexpr (sizeof...(Rest) > 0) {
+    void* q = chain_block_take(m, ch.next, base, sm, gc, r, off + I::kNQ);
+    if (!here && q) p = q;
+  }
+  return p;
+}
+
+template <class S, class I, class... Rest>
+__device__ inline void chain_block_back(Chain<I, Rest...>& ch, S& sm, uint32_t off = 0) {
+  if constexpr (!I::kChunks) block_back(ch.inst, off, sm);
+  if constexpr (sizeof...(Rest) > 0) chain_block_back(ch.next, sm, off + I::kNQ);
+}
+
+template <class S, class I, class... Rest>
+__device__ inline void chain_block_signal(Chain<I, Rest...>& ch, S& sm, uint32_t off = 0) {
+  block_signal(ch.inst, off, sm);
+  if constexpr (sizeof...(Rest) > 0) chain_block_signal(ch.next, sm, off + I::kNQ);
+}
+
+template <class S, class I, class... Rest>
+__device__ inline void chain_block_release(Memory& m, Chain<I, Rest...>& ch, ReuseRing& base, S& sm, uint32_t gc,
+                                           uint32_t r, uint32_t chunk, uint32_t page, uint32_t off = 0) {
+  const bool here = gc != kMarker && gc >= off && gc < off + I::kNQ;
+  if (dev::ballot(here)) {
+    const uint32_t qi = here ? gc - off : 0u;
+    if constexpr (I::kChunks) {
+      block_release_chunks(m, ch.inst, base, here, qi, chunk, page);
+    } else {
+      Ctx<I> x{m, ch.inst, base};
+      put_lanes(x, &ch.inst.q[qi], here, here ? sm.base[gc] + r : 0u, make_index(chunk, page));
+    }
+  }
+  if constexpr (sizeof...(Rest) > 0) chain_block_release(m, ch.next, base, sm, gc, r, chunk, page, off + I::kNQ);
+}
+
 template <class I>
 inline void host_instance_geometry(InstanceGeometry& g) {
   g.kind = (uint32_t)I::kKind;
@@ -146,6 +247,41 @@ struct Ouroboros {
   __device__ void* malloc(size_t size) {
     static_assert(W == 1 || W == 16 || W == 64, "aggregation width must be 1, 16 or 64");
     return detail::chain_malloc<W>(memory, chain, chain.inst.reuse, (uint64_t)size, true);
+  }
+
+  // Block-wide malloc / free: every thread of the workgroup calls together (blockDim.x a multiple of
+  // 64, at most 1024), size 0 / nullptr for a thread without a request; `sm` is the block's
+  // __shared__ Scratch. Requests of one class form one group for the whole block.
+  static constexpr uint32_t kNumClasses = (Is::kNQ + ...);
+  static constexpr bool kAnyChunks = (Is::kChunks || ...);
+  static_assert(kNumClasses <= kBlockMaxClasses, "too many size classes for the block scratch");
+  using Scratch = BlockScratch<kAnyChunks>;
+
+  __device__ void* malloc_block(size_t size, Scratch& sm) {
+    uint32_t gc = kMarker;
+    if (size) {
+      gc = detail::chain_alloc_class<Is...>((uint64_t)size);
+      if (gc == kMarker) detail::set_error(memory, ERR_TOO_LARGE);
+    }
+    const uint32_t r = detail::block_rank(sm, gc, kNumClasses);
+    if (threadIdx.x < kWave) detail::chain_block_claim(memory, chain, chain.inst.reuse, sm, kNumClasses);
+    __syncthreads();
+    return detail::chain_block_take(memory, chain, chain.inst.reuse, sm, gc, r);
+  }
+
+  __device__ void free_block(void* ptr, Scratch& sm) {
+    uint32_t chunk = 0, page_size = 0, page = 0, gc = kMarker;
+    uint64_t offset = 0;
+    if (detail::route_free(memory, ptr, chunk, page_size, offset)) {
+      gc = detail::chain_free_class<Is...>(page_size, offset, page);
+      if (gc == kMarker) detail::set_error(memory, ERR_BAD_FREE);
+    }
+    const uint32_t r = detail::block_rank(sm, gc, kNumClasses);
+    if (threadIdx.x < kWave) detail::chain_block_back(chain, sm);
+    __syncthreads();
+    detail::chain_block_release(memory, chain, chain.inst.reuse, sm, gc, r, chunk, page);
+    __syncthreads();  // every put / release of the block is done: publish, then signal
+    if (threadIdx.x < kWave) detail::chain_block_signal(chain, sm);
   }
 
   template <int W = OURO_DEFAULT_AGG>
@@ -320,6 +456,7 @@ struct Ouroboros {
       const uint64_t qb = queue_bytes(g);
       end -= qb;
       q.ring = qb ? reinterpret_cast<uint32_t*>(end) : nullptr;
+      inst.tab.ring[i] = q.ring;
       q.ring_size = (uint32_t)g.queue_entries;
       q.page_size = I::kSmallest << i;
       q.ppc = I::kChunkSize / q.page_size;

@@ -155,6 +155,46 @@ int ouro_snapshot(ouro_handle h, void* dst);
 
 /* ---- harness -------------------------------------------------------------------------------- */
 int ouro_run_trace(ouro_handle h, const ouro_trace_config* cfg, ouro_trace_result* out);
+/* ---- batched form (no reference counterpart) ------------------------------------------------- */
+/* A flat list of allocation requests served by persistent kernels with block-wide aggregation
+ * (agg 0: every size class of a workgroup's tile claims its pages with one semaphore operation and
+ * one front_ fetch-add) or by the per-thread kernels (agg 1, 16, 64). One round = alloc, write the
+ * requester's id into every word, read it back, free; with graph = 1 the round is captured once in
+ * a hipGraph and replayed. */
+typedef struct {
+  uint32_t requests;       /* allocation requests */
+  uint32_t rounds;         /* timed rounds */
+  uint32_t warmup;         /* untimed rounds before them */
+  int32_t agg;             /* 0 = block-wide (persistent kernels), 1 / 16 / 64 = per-wave widths */
+  uint32_t block;          /* threads per workgroup: a multiple of 64, at most 1024 */
+  uint32_t grid;           /* persistent workgroups (agg 0), 0 = every CU at its occupancy limit */
+  int32_t graph;           /* 1 = replay one captured round graph */
+  int32_t check;           /* 1 = a validation round first (overlap / alignment of the live set) */
+  const uint32_t* sizes;   /* host [requests] */
+  uint64_t* offsets_out;   /* optional host [requests]: the validation round's blocks as byte offsets
+                              from the data region, UINT64_MAX for nullptr */
+} ouro_batch_config;
+
+typedef struct {
+  double alloc_ms;         /* summed over timed rounds, hipEvent-timed alloc kernel */
+  double free_ms;          /* summed over timed rounds, hipEvent-timed free kernel */
+  double touch_ms;         /* write + verify passes */
+  double round_ms;         /* whole rounds (alloc .. free) */
+  uint64_t mallocs;        /* requests x timed rounds */
+  uint64_t null_returns;   /* over the timed rounds */
+  uint64_t verify_errors;
+  uint64_t bounds_errors;
+  uint64_t overlap_errors; /* validation round */
+  uint64_t align_errors;   /* validation round */
+  uint64_t touch_bytes;    /* algorithmic bytes of one round's write + verify passes */
+  uint32_t error_word;
+  uint32_t grid;           /* workgroups of the alloc / free kernels */
+  uint32_t peak_chunks;
+  uint32_t graph_used;     /* 1 if the rounds were graph replays */
+} ouro_batch_result;
+
+int ouro_run_batch(ouro_handle h, const ouro_batch_config* cfg, ouro_batch_result* out);
+
 /* Host trace generator (SplitMix64): dist 0 = fixed size `param`, 1 = uniform over the class
  * sizes 16 << q (q < classes), 2 = Zipf(param) over class ranks. */
 int ouro_gen_sizes(uint64_t seed, uint32_t n, int dist, double param, uint32_t classes, uint32_t* out);

@@ -68,6 +68,7 @@ def _load_sim() -> C.CDLL:
         lib.sim_queue_window.argtypes = [C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32), C.c_uint32]
         lib.sim_queue_window.restype = C.c_uint32
         lib.sim_class_of.argtypes = [C.c_uint32, C.c_uint32]
+        lib.sim_set_block_order.argtypes = [C.c_void_p, C.c_int]
         lib.sim_sem_make.restype = C.c_uint64
         lib.sim_sem_make.argtypes = [C.c_int64, C.c_uint64]
         lib.sim_sem_count.restype = C.c_int64
@@ -161,6 +162,23 @@ class Sim:
         out = np.empty(max(n, 1), dtype=np.uint32)
         self._lib.sim_queue_window(self._h, k, out.ctypes.data_as(C.POINTER(C.c_uint32)), n)
         return out[:n]
+
+    def block_trace(self, sizes, rounds: int) -> np.ndarray:
+        """The batched form run by one 64-thread persistent workgroup: each round allocates the flat
+        request list tile by tile (64 requests, block-wide calls), then frees it tile by tile.
+        Returns offsets [rounds, requests]."""
+        sizes = np.asarray(sizes, dtype=np.uint32)
+        self._lib.sim_set_block_order(self._h, 1)
+        try:
+            out = []
+            for _ in range(rounds):
+                offs = np.concatenate([self.wave_malloc(sizes[t:t + 64]) for t in range(0, len(sizes), 64)])
+                for t in range(0, len(sizes), 64):
+                    self.wave_free(offs[t:t + 64])
+                out.append(offs)
+        finally:
+            self._lib.sim_set_block_order(self._h, 0)
+        return np.stack(out)
 
     def trace(self, sizes, rounds: int) -> np.ndarray:
         """Single-wave trace: `rounds` x (wave malloc, wave free) over <= 64 lanes.

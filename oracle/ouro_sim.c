@@ -81,6 +81,7 @@ struct ouro_sim {
   int variant;
   uint64_t max_chunks, alloc_size;
   uint32_t next_free, error, qc_reserve, qc_next;
+  int block_order; /* 1: groups in class order (block-wide calls of one wavefront) */
   uint32_t* locator;
   chunk_t* chunks;
   inst_t inst[2];
@@ -813,6 +814,16 @@ static void inst_malloc(ouro_sim* s, inst_t* I, const int* lanes, int n, const u
   for (int i = 0; i < n; ++i) cls[i] = (uint32_t)sim_class_of(sizes[lanes[i]], I->smallest);
   group_t g[64];
   const int ng = make_groups(lanes, n, cls, g);
+  if (s->block_order) {
+    /* block-wide calls (Ouroboros::malloc_block): class leaders are the lanes of wave 0 indexed by
+       class, so claims and carves run in class order rather than first-lane order */
+    for (int a = 1; a < ng; ++a)
+      for (int b = a; b > 0 && g[b - 1].qi > g[b].qi; --b) {
+        const group_t t = g[b];
+        g[b] = g[b - 1];
+        g[b - 1] = t;
+      }
+  }
   int ok[64];
   acquire(s, I, g, ng, ok, nact);
   if (!is_chunk_kind(I->kind)) {
@@ -1130,3 +1141,5 @@ uint32_t sim_queue_window(const ouro_sim* s, uint32_t k, uint32_t* out, uint32_t
   }
   return len;
 }
+
+void sim_set_block_order(ouro_sim* s, int on) { s->block_order = on; }

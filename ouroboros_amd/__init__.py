@@ -121,7 +121,50 @@ class TraceResult(C.Structure):
 
 
 # Entry points and their C signatures (include/ouro_capi.h).
+class BatchConfig(C.Structure):
+    _fields_ = [
+        ("requests", C.c_uint32),
+        ("rounds", C.c_uint32),
+        ("warmup", C.c_uint32),
+        ("agg", C.c_int32),
+        ("block", C.c_uint32),
+        ("grid", C.c_uint32),
+        ("graph", C.c_int32),
+        ("check", C.c_int32),
+        ("sizes", C.POINTER(C.c_uint32)),
+        ("offsets_out", C.POINTER(C.c_uint64)),
+    ]
+
+
+class BatchResult(C.Structure):
+    _fields_ = [
+        ("alloc_ms", C.c_double),
+        ("free_ms", C.c_double),
+        ("touch_ms", C.c_double),
+        ("round_ms", C.c_double),
+        ("mallocs", C.c_uint64),
+        ("null_returns", C.c_uint64),
+        ("verify_errors", C.c_uint64),
+        ("bounds_errors", C.c_uint64),
+        ("overlap_errors", C.c_uint64),
+        ("align_errors", C.c_uint64),
+        ("touch_bytes", C.c_uint64),
+        ("error_word", C.c_uint32),
+        ("grid", C.c_uint32),
+        ("peak_chunks", C.c_uint32),
+        ("graph_used", C.c_uint32),
+    ]
+
+    def as_dict(self) -> dict:
+        d = {name: getattr(self, name) for name, _ in self._fields_}
+        d["errors"] = decode_errors(self.error_word)
+        t = (self.alloc_ms + self.free_ms) * 1e-3
+        d["mops"] = 2.0 * self.mallocs / t / 1e6 if t > 0 else 0.0
+        return d
+
+
 SIGNATURES = {
+    "ouro_run_batch": (C.c_int, [C.c_void_p, C.POINTER(BatchConfig), C.POINTER(BatchResult)]),
     "ouro_create": (C.c_int, [C.c_int, C.c_uint64, C.POINTER(C.c_void_p)]),
     "ouro_destroy": (C.c_int, [C.c_void_p]),
     "ouro_device_manager": (C.c_void_p, [C.c_void_p]),
@@ -267,6 +310,39 @@ class MemoryManager:
 
     def free_batch(self, d_ptrs: int, n: int, agg: int = 64, stream: int = 0) -> None:
         _check(lib().ouro_free_batch(self._h, d_ptrs, n, agg, stream), "ouro_free_batch")
+
+    def run_batch(
+        self,
+        sizes: np.ndarray,
+        rounds: int,
+        warmup: int = 1,
+        agg: int = 0,
+        block: int = 256,
+        grid: int = 0,
+        graph: bool = True,
+        check: bool = False,
+        offsets: np.ndarray | None = None,
+    ) -> dict:
+        """The batched form: `sizes` is a flat request list served by persistent block-aggregated
+        kernels (agg 0) or the per-thread kernels (agg 1, 16, 64); rounds replay a captured graph.
+        `offsets`, if given, receives the validation round's blocks (byte offsets, UINT64_MAX = null)."""
+        sizes = np.ascontiguousarray(sizes, dtype=np.uint32)
+        cfg = BatchConfig()
+        cfg.requests = len(sizes)
+        cfg.rounds = rounds
+        cfg.warmup = warmup
+        cfg.agg = agg
+        cfg.block = block
+        cfg.grid = grid
+        cfg.graph = int(graph)
+        cfg.check = int(check or offsets is not None)
+        cfg.sizes = sizes.ctypes.data_as(C.POINTER(C.c_uint32))
+        if offsets is not None:
+            assert offsets.dtype == np.uint64 and offsets.shape == sizes.shape and offsets.flags.c_contiguous
+            cfg.offsets_out = offsets.ctypes.data_as(C.POINTER(C.c_uint64))
+        res = BatchResult()
+        _check(lib().ouro_run_batch(self._h, C.byref(cfg), C.byref(res)), "ouro_run_batch")
+        return res.as_dict()
 
     def run_trace(
         self,

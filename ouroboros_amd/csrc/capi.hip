@@ -66,6 +66,7 @@ __global__ void __launch_bounds__(256) k_touch(void* const* __restrict__ ptrs, c
   const uint32_t bytes = size & ~3u;
   const bool ok = p != nullptr && p >= lo && p + bytes <= hi && (reinterpret_cast<uintptr_t>(p) & 15u) == 0;
   if (p != nullptr && !ok) atomicAdd(&errs[1], 1ull);
+  if (!verify && in && size && p == nullptr) atomicAdd(&errs[2], 1ull);  // requests answered with nullptr
   unsigned long long bad = 0;
   if (ok && bytes <= 256) {
     uint32_t* w = reinterpret_cast<uint32_t*>(const_cast<uint8_t*>(p));
@@ -301,6 +302,140 @@ int ouro_snapshot(ouro_handle h, void* dst) {
              : OURO_E_HIP;
 }
 
+int ouro_run_batch(ouro_handle h, const ouro_batch_config* cfg, ouro_batch_result* out) {
+  if (!h || !cfg || !out || !cfg->sizes) return OURO_E_INVALID;
+  if (cfg->agg != 0 && cfg->agg != 1 && cfg->agg != 16 && cfg->agg != 64) return OURO_E_INVALID;
+  const uint32_t block = cfg->block ? cfg->block : 256;
+  if (block % 64 != 0 || block > 1024) return OURO_E_INVALID;
+  memset(out, 0, sizeof(*out));
+  const uint32_t n = cfg->requests;
+  if (n == 0 || cfg->rounds == 0) return OURO_OK;
+  IManager& m = *h->impl;
+
+  DevBuf d_sizes, d_ptrs, d_errs;
+  OURO_TRY(d_sizes.alloc((size_t)n * 4));
+  OURO_TRY(d_ptrs.alloc((size_t)n * 8));
+  OURO_TRY(d_errs.alloc(32));
+  OURO_TRY(hipMemcpy(d_sizes.p, cfg->sizes, (size_t)n * 4, hipMemcpyHostToDevice));
+  OURO_TRY(hipMemset(d_ptrs.p, 0, (size_t)n * 8));
+  OURO_TRY(hipMemset(d_errs.p, 0, 32));
+
+  int cus = 0;
+  OURO_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, 0));
+  uint32_t grid = (n + block - 1) / block;
+  if (cfg->agg == 0) {
+    const int occ = m.persist_occupancy(block);
+    if (occ <= 0) return OURO_E_INVALID;  // the block size does not fit the persistent kernels
+    // more workgroups than can be resident would serialise behind each other anyway
+    const uint32_t cap = (uint32_t)occ * (uint32_t)cus;
+    grid = std::min<uint32_t>(grid, cfg->grid ? std::min<uint32_t>(cfg->grid, cap) : cap);
+  }
+  out->grid = grid;
+  const uint8_t* lo = reinterpret_cast<const uint8_t*>(m.data_base());
+  const uint8_t* hi = lo + m.max_chunks() * 8320ull;
+  const uint32_t tgrid = (n + 255) / 256;
+
+  hipStream_t s;
+  OURO_TRY(hipStreamCreate(&s));
+  hipEvent_t ev[4];
+  for (auto& e : ev) OURO_TRY(hipEventCreate(&e));
+  void** ptrs = d_ptrs.as<void*>();
+  const uint32_t* sizes = d_sizes.as<uint32_t>();
+  unsigned long long* errs = d_errs.as<unsigned long long>();
+  auto alloc = [&]() -> hipError_t {
+    return cfg->agg == 0 ? m.malloc_persist(sizes, ptrs, n, s, block, grid)
+                         : m.malloc_batch(sizes, ptrs, n, cfg->agg, s, nullptr, block);
+  };
+  auto release = [&]() -> hipError_t {
+    return cfg->agg == 0 ? m.free_persist(ptrs, n, s, block, grid) : m.free_batch(ptrs, n, cfg->agg, s, block);
+  };
+  auto touch = [&]() -> hipError_t {
+    hipLaunchKernelGGL(k_touch, dim3(tgrid), dim3(256), 0, s, ptrs, sizes, n, lo, hi, 0, errs);
+    hipLaunchKernelGGL(k_touch, dim3(tgrid), dim3(256), 0, s, ptrs, sizes, n, lo, hi, 1, errs);
+    return hipGetLastError();
+  };
+  auto round = [&]() -> hipError_t {
+    hipError_t e;
+    if ((e = hipEventRecord(ev[0], s)) != hipSuccess || (e = alloc()) != hipSuccess) return e;
+    if ((e = hipEventRecord(ev[1], s)) != hipSuccess || (e = touch()) != hipSuccess) return e;
+    if ((e = hipEventRecord(ev[2], s)) != hipSuccess || (e = release()) != hipSuccess) return e;
+    return hipEventRecord(ev[3], s);
+  };
+
+  // validation round: the live set after the alloc kernel must be overlap-free and page-aligned
+  if (cfg->check) {
+    OURO_TRY(alloc());
+    OURO_TRY(hipStreamSynchronize(s));
+    std::vector<void*> hp(n);
+    OURO_TRY(hipMemcpy(hp.data(), d_ptrs.p, (size_t)n * 8, hipMemcpyDeviceToHost));
+    std::vector<uint64_t> offs(n);
+    std::vector<uint32_t> sz(cfg->sizes, cfg->sizes + n);
+    for (uint32_t i = 0; i < n; ++i)
+      offs[i] = hp[i] ? (uint64_t)(reinterpret_cast<const uint8_t*>(hp[i]) - lo) : UINT64_MAX;
+    if (cfg->offsets_out) memcpy(cfg->offsets_out, offs.data(), (size_t)n * 8);
+    ouro_trace_result tr;
+    memset(&tr, 0, sizeof(tr));
+    check_live(offs, sz, &tr);
+    out->overlap_errors = tr.overlap_errors;
+    out->align_errors = tr.align_errors;
+    OURO_TRY(touch());
+    OURO_TRY(release());
+    OURO_TRY(hipStreamSynchronize(s));
+  }
+  for (uint32_t w = 0; w < cfg->warmup; ++w) OURO_TRY(round());
+  OURO_TRY(hipStreamSynchronize(s));
+  // counters cover the timed rounds only (reset outside any captured graph)
+  OURO_TRY(hipMemsetAsync(d_errs.p, 0, 32, s));
+  OURO_TRY(hipStreamSynchronize(s));
+
+  hipGraph_t g = nullptr;
+  hipGraphExec_t ge = nullptr;
+  if (cfg->graph) {
+    OURO_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    const hipError_t ce = round();
+    hipGraph_t cg = nullptr;
+    const hipError_t ee = hipStreamEndCapture(s, &cg);
+    if (ce != hipSuccess || ee != hipSuccess) {
+      if (cg) (void)hipGraphDestroy(cg);
+      return OURO_E_HIP;
+    }
+    g = cg;
+    OURO_TRY(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+    out->graph_used = 1;
+  }
+  for (uint32_t r = 0; r < cfg->rounds; ++r) {
+    if (ge) OURO_TRY(hipGraphLaunch(ge, s));
+    else OURO_TRY(round());
+    OURO_TRY(hipEventSynchronize(ev[3]));
+    float t = 0;
+    OURO_TRY(hipEventElapsedTime(&t, ev[0], ev[1]));
+    out->alloc_ms += t;
+    OURO_TRY(hipEventElapsedTime(&t, ev[1], ev[2]));
+    out->touch_ms += t;
+    OURO_TRY(hipEventElapsedTime(&t, ev[2], ev[3]));
+    out->free_ms += t;
+    OURO_TRY(hipEventElapsedTime(&t, ev[0], ev[3]));
+    out->round_ms += t;
+  }
+  OURO_TRY(hipStreamSynchronize(s));
+  if (ge) (void)hipGraphExecDestroy(ge);
+  if (g) (void)hipGraphDestroy(g);
+  unsigned long long e3[3];
+  OURO_TRY(hipMemcpy(e3, d_errs.p, 24, hipMemcpyDeviceToHost));
+  out->verify_errors = e3[0];
+  out->bounds_errors = e3[1];
+  out->null_returns = e3[2];
+  out->mallocs = (uint64_t)n * cfg->rounds;
+  for (uint32_t i = 0; i < n; ++i) out->touch_bytes += 2ull * (cfg->sizes[i] & ~3u);
+  ouro_state st;
+  if (m.query(&st) != hipSuccess) return OURO_E_HIP;
+  out->error_word = st.error;
+  out->peak_chunks = st.next_free_chunk;
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  (void)hipStreamDestroy(s);
+  return OURO_OK;
+}
+
 int ouro_gen_sizes(uint64_t seed, uint32_t n, int dist, double param, uint32_t classes, uint32_t* out) {
   if (!out || classes == 0 || classes > 16) return OURO_E_INVALID;
   if (dist == 0) {
@@ -347,8 +482,8 @@ int ouro_run_trace(ouro_handle h, const ouro_trace_config* cfg, ouro_trace_resul
   }
   OURO_TRY(d_skip_cur.alloc(n));
   OURO_TRY(d_skip_prev.alloc(n));
-  OURO_TRY(d_errs.alloc(16));
-  OURO_TRY(hipMemset(d_errs.p, 0, 16));
+  OURO_TRY(d_errs.alloc(32));
+  OURO_TRY(hipMemset(d_errs.p, 0, 32));
   const uint32_t nlat = (n + 63) / 64;
   if (cfg->latency) OURO_TRY(d_lat.alloc((size_t)nlat * 8));
 

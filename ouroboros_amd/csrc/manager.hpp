@@ -4,6 +4,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <stdlib.h>
 #include <string.h>
 #include <vector>
 
@@ -25,6 +26,13 @@ struct IManager {
   virtual hipError_t free_batch(void* const* d_ptrs, uint32_t n, int agg, hipStream_t s, uint32_t block) = 0;
   virtual hipError_t free_masked(void* const* d_ptrs, const uint8_t* d_skip, uint32_t n, int agg, hipStream_t s,
                                  uint32_t block) = 0;
+  // Batched form: persistent kernels over a flat request list with block-wide aggregation
+  // (Ouroboros::malloc_block / free_block). grid = persistent workgroups.
+  virtual hipError_t malloc_persist(const uint32_t* d_sizes, void** d_out, uint32_t n, hipStream_t s, uint32_t block,
+                                    uint32_t grid) = 0;
+  virtual hipError_t free_persist(void* const* d_ptrs, uint32_t n, hipStream_t s, uint32_t block, uint32_t grid) = 0;
+  // workgroups of `block` threads one CU holds at once for the persistent kernels (the smaller of the two)
+  virtual int persist_occupancy(uint32_t block) = 0;
   virtual hipError_t query(ouro_state* out) = 0;
   // immutable queue placement of class k (chain order)
   virtual void class_ring(uint32_t k, uint32_t** ring, uint32_t* ring_size) = 0;
@@ -78,7 +86,41 @@ __global__ void __launch_bounds__(256) k_free_masked(M* mm, void* const* __restr
   mm->template free<W>(skip[tid] ? nullptr : ptrs[tid]);
 }
 
+// Persistent block-aggregated kernels: each workgroup walks block-sized tiles of the request list
+// (the loop bound is uniform per block, as malloc_block / free_block synchronise the block).
+template <class M>
+__global__ void __launch_bounds__(1024) k_malloc_persist(M* mm, const uint32_t* __restrict__ sizes,
+                                                         void** __restrict__ out, uint32_t n) {
+  __shared__ typename M::Scratch sm;
+  const uint32_t step = gridDim.x * blockDim.x;
+  for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += step) {
+    const uint32_t i = base + threadIdx.x;
+    void* p = mm->malloc_block(i < n ? sizes[i] : 0u, sm);
+    if (i < n) out[i] = p;
+  }
+}
+
+template <class M>
+__global__ void __launch_bounds__(1024) k_free_persist(M* mm, void* const* __restrict__ ptrs, uint32_t n) {
+  __shared__ typename M::Scratch sm;
+  const uint32_t step = gridDim.x * blockDim.x;
+  for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += step) {
+    const uint32_t i = base + threadIdx.x;
+    mm->free_block(i < n ? ptrs[i] : nullptr, sm);
+  }
+}
+
 inline uint32_t grid_for(uint32_t n, uint32_t block) { return (n + block - 1) / block; }
+
+#ifdef OURO_DIAG
+// diagnostic builds: dynamic LDS per block (bytes) from OURO_DIAG_LDS, to cap the occupancy
+inline size_t launch_lds() {
+  const char* e = getenv("OURO_DIAG_LDS");
+  return e ? (size_t)atoi(e) : 0;
+}
+#else
+inline size_t launch_lds() { return 0; }
+#endif
 
 template <class M>
 struct Manager final : IManager {
@@ -97,10 +139,10 @@ struct Manager final : IManager {
                            uint32_t block) {
     M* dm = mm.getDeviceMemoryManager();
     if (d_lat)
-      hipLaunchKernelGGL((k_malloc_lat<M, W>), dim3(grid_for(n, block)), dim3(block), 0, s, dm, d_sizes, d_out, n,
+      hipLaunchKernelGGL((k_malloc_lat<M, W>), dim3(grid_for(n, block)), dim3(block), launch_lds(), s, dm, d_sizes, d_out, n,
                          d_lat);
     else
-      hipLaunchKernelGGL((k_malloc<M, W>), dim3(grid_for(n, block)), dim3(block), 0, s, dm, d_sizes, d_out, n);
+      hipLaunchKernelGGL((k_malloc<M, W>), dim3(grid_for(n, block)), dim3(block), launch_lds(), s, dm, d_sizes, d_out, n);
     return hipGetLastError();
   }
   hipError_t malloc_batch(const uint32_t* d_sizes, void** d_out, uint32_t n, int agg, hipStream_t s, uint64_t* d_lat,
@@ -117,10 +159,28 @@ struct Manager final : IManager {
   hipError_t launch_free(void* const* d_ptrs, const uint8_t* d_skip, uint32_t n, hipStream_t s, uint32_t block) {
     M* dm = mm.getDeviceMemoryManager();
     if (d_skip)
-      hipLaunchKernelGGL((k_free_masked<M, W>), dim3(grid_for(n, block)), dim3(block), 0, s, dm, d_ptrs, d_skip, n);
+      hipLaunchKernelGGL((k_free_masked<M, W>), dim3(grid_for(n, block)), dim3(block), launch_lds(), s, dm, d_ptrs, d_skip, n);
     else
-      hipLaunchKernelGGL((k_free<M, W>), dim3(grid_for(n, block)), dim3(block), 0, s, dm, d_ptrs, n);
+      hipLaunchKernelGGL((k_free<M, W>), dim3(grid_for(n, block)), dim3(block), launch_lds(), s, dm, d_ptrs, n);
     return hipGetLastError();
+  }
+  hipError_t malloc_persist(const uint32_t* d_sizes, void** d_out, uint32_t n, hipStream_t s, uint32_t block,
+                            uint32_t grid) override {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL((k_malloc_persist<M>), dim3(grid), dim3(block), 0, s, mm.getDeviceMemoryManager(), d_sizes,
+                       d_out, n);
+    return hipGetLastError();
+  }
+  hipError_t free_persist(void* const* d_ptrs, uint32_t n, hipStream_t s, uint32_t block, uint32_t grid) override {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL((k_free_persist<M>), dim3(grid), dim3(block), 0, s, mm.getDeviceMemoryManager(), d_ptrs, n);
+    return hipGetLastError();
+  }
+  int persist_occupancy(uint32_t block) override {
+    int a = 0, b = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_malloc_persist<M>, (int)block, 0) != hipSuccess) return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_free_persist<M>, (int)block, 0) != hipSuccess) return 0;
+    return a < b ? a : b;
   }
   hipError_t free_batch(void* const* d_ptrs, uint32_t n, int agg, hipStream_t s, uint32_t block) override {
     return free_masked(d_ptrs, nullptr, n, agg, s, block);

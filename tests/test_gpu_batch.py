@@ -1,0 +1,65 @@
+"""The batched form on the device (MI355X): persistent kernels with block-wide aggregation and
+hipGraph-replayed rounds, through the C ABI (ouro_run_batch).
+
+A single 64-thread persistent workgroup replays deterministically, so its blocks must equal the
+oracle's block-order replay bit for bit; concurrent runs must pass the invariant checkers (no overlap,
+page alignment, every written id read back, a quiescent pool afterwards)."""
+import numpy as np
+import pytest
+
+import oracle
+import ouroboros_amd as ob
+
+pytestmark = pytest.mark.gpu
+
+ALL = ["PQ", "CQ", "VAPQ", "VACQ", "VLPQ", "VLCQ"]
+
+
+def assert_clean(r):
+    assert r["verify_errors"] == 0 and r["bounds_errors"] == 0, r
+    assert r["overlap_errors"] == 0 and r["align_errors"] == 0, r
+    assert r["null_returns"] == 0, r
+    assert r["error_word"] == 0, r["errors"]
+
+
+@pytest.mark.parametrize("kind", ALL)
+def test_single_workgroup_matches_oracle(kind):
+    # 640 requests, mixed classes (both instances), served tile by tile by one 64-thread workgroup
+    sizes = (16 << (np.arange(640) * 7 % 12)).astype(np.uint32)
+    print(f"single-workgroup {kind}", flush=True)
+    sim = oracle.Sim(kind, 512 << 20)
+    want = sim.block_trace(sizes, 6)
+    assert sim.state().error == 0
+    with ob.MemoryManager(kind, 512 << 20) as mm:
+        for call in range(3):
+            # each call runs its validation round (observed) and then one timed round
+            got = np.empty(len(sizes), dtype=np.uint64)
+            res = mm.run_batch(sizes, rounds=1, warmup=0, agg=0, block=64, grid=1, graph=False, offsets=got)
+            assert_clean(res)
+            w = want[2 * call]
+            mism = np.flatnonzero(got != w)
+            assert len(mism) == 0, f"round {2 * call} request {mism[0]}: device {got[mism[0]]} oracle {w[mism[0]]}"
+        assert mm.check_quiescent().violations == 0
+
+
+@pytest.mark.parametrize("kind", ALL)
+@pytest.mark.parametrize("block", [256, 1024])
+def test_persistent_block_aggregated_invariants(kind, block):
+    print(f"persistent {kind} block={block}", flush=True)
+    sizes = ob.gen_sizes(42, 65536, "uniform")
+    with ob.MemoryManager(kind, 1 << 30) as mm:
+        res = mm.run_batch(sizes, rounds=3, warmup=1, agg=0, block=block, graph=True, check=True)
+        assert res["graph_used"] == 1
+        assert_clean(res)
+        chk = mm.check_quiescent()
+        assert chk.violations == 0, chk.message.decode()
+
+
+@pytest.mark.parametrize("agg", [1, 16, 64])
+def test_batch_runner_per_wave_widths(agg):
+    print(f"batch per-wave agg={agg}", flush=True)
+    sizes = ob.gen_sizes(7, 65536, "uniform")
+    with ob.MemoryManager("VAPQ", 1 << 30) as mm:
+        res = mm.run_batch(sizes, rounds=2, warmup=1, agg=agg, block=256, graph=True, check=True)
+        assert_clean(res)
+        assert mm.check_quiescent().violations == 0
