@@ -1,13 +1,21 @@
 #!/usr/bin/env python3
-"""Benchmark: malloc+free Mops/s on one MI355X, 1,048,576 threads, mixed sizes (config 2).
+"""Benchmark: malloc+free Mops/s on one MI355X, 1,048,576 threads, mixed sizes.
 
-A "step" is one round of the config-2 trace: 1,048,576 device threads each malloc a block of their
-size class (uniform over 16 B .. 8 KiB, seed 42), write their id into every word, read it back and
-free it. Mops/s = (mallocs + frees) / (alloc-kernel time + free-kernel time), the kernels timed
-with HIP events on the stream they run on (inside libouro's trace harness). The touch passes
-(write + verify) are timed separately and reported as HBM GB/s.
+The workload is the 1M-thread mixed-size trace: 1,048,576 requests whose sizes are uniform over the
+ten power-of-two classes 16 B .. 8 KiB (SplitMix64, seed 42) on a 4 GiB pool. A "step" is one round
+of it: every request is allocated, its block is written with the requester's id and read back, and
+every block is freed. Mops/s = (mallocs + frees) / (alloc-kernel time + free-kernel time), both
+kernels timed with HIP events on the stream they run on; the write / verify passes are timed apart
+and reported as HBM GB/s.
 
-Prints one JSON line (the driver contract). Only N = 1 is supported.
+The headline is the batched form: persistent kernels with block-wide aggregation, the round captured
+once in a hipGraph and replayed, with the pool geometry, aggregation width and launch shape chosen
+by the autotuner for this trace. Beside it: the plain per-thread path (the reference's own
+aggregation, one kernel thread per request), the 64-lane wave-aggregated path, p50 / p99 allocation
+latency, the atomic roofline, the batch workload set (256 workloads on a 32 GiB pool) and a glibc
+malloc/free baseline on the host cores.
+
+Prints one JSON line. Only N = 1 is supported.
 """
 from __future__ import annotations
 
@@ -24,6 +32,7 @@ sys.path.insert(0, str(ROOT))
 import numpy as np  # noqa: E402
 
 import ouroboros_amd as ob  # noqa: E402
+from ouroboros_amd import tune  # noqa: E402
 
 THREADS = 1 << 20
 POOL = 4 << 30
@@ -33,6 +42,11 @@ HBM_PEAK_GBS = 8000.0
 # queue slot read and clear (4 + 4); free: pointer in (8) + chunk header word (4) + slot write (4)
 ALG_BYTES_MALLOC = 20
 ALG_BYTES_FREE = 16
+KINDS = ["PQ", "CQ", "VAPQ", "VACQ", "VLPQ", "VLCQ"]
+
+
+def log(msg: str) -> None:
+    print(f"# {msg}", file=sys.stderr, flush=True)
 
 
 def mops(res: dict, n: int, steps: int) -> float:
@@ -40,19 +54,12 @@ def mops(res: dict, n: int, steps: int) -> float:
     return 2.0 * n * steps / t / 1e6 if t > 0 else 0.0
 
 
-def run(variant: str, agg: int, sizes: np.ndarray, steps: int, warmup: int, check: bool = False) -> dict:
+def run_trace(variant: str, agg: int, sizes: np.ndarray, steps: int, warmup: int) -> dict:
+    """The per-thread kernels (one request per kernel thread), launched one by one."""
     with ob.MemoryManager(variant, POOL) as mm:
         if warmup:
             mm.run_trace(sizes, warmup, agg=agg)
-        t0 = time.perf_counter()
-        out = mm.run_trace(sizes, steps, agg=agg)
-        wall = time.perf_counter() - t0
-        res = dict(out.result)
-        res["wall_s"] = wall
-        if check:
-            chk = mm.check_quiescent()
-            res["quiescent_violations"] = int(chk.violations)
-            res["quiescent_message"] = chk.message.decode()
+        res = dict(mm.run_trace(sizes, steps, agg=agg).result)
     res["mops"] = mops(res, len(sizes), steps)
     return res
 
@@ -60,8 +67,7 @@ def run(variant: str, agg: int, sizes: np.ndarray, steps: int, warmup: int, chec
 def latency(variant: str, agg: int, sizes: np.ndarray) -> dict:
     with ob.MemoryManager(variant, POOL) as mm:
         mm.run_trace(sizes, 1, agg=agg)
-        out = mm.run_trace(sizes, 3, agg=agg, latency=True)
-    r = out.result
+        r = mm.run_trace(sizes, 3, agg=agg, latency=True).result
     return {"p50_ns": r["lat_p50_ns"], "p99_ns": r["lat_p99_ns"], "max_ns": r["lat_max_ns"], "samples": r["lat_samples"]}
 
 
@@ -81,59 +87,113 @@ def cpu_baseline(sizes: np.ndarray, rounds: int) -> dict:
     }
 
 
+def batch_config4(steps: int) -> dict:
+    """The batch workload set: 256 workloads, lognormal request counts, per-workload Zipf classes,
+    1-4 allocations per request, 32 GiB pool; tuned, then measured on the whole set."""
+    sizes, req, zs = ob.gen_batch(seed=SEED)
+    pool = 32 << 30
+    t0 = time.perf_counter()
+    res = tune.tune(sizes, pool, sample=16, keep=0.15)
+    tune_s = time.perf_counter() - t0
+    c = res.best
+    with ob.MemoryManager(c.variant, pool) as mm:
+        r = mm.run_batch(sizes, rounds=max(2, steps // 4), warmup=1, agg=c.agg, block=c.block, grid=c.grid,
+                         graph=True, check=True)
+    return {
+        "workloads": len(req),
+        "requests": int(req.sum()),
+        "allocations": int(len(sizes)),
+        "mean_bytes": round(float(sizes.mean()), 1),
+        "pool_bytes": pool,
+        "tuned": c.label(),
+        "tune_seconds": round(tune_s, 1),
+        "excluded": res.excluded,
+        "mops": round(r["mops"], 2),
+        "alloc_ms_per_round": round(r["alloc_ms"] / max(1, r["mallocs"] // len(sizes)), 3),
+        "free_ms_per_round": round(r["free_ms"] / max(1, r["mallocs"] // len(sizes)), 3),
+        "touch_gbs": round(r["touch_bytes"] * (r["mallocs"] // len(sizes)) / (r["touch_ms"] * 1e-3) / 1e9, 1)
+        if r["touch_ms"] else None,
+        "invariants": {k: r[k] for k in ("verify_errors", "bounds_errors", "overlap_errors", "align_errors",
+                                          "null_returns")} | {"device_error_bits": r["errors"]},
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--variant", default="VAPQ", choices=list(ob.VARIANTS))
-    ap.add_argument("--no-table", action="store_true", help="skip the six-variant plain/aggregated table")
+    ap.add_argument("--variant", default=None, help="skip the tuner: batched form on this variant, block-wide")
+    ap.add_argument("--no-table", action="store_true", help="skip the per-kind plain / wave / batched table")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-config4", action="store_true", help="skip the 256-workload batch set")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one MI355X only; --gpus {args.gpus} is not supported (the allocator "
-              "shards as one independent pool per GPU, see DESIGN.md)", file=sys.stderr)
+              "shards as one independent pool per GPU)", file=sys.stderr)
         return 2
     ob.lib()  # fails loudly if libouro.so is missing
     sizes = ob.gen_sizes(SEED, THREADS, "uniform")
     steps, warmup = args.steps, args.warmup
 
-    print(f"# headline {args.variant} agg=64 ...", file=sys.stderr, flush=True)
-    head = run(args.variant, 64, sizes, steps, warmup, check=True)
-    print(f"# plain {args.variant} agg=1 ...", file=sys.stderr, flush=True)
-    plain = run(args.variant, 1, sizes, steps, warmup)
+    # ---- the batched form: tuned, then measured
+    t0 = time.perf_counter()
+    if args.variant:
+        best = tune.Candidate(args.variant, 0, 512, 0)
+        tuned = None
+    else:
+        log("tuning the batched form on the 1M-thread trace ...")
+        tuned = tune.tune(sizes, POOL, sample=8)
+        best = tuned.best
+    tune_s = time.perf_counter() - t0
+    log(f"headline {best.label()} ...")
+    with ob.MemoryManager(best.variant, POOL) as mm:
+        head = mm.run_batch(sizes, rounds=steps, warmup=warmup, agg=best.agg, block=best.block, grid=best.grid,
+                            graph=True, check=True)
+        chk = mm.check_quiescent()
+    kind = ob.variant_geometry(best.variant)["kind_name"]
+
+    # ---- the plain per-thread path (reference aggregation) on the same queue kind, and the best of all kinds
+    log(f"plain {kind} agg=1 ...")
+    plain = run_trace(kind, 1, sizes, steps, warmup)
+    log(f"wave-aggregated {kind} agg=64 ...")
+    wave = run_trace(kind, 64, sizes, steps, warmup)
 
     table = {}
     if not args.no_table:
-        for v in ob.VARIANTS:
+        for v in KINDS:
+            row = {}
             for agg in (1, 64):
-                if v == args.variant:
-                    r = head if agg == 64 else plain
+                if v == kind:
+                    r = plain if agg == 1 else wave
                 else:
-                    print(f"# table {v} agg={agg} ...", file=sys.stderr, flush=True)
-                    r = run(v, agg, sizes, max(4, steps // 4), 1)
-                table[f"{v}/agg{agg}"] = {
-                    "mops": round(r["mops"], 2),
-                    "alloc_ms_per_round": round(r["alloc_ms"] / max(1, r["mallocs"] // THREADS), 4),
-                    "free_ms_per_round": round(r["free_ms"] / max(1, r["mallocs"] // THREADS), 4),
-                    "errors": r["errors"],
-                    "verify_errors": r["verify_errors"],
-                }
+                    log(f"table {v} agg={agg} ...")
+                    r = run_trace(v, agg, sizes, max(4, steps // 4), 1)
+                row[f"agg{agg}"] = {"mops": round(r["mops"], 2), "errors": r["errors"], "verify_errors": r["verify_errors"]}
+            log(f"table {v} batched ...")
+            with ob.MemoryManager(v, POOL) as mm:
+                b = mm.run_batch(sizes, rounds=max(4, steps // 4), warmup=1, agg=0, block=512, graph=True)
+            row["batched_block512"] = {"mops": round(b["mops"], 2), "errors": b["errors"]}
+            table[v] = row
+    best_plain = max([plain["mops"]] + [table[v]["agg1"]["mops"] for v in table])
 
-    lat = latency(args.variant, 64, sizes)
+    lat = latency(kind, 64, sizes)
     atom_distinct = ob.bench_atomics(1 << 20, 64, "distinct")
     atom_same_wave = ob.bench_atomics(1 << 20, 16, "per_wave")
+    cfg4 = None if args.no_config4 else batch_config4(steps)
 
+    # ---- roofline of the dominant kernel of the headline
     alloc_avg_s = head["alloc_ms"] / steps * 1e-3
     free_avg_s = head["free_ms"] / steps * 1e-3
     dominant = "alloc" if alloc_avg_s >= free_avg_s else "free"
     dom_s = max(alloc_avg_s, free_avg_s)
     alg_bytes = THREADS * (ALG_BYTES_MALLOC if dominant == "alloc" else ALG_BYTES_FREE)
     achieved_gbs = alg_bytes / dom_s / 1e9
-    pairs_per_s = head["mops"] * 1e6 / 2.0
-    touch_gbs = head["touch_bytes"] / ((head["write_ms"] + head["verify_ms"]) * 1e-3) / 1e9
-
+    touch_gbs = head["touch_bytes"] / (head["touch_ms"] / steps * 1e-3) / 1e9 if head["touch_ms"] else None
     cpu = None if args.no_cpu else cpu_baseline(sizes, 3)
+    kern = "k_malloc_persist" if dominant == "alloc" else "k_free_persist"
+    if best.agg != 0:
+        kern = "k_malloc" if dominant == "alloc" else "k_free"
 
     line = {
         "metric": "malloc+free Mops/s on 1 MI355X (1M threads, mixed sizes); p99 alloc latency",
@@ -149,31 +209,40 @@ def main() -> int:
         "dtype": "u32",
         "data": "synthetic",
         "config": {
-            "workload": f"config 2: MultiOuro{args.variant}, 4 GiB pool, 1,048,576 threads, sizes uniform over "
-            f"16 B..8 KiB power-of-two classes (SplitMix64 seed 42), alloc/write/verify/free per step, "
-            f"64-lane wave-aggregated path",
-            "variant": args.variant,
-            "aggregation_width": 64,
+            "workload": "1,048,576 requests, sizes uniform over the 16 B..8 KiB power-of-two classes (SplitMix64 "
+            "seed 42), 4 GiB pool; per step alloc / write id / verify / free; batched form: persistent "
+            "block-aggregated kernels, hipGraph-replayed rounds, autotuned geometry and launch",
+            "tuned": best.label(),
+            "variant": best.variant,
+            "aggregation": "block" if best.agg == 0 else best.agg,
+            "block": best.block,
+            "grid": head["grid"],
             "threads": THREADS,
             "pool_bytes": POOL,
         },
         "plain_mops": round(plain["mops"], 3),
+        "plain_variant": kind,
         "speedup_vs_plain": round(head["mops"] / plain["mops"], 3) if plain["mops"] else None,
+        "speedup_vs_best_plain": round(head["mops"] / best_plain, 3) if best_plain else None,
+        "wave64_mops": round(wave["mops"], 3),
         "p99_alloc_latency_ns": round(lat["p99_ns"], 1),
         "p50_alloc_latency_ns": round(lat["p50_ns"], 1),
+        "latency_note": f"{kind} device malloc, 64-lane path, wall_clock64 around malloc on one lane per wave",
         "latency_samples": lat["samples"],
-        "touch_pass_gbs": round(touch_gbs, 1),
-        "wall_ms_per_step": round(head["wall_s"] / steps * 1e3, 3),
+        "touch_pass_gbs": round(touch_gbs, 1) if touch_gbs else None,
+        "tune_seconds": round(tune_s, 1),
         "invariants": {
             "verify_errors": head["verify_errors"],
             "bounds_errors": head["bounds_errors"],
+            "overlap_errors": head["overlap_errors"],
+            "align_errors": head["align_errors"],
             "null_returns": head["null_returns"],
             "device_error_bits": head["errors"],
-            "quiescent_violations": head.get("quiescent_violations"),
+            "quiescent_violations": int(chk.violations),
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": f"k_malloc<MultiOuro{args.variant}, 64>" if dominant == "alloc" else f"k_free<MultiOuro{args.variant}, 64>",
+            "kernel": f"{kern}<{best.variant}>",
             "achieved": round(achieved_gbs, 2),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
@@ -183,14 +252,20 @@ def main() -> int:
             "algorithmic_bytes_per_launch": alg_bytes,
         },
         "atomic_roofline": {
-            "achieved_atomics_per_s": pairs_per_s * 2.0,
+            "achieved_atomics_per_s": head["mops"] * 1e6,
             "peak_atomics_per_s": atom_distinct,
-            "frac": round(pairs_per_s * 2.0 / atom_distinct, 4) if atom_distinct else None,
+            "frac": round(head["mops"] * 1e6 / atom_distinct, 4) if atom_distinct else None,
             "per_wave_same_address_atomics_per_s": atom_same_wave,
-            "note": "2 algorithmic atomic RMWs per malloc+free pair vs the measured distinct-address "
-            "agent-scope u32 fetch-add-with-return rate",
+            "note": "2 algorithmic atomic RMWs per malloc+free pair (slot take + slot put, or bit claim + "
+            "release) vs the measured distinct-address agent-scope u32 fetch-add-with-return rate",
         },
         "cpu_baseline": cpu,
+        "batch_config4": cfg4,
+        "tuner": None if tuned is None else {
+            "best": tuned.best.label(),
+            "excluded": tuned.excluded,
+            "stage2": {t.candidate.label(): round(t.mops, 1) for t in tuned.trials if t.stage == 2},
+        },
         "table": table,
     }
     print(json.dumps(line))

@@ -17,7 +17,8 @@
 
 namespace ouro {
 
-template <Kind K, uint32_t SMALLEST, uint32_t NQ>
+// QCAP: entries of each class queue array (0 = the reference's capacity for the kind).
+template <Kind K, uint32_t SMALLEST, uint32_t NQ, uint32_t QCAP = 0u>
 struct Instance {
   static constexpr Kind kKind = K;
   static constexpr uint32_t kSmallest = SMALLEST;
@@ -29,7 +30,9 @@ struct Instance {
   static constexpr bool kIsBase = kAdd == 1;
   static constexpr bool kChunks = is_chunk_kind(K);
   // entries of each class's queue array (standard: slots; VA: queue-chunk pointers; VL: none)
-  static constexpr uint32_t kQueueEntries = is_std(K) ? kStdEntries : (is_va(K) ? kVAEntries : 0u);
+  static constexpr uint32_t kQueueEntries =
+      is_vl(K) ? 0u : (QCAP ? QCAP : (is_std(K) ? kStdEntries : kVAEntries));
+  static_assert(kQueueEntries == 0 || (kQueueEntries & (kQueueEntries - 1)) == 0, "queue capacity must be a power of two");
   static_assert(kChunkSize % kBaseChunk == 0, "instance chunk must be a multiple of the base chunk");
   static_assert(kChunkSize / SMALLEST <= 512, "at most 512 pages per chunk (9 page bits)");
   static_assert(NQ <= 16, "at most 16 classes per instance");
@@ -163,8 +166,8 @@ __device__ inline uint32_t alloc_chunks(Memory& m, ReuseRing& r, uint32_t add_fa
       rbase = readlane(rbase, l);
       if (me < k && !valid) {
         const uint32_t j = rbase + rank_in(miss);
-        chunk = (uint32_t)limit + j;
-        valid = j < m.qc_reserve && chunk < m.max_chunks;
+        chunk = (uint32_t)limit + j * add_factor;
+        valid = (uint64_t)chunk + add_factor <= (uint64_t)m.max_chunks;
       }
     }
   }
@@ -285,25 +288,28 @@ __device__ inline void vl_set_old(Ctx<I>& x, ClassQueue* cq, bool want, uint32_t
   } else {
     free_count = 0;
   }
-  uint32_t cur = free_count ? ld(&cq->old_qc) : 0u;
-  Spin it(&x.m.error);
-  while (ballot(free_count > 0)) {
-    bool put = false;
-    uint32_t victim = 0;
-    if (free_count > 0) {
-      const uint32_t nx = chunk_ok(x.m, cur) ? ld(&hdr<QueueChunkHdr>(x.m, cur)->next) : kMarker;
-      const uint32_t seen = !chunk_ok(x.m, nx) ? ~cur : cas(&cq->old_qc, cur, nx);
-      if (seen == cur) {
-        victim = cur;
-        put = true;
-        cur = nx;
-        --free_count;
-      } else if (chunk_ok(x.m, nx)) {
-        cur = seen;
+  // one lane at a time in lane order, each recycling all of its queue chunks, so the base ring
+  // receives them in an order a sequential replay reproduces
+  uint64_t todo = ballot(free_count > 0);
+  while (todo) {
+    const int l = first(todo);
+    todo &= todo - 1;
+    if ((int)lane() == l) {
+      uint32_t cur = ld(&cq->old_qc);
+      Spin it(&x.m.error);
+      while (free_count > 0) {
+        const uint32_t nx = chunk_ok(x.m, cur) ? ld(&hdr<QueueChunkHdr>(x.m, cur)->next) : kMarker;
+        const uint32_t seen = !chunk_ok(x.m, nx) ? ~cur : cas(&cq->old_qc, cur, nx);
+        if (seen == cur) {
+          ring_put(x.m, x.base_ring, true, cur);
+          cur = nx;
+          --free_count;
+        } else if (chunk_ok(x.m, nx)) {
+          cur = seen;
+        }
+        if (it.expired()) { set_error(x.m, ERR_TIMEOUT); free_count = 0; }
       }
-      if (it.expired()) { set_error(x.m, ERR_TIMEOUT); free_count = 0; }
     }
-    ring_put(x.m, x.base_ring, put, victim);
   }
 }
 
@@ -438,7 +444,8 @@ __device__ inline void put_lanes(Ctx<I>& x, ClassQueue* cq, bool want, uint32_t 
       // lanes that need one form a sub-wave; alloc_chunks hands them out in lane order
       if (need) {
         const uint32_t k = popc(ballot(true));
-        const uint32_t ok = alloc_chunks(x.m, x.base_ring, 1u, k, got, true);
+        // queue chunks share the base ring with the first instance's chunks: same run length
+        const uint32_t ok = alloc_chunks(x.m, x.base_ring, x.m.qc_add, k, got, true);
         (void)ok;
         qc = got;
         if (rank_in(ballot(true)) >= ok) qc = kMarker;
@@ -1141,10 +1148,22 @@ __device__ inline void* alloc_chunk_pages(Ctx<I>& x, uint32_t qi, bool want) {
   uint32_t reclaim = kMarker;
   bool got = false;
   if constexpr (W == 1) {
-    // the reference's per-thread protocol: every lane walks the queue from front_ on its own and
-    // probes each chunk's count with a decrement
+    // the reference's per-thread protocol: every lane walks the queue on its own and probes each
+    // chunk's count with a decrement. Lanes start at the class's page cursor (one add per semaphore
+    // group, rank order inside it) instead of all at front_, which a million lanes turn into one
+    // long contended walk.
+    const int L = first(grp);
+    uint64_t p0 = 0;
+    if (ok && (int)lane() == L) p0 = add(&cq->scan, (uint64_t)popc(grp));
+    p0 = shfl64(p0, L);
     if (ok) {
-      uint32_t v = ld(&cq->front);
+      uint32_t v;
+      {
+        const uint32_t h = (uint32_t)((p0 + rank_in(grp)) >> __builtin_ctz(ppc_of(x, cq)));
+        const uint32_t f = ld(&cq->front), b = ld(&cq->back);
+        const uint32_t len = b - f;
+        v = (len == 0 || len >= 0x80000000u) ? f : (h - f < len ? h : f + (h - f) % len);
+      }
       uint32_t cursor = kMarker, wraps = 0;
       Spin it(&x.m.error);
       while (true) {

@@ -134,6 +134,7 @@ struct alignas(128) Memory {
   uint32_t variant;
   uint32_t initialized;
   uint32_t qc_reserve;      // chunks at the top of the pool that only queue chunks may take
+  uint32_t qc_add;          // base chunks per queue chunk: the first instance's chunk run length
   alignas(128) uint32_t next_free_chunk;
   uint32_t qc_next;         // bump pointer inside the queue-chunk reserve
   alignas(128) uint32_t error;

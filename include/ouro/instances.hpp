@@ -79,4 +79,14 @@ using MultiOuroVACQ = Ouroboros<VACQ, Second<ChunkQueueVA>>;
 using MultiOuroVLPQ = Ouroboros<VLPQ, Second<PageQueueVL>>;
 using MultiOuroVLCQ = Ouroboros<VLCQ, Second<ChunkQueueVL>>;
 
+// Pool geometries the batched form's tuner chooses among (no reference counterpart): a single
+// instance of CLASSES classes whose largest page is the CHUNK-byte instance chunk (a multiple of the
+// 8 KiB base chunk, at most 512 pages per chunk), and chains whose class queue arrays hold QCAP
+// entries instead of the reference's capacity.
+template <template <class> class Q, unsigned CHUNK, unsigned CLASSES, unsigned QCAP = 0>
+using OuroGeometry = Ouroboros<Instance<KindOf<Q>::value, (CHUNK >> (CLASSES - 1)), CLASSES, QCAP>>;
+template <template <class> class Q, unsigned QCAP>
+using MultiOuroCapacity = Ouroboros<Instance<KindOf<Q>::value, SMALLEST_PAGE_SIZE, NUM_QUEUES, QCAP>,
+                                    Instance<KindOf<Q>::value, SMALLEST_PAGE_SIZE << NUM_QUEUES, 2, QCAP>>;
+
 }  // namespace ouro

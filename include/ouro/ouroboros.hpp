@@ -15,6 +15,7 @@
 #include <stdint.h>
 #include <stddef.h>
 #include <string.h>
+#include <tuple>
 #include <utility>
 #include <vector>
 
@@ -228,7 +229,7 @@ template <class I>
 inline void host_instance_geometry(InstanceGeometry& g) {
   g.kind = (uint32_t)I::kKind;
   g.num_classes = I::kNQ;
-  g.queue_entries = is_std(I::kKind) ? kStdEntries : (is_va(I::kKind) ? kVAEntries : 0);
+  g.queue_entries = I::kQueueEntries;
 }
 
 }  // namespace detail
@@ -238,6 +239,8 @@ struct Ouroboros {
   using MyType = Ouroboros<Is...>;
   static constexpr int kNumInstances = sizeof...(Is);
   static constexpr bool kVirtualized = ((!is_std(Is::kKind)) || ...);
+  // base chunks per chunk of the first instance; queue chunks take runs of this length too
+  static constexpr uint32_t kBaseAdd = std::tuple_element_t<0, std::tuple<Is...>>::kAdd;
 
   Memory memory;
   Chain<Is...> chain;
@@ -253,6 +256,11 @@ struct Ouroboros {
   // 64, at most 1024), size 0 / nullptr for a thread without a request; `sm` is the block's
   // __shared__ Scratch. Requests of one class form one group for the whole block.
   static constexpr uint32_t kNumClasses = (Is::kNQ + ...);
+  static constexpr uint32_t kLargestPage = [] {
+    uint32_t m = 0;
+    ((m = Is::kLargest > m ? Is::kLargest : m), ...);
+    return m;
+  }();
   static constexpr bool kAnyChunks = (Is::kChunks || ...);
   static_assert(kNumClasses <= kBlockMaxClasses, "too many size classes for the block scratch");
   using Scratch = BlockScratch<kAnyChunks>;
@@ -382,6 +390,7 @@ struct Ouroboros {
     memory.locator = reinterpret_cast<uint32_t*>(d + L.allocation_size - L.locator_bytes);
     memory.next_free_chunk = 0;
     memory.qc_next = 0;
+    memory.qc_add = kBaseAdd;
     // a virtualized pool keeps its top chunks for queue chunks (at most a quarter of a tiny pool)
     memory.qc_reserve = kVirtualized ? (kQCReserve < L.max_chunks / 4 ? kQCReserve : (uint32_t)(L.max_chunks / 4)) : 0u;
     memory.error = 0;
@@ -399,17 +408,19 @@ struct Ouroboros {
       std::vector<uint32_t> loc((L.locator_bytes + 3) / 4, 0u);
       std::vector<uint8_t> qc(kStride);
       for (uint32_t i = 0; i < qcs.size() && e == hipSuccess; ++i) {
+        const uint32_t c = qcs[i];
         memset(qc.data(), 0xFF, qc.size());
-        QueueChunkHdr h{0u, 0u, 0u, kMarker, i};
+        QueueChunkHdr h{0u, 0u, 0u, kMarker, c};
         memset(qc.data(), 0, kMeta);
         memcpy(qc.data(), &h, sizeof(h));
-        e = hipMemcpy(memory.d_data + (uint64_t)i * kStride, qc.data(), kStride, hipMemcpyHostToDevice);
-        loc[i >> 5] |= 1u << (i & 31u);
+        e = hipMemcpy(memory.d_data + (uint64_t)c * kStride, qc.data(), kStride, hipMemcpyHostToDevice);
+        loc[c >> 5] |= 1u << (c & 31u);
       }
-      if (e == hipSuccess) e = hipMemcpy(memory.locator, loc.data(), ((qcs.size() + 31) / 32) * 4, hipMemcpyHostToDevice);
+      const uint32_t used = (uint32_t)qcs.size() * memory.qc_add;
+      if (e == hipSuccess) e = hipMemcpy(memory.locator, loc.data(), ((used + 31) / 32) * 4, hipMemcpyHostToDevice);
       for (size_t i = 0; i < va_heads.size() && e == hipSuccess; ++i)
         e = hipMemcpy(va_heads[i].first, &va_heads[i].second, 4, hipMemcpyHostToDevice);
-      memory.next_free_chunk = (uint32_t)qcs.size();
+      memory.next_free_chunk = used;
     }
     memory.initialized = 1;
     if (e == hipSuccess) e = hipMemcpy(d, this, sizeof(MyType), hipMemcpyHostToDevice);
@@ -469,7 +480,8 @@ struct Ouroboros {
       q.old_count = 0;
       q.front_qc = q.back_qc = q.old_qc = kMarker;
       if (!is_std(I::kKind)) {
-        const uint32_t c = (uint32_t)qcs.size();
+        // queue chunks take whole runs of the first instance's chunk length
+        const uint32_t c = (uint32_t)qcs.size() * kBaseAdd;
         qcs.push_back(c);
         if (is_vl(I::kKind)) q.front_qc = q.back_qc = q.old_qc = c;
         if (is_va(I::kKind)) va_heads.emplace_back(q.ring, c);

@@ -27,8 +27,29 @@ enum {
   OURO_VAPQ = 2, /* page-based, virtualized array queue     */
   OURO_VACQ = 3, /* chunk-based, virtualized array queue    */
   OURO_VLPQ = 4, /* page-based, virtualized linked list     */
-  OURO_VLCQ = 5  /* chunk-based, virtualized linked list    */
+  OURO_VLCQ = 5, /* chunk-based, virtualized linked list    */
+  /* pool geometries for the batched form's tuner (no reference counterpart): page-based virtualized
+   * array queues on one instance, and MultiOuro chains with other queue-array capacities */
+  OURO_VAPQ_C8K_Q10 = 6,   /* 8 KiB chunks, 10 classes (16 B .. 8 KiB)            */
+  OURO_VAPQ_C8K_Q8 = 7,    /* 8 KiB chunks, 8 classes (64 B .. 8 KiB)             */
+  OURO_VAPQ_C16K_Q10 = 8,  /* 16 KiB chunks, 10 classes (32 B .. 16 KiB)          */
+  OURO_VAPQ_C16K_Q8 = 9,   /* 16 KiB chunks, 8 classes (128 B .. 16 KiB)          */
+  OURO_VAPQ_CAP4K = 10,    /* MultiOuroVAPQ, 4,096 queue chunks per class (2^23 positions) */
+  OURO_PQ_CAP1M = 11,      /* MultiOuroPQ, 2^20-entry class rings                 */
+  OURO_NUM_VARIANTS = 12
 };
+
+/* Geometry of a variant (no pool needed). */
+typedef struct {
+  uint32_t kind;            /* OURO_PQ ... OURO_VLCQ */
+  uint32_t classes;         /* over the whole chain */
+  uint32_t smallest_page;   /* bytes */
+  uint32_t largest_page;    /* bytes */
+  uint32_t chunk_bytes;     /* the first instance's chunk (its largest page) */
+  uint32_t queue_entries;   /* entries of each class queue array of the first instance (0: linked) */
+  uint64_t page_capacity;   /* pages (page queues) or chunks (chunk queues) one class queue can hold */
+} ouro_variant_info;
+int ouro_variant_geometry(int variant, ouro_variant_info* out);
 
 /* Status codes. */
 enum {
@@ -198,6 +219,25 @@ int ouro_run_batch(ouro_handle h, const ouro_batch_config* cfg, ouro_batch_resul
 /* Host trace generator (SplitMix64): dist 0 = fixed size `param`, 1 = uniform over the class
  * sizes 16 << q (q < classes), 2 = Zipf(param) over class ranks. */
 int ouro_gen_sizes(uint64_t seed, uint32_t n, int dist, double param, uint32_t classes, uint32_t* out);
+
+/* Batch workload generator (SplitMix64 keyed by seed and workload): workload k has
+ * R_k = clamp(round(lognormal(mu, sigma)), lo, hi) requests, a Zipf exponent s_k ~ U[s_lo, s_hi] over
+ * the class ranks (P(q) proportional to (q + 1)^-s_k) and 1 .. max_allocs allocations per request.
+ * Writes the flat allocation sizes (16 << q), request after request, to sizes[0 .. cap) and the total
+ * in *n_out (call with sizes = NULL to size the buffer); requests_out / zipf_out are optional
+ * [workloads] arrays. */
+typedef struct {
+  uint64_t seed;
+  uint32_t workloads;
+  uint32_t lo, hi;          /* request-count clamp */
+  uint32_t max_allocs;      /* allocations per request: 1 .. max_allocs */
+  uint32_t classes;
+  uint32_t pad;
+  double mu, sigma;         /* lognormal request count */
+  double s_lo, s_hi;        /* per-workload Zipf exponent range */
+} ouro_batch_workload;
+int ouro_gen_batch(const ouro_batch_workload* w, uint32_t* sizes, uint64_t cap, uint64_t* n_out,
+                   uint32_t* requests_out, double* zipf_out);
 
 /* ---- test hook (no reference counterpart) --------------------------------------------------- */
 /* The device semaphore's carve decision (BulkSemaphore_impl.cuh:236-263 as changed here) evaluated

@@ -16,7 +16,10 @@ import numpy as np
 
 _LIB_PATH = Path(__file__).resolve().parent / "libouro.so"
 
-VARIANTS = {"PQ": 0, "CQ": 1, "VAPQ": 2, "VACQ": 3, "VLPQ": 4, "VLCQ": 5}
+VARIANTS = {"PQ": 0, "CQ": 1, "VAPQ": 2, "VACQ": 3, "VLPQ": 4, "VLCQ": 5,
+            # pool geometries of the batched form's tuner (queue kind / chunk bytes / classes / capacity)
+            "VAPQ-c8k-q10": 6, "VAPQ-c8k-q8": 7, "VAPQ-c16k-q10": 8, "VAPQ-c16k-q8": 9,
+            "VAPQ-cap4k": 10, "PQ-cap1m": 11}
 VARIANT_NAMES = {v: k for k, v in VARIANTS.items()}
 MAX_CLASSES = 16
 
@@ -163,7 +166,38 @@ class BatchResult(C.Structure):
         return d
 
 
+class BatchWorkload(C.Structure):
+    _fields_ = [
+        ("seed", C.c_uint64),
+        ("workloads", C.c_uint32),
+        ("lo", C.c_uint32),
+        ("hi", C.c_uint32),
+        ("max_allocs", C.c_uint32),
+        ("classes", C.c_uint32),
+        ("pad", C.c_uint32),
+        ("mu", C.c_double),
+        ("sigma", C.c_double),
+        ("s_lo", C.c_double),
+        ("s_hi", C.c_double),
+    ]
+
+
+class VariantInfo(C.Structure):
+    _fields_ = [
+        ("kind", C.c_uint32),
+        ("classes", C.c_uint32),
+        ("smallest_page", C.c_uint32),
+        ("largest_page", C.c_uint32),
+        ("chunk_bytes", C.c_uint32),
+        ("queue_entries", C.c_uint32),
+        ("page_capacity", C.c_uint64),
+    ]
+
+
 SIGNATURES = {
+    "ouro_variant_geometry": (C.c_int, [C.c_int, C.POINTER(VariantInfo)]),
+    "ouro_gen_batch": (C.c_int, [C.POINTER(BatchWorkload), C.POINTER(C.c_uint32), C.c_uint64,
+                                 C.POINTER(C.c_uint64), C.POINTER(C.c_uint32), C.POINTER(C.c_double)]),
     "ouro_run_batch": (C.c_int, [C.c_void_p, C.POINTER(BatchConfig), C.POINTER(BatchResult)]),
     "ouro_create": (C.c_int, [C.c_int, C.c_uint64, C.POINTER(C.c_void_p)]),
     "ouro_destroy": (C.c_int, [C.c_void_p]),
@@ -226,6 +260,34 @@ def gen_sizes(seed: int, n: int, dist: str = "uniform", param: float = 0.0, clas
         "ouro_gen_sizes",
     )
     return out
+
+
+def variant_geometry(variant: str) -> dict:
+    """Kind, classes, page range, chunk bytes and per-class queue capacity of a variant."""
+    g = VariantInfo()
+    _check(lib().ouro_variant_geometry(VARIANTS[variant], C.byref(g)), "ouro_variant_geometry")
+    d = {name: getattr(g, name) for name, _ in g._fields_}
+    d["kind_name"] = VARIANT_NAMES[g.kind]
+    return d
+
+
+def gen_batch(seed: int = 42, workloads: int = 256, mu: float = 10.0, sigma: float = 1.5, lo: int = 64,
+              hi: int = 1 << 20, s_lo: float = 0.8, s_hi: float = 1.6, max_allocs: int = 4,
+              classes: int = 10) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+    """The batch workload set (default: 256 workloads, lognormal(10, 1.5) requests clamped to
+    [64, 2^20], Zipf exponents in [0.8, 1.6], 1-4 allocations per request). Returns (flat allocation
+    sizes, requests per workload, Zipf exponent per workload)."""
+    w = BatchWorkload(seed=seed, workloads=workloads, lo=lo, hi=hi, max_allocs=max_allocs, classes=classes,
+                      pad=0, mu=mu, sigma=sigma, s_lo=s_lo, s_hi=s_hi)
+    n = C.c_uint64()
+    req = np.empty(workloads, dtype=np.uint32)
+    zs = np.empty(workloads, dtype=np.float64)
+    _check(lib().ouro_gen_batch(C.byref(w), None, 0, C.byref(n), req.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                zs.ctypes.data_as(C.POINTER(C.c_double))), "ouro_gen_batch")
+    sizes = np.empty(n.value, dtype=np.uint32)
+    _check(lib().ouro_gen_batch(C.byref(w), sizes.ctypes.data_as(C.POINTER(C.c_uint32)), n.value, C.byref(n),
+                                None, None), "ouro_gen_batch")
+    return sizes, req, zs
 
 
 def sem_share(before: int, num: int, ppc: int) -> int:

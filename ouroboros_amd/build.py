@@ -1,5 +1,6 @@
-"""Build libouro.so in-tree: six MultiOuro* translation units plus the C ABI, compiled for gfx950
-with hipcc in parallel and linked into ouroboros_amd/libouro.so."""
+"""Build libouro.so in-tree: one translation unit per allocator instantiation (the six MultiOuro*
+chains and the tuner's pool geometries) plus the C ABI, compiled for gfx950 with hipcc in parallel and
+linked into ouroboros_amd/libouro.so."""
 from __future__ import annotations
 
 import os
@@ -15,6 +16,7 @@ OBJ = PKG / "_obj"
 LIB = PKG / "libouro.so"
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+NUM_VARIANTS = 12  # the six MultiOuro chains and six pool geometries (csrc/variant.hip)
 ARCH = "gfx950"
 FLAGS = [
     f"--offload-arch={ARCH}",
@@ -41,7 +43,7 @@ def build(jobs: int | None = None, verbose: bool = False, extra_flags: list[str]
     lib = out or LIB
     obj_dir = OBJ if not extra_flags else OBJ.with_name("_obj_" + lib.stem)
     obj_dir.mkdir(exist_ok=True)
-    units = [(CSRC / "variant.hip", obj_dir / f"variant{v}.o", [f"-DOURO_VARIANT={v}"]) for v in range(6)]
+    units = [(CSRC / "variant.hip", obj_dir / f"variant{v}.o", [f"-DOURO_VARIANT={v}"]) for v in range(NUM_VARIANTS)]
     units.append((CSRC / "capi.hip", obj_dir / "capi.o", []))
     cmds = [[HIPCC, *FLAGS, *extra_flags, *extra, "-c", str(src), "-o", str(obj)] for src, obj, extra in units]
     jobs = jobs or min(8, os.cpu_count() or 4)

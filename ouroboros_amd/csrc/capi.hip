@@ -30,6 +30,12 @@ IManager* make_variant(int v) {
     case OURO_VACQ: return make_vacq();
     case OURO_VLPQ: return make_vlpq();
     case OURO_VLCQ: return make_vlcq();
+    case 6: return make_geo6();
+    case 7: return make_geo7();
+    case 8: return make_geo8();
+    case 9: return make_geo9();
+    case 10: return make_geo10();
+    case 11: return make_geo11();
     default: return nullptr;
   }
 }
@@ -44,9 +50,9 @@ inline uint64_t splitmix64(uint64_t x) {
 inline uint64_t keyed(uint64_t seed, uint64_t i) { return splitmix64(splitmix64(seed) ^ (i * 0xD1B54A32D192ED03ull)); }
 inline double unit(uint64_t r) { return (double)(r >> 11) * (1.0 / 9007199254740992.0); }
 
-// Page size a request of `size` bytes receives from the MultiOuro chain.
-inline uint32_t page_of(uint32_t size) {
-  uint32_t p = 16;
+// Page size a request of `size` bytes receives from a chain whose smallest page is `smallest`.
+inline uint32_t page_of(uint32_t size, uint32_t smallest = 16) {
+  uint32_t p = smallest;
   while (p < size) p <<= 1;
   return p;
 }
@@ -117,7 +123,10 @@ struct DevBuf {
 
 // Host-side check of the live set: alignment to the page grid of the request's class, no overlap,
 // and the fragmentation figures (chunks holding a live block vs. the ideal packing).
-void check_live(const std::vector<uint64_t>& offs, const std::vector<uint32_t>& sizes, ouro_trace_result* r) {
+// Live-set invariants: no two live blocks overlap; a block of the first instance starts on its
+// class's page grid inside the chunk (8 KiB instance chunks; other geometries get the 16-byte check).
+void check_live(const std::vector<uint64_t>& offs, const std::vector<uint32_t>& sizes, ouro_trace_result* r,
+                const ouro_variant_info& g) {
   struct Iv {
     uint64_t a, b;
   };
@@ -127,14 +136,14 @@ void check_live(const std::vector<uint64_t>& offs, const std::vector<uint32_t>& 
   std::unordered_set<uint64_t> chunks;
   for (size_t i = 0; i < offs.size(); ++i) {
     if (offs[i] == UINT64_MAX) continue;
-    const uint32_t ps = page_of(sizes[i]);
+    const uint32_t ps = page_of(sizes[i], g.smallest_page);
     iv.push_back({offs[i], offs[i] + ps});
-    if (ps <= 8192) {
+    if (ps <= 8192 && g.chunk_bytes == 8192) {
       const uint64_t c = offs[i] / 8320, in = offs[i] - c * 8320;
       if (in < 128 || (in - 128) % ps != 0 || in - 128 + ps > 8192) r->align_errors++;
       chunks.insert(c);
       uint32_t q = 0;
-      while ((16u << q) < ps) ++q;
+      while ((g.smallest_page << q) < ps) ++q;
       live_q[q]++;
     } else if ((offs[i] & 15u) != 0) {
       r->align_errors++;
@@ -144,7 +153,10 @@ void check_live(const std::vector<uint64_t>& offs, const std::vector<uint32_t>& 
   for (size_t i = 1; i < iv.size(); ++i)
     if (iv[i].a < iv[i - 1].b) r->overlap_errors++;
   uint64_t ideal = 0;
-  for (uint32_t q = 0; q < 10; ++q) ideal += (live_q[q] + (512u >> q) - 1) / (512u >> q);
+  for (uint32_t q = 0; q < 10; ++q) {
+    const uint32_t ppc = std::max<uint32_t>(1u, (g.chunk_bytes / g.smallest_page) >> q);
+    ideal += (live_q[q] + ppc - 1) / ppc;
+  }
   if (chunks.size() > r->peak_live_chunks) {
     r->peak_live_chunks = chunks.size();
     r->ideal_chunks = ideal;
@@ -265,13 +277,9 @@ void* ouro_device_manager(ouro_handle h) { return h ? h->impl->device_manager() 
 void* ouro_data_base(ouro_handle h) { return h ? h->impl->data_base() : nullptr; }
 
 int ouro_layout(int variant, uint64_t pool_bytes, uint64_t* max_chunks, uint64_t* allocation_size) {
-  ouro::InstanceGeometry g[2];
-  const ouro::Kind k = (ouro::Kind)variant;
-  if (variant < 0 || variant > 5) return OURO_E_INVALID;
-  const uint64_t entries = ouro::is_std(k) ? ouro::kStdEntries : (ouro::is_va(k) ? ouro::kVAEntries : 0);
-  g[0] = {(uint32_t)variant, 10, entries};
-  g[1] = {(uint32_t)variant, 2, entries};
-  const ouro::PoolLayout L = ouro::compute_layout(pool_bytes, g, 2);
+  std::unique_ptr<IManager> m(make_variant(variant));
+  if (!m) return OURO_E_INVALID;
+  const ouro::PoolLayout L = m->pool_layout(pool_bytes);
   if (max_chunks) *max_chunks = L.max_chunks;
   if (allocation_size) *allocation_size = L.allocation_size;
   return OURO_OK;
@@ -302,6 +310,14 @@ int ouro_snapshot(ouro_handle h, void* dst) {
              : OURO_E_HIP;
 }
 
+int ouro_variant_geometry(int variant, ouro_variant_info* out) {
+  if (!out) return OURO_E_INVALID;
+  std::unique_ptr<IManager> m(make_variant(variant));
+  if (!m) return OURO_E_INVALID;
+  m->geometry(out);
+  return OURO_OK;
+}
+
 int ouro_run_batch(ouro_handle h, const ouro_batch_config* cfg, ouro_batch_result* out) {
   if (!h || !cfg || !out || !cfg->sizes) return OURO_E_INVALID;
   if (cfg->agg != 0 && cfg->agg != 1 && cfg->agg != 16 && cfg->agg != 64) return OURO_E_INVALID;
@@ -309,7 +325,7 @@ int ouro_run_batch(ouro_handle h, const ouro_batch_config* cfg, ouro_batch_resul
   if (block % 64 != 0 || block > 1024) return OURO_E_INVALID;
   memset(out, 0, sizeof(*out));
   const uint32_t n = cfg->requests;
-  if (n == 0 || cfg->rounds == 0) return OURO_OK;
+  if (n == 0 || (cfg->rounds == 0 && !cfg->check)) return OURO_OK;
   IManager& m = *h->impl;
 
   DevBuf d_sizes, d_ptrs, d_errs;
@@ -375,7 +391,9 @@ int ouro_run_batch(ouro_handle h, const ouro_batch_config* cfg, ouro_batch_resul
     if (cfg->offsets_out) memcpy(cfg->offsets_out, offs.data(), (size_t)n * 8);
     ouro_trace_result tr;
     memset(&tr, 0, sizeof(tr));
-    check_live(offs, sz, &tr);
+    ouro_variant_info gi;
+    m.geometry(&gi);
+    check_live(offs, sz, &tr, gi);
     out->overlap_errors = tr.overlap_errors;
     out->align_errors = tr.align_errors;
     OURO_TRY(touch());
@@ -390,7 +408,7 @@ int ouro_run_batch(ouro_handle h, const ouro_batch_config* cfg, ouro_batch_resul
 
   hipGraph_t g = nullptr;
   hipGraphExec_t ge = nullptr;
-  if (cfg->graph) {
+  if (cfg->graph && cfg->rounds) {
     OURO_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     const hipError_t ce = round();
     hipGraph_t cg = nullptr;
@@ -461,6 +479,43 @@ int ouro_gen_sizes(uint64_t seed, uint32_t n, int dist, double param, uint32_t c
     return OURO_OK;
   }
   return OURO_E_INVALID;
+}
+
+int ouro_gen_batch(const ouro_batch_workload* w, uint32_t* sizes, uint64_t cap, uint64_t* n_out,
+                   uint32_t* requests_out, double* zipf_out) {
+  if (!w || !n_out || w->workloads == 0 || w->classes == 0 || w->classes > 16 || w->max_allocs == 0 ||
+      w->lo == 0 || w->lo > w->hi || w->s_lo > w->s_hi)
+    return OURO_E_INVALID;
+  const uint64_t seed = w->seed;
+  uint64_t total = 0;
+  std::vector<double> cdf(w->classes);
+  for (uint32_t k = 0; k < w->workloads; ++k) {
+    // request count: lognormal(mu, sigma) by Box-Muller from two keyed uniforms, rounded, clamped
+    const double u1 = 1.0 - unit(keyed(seed ^ 0x5851F42D4C957F2Dull, k));
+    const double u2 = unit(keyed(seed ^ 0x14057B7EF767814Full, k));
+    const double z = std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+    const double r = std::exp(w->mu + w->sigma * z);
+    const uint32_t req = (uint32_t)std::min<double>(w->hi, std::max<double>(w->lo, std::floor(r + 0.5)));
+    const double s = w->s_lo + (w->s_hi - w->s_lo) * unit(keyed(seed ^ 0x2545F4914F6CDD1Dull, k));
+    if (requests_out) requests_out[k] = req;
+    if (zipf_out) zipf_out[k] = s;
+    double acc = 0;
+    for (uint32_t q = 0; q < w->classes; ++q) cdf[q] = (acc += std::pow((double)(q + 1), -s));
+    for (uint32_t q = 0; q < w->classes; ++q) cdf[q] /= acc;
+    const uint64_t ws = seed ^ (0x9E3779B97F4A7C15ull * (k + 1));
+    for (uint32_t i = 0; i < req; ++i) {
+      const uint32_t na = 1u + (uint32_t)(keyed(ws, i) % w->max_allocs);
+      for (uint32_t j = 0; j < na; ++j, ++total) {
+        if (!sizes || total >= cap) continue;
+        const double u = unit(keyed(ws ^ 0xD6E8FEB86659FD93ull, (uint64_t)i * 4u + j));
+        uint32_t q = 0;
+        while (q + 1 < w->classes && u >= cdf[q]) ++q;
+        sizes[total] = 16u << q;
+      }
+    }
+  }
+  *n_out = total;
+  return OURO_OK;
 }
 
 int ouro_run_trace(ouro_handle h, const ouro_trace_config* cfg, ouro_trace_result* out) {
@@ -547,7 +602,9 @@ int ouro_run_trace(ouro_handle h, const ouro_trace_config* cfg, ouro_trace_resul
             live.push_back(poffs[i]);
             lsz.push_back(psizes[i]);
           }
-      check_live(live, lsz, out);
+      ouro_variant_info gi;
+      m.geometry(&gi);
+      check_live(live, lsz, out, gi);
     }
     OURO_TRY(hipEventRecord(ev[4], s));
     if (held) {

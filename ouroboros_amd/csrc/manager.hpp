@@ -34,6 +34,8 @@ struct IManager {
   // workgroups of `block` threads one CU holds at once for the persistent kernels (the smaller of the two)
   virtual int persist_occupancy(uint32_t block) = 0;
   virtual hipError_t query(ouro_state* out) = 0;
+  virtual void geometry(ouro_variant_info* out) = 0;
+  virtual ouro::PoolLayout pool_layout(uint64_t pool_bytes) = 0;
   // immutable queue placement of class k (chain order)
   virtual void class_ring(uint32_t k, uint32_t** ring, uint32_t* ring_size) = 0;
 #ifdef OURO_DIAG
@@ -48,6 +50,12 @@ IManager* make_vapq();
 IManager* make_vacq();
 IManager* make_vlpq();
 IManager* make_vlcq();
+IManager* make_geo6();
+IManager* make_geo7();
+IManager* make_geo8();
+IManager* make_geo9();
+IManager* make_geo10();
+IManager* make_geo11();
 
 // ------------------------------------------------------------------ harness kernels
 template <class M, int W>
@@ -247,6 +255,20 @@ struct Manager final : IManager {
   }
 #endif
 
+  ouro::PoolLayout pool_layout(uint64_t pool_bytes) override { return M::layout(pool_bytes); }
+  void geometry(ouro_variant_info* out) override {
+    using I = decltype(mm.chain.inst);
+    memset(out, 0, sizeof(*out));
+    out->kind = (uint32_t)I::kKind;
+    out->classes = M::kNumClasses;
+    out->smallest_page = I::kSmallest;
+    out->largest_page = M::kLargestPage;
+    out->chunk_bytes = I::kChunkSize;
+    out->queue_entries = I::kQueueEntries;
+    out->page_capacity = ouro::is_vl(I::kKind) ? 0xFFFFFFFFull
+                         : ouro::is_va(I::kKind) ? (uint64_t)I::kQueueEntries * ouro::kQCSpots - 2ull * ouro::kQCSpots
+                                                 : (uint64_t)I::kQueueEntries;
+  }
   hipError_t query(ouro_state* out) override {
     // the manager state is ~8 KiB; stage it in heap memory, it must not be constructed
     std::vector<uint8_t> buf(sizeof(M));

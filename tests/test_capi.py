@@ -29,9 +29,12 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_layout_agrees_with_oracle():
-    for v in ob.VARIANTS:
+    for v in ["PQ", "CQ", "VAPQ", "VACQ", "VLPQ", "VLCQ"]:
         for pool in [400 << 20, 512 << 20, 4 << 30, 64 << 30]:
             assert ob.layout(v, pool) == oracle.layout(v, pool)
+    # the tuner's geometries: smaller queue arrays leave more of the pool to chunks
+    assert ob.layout("PQ-cap1m", 4 << 30)[0] > ob.layout("PQ", 4 << 30)[0]
+    assert ob.layout("VAPQ-cap4k", 4 << 30)[0] > ob.layout("VAPQ", 4 << 30)[0]
 
 
 def test_trace_generator_is_deterministic_and_uniform():
