@@ -12,8 +12,9 @@ The headline is the batched form: persistent kernels with block-wide aggregation
 once in a hipGraph and replayed, with the pool geometry, aggregation width and launch shape chosen
 by the autotuner for this trace. Beside it: the plain per-thread path (the reference's own
 aggregation, one kernel thread per request), the 64-lane wave-aggregated path, p50 / p99 allocation
-latency, the atomic roofline, the batch workload set (256 workloads on a 32 GiB pool) and a glibc
-malloc/free baseline on the host cores.
+latency, the atomic roofline, the batch workload set (256 workloads on a 32 GiB pool), the serving
+form (Poisson request arrivals over a 64 GiB pool, p50 / p99 request latency and Mops/s against the
+plain per-request launch path) and a glibc malloc/free baseline on the host cores.
 
 Prints one JSON line. Only N = 1 is supported.
 """
@@ -118,6 +119,66 @@ def batch_config4(steps: int) -> dict:
     }
 
 
+def stress_config3(rounds: int = 4) -> dict:
+    """Mixed-size skewed stress on the chunk-based virtualized-array queue: 16 GiB pool, 4,194,304
+    threads, Zipf(1.2) sizes over 16 B .. 8 KiB, half of the blocks held across each round boundary;
+    the live set is checked every round. Fragmentation = chunks holding a live block at the worst
+    round end over the ideal packing (per class, ceil(live pages / pages per chunk))."""
+    n, pool = 1 << 22, 16 << 30
+    sizes = ob.gen_sizes(SEED, n, "zipf", 1.2)
+    out = {}
+    for agg in (1, 64):
+        with ob.MemoryManager("VACQ", pool) as mm:
+            r = mm.run_trace(sizes, rounds, agg=agg, check_rounds=True, hold_permille=500, hold_seed=SEED).result
+            chk = mm.check_quiescent()
+        out[f"agg{agg}"] = {
+            "mops": round(mops(r, n, rounds), 2),
+            "peak_live_chunks": r["peak_live_chunks"],
+            "ideal_chunks": r["ideal_chunks"],
+            "fragmentation": round(r["peak_live_chunks"] / r["ideal_chunks"], 4) if r["ideal_chunks"] else None,
+            "peak_bump_chunks": r["peak_chunks"],
+            "invariants": {k: r[k] for k in ("verify_errors", "bounds_errors", "overlap_errors", "align_errors",
+                                              "null_returns")} | {"device_error_bits": r["errors"],
+                                                                  "quiescent_violations": int(chk.violations)},
+        }
+    return {"kind": "VACQ", "threads": n, "pool_bytes": pool, "rounds": rounds, "zipf": 1.2, "held": 0.5,
+            "runs": out}
+
+
+SERVE_RATES = (2000.0, 5000.0, 10000.0, 20000.0, 50000.0)
+
+
+def serve_config5(kind: str, requests: int) -> dict:
+    """The serving form on a 64 GiB device-resident pool: Poisson arrivals at each offered load, each
+    request one kernel of 2^U[10, 18] threads allocating Zipf(1.1) sizes over 16 B .. 8 KiB, writing,
+    verifying and freeing. Continuous batching (16 slots, 64-lane aggregation, admission by the
+    device free-page count) against the plain per-request launch path (one request at a time,
+    per-thread aggregation), on the same arrival schedule."""
+    pool = 64 << 30
+    rows = {}
+    with ob.MemoryManager(kind, pool) as mm:
+        mm.serve(16, rate=1e5, slots=16, agg=64)  # warm the code objects and the pool's classes
+        for rate in SERVE_RATES:
+            row = {}
+            for name, kw in (("batching", dict(slots=16, agg=64, admit=True)),
+                             ("plain", dict(slots=1, agg=1, admit=False))):
+                log(f"serve {kind} rate={rate:.0f}/s {name} ...")
+                r = mm.serve(requests, rate=rate, zipf=1.1, min_log2=10, max_log2=18, seed=SEED, **kw)
+                row[name] = {
+                    "p50_ms": round(r["p50_ms"], 4), "p99_ms": round(r["p99_ms"], 4),
+                    "mops": round(r["mops"], 2), "achieved_req_per_s": round(r["achieved_rate"], 1),
+                    "max_in_flight": r["max_in_flight"], "deferred": r["deferred"],
+                    "mismatches": r["mismatches"], "nulls": r["nulls"], "device_error_bits": r["errors"],
+                }
+            b, p = row["batching"], row["plain"]
+            row["p99_speedup"] = round(p["p99_ms"] / b["p99_ms"], 3) if b["p99_ms"] else None
+            row["mops_speedup"] = round(b["mops"] / p["mops"], 3) if p["mops"] else None
+            rows[f"{rate:.0f}"] = row
+        chk = mm.check_quiescent()
+    return {"kind": kind, "pool_bytes": pool, "requests_per_load": requests,
+            "quiescent_violations": int(chk.violations), "loads": rows}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -127,6 +188,9 @@ def main() -> int:
     ap.add_argument("--no-table", action="store_true", help="skip the per-kind plain / wave / batched table")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-config4", action="store_true", help="skip the 256-workload batch set")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving form")
+    ap.add_argument("--no-config3", action="store_true", help="skip the held-block stress run")
+    ap.add_argument("--serve-requests", type=int, default=400)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one MI355X only; --gpus {args.gpus} is not supported (the allocator "
@@ -181,6 +245,8 @@ def main() -> int:
     atom_distinct = ob.bench_atomics(1 << 20, 64, "distinct")
     atom_same_wave = ob.bench_atomics(1 << 20, 16, "per_wave")
     cfg4 = None if args.no_config4 else batch_config4(steps)
+    cfg5 = None if args.no_serve else serve_config5(kind, args.serve_requests)
+    cfg3 = None if args.no_config3 else stress_config3()
 
     # ---- roofline of the dominant kernel of the headline
     alloc_avg_s = head["alloc_ms"] / steps * 1e-3
@@ -260,7 +326,9 @@ def main() -> int:
             "release) vs the measured distinct-address agent-scope u32 fetch-add-with-return rate",
         },
         "cpu_baseline": cpu,
+        "stress_config3": cfg3,
         "batch_config4": cfg4,
+        "serve_config5": cfg5,
         "tuner": None if tuned is None else {
             "best": tuned.best.label(),
             "excluded": tuned.excluded,

@@ -78,7 +78,7 @@ template <class I>
 __device__ inline void* block_take_page(Memory& m, I& inst, ReuseRing& base, bool here, uint32_t qi, uint32_t pos) {
   Ctx<I> x{m, inst, base};
   ClassQueue* cq = &inst.q[here ? qi : 0u];
-  const uint32_t idx = take_lanes(x, cq, here, pos);
+  const uint32_t idx = take_lanes<64>(x, cq, here, pos);
   if (!here || idx == kMarker || idx == kNullPage) return nullptr;
   if (!chunk_ok(m, index_chunk(idx)) || index_page(idx) >= ppc_of(x, cq)) {
     set_error(m, ERR_CORRUPT);

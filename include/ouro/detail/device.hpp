@@ -43,13 +43,28 @@ constexpr uint64_t kSpinTicks = 200000000ull;
 
 #ifdef OURO_DIAG
 // Diagnostic event counters (diagnostic builds only).
-static __device__ unsigned long long g_diag[16];
+// [0, 16) counters, [16] events logged, [17, 96) the first 79 event words.
+static __device__ unsigned long long g_diag[96];
 #define OURO_DIAG_ADD(i, n) atomicAdd(&::ouro::g_diag[i], (unsigned long long)(n))
+#define OURO_DIAG_LOG(v)                                                     \
+  do {                                                                       \
+    const unsigned long long _i = atomicAdd(&::ouro::g_diag[16], 1ull);      \
+    if (_i < 79) atomicExch(&::ouro::g_diag[17 + _i], (unsigned long long)(v)); \
+  } while (0)
 #else
 #define OURO_DIAG_ADD(i, n) ((void)0)
+#define OURO_DIAG_LOG(v) ((void)0)
 #endif
 
 namespace dev {
+
+// A freed block's contents may still sit dirty in the L2 of the XCD that wrote them; each of the
+// eight XCDs has its own L2, and agent-scope atomics do not write those lines back. If the block
+// were handed to a wave on another XCD first, a later write-back of the old lines would overwrite
+// the new owner's data. free() therefore releases at agent scope (buffer_wbl2 sc1, then waits)
+// before it publishes a block, which CUDA's device-coherent L2 makes unnecessary on the reference's
+// hardware. A new owner needs no acquire: it writes a block before it reads it back.
+__device__ __forceinline__ void release_block_contents() { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent"); }
 
 // ------------------------------------------------------------------ agent-scope atomics
 template <class T>

@@ -51,6 +51,12 @@ namespace detail {
 using namespace dev;
 
 __device__ __forceinline__ void set_error(Memory& m, uint32_t bits) { or_(&m.error, bits); }
+// a bounded wait gave up; diagnostic builds also log the source line of the wait
+#define OURO_TIMEOUT(mem)                                         \
+  do {                                                            \
+    OURO_DIAG_LOG((3ull << 60) | (unsigned long long)__LINE__);   \
+    ::ouro::detail::set_error((mem), ::ouro::ERR_TIMEOUT);        \
+  } while (0)
 
 __device__ __forceinline__ uint8_t* chunk_addr(const Memory& m, uint32_t c) {
   return m.d_data + (uint64_t)c * kStride;
@@ -96,7 +102,7 @@ __device__ inline void ring_put(Memory& m, ReuseRing& r, bool want, uint32_t val
       uint32_t* slot = &r.ring[(pos + rk) & (r.size - 1)];
       Spin it(&m.error);
       while (cas(slot, kMarker, value) != kMarker) {
-        if (it.expired()) { set_error(m, ERR_TIMEOUT); break; }
+        if (it.expired()) { OURO_TIMEOUT(m); break; }
         backoff(it.it);
       }
     }
@@ -122,7 +128,7 @@ __device__ inline uint32_t ring_take(Memory& m, ReuseRing& r, uint32_t k, int l,
     Spin it(&m.error);
     uint32_t v;
     while ((v = xchg(slot, kMarker)) == kMarker) {
-      if (it.expired()) { set_error(m, ERR_TIMEOUT); break; }
+      if (it.expired()) { OURO_TIMEOUT(m); break; }
       backoff(it.it);
     }
     out = v;
@@ -251,7 +257,7 @@ __device__ inline void vl_set_back(Ctx<I>& x, ClassQueue* cq, uint32_t c) {
     if (cas(&cq->back_qc, c, nx) != c) break;
     if ((ld(&hdr<QueueChunkHdr>(x.m, nx)->count) & 0xFFFFu) == kQCSpots) c = nx;
     else break;
-    if (it.expired()) { set_error(x.m, ERR_TIMEOUT); break; }
+    if (it.expired()) { OURO_TIMEOUT(x.m); break; }
   }
 }
 template <class I>
@@ -307,7 +313,7 @@ __device__ inline void vl_set_old(Ctx<I>& x, ClassQueue* cq, bool want, uint32_t
         } else if (chunk_ok(x.m, nx)) {
           cur = seen;
         }
-        if (it.expired()) { set_error(x.m, ERR_TIMEOUT); free_count = 0; }
+        if (it.expired()) { OURO_TIMEOUT(x.m); free_count = 0; }
       }
     }
   }
@@ -329,7 +335,7 @@ __device__ inline uint32_t vl_locate(Ctx<I>& x, uint32_t start, uint32_t v) {
     if (vs - v / kQCSpots < 0x80000000u) return kMarker;  // already past v: its queue chunk is gone
     uint32_t nx;
     while ((nx = ld(&hdr<QueueChunkHdr>(x.m, c)->next)) == kMarker) {
-      if (it.expired()) { set_error(x.m, ERR_TIMEOUT); return kMarker; }
+      if (it.expired()) { OURO_TIMEOUT(x.m); return kMarker; }
       backoff(it.it);
     }
     if (!chunk_ok(x.m, nx)) {
@@ -338,7 +344,7 @@ __device__ inline uint32_t vl_locate(Ctx<I>& x, uint32_t start, uint32_t v) {
     }
     c = nx;
     OURO_DIAG_ADD(10, 1);
-    if (it.expired()) { set_error(x.m, ERR_TIMEOUT); return kMarker; }
+    if (it.expired()) { OURO_TIMEOUT(x.m); return kMarker; }
   }
   return c;
 }
@@ -357,7 +363,7 @@ __device__ inline uint32_t va_wait_qc(Ctx<I>& x, ClassQueue* cq, uint32_t v) {
   uint32_t qc;
   Spin it(&x.m.error);
   while ((qc = ld(&ring_of(x, cq)[id])) == kMarker || is_qc_tag(qc)) {
-    if (it.expired()) { set_error(x.m, ERR_TIMEOUT); return kMarker; }
+    if (it.expired()) { OURO_TIMEOUT(x.m); return kMarker; }
     backoff(it.it);
   }
   if (!chunk_ok(x.m, qc) || ld(&hdr<QueueChunkHdr>(x.m, qc)->virtual_start) / kQCSpots != v / kQCSpots) {
@@ -401,10 +407,14 @@ __device__ inline void compact(Ctx<I>& x, ClassQueue* cq) {
         if (vs - fq >= 0x80000000u) return;  // front_ptr_ still behind f: its move is in flight
         nf = vs * kQCSpots;
       } else if (ld(&qc_slots(x.m, c)[f % kQCSpots]) != kTomb) {
+        OURO_DIAG_ADD(14, 1);
         return;
       }
     }
-    if (cas(&cq->front, f, nf) != f) return;
+    if (cas(&cq->front, f, nf) != f) {
+      OURO_DIAG_ADD(15, 1);
+      return;
+    }
     // the ring slot becomes writable again for position f + ring_size
     if constexpr (is_std(K)) cas(&ring_of(x, cq)[f & (I::kQueueEntries - 1)], kTomb, kMarker);
     f = nf;
@@ -420,8 +430,10 @@ __device__ inline void compact(Ctx<I>& x, ClassQueue* cq) {
 //              guaranteeWarpSyncPerChunk (:7-72) so a lane never waits on a link that a lane of
 //              the same wave sets later.
 // Queue-chunk allocation happens in a pre-phase in lane order, so chunk indices are handed out
-// in the same order a sequential replay would.
-template <class I>
+// in the same order a sequential replay would. A queue chunk's counter (QueueChunk count_) takes one
+// atomic per queue chunk of a G-lane group (G = 1: one per lane, as the reference), led by the
+// group's lowest lane.
+template <int G, class I>
 __device__ inline void put_lanes(Ctx<I>& x, ClassQueue* cq, bool want, uint32_t v, uint32_t val) {
   constexpr Kind K = I::kKind;
   if constexpr (is_std(K)) {
@@ -431,7 +443,7 @@ __device__ inline void put_lanes(Ctx<I>& x, ClassQueue* cq, bool want, uint32_t 
       while (cas(slot, kMarker, val) != kMarker) {
         // a chunk queue's slot may still hold the tombstone of position v - ring_size
         if constexpr (is_chunk_kind(K)) compact(x, cq);
-        if (it.expired()) { set_error(x.m, ERR_TIMEOUT); break; }
+        if (it.expired()) { OURO_TIMEOUT(x.m); break; }
         backoff(it.it);
       }
     }
@@ -471,20 +483,25 @@ __device__ inline void put_lanes(Ctx<I>& x, ClassQueue* cq, bool want, uint32_t 
     if constexpr (is_va(K)) {
       bool reclaim = false;
       uint32_t victim = 0;
+      uint32_t c = kMarker;
       if (want) {
-        const uint32_t c = va_wait_qc(x, cq, v);
+        c = va_wait_qc(x, cq, v);
         if (c != kMarker) {
           uint32_t* slot = &qc_slots(x.m, c)[v % kQCSpots];
           Spin it(&x.m.error);
           while (cas(slot, kMarker, val) != kMarker) {
-            if (it.expired()) { set_error(x.m, ERR_TIMEOUT); break; }
+            if (it.expired()) { OURO_TIMEOUT(x.m); break; }
             backoff(it.it);
           }
-          const uint32_t old = add(&hdr<QueueChunkHdr>(x.m, c)->count, 0x10001u);
-          if (qc_empty_after_enqueue(old, 0x10001u)) {
-            victim = xchg(&ring_of(x, cq)[(v / kQCSpots) & (I::kQueueEntries - 1)], qc_tag(v / kQCSpots));
-            reclaim = chunk_ok(x.m, victim);
-          }
+        }
+      }
+      const uint64_t grp = match<G>(c != kMarker ? (uint64_t)c : ~0ull);
+      if (c != kMarker && (int)lane() == first(grp)) {
+        const uint32_t added = popc(grp) * 0x10001u;
+        const uint32_t old = add(&hdr<QueueChunkHdr>(x.m, c)->count, added);
+        if (qc_empty_after_enqueue(old, added)) {
+          victim = xchg(&ring_of(x, cq)[(v / kQCSpots) & (I::kQueueEntries - 1)], qc_tag(v / kQCSpots));
+          reclaim = chunk_ok(x.m, victim);
         }
       }
       ring_put(x.m, x.base_ring, reclaim, victim);
@@ -494,7 +511,7 @@ __device__ inline void put_lanes(Ctx<I>& x, ClassQueue* cq, bool want, uint32_t 
       bool todo = want && chunk_ok(x.m, cur);
       if (want && !todo) set_error(x.m, ERR_VSTART);
       bool old_want = false;
-      uint32_t old_free = 0;
+      uint32_t old_free = 0, mine = kMarker;
       Spin it(&x.m.error);
       while (ballot(todo)) {
         if (todo) {
@@ -503,12 +520,7 @@ __device__ inline void put_lanes(Ctx<I>& x, ClassQueue* cq, bool want, uint32_t 
           if (ld(&h->virtual_start) / kQCSpots == v / kQCSpots) {
             if (v % kQCSpots == 0 && qc != kMarker) xchg(&h->next, qc);
             xchg(&qc_slots(x.m, cur)[v % kQCSpots], val);
-            const uint32_t old = add(&h->count, 0x10001u);
-            if ((old & 0xFFFFu) + 1u == kQCSpots) vl_set_back(x, cq, cur);
-            if (qc_empty_after_enqueue(old, 0x10001u)) {
-              old_free = vl_set_front(x, cq, cur);
-              old_want = true;
-            }
+            mine = cur;
             todo = false;
           }
         }
@@ -519,11 +531,24 @@ __device__ inline void put_lanes(Ctx<I>& x, ClassQueue* cq, bool want, uint32_t 
             backoff(it.it);
           }
           if (!chunk_ok(x.m, nx)) {
-            set_error(x.m, nx == kMarker ? ERR_TIMEOUT : ERR_VSTART);
+            if (nx == kMarker) OURO_TIMEOUT(x.m);
+            else set_error(x.m, ERR_VSTART);
             todo = false;
           } else {
             cur = nx;
           }
+        }
+      }
+      // every slot and link of the wave is written: count per queue chunk, then move back_ptr_ /
+      // front_ptr_ over a queue chunk this completed
+      const uint64_t grp = match<G>(mine != kMarker ? (uint64_t)mine : ~0ull);
+      if (mine != kMarker && (int)lane() == first(grp)) {
+        const uint32_t n = popc(grp);
+        const uint32_t old = add(&hdr<QueueChunkHdr>(x.m, mine)->count, n * 0x10001u);
+        if ((old & 0xFFFFu) + n == kQCSpots) vl_set_back(x, cq, mine);
+        if (qc_empty_after_enqueue(old, n * 0x10001u)) {
+          old_free = vl_set_front(x, cq, mine);
+          old_want = true;
         }
       }
       vl_set_old(x, cq, old_want, old_free);
@@ -533,8 +558,8 @@ __device__ inline void put_lanes(Ctx<I>& x, ClassQueue* cq, bool want, uint32_t 
 
 // take_lanes: every lane with want=true removes the element at virtual position v (spinning until
 // it has been published). PageQueue::dequeue (PageQueue_impl.cuh:84-104), PageQueueVA::allocPage
-// (:202-242), QueueChunk::dequeue (QueueChunk_impl.cuh:323-382).
-template <class I>
+// (:202-242), QueueChunk::dequeue (QueueChunk_impl.cuh:323-382). Queue-chunk counters as put_lanes.
+template <int G, class I>
 __device__ inline uint32_t take_lanes(Ctx<I>& x, ClassQueue* cq, bool want, uint32_t v) {
   constexpr Kind K = I::kKind;
   uint32_t val = kMarker;
@@ -543,50 +568,44 @@ __device__ inline uint32_t take_lanes(Ctx<I>& x, ClassQueue* cq, bool want, uint
       uint32_t* slot = &ring_of(x, cq)[v & (I::kQueueEntries - 1)];
       Spin it(&x.m.error);
       while ((val = xchg(slot, kMarker)) == kMarker) {
-        if (it.expired()) { set_error(x.m, ERR_TIMEOUT); break; }
+        if (it.expired()) { OURO_TIMEOUT(x.m); break; }
         backoff(it.it);
       }
     }
-  } else if constexpr (is_va(K)) {
-    bool reclaim = false;
-    uint32_t victim = 0;
-    if (want) {
-      const uint32_t c = va_wait_qc(x, cq, v);
-      if (c != kMarker) {
-        uint32_t* slot = &qc_slots(x.m, c)[v % kQCSpots];
-        Spin it(&x.m.error);
-        while ((val = xchg(slot, kMarker)) == kMarker) {
-          if (it.expired()) { set_error(x.m, ERR_TIMEOUT); break; }
-          backoff(it.it);
-        }
-        const uint32_t old = sub(&hdr<QueueChunkHdr>(x.m, c)->count, 1u << 16);
-        if (qc_empty_after_dequeue(old)) {
-          victim = xchg(&ring_of(x, cq)[(v / kQCSpots) & (I::kQueueEntries - 1)], qc_tag(v / kQCSpots));
-          reclaim = chunk_ok(x.m, victim);
-        }
-      }
-    }
-    ring_put(x.m, x.base_ring, reclaim, victim);
   } else {
-    bool old_want = false;
-    uint32_t old_free = 0;
+    uint32_t c = kMarker;
     if (want) {
-      const uint32_t c = vl_locate(x, ld(&cq->front_qc), v);
+      if constexpr (is_va(K)) c = va_wait_qc(x, cq, v);
+      else c = vl_locate(x, ld(&cq->front_qc), v);
       if (c != kMarker) {
         uint32_t* slot = &qc_slots(x.m, c)[v % kQCSpots];
         Spin it(&x.m.error);
         while ((val = xchg(slot, kMarker)) == kMarker) {
-          if (it.expired()) { set_error(x.m, ERR_TIMEOUT); break; }
+          if (it.expired()) { OURO_TIMEOUT(x.m); break; }
           backoff(it.it);
-        }
-        const uint32_t old = sub(&hdr<QueueChunkHdr>(x.m, c)->count, 1u << 16);
-        if (qc_empty_after_dequeue(old)) {
-          old_free = vl_set_front(x, cq, c);
-          old_want = true;
         }
       }
     }
-    vl_set_old(x, cq, old_want, old_free);
+    const uint64_t grp = match<G>(c != kMarker ? (uint64_t)c : ~0ull);
+    const bool lead = c != kMarker && (int)lane() == first(grp);
+    uint32_t old = 0, taken = 0;
+    if (lead) {
+      taken = popc(grp) << 16;
+      old = sub(&hdr<QueueChunkHdr>(x.m, c)->count, taken);
+    }
+    const bool emptied = lead && old - taken == kQCSpots;  // QueueChunk::checkChunkEmptyDequeue
+    if constexpr (is_va(K)) {
+      uint32_t victim = 0;
+      bool reclaim = false;
+      if (emptied) {
+        victim = xchg(&ring_of(x, cq)[(v / kQCSpots) & (I::kQueueEntries - 1)], qc_tag(v / kQCSpots));
+        reclaim = chunk_ok(x.m, victim);
+      }
+      ring_put(x.m, x.base_ring, reclaim, victim);
+    } else {
+      const uint32_t old_free = emptied ? vl_set_front(x, cq, c) : 0u;
+      vl_set_old(x, cq, emptied, old_free);
+    }
   }
   return val;
 }
@@ -696,7 +715,7 @@ __device__ inline uint32_t carve_pages(Ctx<I>& x, ClassQueue* cq, uint32_t k) {
     const uint32_t e = e0 + me;
     const uint32_t j = (e < n_real ? e : (n_real ? n_real - 1 : 0)) / ppc;
     const uint32_t c = shfl(chunk, lane_of_rank(act, j));
-    put_lanes(x, cq, e < n, v0 + e, e < n_real ? make_index(c, e % ppc) : kNullPage);
+    put_lanes<64>(x, cq, e < n, v0 + e, e < n_real ? make_index(c, e % ppc) : kNullPage);
   }
   if ((int)lane() == l0) sem_signal_expected(cq, n, k * ppc);
   return k_ok;
@@ -716,7 +735,7 @@ __device__ inline void* alloc_pages(Ctx<I>& x, uint32_t qi, bool want) {
   }
   uint32_t n;
   const uint32_t v = fetch_positions<GS>(&cq->front, want, n);
-  const uint32_t idx = take_lanes(x, cq, want, v);
+  const uint32_t idx = take_lanes<W>(x, cq, want, v);
   if (!want || idx == kMarker || idx == kNullPage) return nullptr;
   if (!chunk_ok(x.m, index_chunk(idx)) || index_page(idx) >= ppc_of(x, cq)) {
     set_error(x.m, ERR_CORRUPT);
@@ -731,7 +750,7 @@ __device__ inline void free_pages(Ctx<I>& x, uint32_t qi, bool want, uint32_t ch
   ClassQueue* cq = &x.inst.q[want ? qi : 0];
   uint32_t n;
   const uint32_t v = fetch_positions<GS>(&cq->back, want, n);
-  put_lanes(x, cq, want, v, make_index(chunk, page));
+  put_lanes<W>(x, cq, want, v, make_index(chunk, page));
   // publish, then signal (the reference signals first, PageQueue_impl.cuh:24-31)
   if constexpr (W == 1) {
     if (want) sem_signal(cq, 1);
@@ -761,7 +780,7 @@ __device__ inline void enqueue_chunks(Ctx<I>& x, ClassQueue* cq, bool want, uint
   const uint32_t v = fetch_positions<G>(&cq->back, want, n);
   if (want) st(&hdr<ChunkHdr>(x.m, chunk)->queue_pos, v);
   order();
-  put_lanes(x, cq, want, v, chunk);
+  put_lanes<G>(x, cq, want, v, chunk);
 }
 
 template <class I>
@@ -940,18 +959,20 @@ __device__ inline void ensure_in_queue(Ctx<I>& x, ClassQueue* cq, bool want, uin
 
 // Remove an exhausted chunk from its class queue, wherever it sits (the reference removes with
 // atomicMax(front_, v + 1), ChunkQueue_impl.cuh:432, which can strand live chunks in front of v).
-// The chunk's own queue_pos names its position: the CAS queue_pos -> MARKER admits one remover
-// and ties the removal to that queue entry, whatever slot the caller happened to read. Then the
-// slot is tombstoned and front_ compacted. Returns true if removed; `again` reports a free that
-// raced in (the caller re-enqueues: the free saw queue_pos != MARKER and left it to us).
+// `pos` is the queue position the caller found the chunk at: the CAS queue_pos pos -> MARKER admits
+// one remover and only for that entry. A caller that took the chunk's last page from an entry that
+// has since been removed must not remove the chunk's next entry: a free may already have re-enqueued
+// it at a position whose slot is still being written. Then the slot is tombstoned and front_
+// compacted. Returns true if removed; `again` reports a free that raced in (the caller re-enqueues:
+// the free saw queue_pos != MARKER and left it to us).
 template <class I>
-__device__ inline bool cq_remove(Ctx<I>& x, ClassQueue* cq, uint32_t chunk, uint32_t& cursor, uint32_t& reclaim,
-                                 bool& again) {
+__device__ inline bool cq_remove(Ctx<I>& x, ClassQueue* cq, uint32_t chunk, uint32_t pos, uint32_t& cursor,
+                                 uint32_t& reclaim, bool& again) {
   again = false;
   OURO_DIAG_ADD(7, 1);
   ChunkHdr* h = hdr<ChunkHdr>(x.m, chunk);
-  const uint32_t p = ld(&h->queue_pos);
-  if (p == kMarker || p == kPending || cas(&h->queue_pos, p, kMarker) != p) return false;
+  const uint32_t p = pos;
+  if (cas(&h->queue_pos, p, kMarker) != p) return false;
   OURO_DIAG_ADD(8, 1);
   uint32_t rc = kMarker;
   tomb_slot(x, cq, p, cursor, rc);
@@ -977,7 +998,7 @@ __device__ inline AccessMode access_alloc_lane(Ctx<I>& x, ChunkHdr* h, uint32_t&
   while ((cur = sub(&h->count, 1)) <= 0) {
     if ((cur = add(&h->count, 1)) < 0) return AccessMode::CONTINUE;
     if (cur == 0) mode = AccessMode::RE_ENQUEUE;
-    if (it.expired()) { set_error(x.m, ERR_TIMEOUT); return AccessMode::ERROR; }
+    if (it.expired()) { OURO_TIMEOUT(x.m); return AccessMode::ERROR; }
   }
   if (cur == 1) mode = AccessMode::DEQUEUE;
   const uint32_t nw = mask_words(ld(&h->size));
@@ -997,7 +1018,7 @@ __device__ inline AccessMode access_alloc_lane(Ctx<I>& x, ChunkHdr* h, uint32_t&
       }
       cm = old & ~(1ull << b);
     }
-    if (it.expired()) { set_error(x.m, ERR_TIMEOUT); return AccessMode::ERROR; }
+    if (it.expired()) { OURO_TIMEOUT(x.m); return AccessMode::ERROR; }
   }
 }
 
@@ -1094,7 +1115,7 @@ struct ChunkClaim {
           const uint32_t size = ld(&h->size);
           w = ((size - (uint32_t)cnt) / 64u) % mask_words(size);
           bool again = false;
-          if (cnt == (int32_t)t && cq_remove(x, cq, s, cursor, reclaim, again) && again)
+          if (cnt == (int32_t)t && cq_remove(x, cq, s, v + i, cursor, reclaim, again) && again)
             ensure_in_queue<1>(x, cq, true, s);
           v += cnt == (int32_t)t ? i + 1u : i;
         } else {
@@ -1127,7 +1148,7 @@ struct ChunkClaim {
       }
     }
     if (it.expired()) {
-      set_error(x.m, ERR_TIMEOUT);
+      OURO_TIMEOUT(x.m);
       gave_up = true;
     }
   }
@@ -1188,13 +1209,13 @@ __device__ inline void* alloc_chunk_pages(Ctx<I>& x, uint32_t qi, bool want) {
             got = true;
             bool again = false;
             if (md == AccessMode::RE_ENQUEUE) ensure_in_queue<1>(x, cq, true, c);
-            if (md == AccessMode::DEQUEUE && cq_remove(x, cq, c, cursor, reclaim, again) && again)
+            if (md == AccessMode::DEQUEUE && cq_remove(x, cq, c, v, cursor, reclaim, again) && again)
               ensure_in_queue<1>(x, cq, true, c);
             break;
           }
         }
         v += n;
-        if (it.expired()) { set_error(x.m, ERR_TIMEOUT); break; }
+        if (it.expired()) { OURO_TIMEOUT(x.m); break; }
         backoff(it.it > 64 ? it.it : 0);
       }
     }

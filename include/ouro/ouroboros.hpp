@@ -219,7 +219,7 @@ __device__ inline void chain_block_release(Memory& m, Chain<I, Rest...>& ch, Reu
       block_release_chunks(m, ch.inst, base, here, qi, chunk, page);
     } else {
       Ctx<I> x{m, ch.inst, base};
-      put_lanes(x, &ch.inst.q[qi], here, here ? sm.base[gc] + r : 0u, make_index(chunk, page));
+      put_lanes<64>(x, &ch.inst.q[qi], here, here ? sm.base[gc] + r : 0u, make_index(chunk, page));
     }
   }
   if constexpr (sizeof...(Rest) > 0) chain_block_release(m, ch.next, base, sm, gc, r, chunk, page, off + I::kNQ);
@@ -284,8 +284,15 @@ struct Ouroboros {
       gc = detail::chain_free_class<Is...>(page_size, offset, page);
       if (gc == kMarker) detail::set_error(memory, ERR_BAD_FREE);
     }
+    // Each wave's stores to the blocks it frees complete at its XCD's L2 before the block barrier
+    // inside block_rank; after it, one agent-scope release by wave 0 writes those lines back for the
+    // whole workgroup (buffer_wbl2 acts on the entire L2 of the XCD), instead of one per wave.
+    dev::order();
     const uint32_t r = detail::block_rank(sm, gc, kNumClasses);
-    if (threadIdx.x < kWave) detail::chain_block_back(chain, sm);
+    if (threadIdx.x < kWave) {
+      if (dev::ballot(dev::lane() < kNumClasses && sm.total[dev::lane()] > 0)) dev::release_block_contents();
+      detail::chain_block_back(chain, sm);
+    }
     __syncthreads();
     detail::chain_block_release(memory, chain, chain.inst.reuse, sm, gc, r, chunk, page);
     __syncthreads();  // every put / release of the block is done: publish, then signal
@@ -317,7 +324,10 @@ struct Ouroboros {
       }
       if (!want) detail::set_error(memory, ERR_BAD_FREE);
     }
-    if (dev::ballot(want)) detail::chain_free<W>(memory, chain, chain.inst.reuse, page_size, want, chunk, offset);
+    if (dev::ballot(want)) {
+      dev::release_block_contents();
+      detail::chain_free<W>(memory, chain, chain.inst.reuse, page_size, want, chunk, offset);
+    }
   }
 
   // Bulk preload of a chunk's pages into a class queue (Ouroboros.cuh:291-295). Single thread,
@@ -350,7 +360,7 @@ struct Ouroboros {
       h->number_pages = pages_per_chunk;
       for (uint32_t p = pages_per_chunk - (uint32_t)available_pages; p < pages_per_chunk; ++p) {
         const uint32_t v = dev::add(&cq->back, 1u);
-        detail::put_lanes(x, cq, true, v, make_index(chunk_index, p));
+        detail::put_lanes<1>(x, cq, true, v, make_index(chunk_index, p));
       }
       dev::add(&cq->carved, (uint32_t)available_pages);
     }

@@ -239,6 +239,49 @@ typedef struct {
 int ouro_gen_batch(const ouro_batch_workload* w, uint32_t* sizes, uint64_t cap, uint64_t* n_out,
                    uint32_t* requests_out, double* zipf_out);
 
+/* ---- serving form (no reference counterpart: the reference has no request loop) ---------------
+ * A host request loop in front of one device-resident pool. Request r arrives at a Poisson time
+ * (rate per second), runs one kernel of 2^e threads (e ~ U[min_log2, max_log2]); thread i
+ * allocates a block of class q ~ Zipf(zipf) over the first `classes` class ranks (16 B << q) through
+ * the per-thread device malloc of width `agg`, fills it with a pattern keyed by (request, i), reads
+ * it back and frees it. Up to `slots` requests run at once, one stream each (slots = 1, agg = 1 is
+ * the plain per-request launch path). With admit = 1 a request waits until the pool's free-chunk
+ * count, copied asynchronously from the device state, covers its estimated demand. Latency is
+ * arrival to kernel completion, on the device clock. */
+typedef struct {
+  double rate;             /* arrivals per second */
+  double zipf;             /* class-rank exponent */
+  uint64_t seed;
+  uint32_t requests;
+  uint32_t slots;          /* requests in flight at most (1 .. 64) */
+  int32_t agg;             /* 1 / 16 / 64 */
+  uint32_t min_log2, max_log2; /* threads per request: 2^U[min, max], max <= 24 */
+  uint32_t classes;        /* size classes drawn (1 .. 10) */
+  int32_t admit;           /* 1 = admission by the device free-chunk count */
+  uint32_t pad;
+  unsigned long long* results_out; /* optional host [requests][4]: words intact, words not, nulls, threads */
+  double* latency_out;     /* optional host [requests] ms */
+} ouro_serve_config;
+
+typedef struct {
+  double p50_ms, p99_ms, mean_ms, max_ms;
+  double wall_ms;          /* first arrival to last completion */
+  double offered_ms;       /* first to last arrival */
+  double mops;             /* (mallocs + frees) / wall */
+  double achieved_rate;    /* requests per second */
+  uint64_t mallocs;
+  uint64_t mismatches;     /* words read back wrong, all requests */
+  uint64_t nulls;
+  uint32_t completed;
+  uint32_t max_in_flight;
+  uint32_t deferred;       /* requests the admission test held back at least once */
+  uint32_t error_word;
+} ouro_serve_result;
+
+int ouro_serve(ouro_handle h, const ouro_serve_config* cfg, ouro_serve_result* out);
+/* Threads and intact words request r of `cfg` returns when every allocation succeeds. */
+int ouro_serve_expected(const ouro_serve_config* cfg, uint32_t r, uint64_t* threads, uint64_t* words);
+
 /* ---- test hook (no reference counterpart) --------------------------------------------------- */
 /* The device semaphore's carve decision (BulkSemaphore_impl.cuh:236-263 as changed here) evaluated
  * on the host: *k = chunks a claim of `num` pages must carve, given the word its fetch-add displaced

@@ -301,6 +301,18 @@ static uint32_t vl_locate(ouro_sim* s, uint32_t c, uint32_t v) {
 static void compact(ouro_sim* s, inst_t* I, cls_t* q);
 
 /* ------------------------------------------------------------------ slot sequences */
+/* Queue-chunk counters (QueueChunk count_) change once per queue chunk of a call, at the lowest lane
+ * holding it (the device groups lanes by queue chunk with one atomic per group). group_size returns
+ * the number of lanes holding qc[i] if i is the lowest of them, else 0. */
+static int group_size(const uint32_t* qc, int n, int i) {
+  if (qc[i] == MARKER) return 0;
+  for (int j = 0; j < i; ++j)
+    if (qc[j] == qc[i]) return 0;
+  int m = 0;
+  for (int j = i; j < n; ++j) m += qc[j] == qc[i];
+  return m;
+}
+
 /* put for a set of lanes already sorted in lane order: positions v[i], values val[i].
  * Queue chunks needed by positions at a chunk boundary are allocated first, in lane order
  * (PageQueueVA_impl.cuh:196-229, QueueChunk_impl.cuh:137-185 with the device's pre-phase). */
@@ -336,11 +348,13 @@ static void put_lanes(ouro_sim* s, inst_t* I, cls_t** qs, int n, const uint32_t*
         }
       }
   }
+  uint32_t held[64];
   if (is_va(K)) {
     uint32_t victims[64];
     int nv = 0;
     for (int i = 0; i < n; ++i) {
       cls_t* q = qs[i];
+      held[i] = MARKER;
       const uint32_t id = (v[i] / QC_SPOTS) & (q->ring_size - 1);
       const uint32_t c = q->va[id];
       if (!chunk_ok(s, c) || ch(s, c)->vstart / QC_SPOTS != v[i] / QC_SPOTS) {
@@ -350,33 +364,45 @@ static void put_lanes(ouro_sim* s, inst_t* I, cls_t** qs, int n, const uint32_t*
       chunk_t* x = ch(s, c);
       if (x->slots[v[i] % QC_SPOTS] != MARKER) s->error |= ERR_TIMEOUT;
       x->slots[v[i] % QC_SPOTS] = val[i];
-      const uint32_t old = x->qc_count;
-      x->qc_count += 0x10001u;
-      if (old + 0x10001u == QC_SPOTS) {
-        victims[nv++] = q->va[id];
-        q->va[id] = qtag(v[i] / QC_SPOTS);
+      held[i] = c;
+    }
+    for (int i = 0; i < n; ++i) {
+      const int m = group_size(held, n, i);
+      if (!m) continue;
+      chunk_t* x = ch(s, held[i]);
+      const uint32_t added = (uint32_t)m * 0x10001u, old = x->qc_count;
+      x->qc_count += added;
+      if (old + added == QC_SPOTS) {
+        const uint32_t id = (v[i] / QC_SPOTS) & (qs[i]->ring_size - 1);
+        victims[nv++] = qs[i]->va[id];
+        qs[i]->va[id] = qtag(v[i] / QC_SPOTS);
       }
     }
     for (int i = 0; i < nv; ++i) ring_put(s, &s->inst[0].reuse, victims[i]);
     return;
   }
-  /* VL: walk, link at local position 0, write, count */
+  /* VL: walk, link at local position 0, write; then the counters and the back / front pointers */
   uint32_t oldf[64];
   int oldw[64];
   for (int i = 0; i < n; ++i) {
     cls_t* q = qs[i];
     oldw[i] = 0;
     oldf[i] = 0;
-    const uint32_t c = vl_locate(s, q->back_qc, v[i]);
-    if (c == MARKER) continue;
-    chunk_t* x = ch(s, c);
+    held[i] = vl_locate(s, q->back_qc, v[i]);
+    if (held[i] == MARKER) continue;
+    chunk_t* x = ch(s, held[i]);
     if (v[i] % QC_SPOTS == 0 && qc[i] != MARKER) x->next = qc[i];
     x->slots[v[i] % QC_SPOTS] = val[i];
-    const uint32_t old = x->qc_count;
-    x->qc_count += 0x10001u;
-    if ((old & 0xFFFFu) + 1u == QC_SPOTS) vl_set_back(s, q, c);
-    if (old + 0x10001u == QC_SPOTS) {
-      oldf[i] = vl_set_front(s, q, c);
+  }
+  for (int i = 0; i < n; ++i) {
+    const int m = group_size(held, n, i);
+    if (!m) continue;
+    chunk_t* x = ch(s, held[i]);
+    const uint32_t added = (uint32_t)m * 0x10001u, old = x->qc_count;
+    x->qc_count += added;
+    if ((old & 0xFFFFu) + (uint32_t)m == QC_SPOTS) vl_set_back(s, qs[i], held[i]);
+    if (old + added == QC_SPOTS) {
+      oldf[i] = vl_set_front(s, qs[i], held[i]);
       oldw[i] = 1;
     }
   }
@@ -397,52 +423,50 @@ static void take_lanes(ouro_sim* s, inst_t* I, cls_t** qs, int n, const uint32_t
     }
     return;
   }
-  if (is_va(K)) {
-    uint32_t victims[64];
-    int nv = 0;
-    for (int i = 0; i < n; ++i) {
-      cls_t* q = qs[i];
-      out[i] = MARKER;
-      const uint32_t id = (v[i] / QC_SPOTS) & (q->ring_size - 1);
-      const uint32_t c = q->va[id];
+  uint32_t held[64];
+  for (int i = 0; i < n; ++i) {
+    cls_t* q = qs[i];
+    out[i] = MARKER;
+    held[i] = MARKER;
+    uint32_t c;
+    if (is_va(K)) {
+      c = q->va[(v[i] / QC_SPOTS) & (q->ring_size - 1)];
       if (!chunk_ok(s, c) || ch(s, c)->vstart / QC_SPOTS != v[i] / QC_SPOTS) {
         s->error |= ERR_VSTART;
         continue;
       }
-      chunk_t* x = ch(s, c);
-      out[i] = x->slots[v[i] % QC_SPOTS];
-      if (out[i] == MARKER) s->error |= ERR_TIMEOUT;
-      x->slots[v[i] % QC_SPOTS] = MARKER;
-      const uint32_t old = x->qc_count;
-      x->qc_count -= 0x10000u;
-      if (old - 0x10000u == QC_SPOTS) {
-        victims[nv++] = q->va[id];
-        q->va[id] = qtag(v[i] / QC_SPOTS);
-      }
+    } else {
+      c = vl_locate(s, q->front_qc, v[i]);
+      if (c == MARKER) continue;
     }
-    for (int i = 0; i < nv; ++i) ring_put(s, &s->inst[0].reuse, victims[i]);
-    return;
-  }
-  uint32_t oldf[64];
-  int oldw[64];
-  for (int i = 0; i < n; ++i) {
-    cls_t* q = qs[i];
-    out[i] = MARKER;
-    oldw[i] = 0;
-    oldf[i] = 0;
-    const uint32_t c = vl_locate(s, q->front_qc, v[i]);
-    if (c == MARKER) continue;
     chunk_t* x = ch(s, c);
     out[i] = x->slots[v[i] % QC_SPOTS];
     if (out[i] == MARKER) s->error |= ERR_TIMEOUT;
     x->slots[v[i] % QC_SPOTS] = MARKER;
-    const uint32_t old = x->qc_count;
-    x->qc_count -= 0x10000u;
-    if (old - 0x10000u == QC_SPOTS) {
-      oldf[i] = vl_set_front(s, q, c);
+    held[i] = c;
+  }
+  /* counters per queue chunk; an emptied queue chunk is reclaimed (VA) or passed by front_ptr_ (VL) */
+  uint32_t victims[64], oldf[64];
+  int nv = 0, oldw[64];
+  for (int i = 0; i < n; ++i) {
+    oldw[i] = 0;
+    oldf[i] = 0;
+    const int m = group_size(held, n, i);
+    if (!m) continue;
+    chunk_t* x = ch(s, held[i]);
+    const uint32_t taken = (uint32_t)m << 16, old = x->qc_count;
+    x->qc_count -= taken;
+    if (old - taken != QC_SPOTS) continue;
+    if (is_va(K)) {
+      const uint32_t id = (v[i] / QC_SPOTS) & (qs[i]->ring_size - 1);
+      victims[nv++] = qs[i]->va[id];
+      qs[i]->va[id] = qtag(v[i] / QC_SPOTS);
+    } else {
+      oldf[i] = vl_set_front(s, qs[i], held[i]);
       oldw[i] = 1;
     }
   }
+  for (int i = 0; i < nv; ++i) ring_put(s, &s->inst[0].reuse, victims[i]);
   for (int i = 0; i < n; ++i)
     if (oldw[i]) vl_set_old(s, qs[i], oldf[i]);
 }

@@ -194,7 +194,53 @@ class VariantInfo(C.Structure):
     ]
 
 
+class ServeConfig(C.Structure):
+    _fields_ = [
+        ("rate", C.c_double),
+        ("zipf", C.c_double),
+        ("seed", C.c_uint64),
+        ("requests", C.c_uint32),
+        ("slots", C.c_uint32),
+        ("agg", C.c_int32),
+        ("min_log2", C.c_uint32),
+        ("max_log2", C.c_uint32),
+        ("classes", C.c_uint32),
+        ("admit", C.c_int32),
+        ("pad", C.c_uint32),
+        ("results_out", C.POINTER(C.c_ulonglong)),
+        ("latency_out", C.POINTER(C.c_double)),
+    ]
+
+
+class ServeResult(C.Structure):
+    _fields_ = [
+        ("p50_ms", C.c_double),
+        ("p99_ms", C.c_double),
+        ("mean_ms", C.c_double),
+        ("max_ms", C.c_double),
+        ("wall_ms", C.c_double),
+        ("offered_ms", C.c_double),
+        ("mops", C.c_double),
+        ("achieved_rate", C.c_double),
+        ("mallocs", C.c_uint64),
+        ("mismatches", C.c_uint64),
+        ("nulls", C.c_uint64),
+        ("completed", C.c_uint32),
+        ("max_in_flight", C.c_uint32),
+        ("deferred", C.c_uint32),
+        ("error_word", C.c_uint32),
+    ]
+
+    def as_dict(self) -> dict:
+        d = {name: getattr(self, name) for name, _ in self._fields_}
+        d["errors"] = decode_errors(self.error_word)
+        return d
+
+
 SIGNATURES = {
+    "ouro_serve": (C.c_int, [C.c_void_p, C.POINTER(ServeConfig), C.POINTER(ServeResult)]),
+    "ouro_serve_expected": (C.c_int, [C.POINTER(ServeConfig), C.c_uint32, C.POINTER(C.c_uint64),
+                                      C.POINTER(C.c_uint64)]),
     "ouro_variant_geometry": (C.c_int, [C.c_int, C.POINTER(VariantInfo)]),
     "ouro_gen_batch": (C.c_int, [C.POINTER(BatchWorkload), C.POINTER(C.c_uint32), C.c_uint64,
                                  C.POINTER(C.c_uint64), C.POINTER(C.c_uint32), C.POINTER(C.c_double)]),
@@ -288,6 +334,22 @@ def gen_batch(seed: int = 42, workloads: int = 256, mu: float = 10.0, sigma: flo
     _check(lib().ouro_gen_batch(C.byref(w), sizes.ctypes.data_as(C.POINTER(C.c_uint32)), n.value, C.byref(n),
                                 None, None), "ouro_gen_batch")
     return sizes, req, zs
+
+
+def serve_config(requests: int, rate: float, slots: int = 8, agg: int = 64, min_log2: int = 10,
+                 max_log2: int = 18, zipf: float = 1.1, classes: int = 10, seed: int = 1,
+                 admit: bool = True) -> ServeConfig:
+    cfg = ServeConfig()
+    cfg.rate, cfg.zipf, cfg.seed, cfg.requests, cfg.slots, cfg.agg = rate, zipf, seed, requests, slots, agg
+    cfg.min_log2, cfg.max_log2, cfg.classes, cfg.admit = min_log2, max_log2, classes, int(admit)
+    return cfg
+
+
+def serve_expected(cfg: ServeConfig, r: int) -> tuple[int, int]:
+    """(threads, intact words) of request r when every allocation succeeds (host computation)."""
+    t, w = C.c_uint64(), C.c_uint64()
+    _check(lib().ouro_serve_expected(C.byref(cfg), r, C.byref(t), C.byref(w)), "ouro_serve_expected")
+    return t.value, w.value
 
 
 def sem_share(before: int, num: int, ppc: int) -> int:
@@ -405,6 +467,38 @@ class MemoryManager:
         res = BatchResult()
         _check(lib().ouro_run_batch(self._h, C.byref(cfg), C.byref(res)), "ouro_run_batch")
         return res.as_dict()
+
+    def serve(
+        self,
+        requests: int,
+        rate: float,
+        slots: int = 8,
+        agg: int = 64,
+        min_log2: int = 10,
+        max_log2: int = 18,
+        zipf: float = 1.1,
+        classes: int = 10,
+        seed: int = 1,
+        admit: bool = True,
+        per_request: bool = False,
+    ) -> dict:
+        """The serving form: Poisson arrivals at `rate` per second, each request one kernel of
+        2^U[min_log2, max_log2] threads allocating Zipf(zipf) class sizes; up to `slots` in flight.
+        slots = 1, agg = 1 is the plain per-request launch path. With per_request the result holds
+        `results` ([requests, 4]: words intact, words not, nulls, threads) and `latency_ms`."""
+        cfg = serve_config(requests, rate, slots, agg, min_log2, max_log2, zipf, classes, seed, admit)
+        res_buf = np.zeros((requests, 4), np.uint64) if per_request else None
+        lat_buf = np.zeros(requests, np.float64) if per_request else None
+        if per_request:
+            cfg.results_out = res_buf.ctypes.data_as(C.POINTER(C.c_ulonglong))
+            cfg.latency_out = lat_buf.ctypes.data_as(C.POINTER(C.c_double))
+        res = ServeResult()
+        _check(lib().ouro_serve(self._h, C.byref(cfg), C.byref(res)), "ouro_serve")
+        d = res.as_dict()
+        if per_request:
+            d["results"] = res_buf
+            d["latency_ms"] = lat_buf
+        return d
 
     def run_trace(
         self,

@@ -45,6 +45,7 @@ def build(jobs: int | None = None, verbose: bool = False, extra_flags: list[str]
     obj_dir.mkdir(exist_ok=True)
     units = [(CSRC / "variant.hip", obj_dir / f"variant{v}.o", [f"-DOURO_VARIANT={v}"]) for v in range(NUM_VARIANTS)]
     units.append((CSRC / "capi.hip", obj_dir / "capi.o", []))
+    units.append((CSRC / "serve.hip", obj_dir / "serve.o", []))
     cmds = [[HIPCC, *FLAGS, *extra_flags, *extra, "-c", str(src), "-o", str(obj)] for src, obj, extra in units]
     jobs = jobs or min(8, os.cpu_count() or 4)
     with ThreadPoolExecutor(max_workers=jobs) as ex:

@@ -16,13 +16,7 @@
 
 using namespace ouro_lib;
 
-struct ouro_manager {
-  std::unique_ptr<IManager> impl;
-};
-
-namespace {
-
-IManager* make_variant(int v) {
+IManager* ouro_lib::make_variant(int v) {
   switch (v) {
     case OURO_PQ: return make_pq();
     case OURO_CQ: return make_cq();
@@ -40,15 +34,7 @@ IManager* make_variant(int v) {
   }
 }
 
-inline uint64_t splitmix64(uint64_t x) {
-  x += 0x9E3779B97F4A7C15ull;
-  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
-  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
-  return x ^ (x >> 31);
-}
-// Independent stream per (seed, index).
-inline uint64_t keyed(uint64_t seed, uint64_t i) { return splitmix64(splitmix64(seed) ^ (i * 0xD1B54A32D192ED03ull)); }
-inline double unit(uint64_t r) { return (double)(r >> 11) * (1.0 / 9007199254740992.0); }
+namespace {
 
 // Page size a request of `size` bytes receives from a chain whose smallest page is `smallest`.
 inline uint32_t page_of(uint32_t size, uint32_t smallest = 16) {

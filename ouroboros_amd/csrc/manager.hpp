@@ -6,12 +6,27 @@
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
+#include <memory>
 #include <vector>
 
 #include "ouro/instances.hpp"
 #include "ouro_capi.h"
 
 namespace ouro_lib {
+
+struct ZipfTable;
+
+// Free capacity of a pool: `shared` base chunks any class can take (the bump region left and the
+// reuse ring) and, per class in chain order, its free pages (the class semaphore's count).
+struct PoolFree {
+  uint64_t shared;
+  int64_t pages[16];
+  uint32_t page_size[16];
+  uint32_t ppc[16];
+  uint32_t add[16];  // base chunks one chunk of the class spans
+  double cost[16];   // base chunks a fresh chunk of the class takes, with its share of queue chunks
+  uint32_t classes;
+};
 
 struct IManager {
   virtual ~IManager() {}
@@ -33,6 +48,13 @@ struct IManager {
   virtual hipError_t free_persist(void* const* d_ptrs, uint32_t n, hipStream_t s, uint32_t block, uint32_t grid) = 0;
   // workgroups of `block` threads one CU holds at once for the persistent kernels (the smaller of the two)
   virtual int persist_occupancy(uint32_t block) = 0;
+  // Serving form: one request kernel of n threads (see k_request) on stream s.
+  virtual hipError_t launch_request(uint64_t seed, uint32_t n, const struct ZipfTable& z, unsigned long long* out,
+                                    int agg, hipStream_t s) = 0;
+  // What the pool can still hand out, read from a host copy of the manager state (state_bytes()
+  // bytes at device_manager()).
+  virtual void pool_free(const void* state, struct PoolFree* out) = 0;
+  virtual size_t state_bytes() = 0;
   virtual hipError_t query(ouro_state* out) = 0;
   virtual void geometry(ouro_variant_info* out) = 0;
   virtual ouro::PoolLayout pool_layout(uint64_t pool_bytes) = 0;
@@ -42,6 +64,18 @@ struct IManager {
   virtual hipError_t diag_read(unsigned long long* out16, int reset) = 0;
 #endif
 };
+
+// Host trace randomness: SplitMix64, an independent stream per (seed, index).
+inline uint64_t splitmix64(uint64_t x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+inline uint64_t keyed(uint64_t seed, uint64_t i) { return splitmix64(splitmix64(seed) ^ (i * 0xD1B54A32D192ED03ull)); }
+inline double unit(uint64_t r) { return (double)(r >> 11) * (1.0 / 9007199254740992.0); }
+
+IManager* make_variant(int v);
 
 // Factories, one per variant translation unit.
 IManager* make_pq();
@@ -120,6 +154,66 @@ __global__ void __launch_bounds__(1024) k_free_persist(M* mm, void* const* __res
 
 inline uint32_t grid_for(uint32_t n, uint32_t block) { return (n + block - 1) / block; }
 
+// ------------------------------------------------------------------ serving requests
+// Zipf draw over the class ranks: class q is the first with r < thr[q] (r a 32-bit random number).
+struct ZipfTable {
+  uint32_t thr[16];
+  uint32_t classes;
+};
+
+__device__ __forceinline__ uint64_t dev_splitmix64(uint64_t x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+__device__ __forceinline__ uint64_t dev_keyed(uint64_t seed, uint64_t i) {
+  return dev_splitmix64(dev_splitmix64(seed) ^ (i * 0xD1B54A32D192ED03ull));
+}
+
+// One request of the serving form, an allocation-heavy user kernel: thread i allocates a block of a
+// Zipf-drawn class (16 B << q), fills it with a pattern keyed by (request seed, i), reads it back
+// and frees it. out[0] += words read back intact, out[1] += words that were not, out[2] += null
+// returns: a request's result depends only on its seed and size, never on what runs beside it.
+template <class M, int W>
+__global__ void __launch_bounds__(256) k_request(M* mm, uint64_t seed, uint32_t n, ZipfTable z,
+                                                 unsigned long long* __restrict__ out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long ok = 0, bad = 0, nul = 0;
+  if (i < n) {
+    const uint64_t key = dev_keyed(seed, i);
+    const uint32_t r = (uint32_t)(key >> 32);
+    uint32_t q = 0;
+    while (q + 1 < z.classes && r >= z.thr[q]) ++q;
+    const uint32_t words = (16u << q) / 4u;
+    uint32_t* p = reinterpret_cast<uint32_t*>(mm->template malloc<W>(16u << q));
+    if (p) {
+      const uint32_t v0 = (uint32_t)key;
+      for (uint32_t k = 0; k < words; k += 4)  // blocks are at least 16-byte aligned
+        *reinterpret_cast<uint4*>(p + k) = make_uint4(v0 + k, v0 + k + 1, v0 + k + 2, v0 + k + 3);
+      for (uint32_t k = 0; k < words; k += 4) {
+        const uint4 v = *reinterpret_cast<const uint4*>(p + k);
+        const uint32_t g = (v.x == v0 + k) + (v.y == v0 + k + 1) + (v.z == v0 + k + 2) + (v.w == v0 + k + 3);
+        ok += g;
+        bad += 4u - g;
+      }
+      mm->template free<W>(p);
+    } else {
+      nul = 1;
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    ok += __shfl_xor(ok, off);
+    bad += __shfl_xor(bad, off);
+    nul += __shfl_xor(nul, off);
+  }
+  if (__lane_id() == 0) {
+    if (ok) atomicAdd(&out[0], ok);
+    if (bad) atomicAdd(&out[1], bad);
+    if (nul) atomicAdd(&out[2], nul);
+  }
+}
+
 #ifdef OURO_DIAG
 // diagnostic builds: dynamic LDS per block (bytes) from OURO_DIAG_LDS, to cap the occupancy
 inline size_t launch_lds() {
@@ -184,6 +278,48 @@ struct Manager final : IManager {
     hipLaunchKernelGGL((k_free_persist<M>), dim3(grid), dim3(block), 0, s, mm.getDeviceMemoryManager(), d_ptrs, n);
     return hipGetLastError();
   }
+  template <int W>
+  hipError_t launch_request_w(uint64_t seed, uint32_t n, const ZipfTable& z, unsigned long long* out, hipStream_t s) {
+    hipLaunchKernelGGL((k_request<M, W>), dim3(grid_for(n, 256)), dim3(256), 0, s, mm.getDeviceMemoryManager(), seed,
+                       n, z, out);
+    return hipGetLastError();
+  }
+  hipError_t launch_request(uint64_t seed, uint32_t n, const ZipfTable& z, unsigned long long* out, int agg,
+                            hipStream_t s) override {
+    if (n == 0) return hipSuccess;
+    switch (agg) {
+      case 1: return launch_request_w<1>(seed, n, z, out, s);
+      case 16: return launch_request_w<16>(seed, n, z, out, s);
+      case 64: return launch_request_w<64>(seed, n, z, out, s);
+      default: return hipErrorInvalidValue;
+    }
+  }
+  template <int N, class... Is>
+  static void free_pages_of(const ouro::Chain<>&, PoolFree*) {}
+  template <int N, class I, class... Rest>
+  static void free_pages_of(const ouro::Chain<I, Rest...>& ch, PoolFree* out) {
+    for (uint32_t q = 0; q < I::kNQ && out->classes < 16; ++q, ++out->classes) {
+      const uint32_t k = out->classes;
+      out->pages[k] = ouro::Sem::count(ch.inst.q[q].sem);
+      out->page_size[k] = (1u << I::kSmallLog2) << q;
+      out->ppc[k] = (I::kChunkSize >> I::kSmallLog2) >> q;
+      out->add[k] = I::kAdd;
+      // a virtualized queue stores 2048 entries per queue chunk: one per page (page queues) or one per
+      // chunk (chunk queues)
+      const double entries = is_std(I::kKind) ? 0.0 : I::kChunks ? 1.0 : (double)out->ppc[k];
+      out->cost[k] = I::kAdd + entries / ouro::kQCSpots;
+    }
+    free_pages_of<N + 1, Rest...>(ch.next, out);
+  }
+  size_t state_bytes() override { return sizeof(M); }
+  void pool_free(const void* state, PoolFree* out) override {
+    const M* h = reinterpret_cast<const M*>(state);
+    const uint64_t limit = (uint64_t)h->memory.max_chunks - h->memory.qc_reserve;
+    const int64_t reuse = h->chain.inst.reuse.count;
+    out->shared = (limit > h->memory.next_free_chunk ? limit - h->memory.next_free_chunk : 0) + (reuse > 0 ? reuse : 0);
+    out->classes = 0;
+    free_pages_of<0>(h->chain, out);
+  }
   int persist_occupancy(uint32_t block) override {
     int a = 0, b = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_malloc_persist<M>, (int)block, 0) != hipSuccess) return 0;
@@ -246,9 +382,9 @@ struct Manager final : IManager {
 #ifdef OURO_DIAG
   // each variant translation unit holds its own copy of the diagnostic counters
   hipError_t diag_read(unsigned long long* out16, int reset) override {
-    hipError_t e = hipMemcpyFromSymbol(out16, HIP_SYMBOL(ouro::g_diag), sizeof(unsigned long long) * 16);
+    hipError_t e = hipMemcpyFromSymbol(out16, HIP_SYMBOL(ouro::g_diag), sizeof(unsigned long long) * 96);
     if (e == hipSuccess && reset) {
-      unsigned long long z[16] = {0};
+      unsigned long long z[96] = {0};
       e = hipMemcpyToSymbol(HIP_SYMBOL(ouro::g_diag), z, sizeof(z));
     }
     return e;
@@ -290,3 +426,8 @@ struct Manager final : IManager {
 };
 
 }  // namespace ouro_lib
+
+// The C handle behind ouro_handle.
+struct ouro_manager {
+  std::unique_ptr<ouro_lib::IManager> impl;
+};

@@ -75,3 +75,41 @@ def test_semaphore_share_matches_oracle():
         k = ob.sem_share(word, num, ppc)
         assert k == oracle.sem_share(c, e, num, ppc)
         assert 0 <= k <= -(-num // ppc) + 1
+
+
+def test_serve_expected_restatement():
+    # ouro_serve_expected against a restatement of the request kernel's draws: thread i of request r
+    # draws class q from the top 32 bits of keyed(seed_r, i) against Zipf thresholds and reads back
+    # (16 << q) / 4 words
+    M = (1 << 64) - 1
+
+    def sm(x):
+        x = (x + 0x9E3779B97F4A7C15) & M
+        x = ((x ^ (x >> 30)) * 0xBF58476D1CE4E5B9) & M
+        x = ((x ^ (x >> 27)) * 0x94D049BB133111EB) & M
+        return x ^ (x >> 31)
+
+    def keyed(seed, i):
+        return sm(sm(seed) ^ ((i * 0xD1B54A32D192ED03) & M))
+
+    def unit(r):
+        return (r >> 11) * (1.0 / 9007199254740992.0)
+
+    seed, zipf, classes = 9, 1.1, 10
+    cfg = ob.serve_config(8, 1000.0, min_log2=4, max_log2=9, zipf=zipf, classes=classes, seed=seed)
+    p = np.array([(q + 1) ** -zipf for q in range(classes)])
+    p /= p.sum()
+    thr = [min(4294967295.0, np.floor(c * 4294967296.0)) for c in np.cumsum(p)]
+    thr[-1] = 0xFFFFFFFF
+    for r in range(8):
+        e = 4 + 5 * unit(keyed(seed ^ 0xE7037ED1A0B428DB, r))
+        n = int(np.floor(2.0 ** e))
+        sr = keyed(seed ^ 0x8EBC6AF09C88C6E3, r)
+        words = 0
+        for i in range(n):
+            x = keyed(sr, i) >> 32
+            q = 0
+            while q + 1 < classes and x >= thr[q]:
+                q += 1
+            words += (16 << q) // 4
+        assert ob.serve_expected(cfg, r) == (n, words)

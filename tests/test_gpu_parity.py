@@ -79,6 +79,24 @@ def test_concurrent_mixed_invariants(kind, agg):
         assert chk.violations == 0, chk.message.decode()
 
 
+@pytest.mark.parametrize("kind", ["VACQ", "CQ", "VAPQ"])
+def test_held_blocks_across_rounds(kind):
+    # skewed sizes (Zipf 1.2 over the class ranks) and half of the blocks held across the round
+    # boundary, so live blocks interleave with frees; the live set is checked every round and the
+    # chunks holding a live block never drop below the ideal packing
+    print(f"held {kind}", flush=True)
+    sizes = ob.gen_sizes(7, 65536, "zipf", 1.2)
+    with ob.MemoryManager(kind, 1 << 30) as mm:
+        out = mm.run_trace(sizes, 5, agg=64, check_rounds=True, hold_permille=500, hold_seed=3)
+        r = out.result
+        assert_clean(r)
+        assert r["null_returns"] == 0
+        assert r["frees"] == r["mallocs"]
+        assert 0 < r["ideal_chunks"] <= r["peak_live_chunks"] <= r["peak_chunks"]
+        chk = mm.check_quiescent()
+        assert chk.violations == 0, chk.message.decode()
+
+
 def test_config1_full_concurrent():
     # config 1: MultiOuroPQ, 512 MiB, 8,192 threads x 16 B, 10 rounds, plain path
     with ob.MemoryManager("PQ", 512 << 20) as mm:
