@@ -179,12 +179,49 @@ def serve_config5(kind: str, requests: int) -> dict:
             "quiescent_violations": int(chk.violations), "loads": rows}
 
 
+COPY_BYTES = 1 << 30
+TRAFFIC_FILE = ROOT / "profiles" / "traffic.json"
+
+
+def probe(args, sizes: np.ndarray) -> int:
+    """The profiled command: one device copy of COPY_BYTES (read + write, the counters' calibration),
+    then `steps` rounds of the batched form on --variant / --block (persistent kernels, grid = every
+    CU at its occupancy limit), nothing else."""
+    variant = args.variant or "PQ"
+    gbs = ob.bench_copy(COPY_BYTES, 1)
+    with ob.MemoryManager(variant, POOL) as mm:
+        r = mm.run_batch(sizes, rounds=args.steps, warmup=args.warmup, agg=0, block=args.block,
+                         graph=bool(args.graph))
+    print(json.dumps({"probe": variant, "block": args.block, "copy_gbs": round(gbs, 1), "mops": round(r["mops"], 1),
+                      "alloc_ms": r["alloc_ms"] / args.steps, "free_ms": r["free_ms"] / args.steps}), flush=True)
+    return 0
+
+
+def measured_traffic(kernel: str, variant: str, block: int) -> dict | None:
+    """HBM bytes per launch of `kernel` from the committed FETCH_SIZE / WRITE_SIZE passes (each scaled
+    by what the same counter reads for the calibration copy), when they cover this configuration."""
+    try:
+        t = json.loads(TRAFFIC_FILE.read_text())
+    except (OSError, ValueError):
+        return None
+    k = t.get("kernels", {}).get(kernel)
+    if not k or t.get("variant") != variant or t.get("block") != block:
+        return None
+    return {"bytes": k["fetch_bytes"] + k["write_bytes"], "fetch_bytes": k["fetch_bytes"],
+            "write_bytes": k["write_bytes"], "source": str(TRAFFIC_FILE.relative_to(ROOT))}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--variant", default=None, help="skip the tuner: batched form on this variant, block-wide")
+    ap.add_argument("--block", type=int, default=1024, help="workgroup size with --variant")
+    ap.add_argument("--probe", action="store_true",
+                    help="profiling run: a 1 GiB device copy (counter calibration), then the headline rounds only")
+    ap.add_argument("--graph", type=int, default=1, help="with --probe: 1 = graph-replayed rounds as the headline, "
+                    "0 = plain launches (counter passes)")
     ap.add_argument("--no-table", action="store_true", help="skip the per-kind plain / wave / batched table")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-config4", action="store_true", help="skip the 256-workload batch set")
@@ -202,8 +239,10 @@ def main() -> int:
 
     # ---- the batched form: tuned, then measured
     t0 = time.perf_counter()
+    if args.probe:
+        return probe(args, sizes)
     if args.variant:
-        best = tune.Candidate(args.variant, 0, 512, 0)
+        best = tune.Candidate(args.variant, 0, args.block, 0)
         tuned = None
     else:
         log("tuning the batched form on the 1M-thread trace ...")
@@ -260,6 +299,7 @@ def main() -> int:
     kern = "k_malloc_persist" if dominant == "alloc" else "k_free_persist"
     if best.agg != 0:
         kern = "k_malloc" if dominant == "alloc" else "k_free"
+    traffic = measured_traffic(kern, best.variant, best.block) if best.agg == 0 else None
 
     line = {
         "metric": "malloc+free Mops/s on 1 MI355X (1M threads, mixed sizes); p99 alloc latency",
@@ -313,7 +353,8 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved_gbs / HBM_PEAK_GBS, 5),
-            "traffic": None,
+            "traffic": traffic["bytes"] if traffic else None,
+            "traffic_detail": traffic,
             "avg_launch_ms": round(dom_s * 1e3, 4),
             "algorithmic_bytes_per_launch": alg_bytes,
         },

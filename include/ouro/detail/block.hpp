@@ -15,20 +15,14 @@ namespace ouro {
 constexpr uint32_t kBlockMaxWaves = 16;    // 1024-thread blocks
 constexpr uint32_t kBlockMaxClasses = 16;  // size classes over the whole instance chain
 
-// LDS scratch of one workgroup. The tuple list (chunk queues only) holds the claimed page runs.
+// LDS scratch of one workgroup. Chunk queues write one page index (MemoryIndex) per served rank.
 template <bool kChunks>
 struct BlockScratch {
-  struct Run {
-    uint32_t bc;    // chunk | mask word << 29
-    uint32_t r0;    // first rank served by the run
-    uint64_t bits;  // claimed pages of that mask word
-  };
   uint32_t off[kBlockMaxWaves][kBlockMaxClasses];  // requests of (wave, class), then their rank offset
   uint32_t total[kBlockMaxClasses];
-  uint32_t base[kBlockMaxClasses];    // page queues: first queue position; chunk queues: first run
-  uint32_t nrun[kBlockMaxClasses];    // chunk queues: runs written
+  uint32_t base[kBlockMaxClasses];    // page queues: first queue position; chunk queues: first pidx entry
   uint32_t served[kBlockMaxClasses];  // chunk queues: ranks served (the rest get nullptr)
-  Run run[kChunks ? 1024 : 1];
+  uint32_t pidx[kChunks ? 1024 : 1];  // chunk queues: page of each rank, class stretches in class order
 };
 
 namespace detail {
@@ -106,59 +100,190 @@ __device__ inline void block_signal(I& inst, uint32_t off, S& sm) {
 }
 
 // ---- chunk queues
-// Wave 0: lane gc leads a ChunkClaim for its class's block total; every run of pages it claims is
-// appended to the class's stretch of sm.run (a stretch of `total` entries starting at the class's
-// prefix of totals, enough since every run serves at least one rank).
+// A class's block total is claimed by one whole wave (class gc goes to wave gc mod waves, so the
+// classes of a block proceed in parallel). The wave looks at up to 64 queue positions at once, one
+// chunk per lane: a prefix sum over the chunks' free counts divides the pages in lane order, each
+// lane takes its share with one count CAS, removes a chunk it emptied at the position it found it
+// at, and claims the bits from the chunk's 64-bit mask words, writing the page of each rank it
+// serves into sm.pidx. The reference walks the queue one chunk at a time per thread
+// (ChunkQueue_impl.cuh:88-151); a serial walk per class leader leaves a block waiting on its
+// largest class, where every chunk holds a single page.
 template <class I, class S>
 __device__ inline void block_claim_chunks(Memory& m, I& inst, ReuseRing& base, uint32_t off, S& sm,
                                           uint32_t nclasses) {
+  constexpr Kind K = I::kKind;
+  const uint32_t wave = threadIdx.x >> 6, nwaves = (blockDim.x + 63u) >> 6;
   const uint32_t l = lane();
-  const bool mine = l >= off && l < off + I::kNQ;
-  const uint32_t total = mine ? sm.total[l] : 0u;
-  const bool lead = total > 0;
-  if (!ballot(lead)) return;
   Ctx<I> x{m, inst, base};
-  ClassQueue* cq = &inst.q[mine ? l - off : 0u];
-  auto carve = [&](ClassQueue* lq, uint32_t k) { return carve_chunks(x, lq, k); };
-  sem_claim_n(x, cq, lead, total, carve);
-  // stretch start: prefix of the block totals over the classes before l
-  uint32_t start = 0;
-  for (uint32_t c = 0; c < nclasses; ++c) start += c < l ? sm.total[c] : 0u;
-  ChunkClaim<I> cl(x, cq, lead, total);
-  uint32_t nrun = 0;
-  while (ballot(lead && !cl.finished())) {
-    if (lead && !cl.finished()) {
-      uint32_t bc, r0;
-      uint64_t bits;
-      cl.step(x, cq, bc, bits, r0);
-      if (bc != kMarker) {
-        sm.run[start + nrun] = {bc, r0, bits};
-        ++nrun;
+  for (uint32_t gc = off; gc < off + I::kNQ; ++gc) {
+    if (gc % nwaves != wave) continue;
+    const uint32_t total = sm.total[gc];
+    if (total == 0) continue;
+    ClassQueue* cq = &inst.q[gc - off];
+    uint32_t start = 0;
+    for (uint32_t c = 0; c < gc; ++c) start += sm.total[c];
+    for (uint32_t i = l; i < total; i += kWave) sm.pidx[start + i] = kMarker;
+    auto carve = [&](ClassQueue* lq, uint32_t k) { return carve_chunks(x, lq, k); };
+    sem_claim_n(x, cq, l == 0, total, carve);
+    const uint32_t psize = page_size_of(x, cq);
+    uint32_t v = 0;
+    if (l == 0) {
+      const uint64_t p = add(&cq->scan, (uint64_t)total);
+      const uint32_t h = (uint32_t)(p >> __builtin_ctz(ppc_of(x, cq)));
+      const uint32_t f = ld(&cq->front), b = ld(&cq->back);
+      const uint32_t len = b - f;
+      v = (len == 0 || len >= 0x80000000u) ? f : (h - f < len ? h : f + (h - f) % len);
+    }
+    v = uniform(v);
+    uint32_t assigned = 0, wraps = 0, cursor = kMarker, reclaim = kMarker;
+    Spin it(&m.error);
+    while (assigned < total) {
+      uint32_t b = 0, nb = 0;
+      bool give_up = false;
+      if (l == 0) {
+        b = ld(&cq->back);
+        wraps += scan_wrap<I>(cq, v, cursor, b);
+        nb = b - v;  // 64 positions at a time, not past back_, inside one queue chunk for VA / VL
+        if (nb >= 0x80000000u) nb = 0;
+        if (nb > kWave) nb = kWave;
+        if constexpr (!is_std(K)) nb = nb < kQCSpots - v % kQCSpots ? nb : kQCSpots - v % kQCSpots;
+        give_up = (wraps >= 2 || nb == 0) && pool_exhausted(m);
+      }
+      v = uniform(v);
+      cursor = uniform(cursor);
+      wraps = uniform(wraps);
+      nb = uniform(nb);
+      if (ballot(give_up)) {
+        // the pool ran out and a whole window pass found nothing: unserved ranks get nullptr
+        if (l == 0) sem_signal(cq, total - assigned);
+        break;
+      }
+      if (nb == 0) {
+        if (ballot(it.expired())) {
+          OURO_TIMEOUT(m);
+          break;
+        }
+        backoff(it.it);
+        continue;
+      }
+      // the slots of positions v .. v + nb (one queue chunk for VA / VL)
+      uint32_t s = kMarker, qc = kMarker;
+      uint32_t* slot = nullptr;
+      if constexpr (is_std(K)) {
+        slot = &ring_of(x, cq)[(v + l) & (I::kQueueEntries - 1)];
+        if (l < nb) s = ld(slot);
+      } else {
+        uint32_t fill = kMarker;
+        const uint32_t* sl = nullptr;
+        if (l == 0) sl = slots_at(x, cq, v, cursor, fill, &qc);
+        sl = reinterpret_cast<const uint32_t*>(readlane64(reinterpret_cast<uint64_t>(sl), 0));
+        cursor = uniform(cursor);
+        fill = uniform(fill);
+        qc = uniform(qc);
+        slot = const_cast<uint32_t*>(sl) + l;
+        if (l < nb) s = sl ? ld(&sl[l]) : fill;
+      }
+      ChunkHdr* h = nullptr;
+      int32_t cnt = 0;
+      if (l < nb && chunk_ok(m, s)) {
+        h = hdr<ChunkHdr>(m, s);
+        cnt = ld(&h->count);
+        if (cnt <= 0 || ld(&h->queue_pos) != v + l || ld(&h->page_size) != psize) cnt = 0;
+      }
+      const uint32_t c = cnt > 0 ? (uint32_t)cnt : 0u;
+      uint32_t offered = 0;
+      const uint32_t before = wave_exclusive_sum(c, &offered);
+      const uint32_t rem = total - assigned;
+      // share = clamp(rem - before, 0, c), written branch-free: with the bound inside an `if` the
+      // ROCm 7.2 compiler hoisted the min out of the branch and dropped the before < rem guard
+      const uint32_t owed = before < rem ? rem - before : 0u;
+      uint32_t t = owed < c ? owed : c;
+      if (t && cas(&h->count, cnt, cnt - (int32_t)t) != cnt) t = 0;  // changed meanwhile: skip it
+      uint32_t took = 0;
+      const uint32_t r0 = assigned + wave_exclusive_sum(t, &took);
+      // chunks this claim emptied leave the queue at the position they were found at (cq_remove for
+      // the whole batch: slots tombed in parallel, one count update per queue and queue chunk, one
+      // wave-wide compaction of front_)
+      bool removed = false;
+      if (t && t == c) removed = cas(&h->queue_pos, v + l, kMarker) == v + l;
+      const uint64_t rm = ballot(removed);
+      if (rm) {
+        if (removed) xchg(slot, kTomb);
+        const uint32_t nrm = popc(rm);
+        uint32_t rc = kMarker;
+        if (l == 0) {
+          sub(&cq->count, (int32_t)nrm);
+          if constexpr (!is_std(K)) {
+            const uint32_t old = sub(&hdr<QueueChunkHdr>(m, qc)->count, nrm << 16);
+            if (old - (nrm << 16) == kQCSpots) {  // every position of the queue chunk is dequeued
+              if constexpr (is_va(K)) {
+                const uint32_t victim = xchg(&ring_of(x, cq)[(v / kQCSpots) & (I::kQueueEntries - 1)], qc_tag(v / kQCSpots));
+                if (chunk_ok(m, victim)) rc = victim;
+              } else {
+                rc = vl_set_front(x, cq, qc) | 0x80000000u;
+              }
+            }
+          }
+        }
+        rc = uniform(rc);
+        if (rc != kMarker) {
+          if (reclaim != kMarker) finish_reclaim(x, cq, l == 0, reclaim);
+          reclaim = rc;
+        }
+        order();
+        if (removed && ld(&h->count) > 0) ensure_in_queue<1>(x, cq, true, s);  // a free raced in
+        compact_wave(x, cq);
+      }
+      if (t) {
+        const uint32_t size = ld(&h->size);
+        const uint32_t nw = mask_words(size);
+        uint32_t w = ((size - (uint32_t)cnt) / 64u) % nw;
+        uint32_t left = t, r = start + r0;
+        Spin bits_wait(&m.error);
+        while (left > 0) {
+          const uint64_t take = lowest_bits(ld(&h->mask[w]), left);
+          if (take == 0) {
+            w = (w + 1) % nw;
+          } else {
+            uint64_t got = and_(&h->mask[w], ~take) & take;
+            left -= popc(got);
+            const bool whole = got == take;
+            while (got) {
+              const uint32_t bit = (uint32_t)__builtin_ctzll(got);
+              got &= got - 1;
+              sm.pidx[r++] = make_index(s, w * 64u + bit);
+            }
+            if (whole && left > 0) w = (w + 1) % nw;
+          }
+          if (left > 0 && bits_wait.expired()) {
+            OURO_TIMEOUT(m);
+            break;
+          }
+        }
+      }
+      assigned += took;
+      v += nb;
+      if (assigned < total && ballot(it.expired())) {
+        OURO_TIMEOUT(m);
+        break;
       }
     }
+    if (l == 0) {
+      sm.base[gc] = start;
+      sm.served[gc] = assigned;
+    }
+    finish_reclaim(x, cq, l == 0 && reclaim != kMarker, reclaim);
   }
-  if (lead) {
-    sm.base[l] = start;
-    sm.nrun[l] = nrun;
-    sm.served[l] = cl.assigned;
-  }
-  finish_reclaim(x, cq, lead, cl.reclaim);
+  (void)nclasses;
 }
 
-// Page of rank r from the class's runs (binary search on r0).
+// Page of rank r of class gc, from the claim's page list.
 template <class I, class S>
 __device__ inline void* block_page_of(Memory& m, I& inst, const S& sm, bool here, uint32_t gc, uint32_t qi, uint32_t r) {
   if (!here || r >= sm.served[gc]) return nullptr;
-  uint32_t lo = sm.base[gc], hi = lo + sm.nrun[gc];  // r0 ascending; find the last run with r0 <= r
-  while (hi - lo > 1) {
-    const uint32_t mid = (lo + hi) >> 1;
-    if (sm.run[mid].r0 <= r) lo = mid;
-    else hi = mid;
-  }
-  const auto& run = sm.run[lo];
-  const uint32_t chunk = run.bc & 0x1FFFFFFFu;
-  const uint32_t page = (run.bc >> 29) * 64u + nth_set_bit(run.bits, r - run.r0);
-  return chunk_addr(m, chunk) + kMeta + (uint64_t)page * (I::kSmallest << qi);
+  const uint32_t idx = sm.pidx[sm.base[gc] + r];
+  if (idx == kMarker) return nullptr;
+  return chunk_addr(m, index_chunk(idx)) + kMeta + (uint64_t)index_page(idx) * (I::kSmallest << qi);
 }
 
 // Chunk-queue frees of one instance: bit, then the count per chunk (one add per chunk group of the

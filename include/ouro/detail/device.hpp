@@ -149,6 +149,18 @@ struct Spin {
 
 // ------------------------------------------------------------------ wave primitives
 __device__ __forceinline__ uint32_t lane() { return __lane_id(); }
+__device__ __forceinline__ uint32_t uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+// Exclusive prefix sum over the 64 lanes of a fully active wave; *total gets the wave's sum.
+__device__ __forceinline__ uint32_t wave_exclusive_sum(uint32_t v, uint32_t* total) {
+  uint32_t x = v;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t y = __shfl_up(x, d, 64);
+    if ((int)__lane_id() >= d) x += y;
+  }
+  *total = __builtin_amdgcn_readlane(x, 63);
+  return x - v;
+}
 __device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
 __device__ __forceinline__ uint64_t active() { return __ballot(1); }
 __device__ __forceinline__ uint64_t lt_mask() { return (1ull << lane()) - 1ull; }

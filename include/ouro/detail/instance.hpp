@@ -51,6 +51,13 @@ namespace detail {
 using namespace dev;
 
 __device__ __forceinline__ void set_error(Memory& m, uint32_t bits) { or_(&m.error, bits); }
+
+// Class byte of every base chunk of a freshly carved run (Memory::chunk_log2), published with the
+// chunk's header before any of its pages.
+__device__ __forceinline__ void mark_chunk_class(Memory& m, uint32_t chunk, uint32_t run, uint32_t page_size) {
+  const uint32_t lg = 31u - (uint32_t)__builtin_clz(page_size);
+  for (uint32_t j = 0; j < run; ++j) st(&m.chunk_log2[chunk + j], (uint8_t)(lg | (j << 5)));
+}
 // a bounded wait gave up; diagnostic builds also log the source line of the wait
 #define OURO_TIMEOUT(mem)                                         \
   do {                                                            \
@@ -421,6 +428,79 @@ __device__ inline void compact(Ctx<I>& x, ClassQueue* cq) {
   }
 }
 
+// compact() for a whole, fully active wave: each round looks at up to 64 positions from front_ at
+// once and moves front_ over the leading dead ones with one CAS (a reclaimed VA queue chunk or a VL
+// queue chunk front_ptr_ has left is a single move). It stops where compact() would: at the first
+// position it cannot prove dead, after kCompactSteps moves, or when another lane moved front_.
+template <class I>
+__device__ inline void compact_wave(Ctx<I>& x, ClassQueue* cq) {
+  constexpr Kind K = I::kKind;
+  static_assert(is_chunk_kind(K), "page queues dequeue strictly at front_");
+  const uint32_t l = lane();
+  uint32_t moves = 0;
+  while (moves < kCompactSteps) {
+    uint32_t f = 0, n = 0, jump = kMarker;
+    const uint32_t* sl = nullptr;
+    if (l == 0) {
+      f = ld(&cq->front);
+      n = ld(&cq->back) - f;
+      if (n >= 0x80000000u) n = 0;
+      if (n > kWave) n = kWave;
+      if (n > kCompactSteps - moves) n = kCompactSteps - moves;
+      if constexpr (is_va(K)) {
+        const uint32_t e = ld(&ring_of(x, cq)[(f / kQCSpots) & (I::kQueueEntries - 1)]);
+        if (e == qc_tag(f / kQCSpots)) jump = (f / kQCSpots + 1u) * kQCSpots;
+        else if (chunk_ok(x.m, e) && ld(&hdr<QueueChunkHdr>(x.m, e)->virtual_start) / kQCSpots == f / kQCSpots)
+          sl = qc_slots(x.m, e) + f % kQCSpots;
+        else n = 0;
+      } else if constexpr (is_vl(K)) {
+        const uint32_t c = ld(&cq->front_qc);
+        if (!chunk_ok(x.m, c)) {
+          n = 0;
+        } else {
+          const uint32_t vs = ld(&hdr<QueueChunkHdr>(x.m, c)->virtual_start) / kQCSpots, fq = f / kQCSpots;
+          if (vs == fq) sl = qc_slots(x.m, c) + f % kQCSpots;
+          else if (vs - fq >= 0x80000000u) n = 0;  // front_ptr_ still behind f: its move is in flight
+          else jump = vs * kQCSpots;
+        }
+      }
+      if constexpr (!is_std(K)) {
+        if (n > kQCSpots - f % kQCSpots) n = kQCSpots - f % kQCSpots;
+      }
+    }
+    f = uniform(f);
+    n = uniform(n);
+    jump = uniform(jump);
+    sl = reinterpret_cast<const uint32_t*>(readlane64(reinterpret_cast<uint64_t>(sl), 0));
+    if (n == 0) break;
+    uint32_t nf = jump;
+    if (jump == kMarker) {
+      bool dead = false;
+      if (l < n) {
+        const uint32_t e = is_std(K) ? ld(&ring_of(x, cq)[(f + l) & (I::kQueueEntries - 1)]) : ld(&sl[l]);
+        dead = e == kTomb;
+      }
+      const uint64_t alive = ~ballot(dead);
+      const uint32_t k = alive ? (uint32_t)__builtin_ctzll(alive) : kWave;
+      if (k == 0) break;
+      nf = f + k;
+    }
+    bool moved = false;
+    if (l == 0) moved = cas(&cq->front, f, nf) == f;
+    if (!ballot(moved)) break;
+    // the ring slots become writable again for positions + ring_size
+    if constexpr (is_std(K)) {
+      if (l < nf - f) cas(&ring_of(x, cq)[(f + l) & (I::kQueueEntries - 1)], kTomb, kMarker);
+    }
+    if (jump != kMarker) {
+      moves += 1;
+    } else {
+      moves += nf - f;
+      if (nf - f < n) break;  // stopped at a live position
+    }
+  }
+}
+
 // ------------------------------------------------------------------ slot sequences
 // put_lanes: every lane with want=true stores `val` at virtual position v of its queue cq.
 //   PQ/CQ   : ring slot CAS (PageQueue_impl.cuh:22-36, ChunkQueue_impl.cuh:20-38)
@@ -703,6 +783,7 @@ __device__ inline uint32_t carve_pages(Ctx<I>& x, ClassQueue* cq, uint32_t k) {
     PageChunkHdr* h = hdr<PageChunkHdr>(x.m, chunk);
     st(&h->page_size, psize);
     st(&h->number_pages, ppc);
+    mark_chunk_class(x.m, chunk, I::kAdd, psize);
   }
   order();
   // every promised page gets a queue slot: the carved pages, then a null entry for each page of a
@@ -807,6 +888,7 @@ __device__ inline uint32_t carve_chunks(Ctx<I>& x, ClassQueue* cq, uint32_t k) {
   if (me < k_ok) {
     // ChunkIndexChunk::initializeChunk with all pages free (ChunkAccess ctor, ChunkAccess.cuh:69-89)
     ChunkHdr* h = hdr<ChunkHdr>(x.m, chunk);
+    mark_chunk_class(x.m, chunk, I::kAdd, psize);
     st(&h->page_size, psize);
     st(&h->size, ppc);
     st(&h->count, (int32_t)ppc);
@@ -830,39 +912,53 @@ __device__ inline uint32_t carve_chunks(Ctx<I>& x, ClassQueue* cq, uint32_t k) {
 // batch instead of one per slot.
 constexpr uint32_t kPeek = 8;
 
+// Slots of the queue chunk holding virtual position v (VA / VL), from v on: nullptr when there is
+// none to read, with `fill` what its slots read as (a reclaimed or bypassed queue chunk reads as
+// tombstones, an unpublished one as markers). VL walks from `cursor` (front_ptr_ if unset) and
+// leaves it at the queue chunk found.
+template <class I>
+__device__ inline const uint32_t* slots_at(Ctx<I>& x, ClassQueue* cq, uint32_t v, uint32_t& cursor, uint32_t& fill,
+                                          uint32_t* qc_out = nullptr) {
+  constexpr Kind K = I::kKind;
+  static_assert(!is_std(K), "standard queues index their ring directly");
+  fill = kMarker;
+  if constexpr (is_va(K)) {
+    const uint32_t e = ld(&ring_of(x, cq)[(v / kQCSpots) & (I::kQueueEntries - 1)]);
+    if (e == qc_tag(v / kQCSpots)) fill = kTomb;
+    else if (chunk_ok(x.m, e) && ld(&hdr<QueueChunkHdr>(x.m, e)->virtual_start) / kQCSpots == v / kQCSpots) {
+      if (qc_out) *qc_out = e;
+      return qc_slots(x.m, e) + v % kQCSpots;
+    }
+    return nullptr;
+  } else {
+    if (cursor == kMarker) cursor = ld(&cq->front_qc);
+    uint32_t c = vl_locate(x, cursor, v);
+    if (c == kMarker) {  // the cursor's queue chunk is already past v: retry from front_ptr_
+      cursor = ld(&cq->front_qc);
+      c = vl_locate(x, cursor, v);
+    }
+    if (c != kMarker) {
+      cursor = c;
+      if (qc_out) *qc_out = c;
+      return qc_slots(x.m, c) + v % kQCSpots;
+    }
+    fill = kTomb;
+    return nullptr;
+  }
+}
+
 // Load the slots of virtual positions [v, v + n), n <= kPeek, which lie in one queue chunk for
-// VA / VL. Positions of a reclaimed or bypassed queue chunk read as tombstones, of an unpublished
-// one as markers; lanes past n read markers. (ChunkQueue scan, ChunkQueue_impl.cuh:105-145;
+// VA / VL; entries past n read markers. (ChunkQueue scan, ChunkQueue_impl.cuh:105-145;
 // QueueChunk::access, QueueChunk.cuh:111-114, and the linked walk of accessLinked.)
 template <class I>
 __device__ inline void peek_slots(Ctx<I>& x, ClassQueue* cq, uint32_t v, uint32_t n, uint32_t& cursor,
                                   uint32_t (&s)[kPeek]) {
-  constexpr Kind K = I::kKind;
-  if constexpr (is_std(K)) {
+  if constexpr (is_std(I::kKind)) {
 #pragma unroll
     for (uint32_t i = 0; i < kPeek; ++i) s[i] = i < n ? ld(&ring_of(x, cq)[(v + i) & (I::kQueueEntries - 1)]) : kMarker;
   } else {
-    const uint32_t* sl = nullptr;
-    uint32_t fill = kMarker;
-    if constexpr (is_va(K)) {
-      const uint32_t e = ld(&ring_of(x, cq)[(v / kQCSpots) & (I::kQueueEntries - 1)]);
-      if (e == qc_tag(v / kQCSpots)) fill = kTomb;
-      else if (chunk_ok(x.m, e) && ld(&hdr<QueueChunkHdr>(x.m, e)->virtual_start) / kQCSpots == v / kQCSpots)
-        sl = qc_slots(x.m, e) + v % kQCSpots;
-    } else {
-      if (cursor == kMarker) cursor = ld(&cq->front_qc);
-      uint32_t c = vl_locate(x, cursor, v);
-      if (c == kMarker) {  // the cursor's queue chunk is already past v: retry from front_ptr_
-        cursor = ld(&cq->front_qc);
-        c = vl_locate(x, cursor, v);
-      }
-      if (c != kMarker) {
-        cursor = c;
-        sl = qc_slots(x.m, c) + v % kQCSpots;
-      } else {
-        fill = kTomb;
-      }
-    }
+    uint32_t fill;
+    const uint32_t* sl = slots_at(x, cq, v, cursor, fill);
 #pragma unroll
     for (uint32_t i = 0; i < kPeek; ++i) s[i] = (i < n && sl) ? ld(&sl[i]) : fill;
   }

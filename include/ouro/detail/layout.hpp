@@ -128,6 +128,10 @@ struct alignas(128) Memory {
   uint8_t* d_memory;
   uint8_t* d_data;
   uint32_t* locator;        // ChunkLocator::d_chunk_flags
+  // One byte per base chunk, written when a chunk is carved for a class: log2 of its page size in
+  // bits 0-4 and the base chunk's offset inside its run in bits 5-7; 0 = no data chunk. free() reads
+  // this dense array (128 chunks per cache line) instead of the locator and the chunk header.
+  uint8_t* chunk_log2;
   uint64_t allocation_size;
   uint64_t locator_bytes;
   uint32_t max_chunks;

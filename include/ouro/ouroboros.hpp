@@ -134,13 +134,19 @@ __device__ inline bool route_free(Memory& m, const void* ptr, uint32_t& chunk, u
   bool want = p >= m.d_data && p < m.d_data + (uint64_t)m.max_chunks * kStride;
   if (want) {
     const uint64_t diff = (uint64_t)(p - m.d_data);
-    const uint32_t c = locate_run(m, (uint32_t)(diff / kStride));
-    want = c != kMarker;
+    const uint32_t idx = (uint32_t)(diff / kStride);
+    // the chunk's class byte, in place of ChunkLocator::getChunkIndex and CommonChunk::page_size: a
+    // byte only ever changes from 0 to its final value, so a plain (L2-cached) load is exact unless
+    // it reads 0, which is confirmed with a device-coherent load
+    uint32_t b = m.chunk_log2[idx];
+    if (b == 0) b = dev::ld(&m.chunk_log2[idx]);
+    want = b != 0 && (b >> 5) <= idx;
     if (want) {
+      const uint32_t c = idx - (b >> 5);
       chunk = c;
-      page_size = dev::ld(hdr<uint32_t>(m, c));  // CommonChunk::page_size
+      page_size = 1u << (b & 31u);
       const uint64_t base = (uint64_t)c * kStride + kMeta;
-      want = diff >= base && page_size != 0;
+      want = diff >= base;
       offset = diff - base;
     }
   }
@@ -173,8 +179,9 @@ __device__ inline uint32_t chain_free_class(uint32_t page_size, uint64_t offset,
 template <class S, class I, class... Rest>
 __device__ inline void chain_block_claim(Memory& m, Chain<I, Rest...>& ch, ReuseRing& base, S& sm, uint32_t nclasses,
                                          uint32_t off = 0) {
+  // chunk queues: every wave claims its share of the classes; page queues: wave 0's class lanes
   if constexpr (I::kChunks) block_claim_chunks(m, ch.inst, base, off, sm, nclasses);
-  else block_claim_pages(m, ch.inst, base, off, sm);
+  else if (threadIdx.x < kWave) block_claim_pages(m, ch.inst, base, off, sm);
   if constexpr (sizeof...(Rest) > 0) chain_block_claim(m, ch.next, base, sm, nclasses, off + I::kNQ);
 }
 
@@ -272,7 +279,7 @@ struct Ouroboros {
       if (gc == kMarker) detail::set_error(memory, ERR_TOO_LARGE);
     }
     const uint32_t r = detail::block_rank(sm, gc, kNumClasses);
-    if (threadIdx.x < kWave) detail::chain_block_claim(memory, chain, chain.inst.reuse, sm, kNumClasses);
+    detail::chain_block_claim(memory, chain, chain.inst.reuse, sm, kNumClasses);
     __syncthreads();
     return detail::chain_block_take(memory, chain, chain.inst.reuse, sm, gc, r);
   }
@@ -338,6 +345,7 @@ struct Ouroboros {
     ClassQueue* cq = &inst.q[queue_index];
     using I = decltype(chain.inst);
     detail::Ctx<I> x{memory, inst, inst.reuse};
+    detail::mark_chunk_class(memory, chunk_index, I::kAdd, cq->page_size);
     if constexpr (I::kChunks) {
       ChunkHdr* h = detail::hdr<ChunkHdr>(memory, chunk_index);
       h->page_size = cq->page_size;
@@ -398,6 +406,12 @@ struct Ouroboros {
     memory.max_chunks = (uint32_t)L.max_chunks;
     memory.locator_bytes = L.locator_bytes;
     memory.locator = reinterpret_cast<uint32_t*>(d + L.allocation_size - L.locator_bytes);
+    e = hipMalloc(reinterpret_cast<void**>(&memory.chunk_log2), L.max_chunks);
+    if (e == hipSuccess) e = hipMemset(memory.chunk_log2, 0, L.max_chunks);
+    if (e != hipSuccess) {
+      release();
+      return e;
+    }
     memory.next_free_chunk = 0;
     memory.qc_next = 0;
     memory.qc_add = kBaseAdd;
@@ -446,7 +460,9 @@ struct Ouroboros {
 
   void release() {
     if (memory.d_memory) (void)hipFree(memory.d_memory);
+    if (memory.chunk_log2) (void)hipFree(memory.chunk_log2);
     memory.d_memory = nullptr;
+    memory.chunk_log2 = nullptr;
     memory.initialized = 0;
   }
 

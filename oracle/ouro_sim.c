@@ -831,6 +831,128 @@ static void finish_reclaim(ouro_sim* s, inst_t* I, cls_t* q, uint32_t reclaim) {
   else if (is_vl(I->kind)) vl_set_old(s, q, reclaim & 0x7FFFFFFFu);
 }
 
+/* Block-wide claim of one class by a whole wave (the device's block_claim_chunks): up to 64 queue
+ * positions at a time, one chunk per lane; a prefix sum over the chunks' free counts divides the
+ * pages in lane order; chunks emptied are removed at the position they were found (slots tombed in
+ * lane order, then one compaction of front_, as the lockstep wave does); each lane claims its pages
+ * from the chunk's mask words, lowest bits first. */
+static void claim_wave(ouro_sim* s, inst_t* I, group_t* g, uint64_t* out) {
+  cls_t* q = &I->q[g->qi];
+  const uint32_t total = (uint32_t)g->n;
+  const uint32_t h = (uint32_t)(q->scan / q->ppc);
+  q->scan += (uint64_t)total;
+  const uint32_t f = q->front, len = q->back - q->front;
+  uint32_t v = (len == 0 || len >= 0x80000000u) ? f : (h - f < len ? h : f + (h - f) % len);
+  uint32_t cursor = MARKER, assigned = 0, wraps = 0, rec = MARKER;
+  int guard = 0;
+  while (assigned < total) {
+    if (++guard > 10000000) {
+      s->error |= ERR_TIMEOUT;
+      return;
+    }
+    const uint32_t b = q->back;
+    if (v - b < 0x80000000u || v - q->front >= 0x80000000u) {
+      v = q->front;
+      cursor = MARKER;
+      ++wraps;
+    }
+    /* 64 positions at a time (a single leader peeks PEEK), inside one queue chunk for VA / VL */
+    uint32_t nb = b - v;
+    if (nb >= 0x80000000u) nb = 0;
+    if (nb > 64) nb = 64;
+    if (!is_std(I->kind) && nb > QC_SPOTS - v % QC_SPOTS) nb = QC_SPOTS - v % QC_SPOTS;
+    if ((wraps >= 2 || nb == 0) && (s->error & (ERR_OOM | ERR_QUEUE_FULL))) {
+      q->cnt += total - assigned;
+      break;
+    }
+    if (nb == 0) {
+      s->error |= ERR_TIMEOUT;
+      return;
+    }
+    uint32_t sl[64];
+    for (uint32_t i = 0; i < 64; ++i) sl[i] = MARKER;
+    if (is_std(I->kind)) {
+      for (uint32_t i = 0; i < nb; ++i) sl[i] = *std_slot(q, v + i);
+    } else {
+      uint32_t tmp[PEEK];
+      for (uint32_t i0 = 0; i0 < nb; i0 += PEEK) {
+        const uint32_t m = nb - i0 < PEEK ? nb - i0 : PEEK;
+        peek_slots(s, I, q, v + i0, m, &cursor, tmp);
+        for (uint32_t i = 0; i < m; ++i) sl[i0 + i] = tmp[i];
+      }
+    }
+    int32_t cnt[64];
+    uint32_t t[64];
+    uint32_t before = 0, took = 0;
+    const uint32_t rem = total - assigned;
+    for (uint32_t i = 0; i < 64; ++i) {
+      cnt[i] = 0;
+      t[i] = 0;
+      if (i < nb && chunk_ok(s, sl[i])) {
+        const chunk_t* x = ch(s, sl[i]);
+        if (x->count > 0 && x->queue_pos == v + i && x->page_size == q->page_size) cnt[i] = x->count;
+      }
+      if (cnt[i] > 0 && before < rem) t[i] = (uint32_t)cnt[i] < rem - before ? (uint32_t)cnt[i] : rem - before;
+      before += cnt[i] > 0 ? (uint32_t)cnt[i] : 0u;
+    }
+    uint32_t r0[64];
+    for (uint32_t i = 0; i < 64; ++i) {
+      r0[i] = assigned + took;
+      took += t[i];
+      if (t[i]) ch(s, sl[i])->count -= (int32_t)t[i];
+    }
+    /* removals: the batch's emptied chunks are tombed (lane order), then front_ is compacted once;
+     * a queue chunk the batch empties is reclaimed once, held as the claim's pending reclaim */
+    int removed = 0;
+    for (uint32_t i = 0; i < 64; ++i) {
+      if (!t[i] || t[i] != (uint32_t)cnt[i]) continue;
+      chunk_t* x = ch(s, sl[i]);
+      if (x->queue_pos != v + i) continue;
+      x->queue_pos = MARKER;
+      uint32_t cur = cursor, rc = MARKER;
+      tomb_slot(s, I, q, v + i, &cur, &rc);
+      if (rc != MARKER) {
+        if (rec != MARKER) finish_reclaim(s, I, q, rec);
+        rec = rc;
+      }
+      q->count--;
+      removed = 1;
+    }
+    if (removed) compact(s, I, q);
+    /* mask claims */
+    for (uint32_t i = 0; i < 64; ++i) {
+      if (!t[i]) continue;
+      chunk_t* x = ch(s, sl[i]);
+      const uint32_t nw = mask_words(x->size);
+      uint32_t w = ((x->size - (uint32_t)cnt[i]) / 64u) % nw, left = t[i], r = r0[i];
+      int wguard = 0;
+      while (left > 0 && wguard++ < 64) {
+        const uint64_t mw = x->mask[w];
+        uint64_t take = 0;
+        uint32_t c = 0;
+        for (uint32_t bb = 0; bb < 64 && c < left; ++bb)
+          if ((mw >> bb) & 1ull) {
+            take |= 1ull << bb;
+            ++c;
+          }
+        if (take == 0) {
+          w = (w + 1) % nw;
+          continue;
+        }
+        x->mask[w] &= ~take;
+        for (uint32_t bb = 0; bb < 64; ++bb)
+          if ((take >> bb) & 1ull) out[g->lanes[r++]] = page_offset(s, sl[i], w * 64 + bb, q->page_size);
+        left -= c;
+        if (left > 0) w = (w + 1) % nw;
+      }
+      if (left) s->error |= ERR_TIMEOUT;
+    }
+    assigned += took;
+    v += nb;
+  }
+  if (rec != MARKER) finish_reclaim(s, I, q, rec);
+}
+
 /* nact: active lanes of the whole wave (the device enters each instance's block with all of them). */
 static void inst_malloc(ouro_sim* s, inst_t* I, const int* lanes, int n, const uint32_t* sizes, uint64_t* out,
                         int nact) {
@@ -849,6 +971,14 @@ static void inst_malloc(ouro_sim* s, inst_t* I, const int* lanes, int n, const u
       }
   }
   int ok[64];
+  if (s->block_order && is_chunk_kind(I->kind)) {
+    /* block-wide chunk claims: one wave per class, classes in order, each claim whole before the next */
+    for (int j = 0; j < ng; ++j) {
+      acquire(s, I, &g[j], 1, &ok[j], nact);
+      claim_wave(s, I, &g[j], out);
+    }
+    return;
+  }
   acquire(s, I, g, ng, ok, nact);
   if (!is_chunk_kind(I->kind)) {
     /* dequeue positions: front fetch-add per group, rank order (atomicAggInc) */

@@ -124,8 +124,10 @@ def space(variants: Iterable[str], grids: Iterable[int] = (0,)) -> list[Candidat
 Runner = Callable[[Candidate, np.ndarray], tuple[float, bool, str]]
 
 
-def device_runner(pool_bytes: int, rounds: int = 2, warmup: int = 1) -> Runner:
-    """Measure a candidate on the device: Mops/s of `rounds` graph-replayed batch rounds."""
+def device_runner(pool_bytes: int, rounds: int = 3, warmup: int = 2) -> Runner:
+    """Measure a candidate on the device: Mops/s of `rounds` graph-replayed batch rounds. A fresh
+    pool first serves the list once untimed, so the candidate that happens to run first on a pool
+    does not pay for carving its chunks."""
     import ouroboros_amd as ob
 
     pools: dict[str, object] = {}
@@ -137,6 +139,10 @@ def device_runner(pool_bytes: int, rounds: int = 2, warmup: int = 1) -> Runner:
                 old.close()
             pools.clear()
             mm = pools[c.variant] = ob.MemoryManager(c.variant, pool_bytes)
+            try:
+                mm.run_batch(sizes, rounds=1, warmup=0, agg=0, block=256, graph=False)
+            except RuntimeError:
+                pass
         try:
             r = mm.run_batch(sizes, rounds=rounds, warmup=warmup, agg=c.agg, block=c.block, grid=c.grid, graph=True)
         except RuntimeError as e:  # e.g. a block size the persistent kernels cannot launch

@@ -22,13 +22,7 @@ def assert_clean(r):
     assert r["error_word"] == 0, r["errors"]
 
 
-@pytest.mark.parametrize("kind", [
-    *[k for k in ALL if k != "VLCQ"],
-    pytest.param("VLCQ", marks=pytest.mark.xfail(
-        reason="the device's block-wide claims reproduce the oracle for three rounds; in the fourth the "
-        "chunk chosen for a class diverges (device front_ lags the oracle's by nine positions), "
-        "cause not found; the invariant tests cover this kind", strict=False)),
-])
+@pytest.mark.parametrize("kind", ALL)
 def test_single_workgroup_matches_oracle(kind):
     # 640 requests, mixed classes (both instances), served tile by tile by one 64-thread workgroup
     sizes = (16 << (np.arange(640) * 7 % 12)).astype(np.uint32)

@@ -340,3 +340,22 @@ def test_trace_generator_matches_fixture_restatement():
 
     assert np.array_equal(GOLDEN["uniform_seed42__sizes"], ob.gen_sizes(42, 64, "uniform"))
     assert np.array_equal(GOLDEN["ragged_37__sizes"], ob.gen_sizes(7, 37, "uniform"))
+
+
+@pytest.mark.parametrize("kind", KINDS)
+def test_block_order_replay_invariants(kind):
+    # the block-wide replay (class leaders in class order; chunk queues claimed 64 positions at a
+    # time by a whole wave): every round's blocks are disjoint and on the page grid, the oracle raises
+    # no error, and after the last free every chunk queue holds exactly its chunks with free pages
+    sizes = (16 << (np.arange(1280) * 7 % 12)).astype(np.uint32)
+    s = oracle.Sim(kind, 512 << 20)
+    rounds = s.block_trace(sizes, 5)
+    assert rounds.shape == (5, len(sizes))
+    for r in range(5):
+        _offset_checks(rounds[r], sizes)
+    st = s.state()
+    assert st.error == 0
+    for k in range(st.num_classes):
+        assert st.sem_expected[k] == 0
+    if kind in ("CQ", "VACQ", "VLCQ"):
+        _check_chunk_queues(s, st)
