@@ -275,8 +275,8 @@ def main() -> int:
                 row[f"agg{agg}"] = {"mops": round(r["mops"], 2), "errors": r["errors"], "verify_errors": r["verify_errors"]}
             log(f"table {v} batched ...")
             with ob.MemoryManager(v, POOL) as mm:
-                b = mm.run_batch(sizes, rounds=max(4, steps // 4), warmup=1, agg=0, block=512, graph=True)
-            row["batched_block512"] = {"mops": round(b["mops"], 2), "errors": b["errors"]}
+                b = mm.run_batch(sizes, rounds=max(4, steps // 4), warmup=1, agg=0, block=1024, graph=True)
+            row["batched_block1024"] = {"mops": round(b["mops"], 2), "errors": b["errors"]}
             table[v] = row
     best_plain = max([plain["mops"]] + [table[v]["agg1"]["mops"] for v in table])
 
