@@ -194,6 +194,9 @@ typedef struct {
   const uint32_t* sizes;   /* host [requests] */
   uint64_t* offsets_out;   /* optional host [requests]: the validation round's blocks as byte offsets
                               from the data region, UINT64_MAX for nullptr */
+  int32_t latency;         /* 1 = one more (untimed) round recording the wall-clock time of each
+                              allocation call on lane 0 of every wave */
+  uint32_t pad;
 } ouro_batch_config;
 
 typedef struct {
@@ -212,6 +215,8 @@ typedef struct {
   uint32_t grid;           /* workgroups of the alloc / free kernels */
   uint32_t peak_chunks;
   uint32_t graph_used;     /* 1 if the rounds were graph replays */
+  double lat_p50_ns, lat_p99_ns, lat_max_ns; /* latency round (cfg->latency) */
+  uint64_t lat_samples;
 } ouro_batch_result;
 
 int ouro_run_batch(ouro_handle h, const ouro_batch_config* cfg, ouro_batch_result* out);
@@ -243,22 +248,25 @@ int ouro_gen_batch(const ouro_batch_workload* w, uint32_t* sizes, uint64_t cap, 
  * A host request loop in front of one device-resident pool. Request r arrives at a Poisson time
  * (rate per second), runs one kernel of 2^e threads (e ~ U[min_log2, max_log2]); thread i
  * allocates a block of class q ~ Zipf(zipf) over the first `classes` class ranks (16 B << q) through
- * the per-thread device malloc of width `agg`, fills it with a pattern keyed by (request, i), reads
- * it back and frees it. Up to `slots` requests run at once, one stream each (slots = 1, agg = 1 is
- * the plain per-request launch path). With admit = 1 a request waits until the pool's free-chunk
- * count, copied asynchronously from the device state, covers its estimated demand. Latency is
- * arrival to kernel completion, on the device clock. */
+ * the per-thread device malloc of width `agg` or the block-wide malloc (agg 0), fills it with a
+ * pattern keyed by (request, i), reads it back and frees it. Continuous batching: whenever one of
+ * `slots` launch slots (one stream each) is free, up to `max_batch` requests that have arrived are
+ * served by one launch, each workgroup working on a stretch of one request, and launches overlap.
+ * slots = 1, max_batch = 1, agg = 1 is the plain per-request launch path. With admit = 1 a request
+ * joins a launch only when the pool's free pages, copied asynchronously from the device state, cover
+ * the demand of everything in flight plus its own. Latency is arrival to the completion of the
+ * request's launch, on the device clock. */
 typedef struct {
   double rate;             /* arrivals per second */
   double zipf;             /* class-rank exponent */
   uint64_t seed;
   uint32_t requests;
   uint32_t slots;          /* requests in flight at most (1 .. 64) */
-  int32_t agg;             /* 1 / 16 / 64 */
+  int32_t agg;             /* 0 = block-wide (malloc_block / free_block, 1024-thread blocks), 1 / 16 / 64 */
   uint32_t min_log2, max_log2; /* threads per request: 2^U[min, max], max <= 24 */
   uint32_t classes;        /* size classes drawn (1 .. 10) */
-  int32_t admit;           /* 1 = admission by the device free-chunk count */
-  uint32_t pad;
+  int32_t admit;           /* 1 = admission by the device free-page count */
+  uint32_t max_batch;      /* requests one launch may carry (0 or 1: one request per launch) */
   unsigned long long* results_out; /* optional host [requests][4]: words intact, words not, nulls, threads */
   double* latency_out;     /* optional host [requests] ms */
 } ouro_serve_config;
@@ -276,6 +284,8 @@ typedef struct {
   uint32_t max_in_flight;
   uint32_t deferred;       /* requests the admission test held back at least once */
   uint32_t error_word;
+  uint32_t launches;       /* kernel launches that served the requests */
+  uint32_t pad;
 } ouro_serve_result;
 
 int ouro_serve(ouro_handle h, const ouro_serve_config* cfg, ouro_serve_result* out);

@@ -136,6 +136,8 @@ class BatchConfig(C.Structure):
         ("check", C.c_int32),
         ("sizes", C.POINTER(C.c_uint32)),
         ("offsets_out", C.POINTER(C.c_uint64)),
+        ("latency", C.c_int32),
+        ("pad", C.c_uint32),
     ]
 
 
@@ -156,6 +158,10 @@ class BatchResult(C.Structure):
         ("grid", C.c_uint32),
         ("peak_chunks", C.c_uint32),
         ("graph_used", C.c_uint32),
+        ("lat_p50_ns", C.c_double),
+        ("lat_p99_ns", C.c_double),
+        ("lat_max_ns", C.c_double),
+        ("lat_samples", C.c_uint64),
     ]
 
     def as_dict(self) -> dict:
@@ -206,7 +212,7 @@ class ServeConfig(C.Structure):
         ("max_log2", C.c_uint32),
         ("classes", C.c_uint32),
         ("admit", C.c_int32),
-        ("pad", C.c_uint32),
+        ("max_batch", C.c_uint32),
         ("results_out", C.POINTER(C.c_ulonglong)),
         ("latency_out", C.POINTER(C.c_double)),
     ]
@@ -229,6 +235,8 @@ class ServeResult(C.Structure):
         ("max_in_flight", C.c_uint32),
         ("deferred", C.c_uint32),
         ("error_word", C.c_uint32),
+        ("launches", C.c_uint32),
+        ("pad", C.c_uint32),
     ]
 
     def as_dict(self) -> dict:
@@ -338,10 +346,11 @@ def gen_batch(seed: int = 42, workloads: int = 256, mu: float = 10.0, sigma: flo
 
 def serve_config(requests: int, rate: float, slots: int = 8, agg: int = 64, min_log2: int = 10,
                  max_log2: int = 18, zipf: float = 1.1, classes: int = 10, seed: int = 1,
-                 admit: bool = True) -> ServeConfig:
+                 admit: bool = True, max_batch: int = 1) -> ServeConfig:
     cfg = ServeConfig()
     cfg.rate, cfg.zipf, cfg.seed, cfg.requests, cfg.slots, cfg.agg = rate, zipf, seed, requests, slots, agg
     cfg.min_log2, cfg.max_log2, cfg.classes, cfg.admit = min_log2, max_log2, classes, int(admit)
+    cfg.max_batch = max_batch
     return cfg
 
 
@@ -446,10 +455,12 @@ class MemoryManager:
         graph: bool = True,
         check: bool = False,
         offsets: np.ndarray | None = None,
+        latency: bool = False,
     ) -> dict:
         """The batched form: `sizes` is a flat request list served by persistent block-aggregated
         kernels (agg 0) or the per-thread kernels (agg 1, 16, 64); rounds replay a captured graph.
-        `offsets`, if given, receives the validation round's blocks (byte offsets, UINT64_MAX = null)."""
+        `offsets`, if given, receives the validation round's blocks (byte offsets, UINT64_MAX = null);
+        `latency` adds an untimed round that samples the wall-clock time of each allocation call."""
         sizes = np.ascontiguousarray(sizes, dtype=np.uint32)
         cfg = BatchConfig()
         cfg.requests = len(sizes)
@@ -460,6 +471,7 @@ class MemoryManager:
         cfg.grid = grid
         cfg.graph = int(graph)
         cfg.check = int(check or offsets is not None)
+        cfg.latency = int(latency)
         cfg.sizes = sizes.ctypes.data_as(C.POINTER(C.c_uint32))
         if offsets is not None:
             assert offsets.dtype == np.uint64 and offsets.shape == sizes.shape and offsets.flags.c_contiguous
@@ -481,12 +493,15 @@ class MemoryManager:
         seed: int = 1,
         admit: bool = True,
         per_request: bool = False,
+        max_batch: int = 1,
     ) -> dict:
-        """The serving form: Poisson arrivals at `rate` per second, each request one kernel of
-        2^U[min_log2, max_log2] threads allocating Zipf(zipf) class sizes; up to `slots` in flight.
-        slots = 1, agg = 1 is the plain per-request launch path. With per_request the result holds
+        """The serving form: Poisson arrivals at `rate` per second, each request 2^U[min_log2, max_log2]
+        threads allocating Zipf(zipf) class sizes; up to `slots` launches in flight, each carrying up
+        to `max_batch` of the requests that have arrived.
+        agg 0 serves each request block-wide (malloc_block / free_block); slots = 1, agg = 1 is the
+        plain per-request launch path. With per_request the result holds
         `results` ([requests, 4]: words intact, words not, nulls, threads) and `latency_ms`."""
-        cfg = serve_config(requests, rate, slots, agg, min_log2, max_log2, zipf, classes, seed, admit)
+        cfg = serve_config(requests, rate, slots, agg, min_log2, max_log2, zipf, classes, seed, admit, max_batch)
         res_buf = np.zeros((requests, 4), np.uint64) if per_request else None
         lat_buf = np.zeros(requests, np.float64) if per_request else None
         if per_request:

@@ -426,6 +426,29 @@ int ouro_run_batch(ouro_handle h, const ouro_batch_config* cfg, ouro_batch_resul
   if (g) (void)hipGraphDestroy(g);
   unsigned long long e3[3];
   OURO_TRY(hipMemcpy(e3, d_errs.p, 24, hipMemcpyDeviceToHost));
+  if (cfg->latency) {
+    // one more round, untimed, sampling the allocation call on lane 0 of every wave
+    const uint32_t ns = (n + 63) / 64;
+    DevBuf d_lat;
+    OURO_TRY(d_lat.alloc((size_t)ns * 8));
+    OURO_TRY(hipMemset(d_lat.p, 0, (size_t)ns * 8));
+    OURO_TRY(cfg->agg == 0 ? m.malloc_persist(sizes, ptrs, n, s, block, grid, d_lat.as<uint64_t>())
+                           : m.malloc_batch(sizes, ptrs, n, cfg->agg, s, d_lat.as<uint64_t>(), block));
+    OURO_TRY(touch());
+    OURO_TRY(release());
+    OURO_TRY(hipStreamSynchronize(s));
+    std::vector<uint64_t> lat(ns);
+    OURO_TRY(hipMemcpy(lat.data(), d_lat.p, (size_t)ns * 8, hipMemcpyDeviceToHost));
+    std::sort(lat.begin(), lat.end());
+    int khz = 0;
+    (void)hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, 0);
+    const double ns_per_tick = khz > 0 ? 1e6 / (double)khz : 10.0;
+    auto pct = [&](double q) { return (double)lat[(size_t)std::min<double>(ns - 1, std::floor(q * (ns - 1) + 0.5))] * ns_per_tick; };
+    out->lat_p50_ns = pct(0.50);
+    out->lat_p99_ns = pct(0.99);
+    out->lat_max_ns = (double)lat.back() * ns_per_tick;
+    out->lat_samples = ns;
+  }
   out->verify_errors = e3[0];
   out->bounds_errors = e3[1];
   out->null_returns = e3[2];

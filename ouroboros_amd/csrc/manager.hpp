@@ -15,6 +15,7 @@
 namespace ouro_lib {
 
 struct ZipfTable;
+struct RequestBlock;
 
 // Free capacity of a pool: `shared` base chunks any class can take (the bump region left and the
 // reuse ring) and, per class in chain order, its free pages (the class semaphore's count).
@@ -44,13 +45,14 @@ struct IManager {
   // Batched form: persistent kernels over a flat request list with block-wide aggregation
   // (Ouroboros::malloc_block / free_block). grid = persistent workgroups.
   virtual hipError_t malloc_persist(const uint32_t* d_sizes, void** d_out, uint32_t n, hipStream_t s, uint32_t block,
-                                    uint32_t grid) = 0;
+                                    uint32_t grid, uint64_t* d_lat = nullptr) = 0;
   virtual hipError_t free_persist(void* const* d_ptrs, uint32_t n, hipStream_t s, uint32_t block, uint32_t grid) = 0;
   // workgroups of `block` threads one CU holds at once for the persistent kernels (the smaller of the two)
   virtual int persist_occupancy(uint32_t block) = 0;
-  // Serving form: one request kernel of n threads (see k_request) on stream s.
-  virtual hipError_t launch_request(uint64_t seed, uint32_t n, const struct ZipfTable& z, unsigned long long* out,
-                                    int agg, hipStream_t s) = 0;
+  // Serving form: one launch of `blocks` workgroups described by d_desc (k_request / k_request_block;
+  // 1024 threads per workgroup for agg 0, else 256) on stream s.
+  virtual hipError_t launch_requests(const struct RequestBlock* d_desc, uint32_t blocks, const struct ZipfTable& z,
+                                     unsigned long long* out, int agg, hipStream_t s) = 0;
   // What the pool can still hand out, read from a host copy of the manager state (state_bytes()
   // bytes at device_manager()).
   virtual void pool_free(const void* state, struct PoolFree* out) = 0;
@@ -142,6 +144,26 @@ __global__ void __launch_bounds__(1024) k_malloc_persist(M* mm, const uint32_t* 
   }
 }
 
+// k_malloc_persist with the wall-clock time of each malloc_block call recorded on lane 0 of every
+// wave that holds a request (sample of request i at lat[i / 64]).
+template <class M>
+__global__ void __launch_bounds__(1024) k_malloc_persist_lat(M* mm, const uint32_t* __restrict__ sizes,
+                                                             void** __restrict__ out, uint32_t n,
+                                                             uint64_t* __restrict__ lat) {
+  __shared__ typename M::Scratch sm;
+  const uint32_t step = gridDim.x * blockDim.x;
+  for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += step) {
+    const uint32_t i = base + threadIdx.x;
+    const uint64_t t0 = wall_clock64();
+    void* p = mm->malloc_block(i < n ? sizes[i] : 0u, sm);
+    const uint64_t t1 = wall_clock64();
+    if (i < n) {
+      out[i] = p;
+      if ((threadIdx.x & 63u) == 0) lat[i >> 6] = t1 - t0;
+    }
+  }
+}
+
 template <class M>
 __global__ void __launch_bounds__(1024) k_free_persist(M* mm, void* const* __restrict__ ptrs, uint32_t n) {
   __shared__ typename M::Scratch sm;
@@ -171,47 +193,93 @@ __device__ __forceinline__ uint64_t dev_keyed(uint64_t seed, uint64_t i) {
   return dev_splitmix64(dev_splitmix64(seed) ^ (i * 0xD1B54A32D192ED03ull));
 }
 
-// One request of the serving form, an allocation-heavy user kernel: thread i allocates a block of a
-// Zipf-drawn class (16 B << q), fills it with a pattern keyed by (request seed, i), reads it back
-// and frees it. out[0] += words read back intact, out[1] += words that were not, out[2] += null
-// returns: a request's result depends only on its seed and size, never on what runs beside it.
-template <class M, int W>
-__global__ void __launch_bounds__(256) k_request(M* mm, uint64_t seed, uint32_t n, ZipfTable z,
-                                                 unsigned long long* __restrict__ out) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  unsigned long long ok = 0, bad = 0, nul = 0;
-  if (i < n) {
-    const uint64_t key = dev_keyed(seed, i);
-    const uint32_t r = (uint32_t)(key >> 32);
-    uint32_t q = 0;
-    while (q + 1 < z.classes && r >= z.thr[q]) ++q;
-    const uint32_t words = (16u << q) / 4u;
-    uint32_t* p = reinterpret_cast<uint32_t*>(mm->template malloc<W>(16u << q));
-    if (p) {
-      const uint32_t v0 = (uint32_t)key;
-      for (uint32_t k = 0; k < words; k += 4)  // blocks are at least 16-byte aligned
-        *reinterpret_cast<uint4*>(p + k) = make_uint4(v0 + k, v0 + k + 1, v0 + k + 2, v0 + k + 3);
-      for (uint32_t k = 0; k < words; k += 4) {
-        const uint4 v = *reinterpret_cast<const uint4*>(p + k);
-        const uint32_t g = (v.x == v0 + k) + (v.y == v0 + k + 1) + (v.z == v0 + k + 2) + (v.w == v0 + k + 3);
-        ok += g;
-        bad += 4u - g;
-      }
-      mm->template free<W>(p);
-    } else {
-      nul = 1;
-    }
+// Work of one workgroup of a serving launch: threads first .. first + blockDim.x of request `req`
+// (n threads, pattern seed `seed`). One launch carries the blocks of every request it batches.
+struct RequestBlock {
+  uint64_t seed;
+  uint32_t req;
+  uint32_t first;
+  uint32_t n;
+  uint32_t pad;
+};
+
+// Thread i of a request: a block of class q (16 B << q, q drawn from the top bits of keyed(seed, i)
+// against the Zipf table), filled with a pattern keyed by (seed, i), read back and freed.
+__device__ __forceinline__ void request_draw(const RequestBlock& d, uint32_t i, const ZipfTable& z, uint32_t& q,
+                                             uint32_t& v0) {
+  const uint64_t key = dev_keyed(d.seed, i);
+  const uint32_t r = (uint32_t)(key >> 32);
+  q = 0;
+  while (q + 1 < z.classes && r >= z.thr[q]) ++q;
+  v0 = (uint32_t)key;
+}
+
+__device__ __forceinline__ void request_touch(uint32_t* p, uint32_t q, uint32_t v0, unsigned long long& ok,
+                                              unsigned long long& bad) {
+  const uint32_t words = (16u << q) / 4u;
+  for (uint32_t k = 0; k < words; k += 4)  // blocks are at least 16-byte aligned
+    *reinterpret_cast<uint4*>(p + k) = make_uint4(v0 + k, v0 + k + 1, v0 + k + 2, v0 + k + 3);
+  for (uint32_t k = 0; k < words; k += 4) {
+    const uint4 v = *reinterpret_cast<const uint4*>(p + k);
+    const uint32_t g = (v.x == v0 + k) + (v.y == v0 + k + 1) + (v.z == v0 + k + 2) + (v.w == v0 + k + 3);
+    ok += g;
+    bad += 4u - g;
   }
+}
+
+// Per-request result line: out[4 * req + 0] += words read back intact, + 1 words that were not,
+// + 2 null returns. A request's result depends only on its seed and size, never on what ran beside it.
+__device__ __forceinline__ void request_report(unsigned long long* out, uint32_t req, unsigned long long ok,
+                                               unsigned long long bad, unsigned long long nul) {
   for (int off = 32; off > 0; off >>= 1) {
     ok += __shfl_xor(ok, off);
     bad += __shfl_xor(bad, off);
     nul += __shfl_xor(nul, off);
   }
   if (__lane_id() == 0) {
-    if (ok) atomicAdd(&out[0], ok);
-    if (bad) atomicAdd(&out[1], bad);
-    if (nul) atomicAdd(&out[2], nul);
+    if (ok) atomicAdd(&out[4 * (size_t)req + 0], ok);
+    if (bad) atomicAdd(&out[4 * (size_t)req + 1], bad);
+    if (nul) atomicAdd(&out[4 * (size_t)req + 2], nul);
   }
+}
+
+// A serving launch through the per-thread device malloc of width W.
+template <class M, int W>
+__global__ void __launch_bounds__(256) k_request(M* mm, const RequestBlock* __restrict__ desc, ZipfTable z,
+                                                 unsigned long long* __restrict__ out) {
+  const RequestBlock d = desc[blockIdx.x];
+  const uint32_t i = d.first + threadIdx.x;
+  unsigned long long ok = 0, bad = 0, nul = 0;
+  if (i < d.n) {
+    uint32_t q, v0;
+    request_draw(d, i, z, q, v0);
+    uint32_t* p = reinterpret_cast<uint32_t*>(mm->template malloc<W>(16u << q));
+    if (p) {
+      request_touch(p, q, v0, ok, bad);
+      mm->template free<W>(p);
+    } else {
+      nul = 1;
+    }
+  }
+  request_report(out, d.req, ok, bad, nul);
+}
+
+// The same launch served block-wide (Ouroboros::malloc_block / free_block): every thread of the
+// block calls together, a thread past the request's end with size 0 / nullptr.
+template <class M>
+__global__ void __launch_bounds__(1024) k_request_block(M* mm, const RequestBlock* __restrict__ desc, ZipfTable z,
+                                                        unsigned long long* __restrict__ out) {
+  __shared__ typename M::Scratch sm;
+  const RequestBlock d = desc[blockIdx.x];
+  const uint32_t i = d.first + threadIdx.x;
+  unsigned long long ok = 0, bad = 0, nul = 0;
+  uint32_t q = 0, v0 = 0;
+  if (i < d.n) request_draw(d, i, z, q, v0);
+  uint32_t* p = reinterpret_cast<uint32_t*>(mm->malloc_block(i < d.n ? (16u << q) : 0u, sm));
+  if (p) request_touch(p, q, v0, ok, bad);
+  else if (i < d.n) nul = 1;
+  mm->free_block(p, sm);
+  request_report(out, d.req, ok, bad, nul);
 }
 
 #ifdef OURO_DIAG
@@ -267,10 +335,14 @@ struct Manager final : IManager {
     return hipGetLastError();
   }
   hipError_t malloc_persist(const uint32_t* d_sizes, void** d_out, uint32_t n, hipStream_t s, uint32_t block,
-                            uint32_t grid) override {
+                            uint32_t grid, uint64_t* d_lat) override {
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL((k_malloc_persist<M>), dim3(grid), dim3(block), 0, s, mm.getDeviceMemoryManager(), d_sizes,
-                       d_out, n);
+    if (d_lat)
+      hipLaunchKernelGGL((k_malloc_persist_lat<M>), dim3(grid), dim3(block), 0, s, mm.getDeviceMemoryManager(), d_sizes,
+                         d_out, n, d_lat);
+    else
+      hipLaunchKernelGGL((k_malloc_persist<M>), dim3(grid), dim3(block), 0, s, mm.getDeviceMemoryManager(), d_sizes,
+                         d_out, n);
     return hipGetLastError();
   }
   hipError_t free_persist(void* const* d_ptrs, uint32_t n, hipStream_t s, uint32_t block, uint32_t grid) override {
@@ -278,21 +350,18 @@ struct Manager final : IManager {
     hipLaunchKernelGGL((k_free_persist<M>), dim3(grid), dim3(block), 0, s, mm.getDeviceMemoryManager(), d_ptrs, n);
     return hipGetLastError();
   }
-  template <int W>
-  hipError_t launch_request_w(uint64_t seed, uint32_t n, const ZipfTable& z, unsigned long long* out, hipStream_t s) {
-    hipLaunchKernelGGL((k_request<M, W>), dim3(grid_for(n, 256)), dim3(256), 0, s, mm.getDeviceMemoryManager(), seed,
-                       n, z, out);
-    return hipGetLastError();
-  }
-  hipError_t launch_request(uint64_t seed, uint32_t n, const ZipfTable& z, unsigned long long* out, int agg,
-                            hipStream_t s) override {
-    if (n == 0) return hipSuccess;
+  hipError_t launch_requests(const RequestBlock* d_desc, uint32_t blocks, const ZipfTable& z, unsigned long long* out,
+                             int agg, hipStream_t s) override {
+    if (blocks == 0) return hipSuccess;
+    M* dm = mm.getDeviceMemoryManager();
     switch (agg) {
-      case 1: return launch_request_w<1>(seed, n, z, out, s);
-      case 16: return launch_request_w<16>(seed, n, z, out, s);
-      case 64: return launch_request_w<64>(seed, n, z, out, s);
+      case 0: hipLaunchKernelGGL((k_request_block<M>), dim3(blocks), dim3(1024), 0, s, dm, d_desc, z, out); break;
+      case 1: hipLaunchKernelGGL((k_request<M, 1>), dim3(blocks), dim3(256), 0, s, dm, d_desc, z, out); break;
+      case 16: hipLaunchKernelGGL((k_request<M, 16>), dim3(blocks), dim3(256), 0, s, dm, d_desc, z, out); break;
+      case 64: hipLaunchKernelGGL((k_request<M, 64>), dim3(blocks), dim3(256), 0, s, dm, d_desc, z, out); break;
       default: return hipErrorInvalidValue;
     }
+    return hipGetLastError();
   }
   template <int N, class... Is>
   static void free_pages_of(const ouro::Chain<>&, PoolFree*) {}
@@ -324,6 +393,9 @@ struct Manager final : IManager {
     int a = 0, b = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_malloc_persist<M>, (int)block, 0) != hipSuccess) return 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_free_persist<M>, (int)block, 0) != hipSuccess) return 0;
+    int c = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&c, k_malloc_persist_lat<M>, (int)block, 0) != hipSuccess) return 0;
+    b = b < c ? b : c;
     return a < b ? a : b;
   }
   hipError_t free_batch(void* const* d_ptrs, uint32_t n, int agg, hipStream_t s, uint32_t block) override {

@@ -63,3 +63,16 @@ def test_batch_runner_per_wave_widths(agg):
         res = mm.run_batch(sizes, rounds=2, warmup=1, agg=agg, block=256, graph=True, check=True)
         assert_clean(res)
         assert mm.check_quiescent().violations == 0
+
+
+@pytest.mark.parametrize("agg", [0, 64])
+def test_batch_latency_round(agg):
+    print(f"batch latency agg={agg}", flush=True)
+    sizes = ob.gen_sizes(5, 65536, "uniform")
+    with ob.MemoryManager("PQ", 1 << 30) as mm:
+        r = mm.run_batch(sizes, rounds=1, warmup=1, agg=agg, block=1024 if agg == 0 else 256, graph=False,
+                         latency=True)
+        assert_clean(r)
+        assert r["lat_samples"] == len(sizes) // 64
+        assert 0 < r["lat_p50_ns"] <= r["lat_p99_ns"] <= r["lat_max_ns"] < 1e9
+        assert mm.check_quiescent().violations == 0

@@ -26,14 +26,17 @@ def test_results_same_alone_and_in_the_mix(kind):
     with ob.MemoryManager(kind, 2 << 30) as mm:
         alone = mm.serve(n, rate=2000.0, slots=1, agg=1, admit=False, per_request=True, **kw)
         mix = mm.serve(n, rate=1e6, slots=16, agg=64, admit=True, per_request=True, **kw)
-        for name, r in (("alone", alone), ("mix", mix)):
+        block_mix = mm.serve(n, rate=1e6, slots=4, agg=0, admit=True, per_request=True, max_batch=16, **kw)
+        for name, r in (("alone", alone), ("mix", mix), ("block mix", block_mix)):
             assert r["error_word"] == 0, (name, r["errors"])
             assert r["completed"] == n and r["mismatches"] == 0 and r["nulls"] == 0, (name, r)
             res = r["results"]
             assert np.array_equal(res[:, 3], want[:, 0]), name
             assert np.array_equal(res[:, 0], want[:, 1]), name
         assert np.array_equal(alone["results"], mix["results"])
-        assert mix["max_in_flight"] > 1
+        assert np.array_equal(alone["results"], block_mix["results"])
+        assert mix["max_in_flight"] > 1 and block_mix["max_in_flight"] > 1
+        assert alone["launches"] == n and block_mix["launches"] < n
         assert mm.check_quiescent().violations == 0
 
 
@@ -46,7 +49,7 @@ def test_admission_holds_back_requests_beyond_the_pool():
     want = expected(n, rate=1000.0, **kw)
     print("serve admission", flush=True)
     with ob.MemoryManager("VAPQ", 512 << 20) as mm:
-        r = mm.serve(n, rate=1e6, slots=16, agg=64, admit=True, per_request=True, **kw)
+        r = mm.serve(n, rate=1e6, slots=16, agg=64, admit=True, per_request=True, max_batch=8, **kw)
         assert r["error_word"] == 0, r["errors"]
         assert r["completed"] == n and r["mismatches"] == 0 and r["nulls"] == 0, r
         assert np.array_equal(r["results"][:, 0], want[:, 1])
