@@ -63,9 +63,12 @@ __device__ inline void block_claim_pages(Memory& m, I& inst, ReuseRing& base, ui
   if (!ballot(lead)) return;
   Ctx<I> x{m, inst, base};
   ClassQueue* cq = &inst.q[mine ? l - off : 0u];
+  // front_ positions and the semaphore claim in flight together (see alloc_pages)
+  uint32_t pos = 0;
+  if (lead) pos = add(&cq->front, total);
   auto carve = [&](ClassQueue* lq, uint32_t k) { return carve_pages(x, lq, k); };
   sem_claim_n(x, cq, lead, total, carve);
-  if (lead) sm.base[l] = add(&cq->front, total);
+  if (lead) sm.base[l] = pos;
 }
 
 template <class I>

@@ -810,12 +810,17 @@ __device__ inline void* alloc_pages(Ctx<I>& x, uint32_t qi, bool want) {
   ClassQueue* cq = &x.inst.q[want ? qi : 0];
   const uint64_t key = want ? (uint64_t)qi : ~0ull;
   const uint64_t grp = match<GS>(key);
+  // the group's front_ positions (atomicAggInc) are fetched together with its semaphore claim: the
+  // two atomics are in flight at once, and the positions do not depend on the claim's outcome (a
+  // carve the claim owes fills the queue at back_, before any lane waits on its slot)
+  const int L = first(grp);
+  uint32_t pos = 0;
+  if (want && (int)lane() == L) pos = add(&cq->front, popc(grp));
   {
     auto carve = [&](ClassQueue* lq, uint32_t k) { return carve_pages(x, lq, k); };
     sem_claim(x, cq, grp, want, carve);
   }
-  uint32_t n;
-  const uint32_t v = fetch_positions<GS>(&cq->front, want, n);
+  const uint32_t v = shfl(pos, L) + rank_in(grp);
   const uint32_t idx = take_lanes<W>(x, cq, want, v);
   if (!want || idx == kMarker || idx == kNullPage) return nullptr;
   if (!chunk_ok(x.m, index_chunk(idx)) || index_page(idx) >= ppc_of(x, cq)) {

@@ -262,7 +262,7 @@ typedef struct {
   uint64_t seed;
   uint32_t requests;
   uint32_t slots;          /* requests in flight at most (1 .. 64) */
-  int32_t agg;             /* 0 = block-wide (malloc_block / free_block, 1024-thread blocks), 1 / 16 / 64 */
+  int32_t agg;             /* 0 = block-wide (malloc_block / free_block), 1 / 16 / 64; 256-thread workgroups */
   uint32_t min_log2, max_log2; /* threads per request: 2^U[min, max], max <= 24 */
   uint32_t classes;        /* size classes drawn (1 .. 10) */
   int32_t admit;           /* 1 = admission by the device free-page count */

@@ -49,8 +49,8 @@ struct IManager {
   virtual hipError_t free_persist(void* const* d_ptrs, uint32_t n, hipStream_t s, uint32_t block, uint32_t grid) = 0;
   // workgroups of `block` threads one CU holds at once for the persistent kernels (the smaller of the two)
   virtual int persist_occupancy(uint32_t block) = 0;
-  // Serving form: one launch of `blocks` workgroups described by d_desc (k_request / k_request_block;
-  // 1024 threads per workgroup for agg 0, else 256) on stream s.
+  // Serving form: one launch of `blocks` 256-thread workgroups described by d_desc (k_request /
+  // k_request_block) on stream s.
   virtual hipError_t launch_requests(const struct RequestBlock* d_desc, uint32_t blocks, const struct ZipfTable& z,
                                      unsigned long long* out, int agg, hipStream_t s) = 0;
   // What the pool can still hand out, read from a host copy of the manager state (state_bytes()
@@ -355,7 +355,7 @@ struct Manager final : IManager {
     if (blocks == 0) return hipSuccess;
     M* dm = mm.getDeviceMemoryManager();
     switch (agg) {
-      case 0: hipLaunchKernelGGL((k_request_block<M>), dim3(blocks), dim3(1024), 0, s, dm, d_desc, z, out); break;
+      case 0: hipLaunchKernelGGL((k_request_block<M>), dim3(blocks), dim3(256), 0, s, dm, d_desc, z, out); break;
       case 1: hipLaunchKernelGGL((k_request<M, 1>), dim3(blocks), dim3(256), 0, s, dm, d_desc, z, out); break;
       case 16: hipLaunchKernelGGL((k_request<M, 16>), dim3(blocks), dim3(256), 0, s, dm, d_desc, z, out); break;
       case 64: hipLaunchKernelGGL((k_request<M, 64>), dim3(blocks), dim3(256), 0, s, dm, d_desc, z, out); break;

@@ -87,7 +87,10 @@ int ouro_serve(ouro_handle h, const ouro_serve_config* cfg, ouro_serve_result* o
   IManager& m = *h->impl;
   const uint32_t R = cfg->requests, S = cfg->slots;
   const uint32_t max_batch = cfg->max_batch ? cfg->max_batch : 1u;
-  const uint32_t B = cfg->agg == 0 ? 1024u : 256u;  // threads per workgroup
+  // threads per workgroup: 256 also for block-wide allocation, where smaller workgroups keep more
+  // of them resident per CU to overlap the requests' memory traffic (measured: 1024-thread
+  // workgroups served 26% fewer requests per second at saturation)
+  const uint32_t B = 256u;
 
   ZipfTable z;
   std::vector<double> pq;
