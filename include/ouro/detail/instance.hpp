@@ -392,7 +392,7 @@ template <class I>
 __device__ inline void compact(Ctx<I>& x, ClassQueue* cq) {
   constexpr Kind K = I::kKind;
   static_assert(is_chunk_kind(K), "page queues dequeue strictly at front_");
-  uint32_t f = ld(&cq->front);
+  uint32_t f = ld(&cq->front), lost = 0;
   const uint32_t b = ld(&cq->back);
   for (uint32_t i = 0; i < kCompactSteps && f != b && b - f < 0x80000000u; ++i) {
     uint32_t nf = f + 1u;
@@ -418,9 +418,12 @@ __device__ inline void compact(Ctx<I>& x, ClassQueue* cq) {
         return;
       }
     }
-    if (cas(&cq->front, f, nf) != f) {
+    if (const uint32_t seen = cas(&cq->front, f, nf); seen != f) {
       OURO_DIAG_ADD(15, 1);
-      return;
+      // another lane moved front_: carry on from there (a few times at most), unless it went back
+      if (seen - f >= 0x80000000u || ++lost > 8) return;
+      f = seen;
+      continue;
     }
     // the ring slot becomes writable again for position f + ring_size
     if constexpr (is_std(K)) cas(&ring_of(x, cq)[f & (I::kQueueEntries - 1)], kTomb, kMarker);
@@ -428,24 +431,27 @@ __device__ inline void compact(Ctx<I>& x, ClassQueue* cq) {
   }
 }
 
-// compact() for a whole, fully active wave: each round looks at up to 64 positions from front_ at
-// once and moves front_ over the leading dead ones with one CAS (a reclaimed VA queue chunk or a VL
-// queue chunk front_ptr_ has left is a single move). It stops where compact() would: at the first
-// position it cannot prove dead, after kCompactSteps moves, or when another lane moved front_.
+// compact() for a whole, fully active wave: each round looks at up to 256 positions from front_
+// (four per lane) and moves front_ over the leading dead ones with one CAS (a reclaimed VA queue chunk
+// or a VL queue chunk front_ptr_ has left is a single move). It stops where compact() would: at the
+// first position it cannot prove dead or after kCompactSteps moves. A CAS lost to another compactor
+// that moved front_ forward is not the end: the round is retried from there (a few times at most),
+// since with thousands of concurrent removers the winner may stop short of what this wave saw dead.
 template <class I>
 __device__ inline void compact_wave(Ctx<I>& x, ClassQueue* cq) {
   constexpr Kind K = I::kKind;
   static_assert(is_chunk_kind(K), "page queues dequeue strictly at front_");
+  constexpr uint32_t kPer = 4;  // positions per lane and round
   const uint32_t l = lane();
-  uint32_t moves = 0;
-  while (moves < kCompactSteps) {
+  uint32_t moves = 0, lost = 0;
+  while (moves < kCompactSteps && lost < 8) {
     uint32_t f = 0, n = 0, jump = kMarker;
     const uint32_t* sl = nullptr;
     if (l == 0) {
       f = ld(&cq->front);
       n = ld(&cq->back) - f;
       if (n >= 0x80000000u) n = 0;
-      if (n > kWave) n = kWave;
+      if (n > kWave * kPer) n = kWave * kPer;
       if (n > kCompactSteps - moves) n = kCompactSteps - moves;
       if constexpr (is_va(K)) {
         const uint32_t e = ld(&ring_of(x, cq)[(f / kQCSpots) & (I::kQueueEntries - 1)]);
@@ -475,22 +481,51 @@ __device__ inline void compact_wave(Ctx<I>& x, ClassQueue* cq) {
     if (n == 0) break;
     uint32_t nf = jump;
     if (jump == kMarker) {
-      bool dead = false;
-      if (l < n) {
-        const uint32_t e = is_std(K) ? ld(&ring_of(x, cq)[(f + l) & (I::kQueueEntries - 1)]) : ld(&sl[l]);
-        dead = e == kTomb;
+      uint32_t k = 0;
+#pragma unroll
+      for (uint32_t r = 0; r < kPer; ++r) {
+        const uint32_t i = r * kWave + l;
+        bool dead = false;
+        if (i < n) {
+          const uint32_t e = is_std(K) ? ld(&ring_of(x, cq)[(f + i) & (I::kQueueEntries - 1)]) : ld(&sl[i]);
+          dead = e == kTomb;
+        }
+        const uint64_t alive = ~ballot(dead);
+        const uint32_t run = alive ? (uint32_t)__builtin_ctzll(alive) : kWave;
+        if (k == r * kWave) k += run;  // still inside the leading dead stretch
       }
-      const uint64_t alive = ~ballot(dead);
-      const uint32_t k = alive ? (uint32_t)__builtin_ctzll(alive) : kWave;
+      if (k > n) k = n;
+#ifdef OURO_DIAG
+      if (k < n && l == 0) {
+        const uint32_t e = is_std(K) ? ld(&ring_of(x, cq)[(f + k) & (I::kQueueEntries - 1)]) : ld(&sl[k]);
+        OURO_DIAG_ADD(e == kMarker ? 12 : 13, 1);
+        if (class_index(x, cq) == 9 && chunk_ok(x.m, e)) {
+          const ChunkHdr* h = hdr<ChunkHdr>(x.m, e);
+          OURO_DIAG_LOG((10ull << 60) | ((uint64_t)(k & 0xFF) << 48) | ((uint64_t)(ld(&h->count) & 0xFFFF) << 32) |
+                        (uint64_t)(ld(&h->queue_pos) - (f + k)));
+        }
+      }
+#endif
       if (k == 0) break;
       nf = f + k;
     }
-    bool moved = false;
-    if (l == 0) moved = cas(&cq->front, f, nf) == f;
-    if (!ballot(moved)) break;
+    uint32_t seen = 0;
+    if (l == 0) seen = cas(&cq->front, f, nf);
+    seen = uniform(seen);
+    if (seen != f) {
+      // another compactor moved front_: retry from where it left it, unless it went backwards
+      if (seen - f >= 0x80000000u) break;
+      ++lost;
+      OURO_DIAG_ADD(11, 1);
+      continue;
+    }
     // the ring slots become writable again for positions + ring_size
     if constexpr (is_std(K)) {
-      if (l < nf - f) cas(&ring_of(x, cq)[(f + l) & (I::kQueueEntries - 1)], kTomb, kMarker);
+#pragma unroll
+      for (uint32_t r = 0; r < kPer; ++r) {
+        const uint32_t i = r * kWave + l;
+        if (i < nf - f) cas(&ring_of(x, cq)[(f + i) & (I::kQueueEntries - 1)], kTomb, kMarker);
+      }
     }
     if (jump != kMarker) {
       moves += 1;

@@ -154,6 +154,7 @@ typedef struct {
 /* ---- lifetime ------------------------------------------------------------------------------ */
 /* Ouroboros::initialize(pool_bytes) (device/MemoryInitialization.cuh:123-219) on the current device. */
 int ouro_create(int variant, uint64_t pool_bytes, ouro_handle* out);
+/* Memory::~Memory (device/Ouroboros.cuh:11-15): frees the pool. */
 int ouro_destroy(ouro_handle h);
 /* getDeviceMemoryManager() (device/Ouroboros.cuh:297): pass to kernels built on include/ouro. */
 void* ouro_device_manager(ouro_handle h);
@@ -169,12 +170,16 @@ int ouro_malloc_batch(ouro_handle h, const uint32_t* d_sizes, void** d_out, uint
 int ouro_free_batch(ouro_handle h, void* const* d_ptrs, uint32_t n, int agg, void* stream);
 
 /* ---- state and checks ----------------------------------------------------------------------- */
+/* printFreeResources (device/MemoryQueries.cuh:8-61) as data. */
 int ouro_query(ouro_handle h, ouro_state* out);
+/* Invariant checker of a quiescent pool (no reference counterpart): queues, masks and counts back at
+ * what the carved chunks account for, no error bit set. */
 int ouro_check_quiescent(ouro_handle h, ouro_check_result* out);
 /* Copy the whole pool (the complete allocator state) to dst (device or host memory, allocation_size bytes). */
 int ouro_snapshot(ouro_handle h, void* dst);
 
 /* ---- harness -------------------------------------------------------------------------------- */
+/* The harness loop of src/main.cu:146-172: alloc / write / verify / free rounds, kernels timed apart. */
 int ouro_run_trace(ouro_handle h, const ouro_trace_config* cfg, ouro_trace_result* out);
 /* ---- batched form (no reference counterpart) ------------------------------------------------- */
 /* A flat list of allocation requests served by persistent kernels with block-wide aggregation

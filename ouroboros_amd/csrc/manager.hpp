@@ -131,16 +131,21 @@ __global__ void __launch_bounds__(256) k_free_masked(M* mm, void* const* __restr
 }
 
 // Persistent block-aggregated kernels: each workgroup walks block-sized tiles of the request list
-// (the loop bound is uniform per block, as malloc_block / free_block synchronise the block).
+// (the loop bound is uniform per block, as malloc_block / free_block synchronise the block). The
+// next tile's input is loaded while the current tile is served.
 template <class M>
 __global__ void __launch_bounds__(1024) k_malloc_persist(M* mm, const uint32_t* __restrict__ sizes,
                                                          void** __restrict__ out, uint32_t n) {
   __shared__ typename M::Scratch sm;
   const uint32_t step = gridDim.x * blockDim.x;
-  for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += step) {
-    const uint32_t i = base + threadIdx.x;
-    void* p = mm->malloc_block(i < n ? sizes[i] : 0u, sm);
+  uint32_t base = blockIdx.x * blockDim.x;
+  uint32_t size = base + threadIdx.x < n ? sizes[base + threadIdx.x] : 0u;
+  for (; base < n; base += step) {
+    const uint32_t i = base + threadIdx.x, j = i + step;
+    const uint32_t next = j < n ? sizes[j] : 0u;
+    void* p = mm->malloc_block(i < n ? size : 0u, sm);
     if (i < n) out[i] = p;
+    size = next;
   }
 }
 
@@ -168,9 +173,13 @@ template <class M>
 __global__ void __launch_bounds__(1024) k_free_persist(M* mm, void* const* __restrict__ ptrs, uint32_t n) {
   __shared__ typename M::Scratch sm;
   const uint32_t step = gridDim.x * blockDim.x;
-  for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += step) {
-    const uint32_t i = base + threadIdx.x;
-    mm->free_block(i < n ? ptrs[i] : nullptr, sm);
+  uint32_t base = blockIdx.x * blockDim.x;
+  void* ptr = base + threadIdx.x < n ? ptrs[base + threadIdx.x] : nullptr;
+  for (; base < n; base += step) {
+    const uint32_t j = base + threadIdx.x + step;
+    void* next = j < n ? ptrs[j] : nullptr;
+    mm->free_block(ptr, sm);
+    ptr = next;
   }
 }
 
