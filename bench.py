@@ -99,7 +99,7 @@ def batch_config4(steps: int) -> dict:
     c = res.best
     with ob.MemoryManager(c.variant, pool) as mm:
         r = mm.run_batch(sizes, rounds=max(2, steps // 4), warmup=1, agg=c.agg, block=c.block, grid=c.grid,
-                         graph=True, check=True)
+                         graph=True, check=True, tile=c.tile)
     return {
         "workloads": len(req),
         "requests": int(req.sum()),
@@ -190,19 +190,20 @@ TRAFFIC_FILE = ROOT / "profiles" / "traffic.json"
 
 def probe(args, sizes: np.ndarray) -> int:
     """The profiled command: one device copy of COPY_BYTES (read + write, the counters' calibration),
-    then `steps` rounds of the batched form on --variant / --block (persistent kernels, grid = every
-    CU at its occupancy limit), nothing else."""
+    then `steps` rounds of the batched form on --variant / --block / --tile (persistent kernels, grid =
+    every CU at its occupancy limit), nothing else."""
     variant = args.variant or "PQ"
     gbs = ob.bench_copy(COPY_BYTES, 1)
     with ob.MemoryManager(variant, POOL) as mm:
         r = mm.run_batch(sizes, rounds=args.steps, warmup=args.warmup, agg=0, block=args.block,
-                         graph=bool(args.graph))
-    print(json.dumps({"probe": variant, "block": args.block, "copy_gbs": round(gbs, 1), "mops": round(r["mops"], 1),
+                         graph=bool(args.graph), tile=args.tile)
+    print(json.dumps({"probe": variant, "block": args.block, "tile": args.tile, "copy_gbs": round(gbs, 1),
+                      "mops": round(r["mops"], 1),
                       "alloc_ms": r["alloc_ms"] / args.steps, "free_ms": r["free_ms"] / args.steps}), flush=True)
     return 0
 
 
-def measured_traffic(kernel: str, variant: str, block: int) -> dict | None:
+def measured_traffic(kernel: str, variant: str, block: int, tile: int) -> dict | None:
     """HBM bytes per launch of `kernel` from the committed FETCH_SIZE / WRITE_SIZE passes (each scaled
     by what the same counter reads for the calibration copy), when they cover this configuration."""
     try:
@@ -210,7 +211,7 @@ def measured_traffic(kernel: str, variant: str, block: int) -> dict | None:
     except (OSError, ValueError):
         return None
     k = t.get("kernels", {}).get(kernel)
-    if not k or t.get("variant") != variant or t.get("block") != block:
+    if not k or t.get("variant") != variant or t.get("block") != block or t.get("tile", 1) != tile:
         return None
     return {"bytes": k["fetch_bytes"] + k["write_bytes"], "fetch_bytes": k["fetch_bytes"],
             "write_bytes": k["write_bytes"], "source": str(TRAFFIC_FILE.relative_to(ROOT))}
@@ -223,6 +224,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--variant", default=None, help="skip the tuner: batched form on this variant, block-wide")
     ap.add_argument("--block", type=int, default=1024, help="workgroup size with --variant")
+    ap.add_argument("--tile", type=int, default=1, help="requests per thread and call with --variant (1, 2, 4)")
     ap.add_argument("--probe", action="store_true",
                     help="profiling run: a 1 GiB device copy (counter calibration), then the headline rounds only")
     ap.add_argument("--graph", type=int, default=1, help="with --probe: 1 = graph-replayed rounds as the headline, "
@@ -247,7 +249,7 @@ def main() -> int:
     if args.probe:
         return probe(args, sizes)
     if args.variant:
-        best = tune.Candidate(args.variant, 0, args.block, 0)
+        best = tune.Candidate(args.variant, 0, args.block, 0, args.tile)
         tuned = None
     else:
         log("tuning the batched form on the 1M-thread trace ...")
@@ -257,11 +259,11 @@ def main() -> int:
     log(f"headline {best.label()} ...")
     with ob.MemoryManager(best.variant, POOL) as mm:
         head = mm.run_batch(sizes, rounds=steps, warmup=warmup, agg=best.agg, block=best.block, grid=best.grid,
-                            graph=True, check=True)
+                            graph=True, check=True, tile=best.tile)
         chk = mm.check_quiescent()
         # allocation-call latency of the same form: one more round, sampled on lane 0 of every wave
         lat_head = mm.run_batch(sizes, rounds=1, warmup=0, agg=best.agg, block=best.block, grid=best.grid,
-                                graph=False, latency=True)
+                                graph=False, latency=True, tile=best.tile)
     kind = ob.variant_geometry(best.variant)["kind_name"]
 
     # ---- the plain per-thread path (reference aggregation) on the same queue kind, and the best of all kinds
@@ -271,6 +273,7 @@ def main() -> int:
     wave = run_trace(kind, 64, sizes, steps, warmup)
 
     table = {}
+    table_tile = best.tile if best.agg == 0 else 1
     if not args.no_table:
         for v in KINDS:
             row = {}
@@ -283,8 +286,9 @@ def main() -> int:
                 row[f"agg{agg}"] = {"mops": round(r["mops"], 2), "errors": r["errors"], "verify_errors": r["verify_errors"]}
             log(f"table {v} batched ...")
             with ob.MemoryManager(v, POOL) as mm:
-                b = mm.run_batch(sizes, rounds=max(4, steps // 4), warmup=1, agg=0, block=1024, graph=True)
-            row["batched_block1024"] = {"mops": round(b["mops"], 2), "errors": b["errors"]}
+                b = mm.run_batch(sizes, rounds=max(4, steps // 4), warmup=1, agg=0, block=1024, graph=True,
+                                 tile=table_tile)
+            row["batched_block1024"] = {"mops": round(b["mops"], 2), "tile": table_tile, "errors": b["errors"]}
             table[v] = row
     best_plain = max([plain["mops"]] + [table[v]["agg1"]["mops"] for v in table])
 
@@ -307,7 +311,7 @@ def main() -> int:
     kern = "k_malloc_persist" if dominant == "alloc" else "k_free_persist"
     if best.agg != 0:
         kern = "k_malloc" if dominant == "alloc" else "k_free"
-    traffic = measured_traffic(kern, best.variant, best.block) if best.agg == 0 else None
+    traffic = measured_traffic(kern, best.variant, best.block, best.tile) if best.agg == 0 else None
 
     line = {
         "metric": "malloc+free Mops/s on 1 MI355X (1M threads, mixed sizes); p99 alloc latency",
@@ -330,6 +334,7 @@ def main() -> int:
             "variant": best.variant,
             "aggregation": "block" if best.agg == 0 else best.agg,
             "block": best.block,
+            "tile": best.tile,
             "grid": head["grid"],
             "threads": THREADS,
             "pool_bytes": POOL,

@@ -15,40 +15,61 @@ namespace ouro {
 constexpr uint32_t kBlockMaxWaves = 16;    // 1024-thread blocks
 constexpr uint32_t kBlockMaxClasses = 16;  // size classes over the whole instance chain
 
-// LDS scratch of one workgroup. Chunk queues write one page index (MemoryIndex) per served rank.
-template <bool kChunks>
+// LDS scratch of one workgroup serving kTile requests per thread. Chunk queues write one page index
+// (MemoryIndex) per served rank.
+template <bool kChunks, uint32_t kTile = 1>
 struct BlockScratch {
-  uint32_t off[kBlockMaxWaves][kBlockMaxClasses];  // requests of (wave, class), then their rank offset
+  static constexpr uint32_t kTileSize = kTile;
+  uint32_t off[kTile * kBlockMaxWaves][kBlockMaxClasses];  // requests of (sub-tile, wave, class), then their rank offset
   uint32_t total[kBlockMaxClasses];
   uint32_t base[kBlockMaxClasses];    // page queues: first queue position; chunk queues: first pidx entry
   uint32_t served[kBlockMaxClasses];  // chunk queues: ranks served (the rest get nullptr)
-  uint32_t pidx[kChunks ? 1024 : 1];  // chunk queues: page of each rank, class stretches in class order
+  uint32_t pidx[kChunks ? 1024 * kTile : 1];  // chunk queues: page of each rank, class stretches in class order
 };
 
 namespace detail {
 
-// Rank of this lane among the block's requests of class gc (kMarker: no request), after the class
-// totals and per-wave offsets are in sm. Ends with the block synchronised.
-template <class S>
-__device__ inline uint32_t block_rank(S& sm, uint32_t gc, uint32_t nclasses) {
+// Ranks of this thread's T requests among the block's requests of their classes gc[t] (kMarker: no
+// request), after the class totals and per-(sub-tile, wave) offsets are in sm. Request t of thread i
+// is request t * blockDim.x + i of the tile, and ranks follow that order: a block serving T requests
+// per thread ranks exactly as a block of T * blockDim.x threads would. Ends with the block
+// synchronised.
+template <uint32_t T, class S>
+__device__ inline void block_rank_tile(S& sm, const uint32_t (&gc)[T], uint32_t (&r)[T], uint32_t nclasses) {
+  static_assert(T <= S::kTileSize, "scratch too small for the tile");
   const uint32_t wave = threadIdx.x >> 6, nwaves = (blockDim.x + 63u) >> 6;
-  if (lane() < kBlockMaxClasses) sm.off[wave][lane()] = 0;
-  const bool valid = gc != kMarker;
-  const uint64_t grp = match<64>(valid ? (uint64_t)gc : ~0ull);
-  const uint32_t r = rank_in(grp);
-  if (valid && (int)lane() == first(grp)) sm.off[wave][gc] = popc(grp);
+  uint32_t in[T];
+#pragma unroll
+  for (uint32_t t = 0; t < T; ++t) {
+    uint32_t* row = sm.off[t * kBlockMaxWaves + wave];
+    if (lane() < kBlockMaxClasses) row[lane()] = 0;
+    const bool valid = gc[t] != kMarker;
+    const uint64_t grp = match<64>(valid ? (uint64_t)gc[t] : ~0ull);
+    in[t] = rank_in(grp);
+    if (valid && (int)lane() == first(grp)) row[gc[t]] = popc(grp);
+  }
   __syncthreads();
   if (wave == 0 && lane() < nclasses) {
     uint32_t run = 0;
-    for (uint32_t w = 0; w < nwaves; ++w) {
-      const uint32_t c = sm.off[w][lane()];
-      sm.off[w][lane()] = run;
-      run += c;
-    }
+    for (uint32_t t = 0; t < T; ++t)
+      for (uint32_t w = 0; w < nwaves; ++w) {
+        const uint32_t c = sm.off[t * kBlockMaxWaves + w][lane()];
+        sm.off[t * kBlockMaxWaves + w][lane()] = run;
+        run += c;
+      }
     sm.total[lane()] = run;
   }
   __syncthreads();
-  return valid ? sm.off[wave][gc] + r : 0u;
+#pragma unroll
+  for (uint32_t t = 0; t < T; ++t) r[t] = gc[t] != kMarker ? sm.off[t * kBlockMaxWaves + wave][gc[t]] + in[t] : 0u;
+}
+
+template <class S>
+__device__ inline uint32_t block_rank(S& sm, uint32_t gc, uint32_t nclasses) {
+  const uint32_t g[1] = {gc};
+  uint32_t r[1];
+  block_rank_tile<1>(sm, g, r, nclasses);
+  return r[0];
 }
 
 // ---- page queues

@@ -387,6 +387,8 @@ __device__ inline uint32_t va_wait_qc(Ctx<I>& x, ClassQueue* cq, uint32_t v) {
 // front_ptr_ has already left (those are empty, all 2048 positions dequeued). Any lane may help;
 // each call does bounded work and stops at the first position it cannot prove dead.
 constexpr uint32_t kCompactSteps = 256;
+// Slots a scan or a compaction step loads at once.
+constexpr uint32_t kPeek = 8;
 
 template <class I>
 __device__ inline void compact(Ctx<I>& x, ClassQueue* cq) {
@@ -397,7 +399,17 @@ __device__ inline void compact(Ctx<I>& x, ClassQueue* cq) {
   for (uint32_t i = 0; i < kCompactSteps && f != b && b - f < 0x80000000u; ++i) {
     uint32_t nf = f + 1u;
     if constexpr (is_std(K)) {
-      if (ld(&ring_of(x, cq)[f & (I::kQueueEntries - 1)]) != kTomb) return;
+      // up to kPeek positions per CAS
+      uint32_t e[kPeek];
+      const uint32_t n = b - f < kPeek ? b - f : kPeek;
+#pragma unroll
+      for (uint32_t j = 0; j < kPeek; ++j) e[j] = j < n ? ld(&ring_of(x, cq)[(f + j) & (I::kQueueEntries - 1)]) : kMarker;
+      uint32_t k = 0;
+#pragma unroll
+      for (uint32_t j = 0; j < kPeek; ++j)
+        if (k == j && e[j] == kTomb) ++k;
+      if (k == 0) return;
+      nf = f + k;
     } else if constexpr (is_va(K)) {
       const uint32_t e = ld(&ring_of(x, cq)[(f / kQCSpots) & (I::kQueueEntries - 1)]);
       if (e == qc_tag(f / kQCSpots)) {
@@ -425,34 +437,45 @@ __device__ inline void compact(Ctx<I>& x, ClassQueue* cq) {
       f = seen;
       continue;
     }
-    // the ring slot becomes writable again for position f + ring_size
-    if constexpr (is_std(K)) cas(&ring_of(x, cq)[f & (I::kQueueEntries - 1)], kTomb, kMarker);
+    // the ring slots become writable again for positions + ring_size (only the winner of the move
+    // writes them; the next writer of a slot waits for the marker with a CAS)
+    if constexpr (is_std(K)) {
+      for (uint32_t j = f; j != nf; ++j) st(&ring_of(x, cq)[j & (I::kQueueEntries - 1)], kMarker);
+      if (nf - f < kPeek && nf != b) return;  // stopped at a position it could not prove dead
+    }
     f = nf;
   }
 }
 
-// compact() for a whole, fully active wave: each round looks at up to 256 positions from front_
-// (four per lane) and moves front_ over the leading dead ones with one CAS (a reclaimed VA queue chunk
-// or a VL queue chunk front_ptr_ has left is a single move). It stops where compact() would: at the
-// first position it cannot prove dead or after kCompactSteps moves. A CAS lost to another compactor
-// that moved front_ forward is not the end: the round is retried from there (a few times at most),
-// since with thousands of concurrent removers the winner may stop short of what this wave saw dead.
+// compact() for the active lanes of a wave together (all of them must call): each round looks at up
+// to 16 positions per active lane from front_ and moves front_ over the leading dead ones with one CAS
+// (a reclaimed VA queue chunk or a VL queue chunk front_ptr_ has left is a single move). It stops at
+// the first position it cannot prove dead or after kWaveCompactSteps moves. Every remover of a class
+// compacts, so the front_ CAS is contended and its rate bounds how fast front_ advances: each
+// successful CAS should pass as many positions as it can. A wave whose CAS lost to a compactor that
+// already passed every position this wave saw dead leaves the rest to that winner; otherwise it
+// retries from the new front_ (a few times at most).
+constexpr uint32_t kWaveCompactSteps = 8192;
+
 template <class I>
 __device__ inline void compact_wave(Ctx<I>& x, ClassQueue* cq) {
   constexpr Kind K = I::kKind;
   static_assert(is_chunk_kind(K), "page queues dequeue strictly at front_");
-  constexpr uint32_t kPer = 4;  // positions per lane and round
-  const uint32_t l = lane();
+  constexpr uint32_t kPer = 16;  // positions per lane and round
+  const uint64_t act = active();
+  const int L = first(act);
+  const uint32_t nact = popc(act), me = rank_in(act);
+  const bool lead = (int)lane() == L;
   uint32_t moves = 0, lost = 0;
-  while (moves < kCompactSteps && lost < 8) {
+  while (moves < kWaveCompactSteps && lost < 8) {
     uint32_t f = 0, n = 0, jump = kMarker;
     const uint32_t* sl = nullptr;
-    if (l == 0) {
+    if (lead) {
       f = ld(&cq->front);
       n = ld(&cq->back) - f;
       if (n >= 0x80000000u) n = 0;
-      if (n > kWave * kPer) n = kWave * kPer;
-      if (n > kCompactSteps - moves) n = kCompactSteps - moves;
+      if (n > nact * kPer) n = nact * kPer;
+      if (n > kWaveCompactSteps - moves) n = kWaveCompactSteps - moves;
       if constexpr (is_va(K)) {
         const uint32_t e = ld(&ring_of(x, cq)[(f / kQCSpots) & (I::kQueueEntries - 1)]);
         if (e == qc_tag(f / kQCSpots)) jump = (f / kQCSpots + 1u) * kQCSpots;
@@ -474,57 +497,50 @@ __device__ inline void compact_wave(Ctx<I>& x, ClassQueue* cq) {
         if (n > kQCSpots - f % kQCSpots) n = kQCSpots - f % kQCSpots;
       }
     }
-    f = uniform(f);
-    n = uniform(n);
-    jump = uniform(jump);
-    sl = reinterpret_cast<const uint32_t*>(readlane64(reinterpret_cast<uint64_t>(sl), 0));
+    f = readlane(f, L);
+    n = readlane(n, L);
+    jump = readlane(jump, L);
+    sl = reinterpret_cast<const uint32_t*>(readlane64(reinterpret_cast<uint64_t>(sl), L));
     if (n == 0) break;
     uint32_t nf = jump;
     if (jump == kMarker) {
-      uint32_t k = 0;
+      // position r * nact + rank for the lane of that rank
+      uint32_t k = 0, e[kPer];
+#pragma unroll
+      for (uint32_t r = 0; r < kPer; ++r) {  // all loads first: one round trip per round
+        const uint32_t i = r * nact + me;
+        e[r] = kMarker;
+        if (i < n) e[r] = is_std(K) ? ld(&ring_of(x, cq)[(f + i) & (I::kQueueEntries - 1)]) : ld(&sl[i]);
+      }
 #pragma unroll
       for (uint32_t r = 0; r < kPer; ++r) {
-        const uint32_t i = r * kWave + l;
-        bool dead = false;
-        if (i < n) {
-          const uint32_t e = is_std(K) ? ld(&ring_of(x, cq)[(f + i) & (I::kQueueEntries - 1)]) : ld(&sl[i]);
-          dead = e == kTomb;
-        }
-        const uint64_t alive = ~ballot(dead);
-        const uint32_t run = alive ? (uint32_t)__builtin_ctzll(alive) : kWave;
-        if (k == r * kWave) k += run;  // still inside the leading dead stretch
+        // leading dead ranks of this round: active lanes below the first live one
+        const uint64_t alive = act & ~ballot(e[r] == kTomb);
+        const uint32_t run = alive ? popc(act & ((1ull << __builtin_ctzll(alive)) - 1ull)) : nact;
+        if (k == r * nact) k += run;  // still inside the leading dead stretch
       }
       if (k > n) k = n;
-#ifdef OURO_DIAG
-      if (k < n && l == 0) {
-        const uint32_t e = is_std(K) ? ld(&ring_of(x, cq)[(f + k) & (I::kQueueEntries - 1)]) : ld(&sl[k]);
-        OURO_DIAG_ADD(e == kMarker ? 12 : 13, 1);
-        if (class_index(x, cq) == 9 && chunk_ok(x.m, e)) {
-          const ChunkHdr* h = hdr<ChunkHdr>(x.m, e);
-          OURO_DIAG_LOG((10ull << 60) | ((uint64_t)(k & 0xFF) << 48) | ((uint64_t)(ld(&h->count) & 0xFFFF) << 32) |
-                        (uint64_t)(ld(&h->queue_pos) - (f + k)));
-        }
-      }
-#endif
       if (k == 0) break;
       nf = f + k;
     }
     uint32_t seen = 0;
-    if (l == 0) seen = cas(&cq->front, f, nf);
-    seen = uniform(seen);
+    if (lead) seen = cas(&cq->front, f, nf);
+    seen = readlane(seen, L);
     if (seen != f) {
-      // another compactor moved front_: retry from where it left it, unless it went backwards
+      // another compactor moved front_. If it passed every position this wave saw dead (and this
+      // wave stopped at one it could not), the winner carries on from there; else retry from seen.
       if (seen - f >= 0x80000000u) break;
+      if (jump == kMarker && nf - f < n && seen - f >= nf - f) break;
       ++lost;
-      OURO_DIAG_ADD(11, 1);
       continue;
     }
-    // the ring slots become writable again for positions + ring_size
+    // the ring slots become writable again for positions + ring_size; only the winner of the move
+    // writes them, and the next writer of a slot waits for the marker with a CAS
     if constexpr (is_std(K)) {
 #pragma unroll
       for (uint32_t r = 0; r < kPer; ++r) {
-        const uint32_t i = r * kWave + l;
-        if (i < nf - f) cas(&ring_of(x, cq)[(f + i) & (I::kQueueEntries - 1)], kTomb, kMarker);
+        const uint32_t i = r * nact + me;
+        if (i < nf - f) st(&ring_of(x, cq)[(f + i) & (I::kQueueEntries - 1)], kMarker);
       }
     }
     if (jump != kMarker) {
@@ -950,7 +966,6 @@ __device__ inline uint32_t carve_chunks(Ctx<I>& x, ClassQueue* cq, uint32_t k) {
 // ---- chunk-queue scanning
 // Peeks load kPeek consecutive slots (and then their chunk headers) at once: one round trip per
 // batch instead of one per slot.
-constexpr uint32_t kPeek = 8;
 
 // Slots of the queue chunk holding virtual position v (VA / VL), from v on: nullptr when there is
 // none to read, with `fill` what its slots read as (a reclaimed or bypassed queue chunk reads as
@@ -1098,9 +1113,11 @@ __device__ inline void ensure_in_queue(Ctx<I>& x, ClassQueue* cq, bool want, uin
 // `pos` is the queue position the caller found the chunk at: the CAS queue_pos pos -> MARKER admits
 // one remover and only for that entry. A caller that took the chunk's last page from an entry that
 // has since been removed must not remove the chunk's next entry: a free may already have re-enqueued
-// it at a position whose slot is still being written. Then the slot is tombstoned and front_
-// compacted. Returns true if removed; `again` reports a free that raced in (the caller re-enqueues:
-// the free saw queue_pos != MARKER and left it to us).
+// it at a position whose slot is still being written. Then the slot is tombstoned; the caller
+// compacts front_ once for the wave afterwards (compact_wave where the lanes reconverge: one
+// wave-wide pass instead of a contended serial one per removing lane). Returns true if removed;
+// `again` reports a free that raced in (the caller re-enqueues: the free saw queue_pos != MARKER and
+// left it to us).
 template <class I>
 __device__ inline bool cq_remove(Ctx<I>& x, ClassQueue* cq, uint32_t chunk, uint32_t pos, uint32_t& cursor,
                                  uint32_t& reclaim, bool& again) {
@@ -1119,7 +1136,6 @@ __device__ inline bool cq_remove(Ctx<I>& x, ClassQueue* cq, uint32_t chunk, uint
   sub(&cq->count, 1);
   order();
   again = ld(&h->count) > 0;
-  compact(x, cq);
   return true;
 }
 
@@ -1199,7 +1215,7 @@ template <class I>
 struct ChunkClaim {
   uint32_t num = 0, v = 0, cursor = kMarker, need = 0, assigned = 0, t_rem = 0, w = 0, c = kMarker, wraps = 0;
   uint32_t reclaim = kMarker;
-  bool gave_up = false;
+  bool gave_up = false, removed = false;
   Spin it;
 
   __device__ ChunkClaim(Ctx<I>& x, ClassQueue* cq, bool leader, uint32_t n) : it(&x.m.error) {
@@ -1251,8 +1267,10 @@ struct ChunkClaim {
           const uint32_t size = ld(&h->size);
           w = ((size - (uint32_t)cnt) / 64u) % mask_words(size);
           bool again = false;
-          if (cnt == (int32_t)t && cq_remove(x, cq, s, v + i, cursor, reclaim, again) && again)
-            ensure_in_queue<1>(x, cq, true, s);
+          if (cnt == (int32_t)t && cq_remove(x, cq, s, v + i, cursor, reclaim, again)) {
+            removed = true;
+            if (again) ensure_in_queue<1>(x, cq, true, s);
+          }
           v += cnt == (int32_t)t ? i + 1u : i;
         } else {
           OURO_DIAG_ADD(4, 1);
@@ -1303,7 +1321,7 @@ __device__ inline void* alloc_chunk_pages(Ctx<I>& x, uint32_t qi, bool want) {
   const bool ok = want;
   uint32_t chunk = kMarker, page = 0;
   uint32_t reclaim = kMarker;
-  bool got = false;
+  bool got = false, removed = false;
   if constexpr (W == 1) {
     // the reference's per-thread protocol: every lane walks the queue on its own and probes each
     // chunk's count with a decrement. Lanes start at the class's page cursor (one add per semaphore
@@ -1345,8 +1363,10 @@ __device__ inline void* alloc_chunk_pages(Ctx<I>& x, uint32_t qi, bool want) {
             got = true;
             bool again = false;
             if (md == AccessMode::RE_ENQUEUE) ensure_in_queue<1>(x, cq, true, c);
-            if (md == AccessMode::DEQUEUE && cq_remove(x, cq, c, v, cursor, reclaim, again) && again)
-              ensure_in_queue<1>(x, cq, true, c);
+            if (md == AccessMode::DEQUEUE && cq_remove(x, cq, c, v, cursor, reclaim, again)) {
+              removed = true;
+              if (again) ensure_in_queue<1>(x, cq, true, c);
+            }
             break;
           }
         }
@@ -1383,6 +1403,13 @@ __device__ inline void* alloc_chunk_pages(Ctx<I>& x, uint32_t qi, bool want) {
       }
     }
     reclaim = cl.reclaim;
+    removed = cl.removed;
+  }
+  // the wave compacts each class queue it removed chunks from, one queue at a time
+  for (uint64_t pend = ballot(removed); pend;) {
+    ClassQueue* q = reinterpret_cast<ClassQueue*>(readlane64(reinterpret_cast<uint64_t>(cq), first(pend)));
+    compact_wave(x, q);
+    pend &= ~ballot(cq == q);
   }
   finish_reclaim(x, cq, true, reclaim);
   if (!got) return nullptr;

@@ -270,38 +270,91 @@ struct Ouroboros {
   }();
   static constexpr bool kAnyChunks = (Is::kChunks || ...);
   static_assert(kNumClasses <= kBlockMaxClasses, "too many size classes for the block scratch");
-  using Scratch = BlockScratch<kAnyChunks>;
+  template <uint32_t T>
+  using TileScratch = BlockScratch<kAnyChunks, T>;
+  using Scratch = TileScratch<1>;
 
   __device__ void* malloc_block(size_t size, Scratch& sm) {
-    uint32_t gc = kMarker;
-    if (size) {
-      gc = detail::chain_alloc_class<Is...>((uint64_t)size);
-      if (gc == kMarker) detail::set_error(memory, ERR_TOO_LARGE);
-    }
-    const uint32_t r = detail::block_rank(sm, gc, kNumClasses);
-    detail::chain_block_claim(memory, chain, chain.inst.reuse, sm, kNumClasses);
-    __syncthreads();
-    return detail::chain_block_take(memory, chain, chain.inst.reuse, sm, gc, r);
+    const size_t s[1] = {size};
+    void* p[1];
+    malloc_tile<1>(s, p, sm);
+    return p[0];
   }
 
   __device__ void free_block(void* ptr, Scratch& sm) {
-    uint32_t chunk = 0, page_size = 0, page = 0, gc = kMarker;
-    uint64_t offset = 0;
-    if (detail::route_free(memory, ptr, chunk, page_size, offset)) {
-      gc = detail::chain_free_class<Is...>(page_size, offset, page);
-      if (gc == kMarker) detail::set_error(memory, ERR_BAD_FREE);
+    void* const p[1] = {ptr};
+    free_tile<1>(p, sm);
+  }
+
+  // malloc_block / free_block with T requests per thread: request t of thread i is request
+  // t * blockDim.x + i of a tile of T * blockDim.x, served as one block-wide group per class (one
+  // semaphore claim and one front_ / back_ fetch-add per class for the whole tile), so a workgroup
+  // serves T times the requests per barrier round. `sm` is a __shared__ TileScratch<T>.
+  template <uint32_t T>
+  __device__ void malloc_tile(const size_t (&size)[T], void* (&out)[T], TileScratch<T>& sm) {
+    uint32_t gc[T], r[T];
+#pragma unroll
+    for (uint32_t t = 0; t < T; ++t) {
+      gc[t] = kMarker;
+      if (size[t]) {
+        gc[t] = detail::chain_alloc_class<Is...>((uint64_t)size[t]);
+        if (gc[t] == kMarker) detail::set_error(memory, ERR_TOO_LARGE);
+      }
+    }
+    detail::block_rank_tile<T>(sm, gc, r, kNumClasses);
+    detail::chain_block_claim(memory, chain, chain.inst.reuse, sm, kNumClasses);
+    __syncthreads();
+#pragma unroll
+    for (uint32_t t = 0; t < T; ++t) out[t] = detail::chain_block_take(memory, chain, chain.inst.reuse, sm, gc[t], r[t]);
+  }
+
+  template <uint32_t T>
+  __device__ void free_tile(void* const (&ptr)[T], TileScratch<T>& sm) {
+    if constexpr (T > 1 && kVirtualized && !kAnyChunks) {
+      // VA / VL page queues: a put at a queue-chunk boundary pre-allocates the next queue chunk, and
+      // puts further on wait for it. put_lanes does a wave's pre-allocations before any of its waits,
+      // but sub-tiles released one after the other would wait (sub-tile 0) before pre-allocating
+      // (sub-tile 3), and two workgroups ordered differently in two class queues could then wait on
+      // each other. So the sub-tiles are freed as consecutive block-wide frees, each reserving its own
+      // back_ positions right before its puts.
+#pragma unroll
+      for (uint32_t t = 0; t < T; ++t) {
+        void* const p[1] = {ptr[t]};
+        free_tile_in<1>(p, sm);
+      }
+    } else {
+      free_tile_in<T>(ptr, sm);
+    }
+  }
+
+  template <uint32_t T, class S>
+  __device__ void free_tile_in(void* const (&ptr)[T], S& sm) {
+    uint32_t chunk[T], page[T], gc[T], r[T];
+#pragma unroll
+    for (uint32_t t = 0; t < T; ++t) {
+      uint32_t page_size = 0;
+      uint64_t offset = 0;
+      chunk[t] = 0;
+      page[t] = 0;
+      gc[t] = kMarker;
+      if (detail::route_free(memory, ptr[t], chunk[t], page_size, offset)) {
+        gc[t] = detail::chain_free_class<Is...>(page_size, offset, page[t]);
+        if (gc[t] == kMarker) detail::set_error(memory, ERR_BAD_FREE);
+      }
     }
     // Each wave's stores to the blocks it frees complete at its XCD's L2 before the block barrier
-    // inside block_rank; after it, one agent-scope release by wave 0 writes those lines back for the
-    // whole workgroup (buffer_wbl2 acts on the entire L2 of the XCD), instead of one per wave.
+    // inside block_rank_tile; after it, one agent-scope release by wave 0 writes those lines back for
+    // the whole workgroup (buffer_wbl2 acts on the entire L2 of the XCD), instead of one per wave.
     dev::order();
-    const uint32_t r = detail::block_rank(sm, gc, kNumClasses);
+    detail::block_rank_tile<T>(sm, gc, r, kNumClasses);
     if (threadIdx.x < kWave) {
       if (dev::ballot(dev::lane() < kNumClasses && sm.total[dev::lane()] > 0)) dev::release_block_contents();
       detail::chain_block_back(chain, sm);
     }
     __syncthreads();
-    detail::chain_block_release(memory, chain, chain.inst.reuse, sm, gc, r, chunk, page);
+#pragma unroll
+    for (uint32_t t = 0; t < T; ++t)
+      detail::chain_block_release(memory, chain, chain.inst.reuse, sm, gc[t], r[t], chunk[t], page[t]);
     __syncthreads();  // every put / release of the block is done: publish, then signal
     if (threadIdx.x < kWave) detail::chain_block_signal(chain, sm);
   }

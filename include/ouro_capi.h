@@ -201,7 +201,8 @@ typedef struct {
                               from the data region, UINT64_MAX for nullptr */
   int32_t latency;         /* 1 = one more (untimed) round recording the wall-clock time of each
                               allocation call on lane 0 of every wave */
-  uint32_t pad;
+  uint32_t tile;           /* agg 0: requests per thread and block-wide call, 1 (0), 2 or 4; a tile
+                              of tile * block requests is served as one block-wide group per class */
 } ouro_batch_config;
 
 typedef struct {

@@ -129,9 +129,11 @@ class Sim:
     def __del__(self):
         self.close()
 
-    def wave_malloc(self, sizes) -> np.ndarray:
+    def wave_malloc(self, sizes, block_tile: bool = False) -> np.ndarray:
+        """One wave's malloc (<= 64 lanes), or with block_tile (block order only) one block-wide call
+        of a 64-thread workgroup over up to 4 requests per thread."""
         sizes = np.ascontiguousarray(sizes, dtype=np.uint32)
-        assert len(sizes) <= 64
+        assert len(sizes) <= (256 if block_tile else 64)
         out = np.empty(len(sizes), dtype=np.uint64)
         self._lib.sim_wave_malloc(self._h, len(sizes), sizes.ctypes.data_as(C.POINTER(C.c_uint32)),
                                   out.ctypes.data_as(C.POINTER(C.c_uint64)))
@@ -163,16 +165,19 @@ class Sim:
         self._lib.sim_queue_window(self._h, k, out.ctypes.data_as(C.POINTER(C.c_uint32)), n)
         return out[:n]
 
-    def block_trace(self, sizes, rounds: int) -> np.ndarray:
+    def block_trace(self, sizes, rounds: int, tile: int = 1) -> np.ndarray:
         """The batched form run by one 64-thread persistent workgroup: each round allocates the flat
-        request list tile by tile (64 requests, block-wide calls), then frees it tile by tile.
-        Returns offsets [rounds, requests]."""
+        request list tile by tile (64 * tile requests, block-wide calls), then frees it the same way
+        (a block-wide free of a tile releases its sub-tiles of 64 in order, which is a wave free
+        each). Returns offsets [rounds, requests]."""
         sizes = np.asarray(sizes, dtype=np.uint32)
+        w = 64 * tile
         self._lib.sim_set_block_order(self._h, 1)
         try:
             out = []
             for _ in range(rounds):
-                offs = np.concatenate([self.wave_malloc(sizes[t:t + 64]) for t in range(0, len(sizes), 64)])
+                offs = np.concatenate([self.wave_malloc(sizes[t:t + w], block_tile=True)
+                                       for t in range(0, len(sizes), w)])
                 for t in range(0, len(sizes), 64):
                     self.wave_free(offs[t:t + 64])
                 out.append(offs)

@@ -28,7 +28,10 @@
 #define NULLPAGE 0xFFFFFFFCu  /* page-queue entry for a page the pool could not supply */
 #define TOMB 0xFFFFFFFDu      /* chunk-queue slot whose chunk was removed */
 #define PEEK 8u               /* slots looked at per scan step */
-#define COMPACT_STEPS 256u    /* front moves per compaction call */
+#define MAXL 256              /* requests of one call: a wave's 64, or a block-wide tile of up to 4 per
+                                 thread of a 64-thread workgroup */
+#define COMPACT_STEPS 256u    /* CAS steps per call of the per-lane compaction */
+#define WAVE_COMPACT_STEPS 8192u /* front moves per call of the wave-wide compaction */
 
 enum { ERR_OOM = 1, ERR_QUEUE_FULL = 2, ERR_VSTART = 4, ERR_TIMEOUT = 8, ERR_BAD_FREE = 16, ERR_TOO_LARGE = 32,
        ERR_REUSE_FULL = 64, ERR_CORRUPT = 256 };
@@ -298,7 +301,7 @@ static uint32_t vl_locate(ouro_sim* s, uint32_t c, uint32_t v) {
   }
 }
 
-static void compact(ouro_sim* s, inst_t* I, cls_t* q);
+static void compact(ouro_sim* s, inst_t* I, cls_t* q, int wave);
 
 /* ------------------------------------------------------------------ slot sequences */
 /* Queue-chunk counters (QueueChunk count_) change once per queue chunk of a call, at the lowest lane
@@ -321,7 +324,7 @@ static void put_lanes(ouro_sim* s, inst_t* I, cls_t** qs, int n, const uint32_t*
   if (is_std(K)) {
     for (int i = 0; i < n; ++i) {
       uint32_t* slot = std_slot(qs[i], v[i]);
-      if (*slot == TOMB && is_chunk_kind(K)) compact(s, I, qs[i]);
+      if (*slot == TOMB && is_chunk_kind(K)) compact(s, I, qs[i], 0);
       if (*slot != MARKER) s->error |= ERR_TIMEOUT;
       else *slot = val[i];
     }
@@ -522,16 +525,32 @@ static uint32_t find_chunk(ouro_sim* s, inst_t* I, cls_t* q, uint32_t v, uint32_
   return n;
 }
 
-/* Advance front over dead positions, at most COMPACT_STEPS moves: tombstones, a VA queue chunk whose
- * pointer-ring entry holds its reclaimed tag, VL queue chunks front_ptr_ has already left. */
-static void compact(ouro_sim* s, inst_t* I, cls_t* q) {
+/* Advance front over dead positions: tombstones, a VA queue chunk whose pointer-ring entry holds its
+ * reclaimed tag, VL queue chunks front_ptr_ has already left. The per-lane form (wave = 0) takes at
+ * most COMPACT_STEPS steps, a step on a standard ring passing up to PEEK leading tombstones at once;
+ * the wave-wide form makes at most WAVE_COMPACT_STEPS moves, a move being one position or one jump
+ * over a whole queue chunk. */
+static void compact(ouro_sim* s, inst_t* I, cls_t* q, int wave) {
   uint32_t f = q->front;
   const uint32_t b = q->back;
-  for (uint32_t i = 0; i < COMPACT_STEPS && f != b && b - f < 0x80000000u; ++i) {
+  if (is_std(I->kind)) {
+    const uint32_t steps = wave ? WAVE_COMPACT_STEPS : COMPACT_STEPS, per = wave ? 1u : PEEK;
+    for (uint32_t i = 0; i < steps && f != b && b - f < 0x80000000u; ++i) {
+      const uint32_t n = b - f < per ? b - f : per;
+      uint32_t k = 0;
+      while (k < n && *std_slot(q, f + k) == TOMB) ++k;
+      if (k == 0) return;
+      for (uint32_t j = 0; j < k; ++j) *std_slot(q, f + j) = MARKER;
+      f += k;
+      q->front = f;
+      if (k < n) return;
+    }
+    return;
+  }
+  const uint32_t steps = wave ? WAVE_COMPACT_STEPS : COMPACT_STEPS;
+  for (uint32_t i = 0; i < steps && f != b && b - f < 0x80000000u; ++i) {
     uint32_t nf = f + 1u;
-    if (is_std(I->kind)) {
-      if (*std_slot(q, f) != TOMB) return;
-    } else if (is_va(I->kind)) {
+    if (is_va(I->kind)) {
       const uint32_t e = q->va[(f / QC_SPOTS) & (q->ring_size - 1)];
       if (e == qtag(f / QC_SPOTS)) {
         nf = (f / QC_SPOTS + 1u) * QC_SPOTS;
@@ -551,7 +570,6 @@ static void compact(ouro_sim* s, inst_t* I, cls_t* q) {
       }
     }
     q->front = nf;
-    if (is_std(I->kind) && *std_slot(q, f) == TOMB) *std_slot(q, f) = MARKER;
     f = nf;
   }
 }
@@ -596,7 +614,7 @@ static void tomb_slot(ouro_sim* s, inst_t* I, cls_t* q, uint32_t p, uint32_t* cu
 static void finish_reclaim(ouro_sim* s, inst_t* I, cls_t* q, uint32_t reclaim);
 
 /* Remove an exhausted chunk wherever it sits, at the position its queue_pos names (the device's
- * cq_remove), then compact the front. Returns 1 when removed. */
+ * cq_remove). The caller compacts the front once for the wave. Returns 1 when removed. */
 static int cq_remove(ouro_sim* s, inst_t* I, cls_t* q, uint32_t chunk, uint32_t* cursor, uint32_t* reclaim) {
   chunk_t* x = ch(s, chunk);
   const uint32_t p = x->queue_pos;
@@ -609,8 +627,7 @@ static int cq_remove(ouro_sim* s, inst_t* I, cls_t* q, uint32_t chunk, uint32_t*
     *reclaim = rc;
   }
   q->count--;
-  compact(s, I, q);
-  return 1;
+  return 1; /* the wave compacts front_ once afterwards (inst_malloc) */
 }
 
 /* ------------------------------------------------------------------ carves */
@@ -693,9 +710,10 @@ static uint32_t carve(ouro_sim* s, inst_t* I, cls_t* q, uint32_t k) {
 
 /* ------------------------------------------------------------------ wave ops */
 typedef struct {
-  int lanes[64];
+  int lanes[MAXL];
   int n;
   uint32_t qi;
+  int removed; /* the group's claim removed a chunk from its queue */
 } group_t;
 
 /* Groups of lanes with equal class, ordered by their lowest lane (match<64> + peeling). */
@@ -709,6 +727,7 @@ static int make_groups(const int* lanes, int n, const uint32_t* cls, group_t* g)
       found = ng++;
       g[found].n = 0;
       g[found].qi = cls[i];
+      g[found].removed = 0;
     }
     g[found].lanes[g[found].n++] = lanes[i];
   }
@@ -720,7 +739,6 @@ static int make_groups(const int* lanes, int n, const uint32_t* cls, group_t* g)
 /* Claims in lane order of the group leaders (one fetch-add each on the device, on different words),
  * then the promised carves, one group at a time in lane order. */
 static void acquire(ouro_sim* s, inst_t* I, group_t* g, int ng, int* ok, int nact) {
-  (void)nact;
   uint32_t carve_k[64];
   for (int j = 0; j < ng; ++j) {
     cls_t* q = &I->q[g[j].qi];
@@ -730,8 +748,10 @@ static void acquire(ouro_sim* s, inst_t* I, group_t* g, int ng, int* ok, int nac
     q->exp += (int64_t)carve_k[j] * q->ppc;
     ok[j] = 1;
   }
+  /* a carve hands one chunk to each active lane: larger promises go in batches of nact */
   for (int j = 0; j < ng; ++j)
-    if (carve_k[j]) carve(s, I, &I->q[g[j].qi], carve_k[j]);
+    for (uint32_t d = 0; d < carve_k[j]; d += (uint32_t)nact)
+      carve(s, I, &I->q[g[j].qi], carve_k[j] - d < (uint32_t)nact ? carve_k[j] - d : (uint32_t)nact);
 }
 
 static uint64_t page_offset(const ouro_sim* s, uint32_t c, uint32_t page, uint32_t page_size) {
@@ -795,7 +815,7 @@ static void claim_group(ouro_sim* s, inst_t* I, group_t* g, uint64_t* out, uint3
     need -= t;
     const uint32_t nw = mask_words(x->size);
     uint32_t w = ((x->size - (uint32_t)cnt) / 64u) % nw;
-    if (cnt == (int32_t)t) cq_remove(s, I, q, sl, &cursor, reclaim);
+    if (cnt == (int32_t)t && cq_remove(s, I, q, sl, &cursor, reclaim)) g->removed = 1;
     v += cnt == (int32_t)t ? i + 1u : i;
     uint32_t t_rem = t;
     int wguard = 0;
@@ -918,7 +938,7 @@ static void claim_wave(ouro_sim* s, inst_t* I, group_t* g, uint64_t* out) {
       q->count--;
       removed = 1;
     }
-    if (removed) compact(s, I, q);
+    if (removed) compact(s, I, q, 1);
     /* mask claims */
     for (uint32_t i = 0; i < 64; ++i) {
       if (!t[i]) continue;
@@ -960,30 +980,12 @@ static void inst_malloc(ouro_sim* s, inst_t* I, const int* lanes, int n, const u
   for (int i = 0; i < n; ++i) cls[i] = (uint32_t)sim_class_of(sizes[lanes[i]], I->smallest);
   group_t g[64];
   const int ng = make_groups(lanes, n, cls, g);
-  if (s->block_order) {
-    /* block-wide calls (Ouroboros::malloc_block): class leaders are the lanes of wave 0 indexed by
-       class, so claims and carves run in class order rather than first-lane order */
-    for (int a = 1; a < ng; ++a)
-      for (int b = a; b > 0 && g[b - 1].qi > g[b].qi; --b) {
-        const group_t t = g[b];
-        g[b] = g[b - 1];
-        g[b - 1] = t;
-      }
-  }
   int ok[64];
-  if (s->block_order && is_chunk_kind(I->kind)) {
-    /* block-wide chunk claims: one wave per class, classes in order, each claim whole before the next */
-    for (int j = 0; j < ng; ++j) {
-      acquire(s, I, &g[j], 1, &ok[j], nact);
-      claim_wave(s, I, &g[j], out);
-    }
-    return;
-  }
   acquire(s, I, g, ng, ok, nact);
   if (!is_chunk_kind(I->kind)) {
     /* dequeue positions: front fetch-add per group, rank order (atomicAggInc) */
-    cls_t* qs[64];
-    uint32_t v[64], got[64];
+    cls_t* qs[64] = {0};
+    uint32_t v[64] = {0}, got[64];
     int owner[64], m = 0;
     /* lanes in lane order, each with its group's front position */
     uint32_t base[64];
@@ -1021,11 +1023,104 @@ static void inst_malloc(ouro_sim* s, inst_t* I, const int* lanes, int n, const u
     rec[j] = MARKER;
     if (ok[j]) claim_group(s, I, &g[j], out, &rec[j]);
   }
+  /* one wave-wide compaction per class queue the claims removed chunks from */
+  for (int j = 0; j < ng; ++j) {
+    int first = g[j].removed;
+    for (int i = 0; i < j; ++i)
+      if (g[i].qi == g[j].qi && g[i].removed) first = 0;
+    if (first) compact(s, I, &I->q[g[j].qi], 1);
+  }
   /* deferred queue-chunk reclamation, by leader lane */
   for (int j = 0; j < ng; ++j) finish_reclaim(s, I, &I->q[g[j].qi], rec[j]);
 }
 
+/* Block-wide allocation of n <= MAXL requests by a 64-thread workgroup (Ouroboros::malloc_tile, the
+ * request of lane l in sub-tile t at index 64 t + l). Phase 1, instance by instance: class leaders are
+ * the lanes of wave 0 indexed by class, so claims and carves run in class order; a chunk-queue class
+ * is claimed whole (pages assigned in rank order) before the next; a page-queue class takes one
+ * front_ fetch-add for all its requests, ranked by request index. Phase 2 (after the block barrier),
+ * sub-tile by sub-tile and instance by instance: the page-queue slots are dequeued. */
+static void block_malloc(ouro_sim* s, int n, const uint32_t* sizes, uint64_t* out) {
+  enum { NI = 4 };
+  group_t g[NI][16];
+  int ng[NI] = {0}, ok[NI][16];
+  uint32_t base[NI][16];
+  for (int i = 0; i < n; ++i) out[i] = UINT64_MAX;
+  int lanes[MAXL];
+  for (int k = 0; k < s->ninst && k < NI; ++k) {
+    inst_t* I = &s->inst[k];
+    int nh = 0;
+    uint32_t cls[MAXL];
+    for (int i = 0; i < n; ++i) {
+      if (!sizes[i] || sizes[i] > I->largest) continue;
+      int prior = 0;
+      for (int j = 0; j < k; ++j) prior |= sizes[i] <= s->inst[j].largest;
+      if (prior) continue;
+      lanes[nh] = i;
+      cls[nh++] = (uint32_t)sim_class_of(sizes[i], I->smallest);
+    }
+    if (!nh) continue;
+    ng[k] = make_groups(lanes, nh, cls, g[k]);
+    for (int a = 1; a < ng[k]; ++a)
+      for (int b = a; b > 0 && g[k][b - 1].qi > g[k][b].qi; --b) {
+        const group_t t = g[k][b];
+        g[k][b] = g[k][b - 1];
+        g[k][b - 1] = t;
+      }
+    if (is_chunk_kind(I->kind)) {
+      for (int j = 0; j < ng[k]; ++j) {
+        acquire(s, I, &g[k][j], 1, &ok[k][j], 64);
+        claim_wave(s, I, &g[k][j], out);
+      }
+    } else {
+      acquire(s, I, g[k], ng[k], ok[k], 64);
+      for (int j = 0; j < ng[k]; ++j)
+        if (ok[k][j]) {
+          base[k][j] = I->q[g[k][j].qi].front;
+          I->q[g[k][j].qi].front += (uint32_t)g[k][j].n;
+        }
+    }
+  }
+  for (int i = 0; i < n; ++i) {
+    int fits = 0;
+    for (int k = 0; k < s->ninst; ++k) fits |= sizes[i] <= s->inst[k].largest;
+    if (sizes[i] && !fits) s->error |= ERR_TOO_LARGE;
+  }
+  for (int t = 0; t * 64 < n; ++t)
+    for (int k = 0; k < s->ninst && k < NI; ++k) {
+      inst_t* I = &s->inst[k];
+      if (is_chunk_kind(I->kind) || !ng[k]) continue;
+      cls_t* qs[64];
+      uint32_t v[64], got[64];
+      int owner[64], m = 0;
+      for (int l = t * 64; l < n && l < t * 64 + 64; ++l)
+        for (int j = 0; j < ng[k]; ++j) {
+          if (!ok[k][j]) continue;
+          for (int r = 0; r < g[k][j].n; ++r)
+            if (g[k][j].lanes[r] == l) {
+              qs[m] = &I->q[g[k][j].qi];
+              v[m] = base[k][j] + (uint32_t)r;
+              owner[m++] = l;
+            }
+        }
+      if (!m) continue;
+      take_lanes(s, I, qs, m, v, got);
+      for (int i = 0; i < m; ++i) {
+        const uint32_t c = got[i] >> PAGE_BITS, p = got[i] & 511u;
+        if (got[i] == MARKER || got[i] == NULLPAGE || !chunk_ok(s, c) || p >= qs[i]->ppc) {
+          if (got[i] != MARKER && got[i] != NULLPAGE) s->error |= ERR_CORRUPT;
+          continue;
+        }
+        out[owner[i]] = page_offset(s, c, p, qs[i]->page_size);
+      }
+    }
+}
+
 void sim_wave_malloc(ouro_sim* s, int n, const uint32_t* sizes, uint64_t* out) {
+  if (s->block_order) {
+    block_malloc(s, n, sizes, out);
+    return;
+  }
   int rest[64], nr = 0;
   for (int i = 0; i < n; ++i) {
     out[i] = UINT64_MAX;
@@ -1051,9 +1146,9 @@ static void inst_free(ouro_sim* s, inst_t* I, const int* lanes, int n, const uin
                       const uint32_t* page) {
   if (!is_chunk_kind(I->kind)) {
     /* back positions per class group in lane order, publish, then signal (PageQueue_impl.cuh:137-143) */
-    cls_t* qs[64];
-    uint32_t v[64], val[64];
-    uint32_t cls[64];
+    cls_t* qs[64] = {0};
+    uint32_t v[64] = {0}, val[64] = {0};
+    uint32_t cls[64] = {0};
     for (int i = 0; i < n; ++i) cls[i] = qi[i];
     group_t g[64];
     const int ng = make_groups(lanes, n, cls, g);

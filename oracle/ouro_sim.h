@@ -32,7 +32,8 @@ void sim_wave_malloc(ouro_sim* s, int n, const uint32_t* sizes, uint64_t* out);
 /* offsets[i] == UINT64_MAX marks a lane that frees nullptr (or is inactive). */
 void sim_wave_free(ouro_sim* s, int n, const uint64_t* offsets);
 /* on = 1: later wave_malloc calls follow the block-wide call of a one-wavefront workgroup
- * (Ouroboros::malloc_block), whose class leaders claim and carve in class order. */
+ * (Ouroboros::malloc_tile): n <= 256 requests, request 64 t + l on lane l of sub-tile t; class
+ * leaders claim and carve in class order, then the sub-tiles take their pages in order. */
 void sim_set_block_order(ouro_sim* s, int on);
 
 typedef struct {

@@ -137,7 +137,7 @@ class BatchConfig(C.Structure):
         ("sizes", C.POINTER(C.c_uint32)),
         ("offsets_out", C.POINTER(C.c_uint64)),
         ("latency", C.c_int32),
-        ("pad", C.c_uint32),
+        ("tile", C.c_uint32),
     ]
 
 
@@ -456,11 +456,13 @@ class MemoryManager:
         check: bool = False,
         offsets: np.ndarray | None = None,
         latency: bool = False,
+        tile: int = 1,
     ) -> dict:
         """The batched form: `sizes` is a flat request list served by persistent block-aggregated
-        kernels (agg 0) or the per-thread kernels (agg 1, 16, 64); rounds replay a captured graph.
-        `offsets`, if given, receives the validation round's blocks (byte offsets, UINT64_MAX = null);
-        `latency` adds an untimed round that samples the wall-clock time of each allocation call."""
+        kernels (agg 0, `tile` = 1, 2 or 4 requests per thread and call) or the per-thread kernels
+        (agg 1, 16, 64); rounds replay a captured graph. `offsets`, if given, receives the validation
+        round's blocks (byte offsets, UINT64_MAX = null); `latency` adds an untimed round that samples
+        the wall-clock time of each allocation call."""
         sizes = np.ascontiguousarray(sizes, dtype=np.uint32)
         cfg = BatchConfig()
         cfg.requests = len(sizes)
@@ -472,6 +474,7 @@ class MemoryManager:
         cfg.graph = int(graph)
         cfg.check = int(check or offsets is not None)
         cfg.latency = int(latency)
+        cfg.tile = tile
         cfg.sizes = sizes.ctypes.data_as(C.POINTER(C.c_uint32))
         if offsets is not None:
             assert offsets.dtype == np.uint64 and offsets.shape == sizes.shape and offsets.flags.c_contiguous

@@ -309,6 +309,9 @@ int ouro_run_batch(ouro_handle h, const ouro_batch_config* cfg, ouro_batch_resul
   if (cfg->agg != 0 && cfg->agg != 1 && cfg->agg != 16 && cfg->agg != 64) return OURO_E_INVALID;
   const uint32_t block = cfg->block ? cfg->block : 256;
   if (block % 64 != 0 || block > 1024) return OURO_E_INVALID;
+  const uint32_t tile = cfg->tile ? cfg->tile : 1;
+  if (tile != 1 && tile != 2 && tile != 4) return OURO_E_INVALID;
+  if (tile != 1 && cfg->agg != 0) return OURO_E_INVALID;  // tiles are a form of the block-wide kernels
   memset(out, 0, sizeof(*out));
   const uint32_t n = cfg->requests;
   if (n == 0 || (cfg->rounds == 0 && !cfg->check)) return OURO_OK;
@@ -326,7 +329,8 @@ int ouro_run_batch(ouro_handle h, const ouro_batch_config* cfg, ouro_batch_resul
   OURO_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, 0));
   uint32_t grid = (n + block - 1) / block;
   if (cfg->agg == 0) {
-    const int occ = m.persist_occupancy(block);
+    grid = (n + block * tile - 1) / (block * tile);
+    const int occ = m.persist_occupancy(block, tile);
     if (occ <= 0) return OURO_E_INVALID;  // the block size does not fit the persistent kernels
     // more workgroups than can be resident would serialise behind each other anyway
     const uint32_t cap = (uint32_t)occ * (uint32_t)cus;
@@ -345,11 +349,11 @@ int ouro_run_batch(ouro_handle h, const ouro_batch_config* cfg, ouro_batch_resul
   const uint32_t* sizes = d_sizes.as<uint32_t>();
   unsigned long long* errs = d_errs.as<unsigned long long>();
   auto alloc = [&]() -> hipError_t {
-    return cfg->agg == 0 ? m.malloc_persist(sizes, ptrs, n, s, block, grid)
+    return cfg->agg == 0 ? m.malloc_persist(sizes, ptrs, n, s, block, grid, tile)
                          : m.malloc_batch(sizes, ptrs, n, cfg->agg, s, nullptr, block);
   };
   auto release = [&]() -> hipError_t {
-    return cfg->agg == 0 ? m.free_persist(ptrs, n, s, block, grid) : m.free_batch(ptrs, n, cfg->agg, s, block);
+    return cfg->agg == 0 ? m.free_persist(ptrs, n, s, block, grid, tile) : m.free_batch(ptrs, n, cfg->agg, s, block);
   };
   auto touch = [&]() -> hipError_t {
     hipLaunchKernelGGL(k_touch, dim3(tgrid), dim3(256), 0, s, ptrs, sizes, n, lo, hi, 0, errs);
@@ -432,7 +436,7 @@ int ouro_run_batch(ouro_handle h, const ouro_batch_config* cfg, ouro_batch_resul
     DevBuf d_lat;
     OURO_TRY(d_lat.alloc((size_t)ns * 8));
     OURO_TRY(hipMemset(d_lat.p, 0, (size_t)ns * 8));
-    OURO_TRY(cfg->agg == 0 ? m.malloc_persist(sizes, ptrs, n, s, block, grid, d_lat.as<uint64_t>())
+    OURO_TRY(cfg->agg == 0 ? m.malloc_persist(sizes, ptrs, n, s, block, grid, tile, d_lat.as<uint64_t>())
                            : m.malloc_batch(sizes, ptrs, n, cfg->agg, s, d_lat.as<uint64_t>(), block));
     OURO_TRY(touch());
     OURO_TRY(release());

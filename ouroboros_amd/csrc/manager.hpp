@@ -43,12 +43,15 @@ struct IManager {
   virtual hipError_t free_masked(void* const* d_ptrs, const uint8_t* d_skip, uint32_t n, int agg, hipStream_t s,
                                  uint32_t block) = 0;
   // Batched form: persistent kernels over a flat request list with block-wide aggregation
-  // (Ouroboros::malloc_block / free_block). grid = persistent workgroups.
+  // (Ouroboros::malloc_tile / free_tile, `tile` = 1, 2 or 4 requests per thread). grid = persistent
+  // workgroups.
   virtual hipError_t malloc_persist(const uint32_t* d_sizes, void** d_out, uint32_t n, hipStream_t s, uint32_t block,
-                                    uint32_t grid, uint64_t* d_lat = nullptr) = 0;
-  virtual hipError_t free_persist(void* const* d_ptrs, uint32_t n, hipStream_t s, uint32_t block, uint32_t grid) = 0;
-  // workgroups of `block` threads one CU holds at once for the persistent kernels (the smaller of the two)
-  virtual int persist_occupancy(uint32_t block) = 0;
+                                    uint32_t grid, uint32_t tile, uint64_t* d_lat = nullptr) = 0;
+  virtual hipError_t free_persist(void* const* d_ptrs, uint32_t n, hipStream_t s, uint32_t block, uint32_t grid,
+                                  uint32_t tile) = 0;
+  // workgroups of `block` threads one CU holds at once for the persistent kernels of `tile` (the
+  // smallest over the three)
+  virtual int persist_occupancy(uint32_t block, uint32_t tile) = 0;
   // Serving form: one launch of `blocks` 256-thread workgroups described by d_desc (k_request /
   // k_request_block) on stream s.
   virtual hipError_t launch_requests(const struct RequestBlock* d_desc, uint32_t blocks, const struct ZipfTable& z,
@@ -130,56 +133,91 @@ __global__ void __launch_bounds__(256) k_free_masked(M* mm, void* const* __restr
   mm->template free<W>(skip[tid] ? nullptr : ptrs[tid]);
 }
 
-// Persistent block-aggregated kernels: each workgroup walks block-sized tiles of the request list
-// (the loop bound is uniform per block, as malloc_block / free_block synchronise the block). The
-// next tile's input is loaded while the current tile is served.
-template <class M>
+// Persistent block-aggregated kernels: each workgroup walks tiles of T * blockDim.x requests (request
+// t * blockDim.x + i of a tile goes to thread i; the loop bound is uniform per block, as malloc_tile /
+// free_tile synchronise the block). The next tile's input is loaded while the current tile is served.
+template <class M, uint32_t T>
 __global__ void __launch_bounds__(1024) k_malloc_persist(M* mm, const uint32_t* __restrict__ sizes,
                                                          void** __restrict__ out, uint32_t n) {
-  __shared__ typename M::Scratch sm;
-  const uint32_t step = gridDim.x * blockDim.x;
-  uint32_t base = blockIdx.x * blockDim.x;
-  uint32_t size = base + threadIdx.x < n ? sizes[base + threadIdx.x] : 0u;
-  for (; base < n; base += step) {
-    const uint32_t i = base + threadIdx.x, j = i + step;
-    const uint32_t next = j < n ? sizes[j] : 0u;
-    void* p = mm->malloc_block(i < n ? size : 0u, sm);
-    if (i < n) out[i] = p;
-    size = next;
+  __shared__ typename M::template TileScratch<T> sm;
+  const uint32_t tile = T * blockDim.x, step = gridDim.x * tile;
+  uint32_t base = blockIdx.x * tile;
+  size_t size[T];
+#pragma unroll
+  for (uint32_t t = 0; t < T; ++t) {
+    const uint32_t i = base + t * blockDim.x + threadIdx.x;
+    size[t] = i < n ? sizes[i] : 0u;
   }
-}
-
-// k_malloc_persist with the wall-clock time of each malloc_block call recorded on lane 0 of every
-// wave that holds a request (sample of request i at lat[i / 64]).
-template <class M>
-__global__ void __launch_bounds__(1024) k_malloc_persist_lat(M* mm, const uint32_t* __restrict__ sizes,
-                                                             void** __restrict__ out, uint32_t n,
-                                                             uint64_t* __restrict__ lat) {
-  __shared__ typename M::Scratch sm;
-  const uint32_t step = gridDim.x * blockDim.x;
-  for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += step) {
-    const uint32_t i = base + threadIdx.x;
-    const uint64_t t0 = wall_clock64();
-    void* p = mm->malloc_block(i < n ? sizes[i] : 0u, sm);
-    const uint64_t t1 = wall_clock64();
-    if (i < n) {
-      out[i] = p;
-      if ((threadIdx.x & 63u) == 0) lat[i >> 6] = t1 - t0;
+  for (; base < n; base += step) {
+    uint32_t next[T];
+#pragma unroll
+    for (uint32_t t = 0; t < T; ++t) {
+      const uint32_t j = base + step + t * blockDim.x + threadIdx.x;
+      next[t] = j < n ? sizes[j] : 0u;
+    }
+    void* p[T];
+    mm->template malloc_tile<T>(size, p, sm);
+#pragma unroll
+    for (uint32_t t = 0; t < T; ++t) {
+      const uint32_t i = base + t * blockDim.x + threadIdx.x;
+      if (i < n) out[i] = p[t];
+      size[t] = next[t];
     }
   }
 }
 
-template <class M>
+// k_malloc_persist with the wall-clock time of each malloc_tile call recorded on lane 0 of every
+// wave that holds a request (sample of request i at lat[i / 64]; the T requests of a wave's lanes
+// share one call).
+template <class M, uint32_t T>
+__global__ void __launch_bounds__(1024) k_malloc_persist_lat(M* mm, const uint32_t* __restrict__ sizes,
+                                                             void** __restrict__ out, uint32_t n,
+                                                             uint64_t* __restrict__ lat) {
+  __shared__ typename M::template TileScratch<T> sm;
+  const uint32_t tile = T * blockDim.x, step = gridDim.x * tile;
+  for (uint32_t base = blockIdx.x * tile; base < n; base += step) {
+    size_t size[T];
+#pragma unroll
+    for (uint32_t t = 0; t < T; ++t) {
+      const uint32_t i = base + t * blockDim.x + threadIdx.x;
+      size[t] = i < n ? sizes[i] : 0u;
+    }
+    void* p[T];
+    const uint64_t t0 = wall_clock64();
+    mm->template malloc_tile<T>(size, p, sm);
+    const uint64_t t1 = wall_clock64();
+#pragma unroll
+    for (uint32_t t = 0; t < T; ++t) {
+      const uint32_t i = base + t * blockDim.x + threadIdx.x;
+      if (i < n) {
+        out[i] = p[t];
+        if ((threadIdx.x & 63u) == 0) lat[i >> 6] = t1 - t0;
+      }
+    }
+  }
+}
+
+template <class M, uint32_t T>
 __global__ void __launch_bounds__(1024) k_free_persist(M* mm, void* const* __restrict__ ptrs, uint32_t n) {
-  __shared__ typename M::Scratch sm;
-  const uint32_t step = gridDim.x * blockDim.x;
-  uint32_t base = blockIdx.x * blockDim.x;
-  void* ptr = base + threadIdx.x < n ? ptrs[base + threadIdx.x] : nullptr;
+  __shared__ typename M::template TileScratch<T> sm;
+  const uint32_t tile = T * blockDim.x, step = gridDim.x * tile;
+  uint32_t base = blockIdx.x * tile;
+  void* ptr[T];
+#pragma unroll
+  for (uint32_t t = 0; t < T; ++t) {
+    const uint32_t i = base + t * blockDim.x + threadIdx.x;
+    ptr[t] = i < n ? ptrs[i] : nullptr;
+  }
   for (; base < n; base += step) {
-    const uint32_t j = base + threadIdx.x + step;
-    void* next = j < n ? ptrs[j] : nullptr;
-    mm->free_block(ptr, sm);
-    ptr = next;
+    void* next[T];
+#pragma unroll
+    for (uint32_t t = 0; t < T; ++t) {
+      const uint32_t j = base + step + t * blockDim.x + threadIdx.x;
+      next[t] = j < n ? ptrs[j] : nullptr;
+    }
+    mm->template free_tile<T>(ptr, sm);
+#pragma unroll
+    for (uint32_t t = 0; t < T; ++t) ptr[t] = next[t];
   }
 }
 
@@ -343,20 +381,36 @@ struct Manager final : IManager {
       hipLaunchKernelGGL((k_free<M, W>), dim3(grid_for(n, block)), dim3(block), launch_lds(), s, dm, d_ptrs, n);
     return hipGetLastError();
   }
-  hipError_t malloc_persist(const uint32_t* d_sizes, void** d_out, uint32_t n, hipStream_t s, uint32_t block,
-                            uint32_t grid, uint64_t* d_lat) override {
-    if (n == 0) return hipSuccess;
+  template <uint32_t T>
+  hipError_t launch_malloc_persist(const uint32_t* d_sizes, void** d_out, uint32_t n, hipStream_t s, uint32_t block,
+                                   uint32_t grid, uint64_t* d_lat) {
+    M* dm = mm.getDeviceMemoryManager();
     if (d_lat)
-      hipLaunchKernelGGL((k_malloc_persist_lat<M>), dim3(grid), dim3(block), 0, s, mm.getDeviceMemoryManager(), d_sizes,
-                         d_out, n, d_lat);
+      hipLaunchKernelGGL((k_malloc_persist_lat<M, T>), dim3(grid), dim3(block), 0, s, dm, d_sizes, d_out, n, d_lat);
     else
-      hipLaunchKernelGGL((k_malloc_persist<M>), dim3(grid), dim3(block), 0, s, mm.getDeviceMemoryManager(), d_sizes,
-                         d_out, n);
+      hipLaunchKernelGGL((k_malloc_persist<M, T>), dim3(grid), dim3(block), 0, s, dm, d_sizes, d_out, n);
     return hipGetLastError();
   }
-  hipError_t free_persist(void* const* d_ptrs, uint32_t n, hipStream_t s, uint32_t block, uint32_t grid) override {
+  hipError_t malloc_persist(const uint32_t* d_sizes, void** d_out, uint32_t n, hipStream_t s, uint32_t block,
+                            uint32_t grid, uint32_t tile, uint64_t* d_lat) override {
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL((k_free_persist<M>), dim3(grid), dim3(block), 0, s, mm.getDeviceMemoryManager(), d_ptrs, n);
+    switch (tile) {
+      case 1: return launch_malloc_persist<1>(d_sizes, d_out, n, s, block, grid, d_lat);
+      case 2: return launch_malloc_persist<2>(d_sizes, d_out, n, s, block, grid, d_lat);
+      case 4: return launch_malloc_persist<4>(d_sizes, d_out, n, s, block, grid, d_lat);
+      default: return hipErrorInvalidValue;
+    }
+  }
+  hipError_t free_persist(void* const* d_ptrs, uint32_t n, hipStream_t s, uint32_t block, uint32_t grid,
+                          uint32_t tile) override {
+    if (n == 0) return hipSuccess;
+    M* dm = mm.getDeviceMemoryManager();
+    switch (tile) {
+      case 1: hipLaunchKernelGGL((k_free_persist<M, 1>), dim3(grid), dim3(block), 0, s, dm, d_ptrs, n); break;
+      case 2: hipLaunchKernelGGL((k_free_persist<M, 2>), dim3(grid), dim3(block), 0, s, dm, d_ptrs, n); break;
+      case 4: hipLaunchKernelGGL((k_free_persist<M, 4>), dim3(grid), dim3(block), 0, s, dm, d_ptrs, n); break;
+      default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
   }
   hipError_t launch_requests(const RequestBlock* d_desc, uint32_t blocks, const ZipfTable& z, unsigned long long* out,
@@ -398,14 +452,22 @@ struct Manager final : IManager {
     out->classes = 0;
     free_pages_of<0>(h->chain, out);
   }
-  int persist_occupancy(uint32_t block) override {
-    int a = 0, b = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_malloc_persist<M>, (int)block, 0) != hipSuccess) return 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_free_persist<M>, (int)block, 0) != hipSuccess) return 0;
-    int c = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&c, k_malloc_persist_lat<M>, (int)block, 0) != hipSuccess) return 0;
+  template <uint32_t T>
+  static int occupancy_of(uint32_t block) {
+    int a = 0, b = 0, c = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_malloc_persist<M, T>, (int)block, 0) != hipSuccess) return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_free_persist<M, T>, (int)block, 0) != hipSuccess) return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&c, k_malloc_persist_lat<M, T>, (int)block, 0) != hipSuccess) return 0;
     b = b < c ? b : c;
     return a < b ? a : b;
+  }
+  int persist_occupancy(uint32_t block, uint32_t tile) override {
+    switch (tile) {
+      case 1: return occupancy_of<1>(block);
+      case 2: return occupancy_of<2>(block);
+      case 4: return occupancy_of<4>(block);
+      default: return 0;
+    }
   }
   hipError_t free_batch(void* const* d_ptrs, uint32_t n, int agg, hipStream_t s, uint32_t block) override {
     return free_masked(d_ptrs, nullptr, n, agg, s, block);

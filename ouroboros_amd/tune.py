@@ -1,8 +1,8 @@
 """Autotuner of the batched form: for one request list it picks the pool geometry (queue kind, chunk
 size, class count, queue capacity), the aggregation width and the launch shape of the persistent
-kernels by measuring them on the device.
+kernels (workgroup size, workgroups, requests per thread) by measuring them on the device.
 
-Search: every feasible (variant, aggregation, block, grid) candidate runs a short batch on a strided
+Search: every feasible (variant, aggregation, block, tile, grid) candidate runs a short batch on a strided
 sample of the requests (same class mix, fewer of them); the best quarter runs again on the whole list
 and the fastest malloc+free Mops/s wins. Infeasible geometries are dropped before any launch: a class
 queue must be able to hold every page its class needs at once, and the largest page must cover the
@@ -39,6 +39,7 @@ GEOMETRIES = {
 }
 AGGREGATION = (0, 1, 16, 64)  # 0 = block-wide (persistent kernels)
 BLOCKS = (256, 512, 1024)     # block-wide path; the per-wave kernels use 256
+TILES = (1, 2, 4)             # block-wide path: requests per thread and block-wide call
 
 
 @dataclass(frozen=True)
@@ -47,11 +48,13 @@ class Candidate:
     agg: int
     block: int
     grid: int = 0  # persistent workgroups; 0 = every CU at its occupancy limit
+    tile: int = 1  # block-wide path: requests per thread and call
 
     def label(self) -> str:
         a = "block" if self.agg == 0 else f"w{self.agg}"
         g = "" if self.agg != 0 else f"/g{self.grid or 'occ'}"
-        return f"{self.variant}/{a}/b{self.block}{g}"
+        t = f"/t{self.tile}" if self.tile != 1 else ""
+        return f"{self.variant}/{a}/b{self.block}{t}{g}"
 
 
 @dataclass
@@ -110,12 +113,13 @@ def feasible(geometry: dict, sizes: np.ndarray, pool_bytes: int) -> str:
     return ""
 
 
-def space(variants: Iterable[str], grids: Iterable[int] = (0,)) -> list[Candidate]:
+def space(variants: Iterable[str], grids: Iterable[int] = (0,), tiles: Iterable[int] = TILES) -> list[Candidate]:
     out = []
+    tiles = tuple(tiles)
     for v in variants:
         for agg in AGGREGATION:
             if agg == 0:
-                out += [Candidate(v, 0, b, g) for b in BLOCKS for g in grids]
+                out += [Candidate(v, 0, b, g, t) for b in BLOCKS for t in tiles for g in grids]
             else:
                 out.append(Candidate(v, agg, 256))
     return out
@@ -144,7 +148,8 @@ def device_runner(pool_bytes: int, rounds: int = 3, warmup: int = 2) -> Runner:
             except RuntimeError:
                 pass
         try:
-            r = mm.run_batch(sizes, rounds=rounds, warmup=warmup, agg=c.agg, block=c.block, grid=c.grid, graph=True)
+            r = mm.run_batch(sizes, rounds=rounds, warmup=warmup, agg=c.agg, block=c.block, grid=c.grid, graph=True,
+                             tile=c.tile)
         except RuntimeError as e:  # e.g. a block size the persistent kernels cannot launch
             return 0.0, False, str(e)
         ok = r["error_word"] == 0 and r["null_returns"] == 0 and r["verify_errors"] == 0
@@ -155,7 +160,8 @@ def device_runner(pool_bytes: int, rounds: int = 3, warmup: int = 2) -> Runner:
 
 
 def tune(sizes: np.ndarray, pool_bytes: int, variants: Iterable[str] | None = None, runner: Runner | None = None,
-         sample: int = 8, keep: float = 0.25, grids: Iterable[int] = (0,), geometry=None) -> TuneResult:
+         sample: int = 8, keep: float = 0.25, grids: Iterable[int] = (0,), geometry=None,
+         tiles: Iterable[int] = TILES) -> TuneResult:
     """Pick the fastest feasible candidate for `sizes` (see the module docstring)."""
     import ouroboros_amd as ob
 
@@ -178,7 +184,7 @@ def tune(sizes: np.ndarray, pool_bytes: int, variants: Iterable[str] | None = No
     try:
         # stage 1: a strided sample keeps the class mix
         part = sizes[::sample] if sample > 1 else sizes
-        cands = space(usable, grids)
+        cands = space(usable, grids, tiles)
         scored = []
         for c in sorted(cands, key=lambda c: c.variant):  # one pool per variant at a time
             mops, ok, note = runner(c, part)

@@ -22,19 +22,23 @@ def assert_clean(r):
     assert r["error_word"] == 0, r["errors"]
 
 
+@pytest.mark.parametrize("tile", [1, 2, 4])
 @pytest.mark.parametrize("kind", ALL)
-def test_single_workgroup_matches_oracle(kind):
-    # 640 requests, mixed classes (both instances), served tile by tile by one 64-thread workgroup
+def test_single_workgroup_matches_oracle(kind, tile):
+    # 640 requests, mixed classes (both instances), served tile by tile by one 64-thread workgroup,
+    # `tile` requests per thread and block-wide call (640 = 2.5 tiles of 256 at tile 4: the last one
+    # is partly empty)
     sizes = (16 << (np.arange(640) * 7 % 12)).astype(np.uint32)
-    print(f"single-workgroup {kind}", flush=True)
+    print(f"single-workgroup {kind} tile={tile}", flush=True)
     sim = oracle.Sim(kind, 512 << 20)
-    want = sim.block_trace(sizes, 6)
+    want = sim.block_trace(sizes, 6, tile=tile)
     assert sim.state().error == 0
     with ob.MemoryManager(kind, 512 << 20) as mm:
         for call in range(3):
             # each call runs its validation round (observed) and then one timed round
             got = np.empty(len(sizes), dtype=np.uint64)
-            res = mm.run_batch(sizes, rounds=1, warmup=0, agg=0, block=64, grid=1, graph=False, offsets=got)
+            res = mm.run_batch(sizes, rounds=1, warmup=0, agg=0, block=64, grid=1, graph=False, offsets=got,
+                               tile=tile)
             assert_clean(res)
             w = want[2 * call]
             mism = np.flatnonzero(got != w)
@@ -43,16 +47,33 @@ def test_single_workgroup_matches_oracle(kind):
 
 
 @pytest.mark.parametrize("kind", ALL)
-@pytest.mark.parametrize("block", [256, 1024])
-def test_persistent_block_aggregated_invariants(kind, block):
-    print(f"persistent {kind} block={block}", flush=True)
+@pytest.mark.parametrize("block,tile", [(256, 1), (1024, 1), (256, 4), (1024, 2)])
+def test_persistent_block_aggregated_invariants(kind, block, tile):
+    print(f"persistent {kind} block={block} tile={tile}", flush=True)
     sizes = ob.gen_sizes(42, 65536, "uniform")
     with ob.MemoryManager(kind, 1 << 30) as mm:
-        res = mm.run_batch(sizes, rounds=3, warmup=1, agg=0, block=block, graph=True, check=True)
+        res = mm.run_batch(sizes, rounds=3, warmup=1, agg=0, block=block, graph=True, check=True, tile=tile)
         assert res["graph_used"] == 1
         assert_clean(res)
         chk = mm.check_quiescent()
         assert chk.violations == 0, chk.message.decode()
+
+
+@pytest.mark.parametrize("kind", ["VAPQ", "VLPQ"])
+@pytest.mark.parametrize("block,tile", [(256, 2), (256, 4), (1024, 4)])
+def test_virtual_page_queue_tiles_at_1m(kind, block, tile):
+    # 1M requests on one pool after rounds at one request per thread: with several requests per
+    # thread, frees into VA / VL page queues must not wait on a queue chunk that another workgroup
+    # only pre-allocates in a later sub-tile (each sub-tile reserves its positions right before its
+    # puts), so no bounded wait may expire
+    print(f"virtual page queue tiles {kind} block={block} tile={tile}", flush=True)
+    sizes = ob.gen_sizes(42, 1 << 20, "uniform")
+    with ob.MemoryManager(kind, 4 << 30) as mm:
+        warm = mm.run_batch(sizes, rounds=2, warmup=1, agg=0, block=block, graph=True)
+        assert_clean(warm)
+        res = mm.run_batch(sizes, rounds=4, warmup=1, agg=0, block=block, graph=True, check=True, tile=tile)
+        assert_clean(res)
+        assert mm.check_quiescent().violations == 0
 
 
 @pytest.mark.parametrize("agg", [1, 16, 64])
@@ -65,13 +86,13 @@ def test_batch_runner_per_wave_widths(agg):
         assert mm.check_quiescent().violations == 0
 
 
-@pytest.mark.parametrize("agg", [0, 64])
-def test_batch_latency_round(agg):
-    print(f"batch latency agg={agg}", flush=True)
+@pytest.mark.parametrize("agg,tile", [(0, 1), (0, 2), (64, 1)])
+def test_batch_latency_round(agg, tile):
+    print(f"batch latency agg={agg} tile={tile}", flush=True)
     sizes = ob.gen_sizes(5, 65536, "uniform")
     with ob.MemoryManager("PQ", 1 << 30) as mm:
         r = mm.run_batch(sizes, rounds=1, warmup=1, agg=agg, block=1024 if agg == 0 else 256, graph=False,
-                         latency=True)
+                         latency=True, tile=tile)
         assert_clean(r)
         assert r["lat_samples"] == len(sizes) // 64
         assert 0 < r["lat_p50_ns"] <= r["lat_p99_ns"] <= r["lat_max_ns"] < 1e9

@@ -342,14 +342,16 @@ def test_trace_generator_matches_fixture_restatement():
     assert np.array_equal(GOLDEN["ragged_37__sizes"], ob.gen_sizes(7, 37, "uniform"))
 
 
+@pytest.mark.parametrize("tile", [1, 2, 4])
 @pytest.mark.parametrize("kind", KINDS)
-def test_block_order_replay_invariants(kind):
+def test_block_order_replay_invariants(kind, tile):
     # the block-wide replay (class leaders in class order; chunk queues claimed 64 positions at a
-    # time by a whole wave): every round's blocks are disjoint and on the page grid, the oracle raises
-    # no error, and after the last free every chunk queue holds exactly its chunks with free pages
+    # time by a whole wave; `tile` requests per thread and call): every round's blocks are disjoint
+    # and on the page grid, the oracle raises no error, and after the last free every chunk queue
+    # holds exactly its chunks with free pages
     sizes = (16 << (np.arange(1280) * 7 % 12)).astype(np.uint32)
     s = oracle.Sim(kind, 512 << 20)
-    rounds = s.block_trace(sizes, 5)
+    rounds = s.block_trace(sizes, 5, tile=tile)
     assert rounds.shape == (5, len(sizes))
     for r in range(5):
         _offset_checks(rounds[r], sizes)
