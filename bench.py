@@ -210,11 +210,13 @@ def measured_traffic(kernel: str, variant: str, block: int, tile: int) -> dict |
         t = json.loads(TRAFFIC_FILE.read_text())
     except (OSError, ValueError):
         return None
+    t = t.get("configs", {}).get(f"{variant}/b{block}/t{tile}", {})
     k = t.get("kernels", {}).get(kernel)
-    if not k or t.get("variant") != variant or t.get("block") != block or t.get("tile", 1) != tile:
+    if not k:
         return None
     return {"bytes": k["fetch_bytes"] + k["write_bytes"], "fetch_bytes": k["fetch_bytes"],
-            "write_bytes": k["write_bytes"], "source": str(TRAFFIC_FILE.relative_to(ROOT))}
+            "write_bytes": k["write_bytes"], "source": str(TRAFFIC_FILE.relative_to(ROOT)),
+            "profile": t.get("profile")}
 
 
 def main() -> int:
@@ -286,8 +288,9 @@ def main() -> int:
                 row[f"agg{agg}"] = {"mops": round(r["mops"], 2), "errors": r["errors"], "verify_errors": r["verify_errors"]}
             log(f"table {v} batched ...")
             with ob.MemoryManager(v, POOL) as mm:
-                b = mm.run_batch(sizes, rounds=max(4, steps // 4), warmup=1, agg=0, block=1024, graph=True,
-                                 tile=table_tile)
+                # like the headline: a validation round and two warm-up rounds before the timed ones
+                b = mm.run_batch(sizes, rounds=max(4, steps // 2), warmup=2, agg=0, block=1024, graph=True,
+                                 check=True, tile=table_tile)
             row["batched_block1024"] = {"mops": round(b["mops"], 2), "tile": table_tile, "errors": b["errors"]}
             table[v] = row
     best_plain = max([plain["mops"]] + [table[v]["agg1"]["mops"] for v in table])

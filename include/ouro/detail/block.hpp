@@ -105,6 +105,69 @@ __device__ inline void* block_take_page(Memory& m, I& inst, ReuseRing& base, boo
   return chunk_addr(m, index_chunk(idx)) + kMeta + (uint64_t)index_page(idx) * page_size_of(x, cq);
 }
 
+// ---- standard page rings, a whole tile at once
+// The T requests of a thread take (or return) their ring slots with all T loads in flight together;
+// only a slot found not yet written (take) or not yet emptied (put) is waited for. A standard ring
+// changes nothing but the slots, so the order of the T hand-overs does not matter. Each slot has one
+// writer and one reader per lap (ring_slot_put / ring_slot_take).
+template <uint32_t T, class I, class S>
+__device__ inline void block_take_ring_tile(Memory& m, I& inst, uint32_t off, const S& sm, const uint32_t (&gc)[T],
+                                            const uint32_t (&r)[T], void* (&out)[T]) {
+  static_assert(!I::kChunks && is_std(I::kKind), "standard page rings only");
+  uint32_t* slot[T];
+  uint32_t val[T];
+#pragma unroll
+  for (uint32_t t = 0; t < T; ++t) {
+    slot[t] = nullptr;
+    val[t] = kMarker;
+    if (gc[t] != kMarker && gc[t] >= off && gc[t] < off + I::kNQ) {
+      slot[t] = &inst.tab.ring[gc[t] - off][(sm.base[gc[t]] + r[t]) & (I::kQueueEntries - 1)];
+      val[t] = ld(slot[t]);
+    }
+  }
+#pragma unroll
+  for (uint32_t t = 0; t < T; ++t) {
+    if (!slot[t]) continue;
+    if (val[t] == kMarker) val[t] = ring_slot_take(m, slot[t]);  // not yet published: wait for it
+    else st(slot[t], kMarker);
+    void* p = nullptr;
+    const uint32_t qi = gc[t] - off;
+    if (val[t] != kMarker && val[t] != kNullPage) {
+      if (!chunk_ok(m, index_chunk(val[t])) || index_page(val[t]) >= ((I::kChunkSize >> I::kSmallLog2) >> qi))
+        set_error(m, ERR_CORRUPT);
+      else
+        p = chunk_addr(m, index_chunk(val[t])) + kMeta + (uint64_t)index_page(val[t]) * (I::kSmallest << qi);
+    }
+    out[t] = p;
+  }
+}
+
+template <uint32_t T, class I, class S>
+__device__ inline void block_put_ring_tile(Memory& m, I& inst, uint32_t off, const S& sm, const uint32_t (&gc)[T],
+                                           const uint32_t (&r)[T], const uint32_t (&chunk)[T],
+                                           const uint32_t (&page)[T]) {
+  static_assert(!I::kChunks && is_std(I::kKind), "standard page rings only");
+  uint32_t* slot[T];
+  uint32_t cur[T];
+#pragma unroll
+  for (uint32_t t = 0; t < T; ++t) {
+    slot[t] = nullptr;
+    cur[t] = kMarker;
+    if (gc[t] != kMarker && gc[t] >= off && gc[t] < off + I::kNQ) {
+      slot[t] = &inst.tab.ring[gc[t] - off][(sm.base[gc[t]] + r[t]) & (I::kQueueEntries - 1)];
+      cur[t] = ld(slot[t]);
+    }
+  }
+#pragma unroll
+  for (uint32_t t = 0; t < T; ++t) {
+    if (!slot[t]) continue;
+    // a slot that still holds the entry of position - ring_size waits for its take
+    if (cur[t] == kMarker) st(slot[t], make_index(chunk[t], page[t]));
+    else ring_slot_put(m, slot[t], make_index(chunk[t], page[t]));
+  }
+  order();  // the slots are written before the block signals the pages
+}
+
 // Wave 0: back_ fetch-add per class of the block's frees.
 template <class I, class S>
 __device__ inline void block_back(I& inst, uint32_t off, S& sm) {

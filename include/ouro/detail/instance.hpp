@@ -553,8 +553,36 @@ __device__ inline void compact_wave(Ctx<I>& x, ClassQueue* cq) {
 }
 
 // ------------------------------------------------------------------ slot sequences
+// ---- standard page-ring slots
+// A page ring's slot has one writer and one reader per lap: the put and the take of its position,
+// each unique (back_ / front_ fetch-adds). The slot itself is the hand-over flag: the put waits for
+// MARKER (the previous lap's take is done) and stores the page; the take waits for a page and stores
+// MARKER. Between a side's load and its store nobody else may write the slot, so plain device-coherent
+// loads and stores suffice, and the lanes of a class, whose positions are consecutive, hit
+// consecutive slots: a wave's accesses coalesce where per-lane atomics would serialise per address.
+// (The reference exchanges / CASes each slot, PageQueue_impl.cuh:22-36, 84-104.)
+__device__ inline void ring_slot_put(Memory& m, uint32_t* slot, uint32_t val) {
+  Spin it(&m.error);
+  while (ld(slot) != kMarker) {
+    if (it.expired()) { OURO_TIMEOUT(m); return; }
+    backoff(it.it);
+  }
+  st(slot, val);
+}
+__device__ inline uint32_t ring_slot_take(Memory& m, uint32_t* slot) {
+  uint32_t val;
+  Spin it(&m.error);
+  while ((val = ld(slot)) == kMarker) {
+    if (it.expired()) { OURO_TIMEOUT(m); return kMarker; }
+    backoff(it.it);
+  }
+  st(slot, kMarker);
+  return val;
+}
+
 // put_lanes: every lane with want=true stores `val` at virtual position v of its queue cq.
-//   PQ/CQ   : ring slot CAS (PageQueue_impl.cuh:22-36, ChunkQueue_impl.cuh:20-38)
+//   PQ      : ring slot handed over with plain coherent loads and stores (below)
+//   CQ      : ring slot CAS (PageQueue_impl.cuh:22-36, ChunkQueue_impl.cuh:20-38)
 //   VAPQ/VACQ: queue chunk via the pointer ring; position 0 of a queue chunk pre-allocates the next
 //              one (PageQueueVA_impl.cuh:196-229, ChunkQueueVA_impl.cuh:276-306)
 //   VLPQ/VLCQ: linked queue chunks (QueueChunk_impl.cuh:137-185), with the lockstep traversal of
@@ -567,7 +595,10 @@ __device__ inline void compact_wave(Ctx<I>& x, ClassQueue* cq) {
 template <int G, class I>
 __device__ inline void put_lanes(Ctx<I>& x, ClassQueue* cq, bool want, uint32_t v, uint32_t val) {
   constexpr Kind K = I::kKind;
-  if constexpr (is_std(K)) {
+  if constexpr (is_std(K) && !is_chunk_kind(K)) {
+    if (want) ring_slot_put(x.m, &ring_of(x, cq)[v & (I::kQueueEntries - 1)], val);
+    order();  // the slots are written before the caller signals the pages
+  } else if constexpr (is_std(K)) {
     if (want) {
       uint32_t* slot = &ring_of(x, cq)[v & (I::kQueueEntries - 1)];
       Spin it(&x.m.error);
@@ -695,14 +726,7 @@ __device__ inline uint32_t take_lanes(Ctx<I>& x, ClassQueue* cq, bool want, uint
   constexpr Kind K = I::kKind;
   uint32_t val = kMarker;
   if constexpr (is_std(K)) {
-    if (want) {
-      uint32_t* slot = &ring_of(x, cq)[v & (I::kQueueEntries - 1)];
-      Spin it(&x.m.error);
-      while ((val = xchg(slot, kMarker)) == kMarker) {
-        if (it.expired()) { OURO_TIMEOUT(x.m); break; }
-        backoff(it.it);
-      }
-    }
+    if (want) val = ring_slot_take(x.m, &ring_of(x, cq)[v & (I::kQueueEntries - 1)]);
   } else {
     uint32_t c = kMarker;
     if (want) {

@@ -204,6 +204,22 @@ __device__ inline void* chain_block_take(Memory& m, Chain<I, Rest...>& ch, Reuse
   return p;
 }
 
+// Whole-tile takes / puts of chains whose instances all use standard page rings.
+template <uint32_t T, class S, class I, class... Rest>
+__device__ inline void chain_take_ring_tile(Memory& m, Chain<I, Rest...>& ch, S& sm, const uint32_t (&gc)[T],
+                                            const uint32_t (&r)[T], void* (&out)[T], uint32_t off = 0) {
+  block_take_ring_tile<T>(m, ch.inst, off, sm, gc, r, out);
+  if constexpr (sizeof...(Rest) > 0) chain_take_ring_tile<T>(m, ch.next, sm, gc, r, out, off + I::kNQ);
+}
+
+template <uint32_t T, class S, class I, class... Rest>
+__device__ inline void chain_put_ring_tile(Memory& m, Chain<I, Rest...>& ch, S& sm, const uint32_t (&gc)[T],
+                                           const uint32_t (&r)[T], const uint32_t (&chunk)[T],
+                                           const uint32_t (&page)[T], uint32_t off = 0) {
+  block_put_ring_tile<T>(m, ch.inst, off, sm, gc, r, chunk, page);
+  if constexpr (sizeof...(Rest) > 0) chain_put_ring_tile<T>(m, ch.next, sm, gc, r, chunk, page, off + I::kNQ);
+}
+
 template <class S, class I, class... Rest>
 __device__ inline void chain_block_back(Chain<I, Rest...>& ch, S& sm, uint32_t off = 0) {
   if constexpr (!I::kChunks) block_back(ch.inst, off, sm);
@@ -269,6 +285,8 @@ struct Ouroboros {
     return m;
   }();
   static constexpr bool kAnyChunks = (Is::kChunks || ...);
+  // every instance a standard page ring: block-wide takes and puts of a tile go out together
+  static constexpr bool kRingPages = !kAnyChunks && !kVirtualized;
   static_assert(kNumClasses <= kBlockMaxClasses, "too many size classes for the block scratch");
   template <uint32_t T>
   using TileScratch = BlockScratch<kAnyChunks, T>;
@@ -304,8 +322,15 @@ struct Ouroboros {
     detail::block_rank_tile<T>(sm, gc, r, kNumClasses);
     detail::chain_block_claim(memory, chain, chain.inst.reuse, sm, kNumClasses);
     __syncthreads();
+    if constexpr (kRingPages) {
 #pragma unroll
-    for (uint32_t t = 0; t < T; ++t) out[t] = detail::chain_block_take(memory, chain, chain.inst.reuse, sm, gc[t], r[t]);
+      for (uint32_t t = 0; t < T; ++t) out[t] = nullptr;
+      detail::chain_take_ring_tile<T>(memory, chain, sm, gc, r, out);
+    } else {
+#pragma unroll
+      for (uint32_t t = 0; t < T; ++t)
+        out[t] = detail::chain_block_take(memory, chain, chain.inst.reuse, sm, gc[t], r[t]);
+    }
   }
 
   template <uint32_t T>
@@ -352,9 +377,13 @@ struct Ouroboros {
       detail::chain_block_back(chain, sm);
     }
     __syncthreads();
+    if constexpr (kRingPages) {
+      detail::chain_put_ring_tile<T>(memory, chain, sm, gc, r, chunk, page);
+    } else {
 #pragma unroll
-    for (uint32_t t = 0; t < T; ++t)
-      detail::chain_block_release(memory, chain, chain.inst.reuse, sm, gc[t], r[t], chunk[t], page[t]);
+      for (uint32_t t = 0; t < T; ++t)
+        detail::chain_block_release(memory, chain, chain.inst.reuse, sm, gc[t], r[t], chunk[t], page[t]);
+    }
     __syncthreads();  // every put / release of the block is done: publish, then signal
     if (threadIdx.x < kWave) detail::chain_block_signal(chain, sm);
   }
