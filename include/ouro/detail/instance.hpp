@@ -581,7 +581,8 @@ __device__ inline uint32_t ring_slot_take(Memory& m, uint32_t* slot) {
 }
 
 // put_lanes: every lane with want=true stores `val` at virtual position v of its queue cq.
-//   PQ      : ring slot handed over with plain coherent loads and stores (below)
+//   PQ      : ring slot handed over with plain coherent loads and stores (below; VA / VL page
+//             queues hand their queue-chunk slots over the same way)
 //   CQ      : ring slot CAS (PageQueue_impl.cuh:22-36, ChunkQueue_impl.cuh:20-38)
 //   VAPQ/VACQ: queue chunk via the pointer ring; position 0 of a queue chunk pre-allocates the next
 //              one (PageQueueVA_impl.cuh:196-229, ChunkQueueVA_impl.cuh:276-306)
@@ -650,13 +651,19 @@ __device__ inline void put_lanes(Ctx<I>& x, ClassQueue* cq, bool want, uint32_t 
         c = va_wait_qc(x, cq, v);
         if (c != kMarker) {
           uint32_t* slot = &qc_slots(x.m, c)[v % kQCSpots];
-          Spin it(&x.m.error);
-          while (cas(slot, kMarker, val) != kMarker) {
-            if (it.expired()) { OURO_TIMEOUT(x.m); break; }
-            backoff(it.it);
+          if constexpr (is_chunk_kind(K)) {
+            Spin it(&x.m.error);
+            while (cas(slot, kMarker, val) != kMarker) {
+              if (it.expired()) { OURO_TIMEOUT(x.m); break; }
+              backoff(it.it);
+            }
+          } else {
+            ring_slot_put(x.m, slot, val);  // one writer and one reader per position, as a page ring
           }
         }
       }
+      // the slots are written before the queue-chunk count, whose last update may recycle it
+      if constexpr (!is_chunk_kind(K)) order();
       const uint64_t grp = match<G>(c != kMarker ? (uint64_t)c : ~0ull);
       if (c != kMarker && (int)lane() == first(grp)) {
         const uint32_t added = popc(grp) * 0x10001u;
@@ -681,7 +688,8 @@ __device__ inline void put_lanes(Ctx<I>& x, ClassQueue* cq, bool want, uint32_t 
           QueueChunkHdr* h = hdr<QueueChunkHdr>(x.m, cur);
           if (ld(&h->virtual_start) / kQCSpots == v / kQCSpots) {
             if (v % kQCSpots == 0 && qc != kMarker) xchg(&h->next, qc);
-            xchg(&qc_slots(x.m, cur)[v % kQCSpots], val);
+            if constexpr (is_chunk_kind(K)) xchg(&qc_slots(x.m, cur)[v % kQCSpots], val);
+            else st(&qc_slots(x.m, cur)[v % kQCSpots], val);  // a fresh queue chunk: the slot is MARKER
             mine = cur;
             todo = false;
           }
@@ -703,6 +711,7 @@ __device__ inline void put_lanes(Ctx<I>& x, ClassQueue* cq, bool want, uint32_t 
       }
       // every slot and link of the wave is written: count per queue chunk, then move back_ptr_ /
       // front_ptr_ over a queue chunk this completed
+      if constexpr (!is_chunk_kind(K)) order();  // plain slot stores complete before the count
       const uint64_t grp = match<G>(mine != kMarker ? (uint64_t)mine : ~0ull);
       if (mine != kMarker && (int)lane() == first(grp)) {
         const uint32_t n = popc(grp);
@@ -732,15 +741,10 @@ __device__ inline uint32_t take_lanes(Ctx<I>& x, ClassQueue* cq, bool want, uint
     if (want) {
       if constexpr (is_va(K)) c = va_wait_qc(x, cq, v);
       else c = vl_locate(x, ld(&cq->front_qc), v);
-      if (c != kMarker) {
-        uint32_t* slot = &qc_slots(x.m, c)[v % kQCSpots];
-        Spin it(&x.m.error);
-        while ((val = xchg(slot, kMarker)) == kMarker) {
-          if (it.expired()) { OURO_TIMEOUT(x.m); break; }
-          backoff(it.it);
-        }
-      }
+      if (c != kMarker) val = ring_slot_take(x.m, &qc_slots(x.m, c)[v % kQCSpots]);
     }
+    // the slots are emptied before the queue-chunk count, whose last update may recycle it
+    order();
     const uint64_t grp = match<G>(c != kMarker ? (uint64_t)c : ~0ull);
     const bool lead = c != kMarker && (int)lane() == first(grp);
     uint32_t old = 0, taken = 0;
