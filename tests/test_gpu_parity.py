@@ -79,6 +79,29 @@ def test_concurrent_mixed_invariants(kind, agg):
         assert chk.violations == 0, chk.message.decode()
 
 
+@pytest.mark.parametrize("kind", ["CQ", "VACQ", "VLCQ"])
+@pytest.mark.parametrize("form", ["agg1", "agg64", "block_t4"])
+def test_chunk_queue_front_keeps_up(kind, form):
+    # chunk queues remove emptied chunks wherever they sit and compact front_ over the tombstones;
+    # across rounds the window back_ - front_ must stay within a bounded distance of the live entry
+    # count (a compaction that falls behind lets it grow by a fixed amount every round until the
+    # ring wraps onto uncompacted slots)
+    print(f"front keeps up {kind} {form}", flush=True)
+    sizes = ob.gen_sizes(42, 262144, "uniform")
+    gaps = []
+    with ob.MemoryManager(kind, 2 << 30) as mm:
+        for _ in range(6):
+            if form == "block_t4":
+                r = mm.run_batch(sizes, rounds=1, warmup=0, agg=0, block=1024, graph=False, tile=4)
+            else:
+                r = mm.run_trace(sizes, 1, agg=int(form[3:])).result
+            assert r["error_word"] == 0, r["errors"]
+            st = mm.query()
+            gaps.append(max((st.back[k] - st.front[k]) % (1 << 32) - st.queue_count[k] for k in range(10)))
+        assert mm.check_quiescent().violations == 0
+    assert max(gaps) < 32768, gaps
+
+
 @pytest.mark.parametrize("kind", ["VACQ", "CQ", "VAPQ"])
 def test_held_blocks_across_rounds(kind):
     # skewed sizes (Zipf 1.2 over the class ranks) and half of the blocks held across the round
