@@ -221,14 +221,17 @@ TRAFFIC_FILE = ROOT / "profiles" / "traffic.json"
 def probe(args, sizes: np.ndarray) -> int:
     """The profiled command: one device copy of COPY_BYTES (read + write, the counters' calibration),
     then `steps` rounds of the batched form on --variant / --block / --tile (persistent kernels, grid =
-    every CU at its occupancy limit), nothing else."""
+    every CU at its occupancy limit), followed by the runner's kernel-only timing rounds (up to 20
+    plain launches, each kernel timed by its own dispatch's events), nothing else."""
     variant = args.variant or "PQ"
     gbs = ob.bench_copy(COPY_BYTES, 1)
     with ob.MemoryManager(variant, POOL) as mm:
         r = mm.run_batch(sizes, rounds=args.steps, warmup=args.warmup, agg=0, block=args.block,
                          graph=bool(args.graph), tile=args.tile)
+    kr = max(1, r["kernel_rounds"])
     print(json.dumps({"probe": variant, "block": args.block, "tile": args.tile, "copy_gbs": round(gbs, 1),
-                      "mops": round(r["mops"], 1),
+                      "mops": round(r["mops"], 1), "alloc_kernel_ms": r["alloc_kernel_ms"] / kr,
+                      "free_kernel_ms": r["free_kernel_ms"] / kr, "kernel_rounds": r["kernel_rounds"],
                       "alloc_ms": r["alloc_ms"] / args.steps, "free_ms": r["free_ms"] / args.steps}), flush=True)
     return 0
 
@@ -333,8 +336,17 @@ def main() -> int:
     cfg3 = None if args.no_config3 else stress_config3()
 
     # ---- roofline of the dominant kernel of the headline
-    alloc_avg_s = head["alloc_ms"] / steps * 1e-3
-    free_avg_s = head["free_ms"] / steps * 1e-3
+    # kernel-only durations (events recorded by each kernel's own dispatch, plain-launch rounds after
+    # the timed ones) where the form has them; otherwise the timed rounds' per-kernel events
+    if head.get("kernel_rounds"):
+        alloc_avg_s = head["alloc_kernel_ms"] / head["kernel_rounds"] * 1e-3
+        free_avg_s = head["free_kernel_ms"] / head["kernel_rounds"] * 1e-3
+        launch_timing = (f"kernel-only: start / end events of each kernel's own dispatch (hipExtLaunchKernel), "
+                         f"{head['kernel_rounds']} plain-launch rounds after the timed ones")
+    else:
+        alloc_avg_s = head["alloc_ms"] / steps * 1e-3
+        free_avg_s = head["free_ms"] / steps * 1e-3
+        launch_timing = "hipEvents around each kernel of the timed rounds"
     dominant = "alloc" if alloc_avg_s >= free_avg_s else "free"
     dom_s = max(alloc_avg_s, free_avg_s)
     alg_bytes = THREADS * (ALG_BYTES_MALLOC if dominant == "alloc" else ALG_BYTES_FREE)
@@ -406,6 +418,8 @@ def main() -> int:
             "traffic": traffic["bytes"] if traffic else None,
             "traffic_detail": traffic,
             "avg_launch_ms": round(dom_s * 1e3, 4),
+            "launch_timing": launch_timing,
+            "avg_launch_ms_graph_node": round((head["alloc_ms"] if dominant == "alloc" else head["free_ms"]) / steps, 4),
             "algorithmic_bytes_per_launch": alg_bytes,
         },
         "atomic_roofline": atomic_roofline(best, kind, sizes, head, steps, atom_distinct, atom_same_wave),

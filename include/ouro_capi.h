@@ -223,6 +223,12 @@ typedef struct {
   uint32_t graph_used;     /* 1 if the rounds were graph replays */
   double lat_p50_ns, lat_p99_ns, lat_max_ns; /* latency round (cfg->latency) */
   uint64_t lat_samples;
+  /* agg 0: kernel-only durations of the persistent alloc / free kernels, summed over kernel_rounds
+     extra rounds of plain launches after the timed ones, each kernel timed by events its own
+     dispatch records (hipExtLaunchKernel); the timed rounds above bracket each kernel, or graph
+     node, with separate event packets */
+  double alloc_kernel_ms, free_kernel_ms;
+  uint64_t kernel_rounds;
 } ouro_batch_result;
 
 int ouro_run_batch(ouro_handle h, const ouro_batch_config* cfg, ouro_batch_result* out);
@@ -304,6 +310,12 @@ int ouro_serve_expected(const ouro_serve_config* cfg, uint32_t r, uint64_t* thre
  * (`before`); ppc a power of two. sem_make packs (count, expected) the way the device does. */
 int ouro_sem_share(uint64_t before, int64_t num, int64_t ppc, uint32_t* k);
 uint64_t ouro_sem_make(int64_t count, uint64_t expected);
+
+/* Layout check for host bindings (no reference counterpart): out[i] = sizeof of the i-th public
+ * struct, in the order ouro_variant_info, ouro_state, ouro_check_result, ouro_trace_config,
+ * ouro_trace_result, ouro_batch_config, ouro_batch_result, ouro_batch_workload, ouro_serve_config,
+ * ouro_serve_result; n <= 10 entries are written. Host only. */
+int ouro_abi_sizes(uint32_t* out, uint32_t n);
 
 /* ---- measurement helpers (no reference counterpart) ---------------------------------------- */
 /* Peak device-scope atomic throughput: `threads` threads each issue `iters` u32 fetch-adds with

@@ -162,6 +162,9 @@ class BatchResult(C.Structure):
         ("lat_p99_ns", C.c_double),
         ("lat_max_ns", C.c_double),
         ("lat_samples", C.c_uint64),
+        ("alloc_kernel_ms", C.c_double),
+        ("free_kernel_ms", C.c_double),
+        ("kernel_rounds", C.c_uint64),
     ]
 
     def as_dict(self) -> dict:
@@ -267,6 +270,7 @@ SIGNATURES = {
     "ouro_gen_sizes": (C.c_int, [C.c_uint64, C.c_uint32, C.c_int, C.c_double, C.c_uint32, C.POINTER(C.c_uint32)]),
     "ouro_sem_share": (C.c_int, [C.c_uint64, C.c_int64, C.c_int64, C.POINTER(C.c_uint32)]),
     "ouro_sem_make": (C.c_uint64, [C.c_int64, C.c_uint64]),
+    "ouro_abi_sizes": (C.c_int, [C.POINTER(C.c_uint32), C.c_uint32]),
     "ouro_bench_atomics": (C.c_int, [C.c_uint32, C.c_uint32, C.c_int, C.POINTER(C.c_double)]),
     "ouro_bench_copy": (C.c_int, [C.c_uint64, C.c_uint32, C.POINTER(C.c_double)]),
 }

@@ -186,6 +186,16 @@ int ouro_sem_share(uint64_t before, int64_t num, int64_t ppc, uint32_t* k) {
 
 uint64_t ouro_sem_make(int64_t count, uint64_t expected) { return ouro::Sem::make(count, expected); }
 
+int ouro_abi_sizes(uint32_t* out, uint32_t n) {
+  if (!out) return OURO_E_INVALID;
+  const uint32_t sizes[] = {sizeof(ouro_variant_info), sizeof(ouro_state),         sizeof(ouro_check_result),
+                            sizeof(ouro_trace_config), sizeof(ouro_trace_result),  sizeof(ouro_batch_config),
+                            sizeof(ouro_batch_result), sizeof(ouro_batch_workload), sizeof(ouro_serve_config),
+                            sizeof(ouro_serve_result)};
+  for (uint32_t i = 0; i < n && i < sizeof(sizes) / sizeof(sizes[0]); ++i) out[i] = sizes[i];
+  return OURO_OK;
+}
+
 int ouro_bench_atomics(uint32_t threads, uint32_t iters, int pattern, double* ops_per_s) {
   if (!ops_per_s || threads == 0 || threads % 256 || iters == 0 || pattern < 0 || pattern > 2) return OURO_E_INVALID;
   DevBuf buf, sink;
@@ -430,6 +440,26 @@ int ouro_run_batch(ouro_handle h, const ouro_batch_config* cfg, ouro_batch_resul
   if (g) (void)hipGraphDestroy(g);
   unsigned long long e3[3];
   OURO_TRY(hipMemcpy(e3, d_errs.p, 24, hipMemcpyDeviceToHost));
+  if (cfg->agg == 0 && cfg->rounds) {
+    // kernel-only durations: the same rounds again as plain launches, each persistent kernel timed by
+    // the start / end events its own dispatch records
+    hipEvent_t ek[4];
+    for (auto& e : ek) OURO_TRY(hipEventCreate(&e));
+    const uint32_t kr = std::min<uint32_t>(cfg->rounds, 20);
+    for (uint32_t r = 0; r < kr; ++r) {
+      OURO_TRY(m.malloc_persist(sizes, ptrs, n, s, block, grid, tile, nullptr, ek[0], ek[1]));
+      OURO_TRY(touch());
+      OURO_TRY(m.free_persist(ptrs, n, s, block, grid, tile, ek[2], ek[3]));
+      OURO_TRY(hipEventSynchronize(ek[3]));
+      float t = 0;
+      OURO_TRY(hipEventElapsedTime(&t, ek[0], ek[1]));
+      out->alloc_kernel_ms += t;
+      OURO_TRY(hipEventElapsedTime(&t, ek[2], ek[3]));
+      out->free_kernel_ms += t;
+    }
+    out->kernel_rounds = kr;
+    for (auto& e : ek) (void)hipEventDestroy(e);
+  }
   if (cfg->latency) {
     // one more round, untimed, sampling the allocation call on lane 0 of every wave
     const uint32_t ns = (n + 63) / 64;

@@ -2,6 +2,7 @@
 // Each variant is compiled in its own translation unit (variant.hip) so the six instantiations
 // build in parallel.
 #pragma once
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <stdlib.h>
@@ -45,10 +46,13 @@ struct IManager {
   // Batched form: persistent kernels over a flat request list with block-wide aggregation
   // (Ouroboros::malloc_tile / free_tile, `tile` = 1, 2 or 4 requests per thread). grid = persistent
   // workgroups.
+  // e0 / e1 (optional): events the kernel's own dispatch records at its start and end
+  // (hipExtLaunchKernel), for kernel-only durations outside any captured graph.
   virtual hipError_t malloc_persist(const uint32_t* d_sizes, void** d_out, uint32_t n, hipStream_t s, uint32_t block,
-                                    uint32_t grid, uint32_t tile, uint64_t* d_lat = nullptr) = 0;
+                                    uint32_t grid, uint32_t tile, uint64_t* d_lat = nullptr,
+                                    hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) = 0;
   virtual hipError_t free_persist(void* const* d_ptrs, uint32_t n, hipStream_t s, uint32_t block, uint32_t grid,
-                                  uint32_t tile) = 0;
+                                  uint32_t tile, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) = 0;
   // workgroups of `block` threads one CU holds at once for the persistent kernels of `tile` (the
   // smallest over the three)
   virtual int persist_occupancy(uint32_t block, uint32_t tile) = 0;
@@ -383,35 +387,45 @@ struct Manager final : IManager {
   }
   template <uint32_t T>
   hipError_t launch_malloc_persist(const uint32_t* d_sizes, void** d_out, uint32_t n, hipStream_t s, uint32_t block,
-                                   uint32_t grid, uint64_t* d_lat) {
+                                   uint32_t grid, uint64_t* d_lat, hipEvent_t e0, hipEvent_t e1) {
     M* dm = mm.getDeviceMemoryManager();
     if (d_lat)
       hipLaunchKernelGGL((k_malloc_persist_lat<M, T>), dim3(grid), dim3(block), 0, s, dm, d_sizes, d_out, n, d_lat);
+    else if (e0 || e1)
+      hipExtLaunchKernelGGL((k_malloc_persist<M, T>), dim3(grid), dim3(block), 0, s, e0, e1, 0, dm, d_sizes, d_out, n);
     else
       hipLaunchKernelGGL((k_malloc_persist<M, T>), dim3(grid), dim3(block), 0, s, dm, d_sizes, d_out, n);
     return hipGetLastError();
   }
   hipError_t malloc_persist(const uint32_t* d_sizes, void** d_out, uint32_t n, hipStream_t s, uint32_t block,
-                            uint32_t grid, uint32_t tile, uint64_t* d_lat) override {
+                            uint32_t grid, uint32_t tile, uint64_t* d_lat, hipEvent_t e0, hipEvent_t e1) override {
     if (n == 0) return hipSuccess;
     switch (tile) {
-      case 1: return launch_malloc_persist<1>(d_sizes, d_out, n, s, block, grid, d_lat);
-      case 2: return launch_malloc_persist<2>(d_sizes, d_out, n, s, block, grid, d_lat);
-      case 4: return launch_malloc_persist<4>(d_sizes, d_out, n, s, block, grid, d_lat);
+      case 1: return launch_malloc_persist<1>(d_sizes, d_out, n, s, block, grid, d_lat, e0, e1);
+      case 2: return launch_malloc_persist<2>(d_sizes, d_out, n, s, block, grid, d_lat, e0, e1);
+      case 4: return launch_malloc_persist<4>(d_sizes, d_out, n, s, block, grid, d_lat, e0, e1);
       default: return hipErrorInvalidValue;
     }
   }
-  hipError_t free_persist(void* const* d_ptrs, uint32_t n, hipStream_t s, uint32_t block, uint32_t grid,
-                          uint32_t tile) override {
-    if (n == 0) return hipSuccess;
+  template <uint32_t T>
+  hipError_t launch_free_persist(void* const* d_ptrs, uint32_t n, hipStream_t s, uint32_t block, uint32_t grid,
+                                 hipEvent_t e0, hipEvent_t e1) {
     M* dm = mm.getDeviceMemoryManager();
+    if (e0 || e1)
+      hipExtLaunchKernelGGL((k_free_persist<M, T>), dim3(grid), dim3(block), 0, s, e0, e1, 0, dm, d_ptrs, n);
+    else
+      hipLaunchKernelGGL((k_free_persist<M, T>), dim3(grid), dim3(block), 0, s, dm, d_ptrs, n);
+    return hipGetLastError();
+  }
+  hipError_t free_persist(void* const* d_ptrs, uint32_t n, hipStream_t s, uint32_t block, uint32_t grid,
+                          uint32_t tile, hipEvent_t e0, hipEvent_t e1) override {
+    if (n == 0) return hipSuccess;
     switch (tile) {
-      case 1: hipLaunchKernelGGL((k_free_persist<M, 1>), dim3(grid), dim3(block), 0, s, dm, d_ptrs, n); break;
-      case 2: hipLaunchKernelGGL((k_free_persist<M, 2>), dim3(grid), dim3(block), 0, s, dm, d_ptrs, n); break;
-      case 4: hipLaunchKernelGGL((k_free_persist<M, 4>), dim3(grid), dim3(block), 0, s, dm, d_ptrs, n); break;
+      case 1: return launch_free_persist<1>(d_ptrs, n, s, block, grid, e0, e1);
+      case 2: return launch_free_persist<2>(d_ptrs, n, s, block, grid, e0, e1);
+      case 4: return launch_free_persist<4>(d_ptrs, n, s, block, grid, e0, e1);
       default: return hipErrorInvalidValue;
     }
-    return hipGetLastError();
   }
   hipError_t launch_requests(const RequestBlock* d_desc, uint32_t blocks, const ZipfTable& z, unsigned long long* out,
                              int agg, hipStream_t s) override {

@@ -28,6 +28,16 @@ def test_library_exports_every_declared_symbol():
     assert names <= set(ob.SIGNATURES), names - set(ob.SIGNATURES)
 
 
+def test_struct_mirrors_match_the_c_layout():
+    # every ctypes mirror has the size the library was compiled with (a field added on one side only
+    # would make the library write past the Python buffer)
+    mirrors = [ob.VariantInfo, ob.State, ob.CheckResult, ob.TraceConfig, ob.TraceResult, ob.BatchConfig,
+               ob.BatchResult, ob.BatchWorkload, ob.ServeConfig, ob.ServeResult]
+    got = (ob.C.c_uint32 * len(mirrors))()
+    assert ob.lib().ouro_abi_sizes(got, len(mirrors)) == 0
+    assert [ob.C.sizeof(m) for m in mirrors] == list(got)
+
+
 def test_layout_agrees_with_oracle():
     for v in ["PQ", "CQ", "VAPQ", "VACQ", "VLPQ", "VLCQ"]:
         for pool in [400 << 20, 512 << 20, 4 << 30, 64 << 30]:
