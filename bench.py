@@ -227,7 +227,7 @@ def probe(args, sizes: np.ndarray) -> int:
     gbs = ob.bench_copy(COPY_BYTES, 1)
     with ob.MemoryManager(variant, POOL) as mm:
         r = mm.run_batch(sizes, rounds=args.steps, warmup=args.warmup, agg=0, block=args.block,
-                         graph=bool(args.graph), tile=args.tile)
+                         graph=bool(args.graph), tile=args.tile, kernel_timing=True)
     kr = max(1, r["kernel_rounds"])
     print(json.dumps({"probe": variant, "block": args.block, "tile": args.tile, "copy_gbs": round(gbs, 1),
                       "mops": round(r["mops"], 1), "alloc_kernel_ms": r["alloc_kernel_ms"] / kr,
@@ -294,7 +294,7 @@ def main() -> int:
     log(f"headline {best.label()} ...")
     with ob.MemoryManager(best.variant, POOL) as mm:
         head = mm.run_batch(sizes, rounds=steps, warmup=warmup, agg=best.agg, block=best.block, grid=best.grid,
-                            graph=True, check=True, tile=best.tile)
+                            graph=True, check=True, tile=best.tile, kernel_timing=True)
         chk = mm.check_quiescent()
         # allocation-call latency of the same form: one more round, sampled on lane 0 of every wave
         lat_head = mm.run_batch(sizes, rounds=1, warmup=0, agg=best.agg, block=best.block, grid=best.grid,

@@ -203,6 +203,9 @@ typedef struct {
                               allocation call on lane 0 of every wave */
   uint32_t tile;           /* agg 0: requests per thread and block-wide call, 1 (0), 2 or 4; a tile
                               of tile * block requests is served as one block-wide group per class */
+  int32_t kernel_timing;   /* agg 0: 1 = after the timed rounds, up to 20 more rounds of plain launches
+                              whose persistent kernels are timed by their own dispatch's events */
+  uint32_t pad;
 } ouro_batch_config;
 
 typedef struct {
@@ -223,10 +226,10 @@ typedef struct {
   uint32_t graph_used;     /* 1 if the rounds were graph replays */
   double lat_p50_ns, lat_p99_ns, lat_max_ns; /* latency round (cfg->latency) */
   uint64_t lat_samples;
-  /* agg 0: kernel-only durations of the persistent alloc / free kernels, summed over kernel_rounds
-     extra rounds of plain launches after the timed ones, each kernel timed by events its own
-     dispatch records (hipExtLaunchKernel); the timed rounds above bracket each kernel, or graph
-     node, with separate event packets */
+  /* cfg->kernel_timing: kernel-only durations of the persistent alloc / free kernels, summed over
+     kernel_rounds extra rounds of plain launches after the timed ones, each kernel timed by events
+     its own dispatch records (hipExtLaunchKernel); the timed rounds above bracket each kernel, or
+     graph node, with separate event packets */
   double alloc_kernel_ms, free_kernel_ms;
   uint64_t kernel_rounds;
 } ouro_batch_result;

@@ -138,6 +138,8 @@ class BatchConfig(C.Structure):
         ("offsets_out", C.POINTER(C.c_uint64)),
         ("latency", C.c_int32),
         ("tile", C.c_uint32),
+        ("kernel_timing", C.c_int32),
+        ("pad", C.c_uint32),
     ]
 
 
@@ -461,12 +463,14 @@ class MemoryManager:
         offsets: np.ndarray | None = None,
         latency: bool = False,
         tile: int = 1,
+        kernel_timing: bool = False,
     ) -> dict:
         """The batched form: `sizes` is a flat request list served by persistent block-aggregated
         kernels (agg 0, `tile` = 1, 2 or 4 requests per thread and call) or the per-thread kernels
         (agg 1, 16, 64); rounds replay a captured graph. `offsets`, if given, receives the validation
         round's blocks (byte offsets, UINT64_MAX = null); `latency` adds an untimed round that samples
-        the wall-clock time of each allocation call."""
+        the wall-clock time of each allocation call; `kernel_timing` (agg 0) adds up to 20 rounds that
+        time each persistent kernel by its own dispatch's events (alloc_kernel_ms, free_kernel_ms)."""
         sizes = np.ascontiguousarray(sizes, dtype=np.uint32)
         cfg = BatchConfig()
         cfg.requests = len(sizes)
@@ -479,6 +483,7 @@ class MemoryManager:
         cfg.check = int(check or offsets is not None)
         cfg.latency = int(latency)
         cfg.tile = tile
+        cfg.kernel_timing = int(kernel_timing)
         cfg.sizes = sizes.ctypes.data_as(C.POINTER(C.c_uint32))
         if offsets is not None:
             assert offsets.dtype == np.uint64 and offsets.shape == sizes.shape and offsets.flags.c_contiguous

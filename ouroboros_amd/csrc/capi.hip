@@ -440,7 +440,7 @@ int ouro_run_batch(ouro_handle h, const ouro_batch_config* cfg, ouro_batch_resul
   if (g) (void)hipGraphDestroy(g);
   unsigned long long e3[3];
   OURO_TRY(hipMemcpy(e3, d_errs.p, 24, hipMemcpyDeviceToHost));
-  if (cfg->agg == 0 && cfg->rounds) {
+  if (cfg->kernel_timing && cfg->agg == 0 && cfg->rounds) {
     // kernel-only durations: the same rounds again as plain launches, each persistent kernel timed by
     // the start / end events its own dispatch records
     hipEvent_t ek[4];

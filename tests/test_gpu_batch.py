@@ -76,6 +76,17 @@ def test_virtual_page_queue_tiles_at_1m(kind, block, tile):
         assert mm.check_quiescent().violations == 0
 
 
+def test_kernel_only_timing_rounds():
+    # the runner's extra plain-launch rounds, each persistent kernel timed by its own dispatch events
+    print("kernel-only timing", flush=True)
+    sizes = ob.gen_sizes(11, 65536, "uniform")
+    with ob.MemoryManager("PQ", 1 << 30) as mm:
+        r = mm.run_batch(sizes, rounds=3, warmup=1, agg=0, block=1024, graph=True, tile=4, kernel_timing=True)
+        assert_clean(r)
+        assert r["kernel_rounds"] == 3 and r["alloc_kernel_ms"] > 0 and r["free_kernel_ms"] > 0
+        assert mm.check_quiescent().violations == 0
+
+
 @pytest.mark.parametrize("agg", [1, 16, 64])
 def test_batch_runner_per_wave_widths(agg):
     print(f"batch per-wave agg={agg}", flush=True)
