@@ -150,3 +150,25 @@ def test_out_of_memory_concurrent(kind):
         chk = mm.check_quiescent()
         # the only violation allowed is the sticky error word itself
         assert chk.violations == 1 and "error word" in chk.message.decode(), chk.message.decode()
+
+
+@pytest.mark.parametrize("kind", ALL)
+@pytest.mark.parametrize("tile", [1, 4])
+def test_out_of_memory_block_wide(kind, tile):
+    # the batched form running out of pool mid-round: block-wide claims the pool cannot supply end in
+    # nullptr (page queues: null page entries keep every promised position filled; chunk queues: the
+    # claim gives its unserved pages back), never in a time-out, and what is handed out never overlaps
+    print(f"oom block {kind} tile={tile}", flush=True)
+    pool = (400 << 20) if kind in ("PQ", "CQ") else (24 << 20)
+    mc, _ = oracle.layout(kind, pool)
+    n = 16384
+    with ob.MemoryManager(kind, pool) as mm:
+        r = mm.run_batch(np.full(n, 8192, np.uint32), rounds=2, warmup=0, agg=0, block=1024, graph=False,
+                         check=True, tile=tile)
+        assert r["verify_errors"] == 0 and r["bounds_errors"] == 0
+        assert r["overlap_errors"] == 0 and r["align_errors"] == 0
+        assert "OOM" in r["errors"] and "TIMEOUT" not in r["errors"], r["errors"]
+        served = 2 * n - r["null_returns"]
+        assert 0 < served <= 2 * (mc - 1)
+        chk = mm.check_quiescent()
+        assert chk.violations == 1 and "error word" in chk.message.decode(), chk.message.decode()
