@@ -284,6 +284,8 @@ typedef struct {
   uint32_t max_batch;      /* requests one launch may carry (0 or 1: one request per launch) */
   unsigned long long* results_out; /* optional host [requests][4]: words intact, words not, nulls, threads */
   double* latency_out;     /* optional host [requests] ms */
+  uint32_t tile;           /* agg 0: request threads per GPU thread, 1 (0), 2 or 4 */
+  uint32_t pad;
 } ouro_serve_config;
 
 typedef struct {

@@ -220,6 +220,8 @@ class ServeConfig(C.Structure):
         ("max_batch", C.c_uint32),
         ("results_out", C.POINTER(C.c_ulonglong)),
         ("latency_out", C.POINTER(C.c_double)),
+        ("tile", C.c_uint32),
+        ("pad", C.c_uint32),
     ]
 
 
@@ -352,11 +354,12 @@ def gen_batch(seed: int = 42, workloads: int = 256, mu: float = 10.0, sigma: flo
 
 def serve_config(requests: int, rate: float, slots: int = 8, agg: int = 64, min_log2: int = 10,
                  max_log2: int = 18, zipf: float = 1.1, classes: int = 10, seed: int = 1,
-                 admit: bool = True, max_batch: int = 1) -> ServeConfig:
+                 admit: bool = True, max_batch: int = 1, tile: int = 1) -> ServeConfig:
     cfg = ServeConfig()
     cfg.rate, cfg.zipf, cfg.seed, cfg.requests, cfg.slots, cfg.agg = rate, zipf, seed, requests, slots, agg
     cfg.min_log2, cfg.max_log2, cfg.classes, cfg.admit = min_log2, max_log2, classes, int(admit)
     cfg.max_batch = max_batch
+    cfg.tile = tile
     return cfg
 
 
@@ -506,14 +509,16 @@ class MemoryManager:
         admit: bool = True,
         per_request: bool = False,
         max_batch: int = 1,
+        tile: int = 1,
     ) -> dict:
         """The serving form: Poisson arrivals at `rate` per second, each request 2^U[min_log2, max_log2]
         threads allocating Zipf(zipf) class sizes; up to `slots` launches in flight, each carrying up
         to `max_batch` of the requests that have arrived.
-        agg 0 serves each request block-wide (malloc_block / free_block); slots = 1, agg = 1 is the
-        plain per-request launch path. With per_request the result holds
+        agg 0 serves each request block-wide (malloc_tile / free_tile, `tile` request threads per GPU
+        thread); slots = 1, agg = 1 is the plain per-request launch path. With per_request the result holds
         `results` ([requests, 4]: words intact, words not, nulls, threads) and `latency_ms`."""
-        cfg = serve_config(requests, rate, slots, agg, min_log2, max_log2, zipf, classes, seed, admit, max_batch)
+        cfg = serve_config(requests, rate, slots, agg, min_log2, max_log2, zipf, classes, seed, admit, max_batch,
+                           tile)
         res_buf = np.zeros((requests, 4), np.uint64) if per_request else None
         lat_buf = np.zeros(requests, np.float64) if per_request else None
         if per_request:

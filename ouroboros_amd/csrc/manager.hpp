@@ -57,9 +57,9 @@ struct IManager {
   // smallest over the three)
   virtual int persist_occupancy(uint32_t block, uint32_t tile) = 0;
   // Serving form: one launch of `blocks` 256-thread workgroups described by d_desc (k_request /
-  // k_request_block) on stream s.
+  // k_request_tile, `tile` request threads per thread for agg 0) on stream s.
   virtual hipError_t launch_requests(const struct RequestBlock* d_desc, uint32_t blocks, const struct ZipfTable& z,
-                                     unsigned long long* out, int agg, hipStream_t s) = 0;
+                                     unsigned long long* out, int agg, hipStream_t s, uint32_t tile = 1) = 0;
   // What the pool can still hand out, read from a host copy of the manager state (state_bytes()
   // bytes at device_manager()).
   virtual void pool_free(const void* state, struct PoolFree* out) = 0;
@@ -315,21 +315,36 @@ __global__ void __launch_bounds__(256) k_request(M* mm, const RequestBlock* __re
   request_report(out, d.req, ok, bad, nul);
 }
 
-// The same launch served block-wide (Ouroboros::malloc_block / free_block): every thread of the
-// block calls together, a thread past the request's end with size 0 / nullptr.
-template <class M>
-__global__ void __launch_bounds__(1024) k_request_block(M* mm, const RequestBlock* __restrict__ desc, ZipfTable z,
-                                                        unsigned long long* __restrict__ out) {
-  __shared__ typename M::Scratch sm;
+// The same launch served block-wide (Ouroboros::malloc_tile / free_tile): every thread of the block
+// calls together and serves T request threads, request thread first + t * blockDim.x + i on thread
+// i; a thread past the request's end passes size 0 / nullptr.
+template <class M, uint32_t T>
+__global__ void __launch_bounds__(1024) k_request_tile(M* mm, const RequestBlock* __restrict__ desc, ZipfTable z,
+                                                       unsigned long long* __restrict__ out) {
+  __shared__ typename M::template TileScratch<T> sm;
   const RequestBlock d = desc[blockIdx.x];
-  const uint32_t i = d.first + threadIdx.x;
   unsigned long long ok = 0, bad = 0, nul = 0;
-  uint32_t q = 0, v0 = 0;
-  if (i < d.n) request_draw(d, i, z, q, v0);
-  uint32_t* p = reinterpret_cast<uint32_t*>(mm->malloc_block(i < d.n ? (16u << q) : 0u, sm));
-  if (p) request_touch(p, q, v0, ok, bad);
-  else if (i < d.n) nul = 1;
-  mm->free_block(p, sm);
+  size_t size[T];
+  uint32_t q[T], v0[T];
+#pragma unroll
+  for (uint32_t t = 0; t < T; ++t) {
+    const uint32_t i = d.first + t * blockDim.x + threadIdx.x;
+    q[t] = 0;
+    v0[t] = 0;
+    size[t] = 0;
+    if (i < d.n) {
+      request_draw(d, i, z, q[t], v0[t]);
+      size[t] = 16u << q[t];
+    }
+  }
+  void* p[T];
+  mm->template malloc_tile<T>(size, p, sm);
+#pragma unroll
+  for (uint32_t t = 0; t < T; ++t) {
+    if (p[t]) request_touch(reinterpret_cast<uint32_t*>(p[t]), q[t], v0[t], ok, bad);
+    else if (size[t]) ++nul;
+  }
+  mm->template free_tile<T>(p, sm);
   request_report(out, d.req, ok, bad, nul);
 }
 
@@ -428,11 +443,16 @@ struct Manager final : IManager {
     }
   }
   hipError_t launch_requests(const RequestBlock* d_desc, uint32_t blocks, const ZipfTable& z, unsigned long long* out,
-                             int agg, hipStream_t s) override {
+                             int agg, hipStream_t s, uint32_t tile) override {
     if (blocks == 0) return hipSuccess;
     M* dm = mm.getDeviceMemoryManager();
+    if (agg == 0 && tile != 1 && tile != 2 && tile != 4) return hipErrorInvalidValue;
     switch (agg) {
-      case 0: hipLaunchKernelGGL((k_request_block<M>), dim3(blocks), dim3(256), 0, s, dm, d_desc, z, out); break;
+      case 0:
+        if (tile == 4) hipLaunchKernelGGL((k_request_tile<M, 4>), dim3(blocks), dim3(256), 0, s, dm, d_desc, z, out);
+        else if (tile == 2) hipLaunchKernelGGL((k_request_tile<M, 2>), dim3(blocks), dim3(256), 0, s, dm, d_desc, z, out);
+        else hipLaunchKernelGGL((k_request_tile<M, 1>), dim3(blocks), dim3(256), 0, s, dm, d_desc, z, out);
+        break;
       case 1: hipLaunchKernelGGL((k_request<M, 1>), dim3(blocks), dim3(256), 0, s, dm, d_desc, z, out); break;
       case 16: hipLaunchKernelGGL((k_request<M, 16>), dim3(blocks), dim3(256), 0, s, dm, d_desc, z, out); break;
       case 64: hipLaunchKernelGGL((k_request<M, 64>), dim3(blocks), dim3(256), 0, s, dm, d_desc, z, out); break;

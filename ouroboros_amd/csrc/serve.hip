@@ -83,14 +83,17 @@ int ouro_serve(ouro_handle h, const ouro_serve_config* cfg, ouro_serve_result* o
       cfg->min_log2 > cfg->max_log2 || cfg->max_log2 > 24 || cfg->classes == 0 || cfg->classes > 10)
     return OURO_E_INVALID;
   if (cfg->agg != 0 && cfg->agg != 1 && cfg->agg != 16 && cfg->agg != 64) return OURO_E_INVALID;
+  const uint32_t tile = cfg->agg == 0 && cfg->tile ? cfg->tile : 1u;
+  if (tile != 1 && tile != 2 && tile != 4) return OURO_E_INVALID;
   memset(out, 0, sizeof(*out));
   IManager& m = *h->impl;
   const uint32_t R = cfg->requests, S = cfg->slots;
   const uint32_t max_batch = cfg->max_batch ? cfg->max_batch : 1u;
   // threads per workgroup: 256 also for block-wide allocation, where smaller workgroups keep more
   // of them resident per CU to overlap the requests' memory traffic (measured: 1024-thread
-  // workgroups served 26% fewer requests per second at saturation)
-  const uint32_t B = 256u;
+  // workgroups served 26% fewer requests per second at saturation); B = request threads per
+  // workgroup, `tile` per thread when block-wide
+  const uint32_t B = 256u * tile;
 
   ZipfTable z;
   std::vector<double> pq;
@@ -255,7 +258,7 @@ int ouro_serve(ouro_handle h, const ouro_serve_config* cfg, ouro_serve_result* o
       gd.events.push_back(bt.done);
       OURO_TRY_HIP(hipMemcpyAsync(d_desc[s], h_desc[s], (size_t)blocks * sizeof(RequestBlock), hipMemcpyHostToDevice,
                                   gd.streams[s]));
-      OURO_TRY_HIP(m.launch_requests(d_desc[s], blocks, z, d_res, cfg->agg, gd.streams[s]));
+      OURO_TRY_HIP(m.launch_requests(d_desc[s], blocks, z, d_res, cfg->agg, gd.streams[s], tile));
       OURO_TRY_HIP(hipEventRecord(bt.done, gd.streams[s]));
       slot_batch[s] = (int)batches.size();
       batches.push_back(bt);

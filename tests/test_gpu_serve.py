@@ -27,7 +27,8 @@ def test_results_same_alone_and_in_the_mix(kind):
         alone = mm.serve(n, rate=2000.0, slots=1, agg=1, admit=False, per_request=True, **kw)
         mix = mm.serve(n, rate=1e6, slots=16, agg=64, admit=True, per_request=True, **kw)
         block_mix = mm.serve(n, rate=1e6, slots=4, agg=0, admit=True, per_request=True, max_batch=16, **kw)
-        for name, r in (("alone", alone), ("mix", mix), ("block mix", block_mix)):
+        tile_mix = mm.serve(n, rate=1e6, slots=4, agg=0, admit=True, per_request=True, max_batch=16, tile=4, **kw)
+        for name, r in (("alone", alone), ("mix", mix), ("block mix", block_mix), ("tile mix", tile_mix)):
             assert r["error_word"] == 0, (name, r["errors"])
             assert r["completed"] == n and r["mismatches"] == 0 and r["nulls"] == 0, (name, r)
             res = r["results"]
@@ -35,6 +36,7 @@ def test_results_same_alone_and_in_the_mix(kind):
             assert np.array_equal(res[:, 0], want[:, 1]), name
         assert np.array_equal(alone["results"], mix["results"])
         assert np.array_equal(alone["results"], block_mix["results"])
+        assert np.array_equal(alone["results"], tile_mix["results"])
         assert mix["max_in_flight"] > 1 and block_mix["max_in_flight"] > 1
         assert alone["launches"] == n and block_mix["launches"] < n
         assert mm.check_quiescent().violations == 0
