@@ -299,6 +299,11 @@ def main() -> int:
         # allocation-call latency of the same form: one more round, sampled on lane 0 of every wave
         lat_head = mm.run_batch(sizes, rounds=1, warmup=0, agg=best.agg, block=best.block, grid=best.grid,
                                 graph=False, latency=True, tile=best.tile)
+        # the same geometry with one request per thread and call: shorter calls, more of them
+        lat_t1 = None
+        if best.agg == 0 and best.tile != 1:
+            lat_t1 = mm.run_batch(sizes, rounds=1, warmup=1, agg=0, block=best.block, grid=best.grid,
+                                  graph=False, latency=True, tile=1)
     kind = ob.variant_geometry(best.variant)["kind_name"]
 
     # ---- the plain per-thread path (reference aggregation) on the same queue kind, and the best of all kinds
@@ -391,8 +396,11 @@ def main() -> int:
         "wave64_mops": round(wave["mops"], 3),
         "p99_alloc_latency_ns": round(lat_head["lat_p99_ns"], 1),
         "p50_alloc_latency_ns": round(lat_head["lat_p50_ns"], 1),
-        "latency_note": f"headline form ({best.label()}): wall_clock64 around each malloc_block call on lane 0 "
-        f"of every wave, {lat_head['lat_samples']} samples, the whole 1M-request round in flight",
+        "latency_note": f"headline form ({best.label()}): wall_clock64 around each malloc_tile call on lane 0 "
+        f"of every wave (one call serves {best.tile} request(s) per thread), {lat_head['lat_samples']} samples, "
+        "the whole 1M-request round in flight",
+        "p99_alloc_latency_ns_tile1": round(lat_t1["lat_p99_ns"], 1) if lat_t1 else None,
+        "p50_alloc_latency_ns_tile1": round(lat_t1["lat_p50_ns"], 1) if lat_t1 else None,
         "p99_alloc_latency_ns_wave64": round(lat["p99_ns"], 1),
         "p50_alloc_latency_ns_wave64": round(lat["p50_ns"], 1),
         "latency_note_wave64": f"{kind} device malloc<64>, per-thread kernel, lane 0 of every wave, "
