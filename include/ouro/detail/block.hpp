@@ -275,7 +275,7 @@ __device__ inline void block_claim_chunks(Memory& m, I& inst, ReuseRing& base, u
       if (l < nb && chunk_ok(m, s)) {
         h = hdr<ChunkHdr>(m, s);
         cnt = ld(&h->count);
-        if (cnt <= 0 || ld(&h->queue_pos) != v + l || ld(&h->page_size) != psize) cnt = 0;
+        if (cnt <= 0 || ld(&h->queue_pos) != (uint64_t)(v + l) || ld(&h->page_size) != psize) cnt = 0;
       }
       const uint32_t c = cnt > 0 ? (uint32_t)cnt : 0u;
       uint32_t offered = 0;
@@ -292,7 +292,7 @@ __device__ inline void block_claim_chunks(Memory& m, I& inst, ReuseRing& base, u
       // the whole batch: slots tombed in parallel, one count update per queue and queue chunk, one
       // wave-wide compaction of front_)
       bool removed = false;
-      if (t && t == c) removed = cas(&h->queue_pos, v + l, kMarker) == v + l;
+      if (t && t == c) removed = cas(&h->queue_pos, (uint64_t)(v + l), kNotQueued) == (uint64_t)(v + l);
       const uint64_t rm = ballot(removed);
       if (rm) {
         if (removed) xchg(slot, kTomb);

@@ -36,6 +36,8 @@ struct Instance {
   static_assert(kChunkSize % kBaseChunk == 0, "instance chunk must be a multiple of the base chunk");
   static_assert(kChunkSize / SMALLEST <= 512, "at most 512 pages per chunk (9 page bits)");
   static_assert(NQ <= 16, "at most 16 classes per instance");
+  // the chunk class byte keeps a base chunk's offset inside its run in 3 bits
+  static_assert(kAdd <= 8, "an instance chunk spans at most 8 base chunks");
 
   // Read-only after initialize: the class queue arrays. Kept off the lines of the class counters,
   // whose atomics slow down when plain loads of the same line keep arriving.
@@ -336,10 +338,13 @@ __device__ inline uint32_t vl_locate(Ctx<I>& x, uint32_t start, uint32_t v) {
     set_error(x.m, ERR_VSTART);
     return kMarker;
   }
+  const uint32_t vb = v - v % kQCSpots;  // virtual start of the queue chunk holding v
   while (true) {
-    const uint32_t vs = ld(&hdr<QueueChunkHdr>(x.m, c)->virtual_start) / kQCSpots;
-    if (vs == v / kQCSpots) break;
-    if (vs - v / kQCSpots < 0x80000000u) return kMarker;  // already past v: its queue chunk is gone
+    // virtual starts are compared as full 32-bit positions, whose differences wrap consistently
+    // (queue-chunk numbers, v / 2048, only span 21 bits)
+    const uint32_t vs = ld(&hdr<QueueChunkHdr>(x.m, c)->virtual_start);
+    if (vs == vb) break;
+    if (vs - vb < 0x80000000u) return kMarker;  // already past v: its queue chunk is gone
     uint32_t nx;
     while ((nx = ld(&hdr<QueueChunkHdr>(x.m, c)->next)) == kMarker) {
       if (it.expired()) { OURO_TIMEOUT(x.m); return kMarker; }
@@ -421,10 +426,11 @@ __device__ inline void compact(Ctx<I>& x, ClassQueue* cq) {
     } else {
       const uint32_t c = ld(&cq->front_qc);
       if (!chunk_ok(x.m, c)) return;
-      const uint32_t vs = ld(&hdr<QueueChunkHdr>(x.m, c)->virtual_start) / kQCSpots, fq = f / kQCSpots;
-      if (vs != fq) {
-        if (vs - fq >= 0x80000000u) return;  // front_ptr_ still behind f: its move is in flight
-        nf = vs * kQCSpots;
+      // full 32-bit virtual starts: their differences wrap consistently with the positions
+      const uint32_t vs = ld(&hdr<QueueChunkHdr>(x.m, c)->virtual_start), fb = f - f % kQCSpots;
+      if (vs != fb) {
+        if (vs - fb >= 0x80000000u) return;  // front_ptr_ still behind f: its move is in flight
+        nf = vs;
       } else if (ld(&qc_slots(x.m, c)[f % kQCSpots]) != kTomb) {
         OURO_DIAG_ADD(14, 1);
         return;
@@ -487,10 +493,10 @@ __device__ inline void compact_wave(Ctx<I>& x, ClassQueue* cq) {
         if (!chunk_ok(x.m, c)) {
           n = 0;
         } else {
-          const uint32_t vs = ld(&hdr<QueueChunkHdr>(x.m, c)->virtual_start) / kQCSpots, fq = f / kQCSpots;
-          if (vs == fq) sl = qc_slots(x.m, c) + f % kQCSpots;
-          else if (vs - fq >= 0x80000000u) n = 0;  // front_ptr_ still behind f: its move is in flight
-          else jump = vs * kQCSpots;
+          const uint32_t vs = ld(&hdr<QueueChunkHdr>(x.m, c)->virtual_start), fb = f - f % kQCSpots;
+          if (vs == fb) sl = qc_slots(x.m, c) + f % kQCSpots;
+          else if (vs - fb >= 0x80000000u) n = 0;  // front_ptr_ still behind f: its move is in flight
+          else jump = vs;
         }
       }
       if constexpr (!is_std(K)) {
@@ -943,7 +949,7 @@ __device__ inline void enqueue_chunks(Ctx<I>& x, ClassQueue* cq, bool want, uint
   const int L = first(grp);
   if (want && (int)lane() == L) add(&cq->count, (int32_t)popc(grp));
   const uint32_t v = fetch_positions<G>(&cq->back, want, n);
-  if (want) st(&hdr<ChunkHdr>(x.m, chunk)->queue_pos, v);
+  if (want) st(&hdr<ChunkHdr>(x.m, chunk)->queue_pos, (uint64_t)v);
   order();
   put_lanes<G>(x, cq, want, v, chunk);
 }
@@ -976,7 +982,7 @@ __device__ inline uint32_t carve_chunks(Ctx<I>& x, ClassQueue* cq, uint32_t k) {
     st(&h->page_size, psize);
     st(&h->size, ppc);
     st(&h->count, (int32_t)ppc);
-    st(&h->queue_pos, kMarker);
+    st(&h->queue_pos, kNotQueued);
     for (uint32_t w = 0; w < kMaskWords; ++w) {
       const uint32_t lo = w * 64u;
       uint64_t bits = 0;
@@ -1056,7 +1062,8 @@ __device__ inline uint32_t find_chunk(Ctx<I>& x, ClassQueue* cq, uint32_t v, uin
   uint32_t s[kPeek];
   peek_slots(x, cq, v, n, cursor, s);
   int32_t c_[kPeek];
-  uint32_t qp[kPeek], ps[kPeek];
+  uint64_t qp[kPeek];
+  uint32_t ps[kPeek];
 #pragma unroll
   for (uint32_t i = 0; i < kPeek; ++i) {
     const ChunkHdr* h = hdr<ChunkHdr>(x.m, chunk_ok(x.m, s[i]) ? s[i] : 0u);
@@ -1067,7 +1074,7 @@ __device__ inline uint32_t find_chunk(Ctx<I>& x, ClassQueue* cq, uint32_t v, uin
   const uint32_t psize = page_size_of(x, cq);
 #pragma unroll
   for (uint32_t i = 0; i < kPeek; ++i) {
-    if (i < n && chunk_ok(x.m, s[i]) && c_[i] > 0 && qp[i] == v + i && ps[i] == psize) {
+    if (i < n && chunk_ok(x.m, s[i]) && c_[i] > 0 && qp[i] == (uint64_t)(v + i) && ps[i] == psize) {
       chunk = s[i];
       cnt = c_[i];
       return i;
@@ -1123,28 +1130,25 @@ __device__ inline void finish_reclaim(Ctx<I>& x, ClassQueue* cq, bool want, uint
   }
 }
 
-// queue_pos while a (re-)enqueue of the chunk is in flight.
-constexpr uint32_t kPending = 0xFFFFFFFEu;
-
 // Lanes that moved a chunk's free count from <= 0 to > 0 make sure the chunk is in its class queue
 // (FIRST_FREE, ChunkQueue_impl.cuh:166-170, and RE_ENQUEUE_CHUNK, :423-428). The queue_pos
-// MARKER -> PENDING CAS admits exactly one enqueue however many lanes or removers race.
+// not-queued -> pending CAS admits exactly one enqueue however many lanes or removers race.
 template <int G, class I>
 __device__ inline void ensure_in_queue(Ctx<I>& x, ClassQueue* cq, bool want, uint32_t chunk) {
   bool won = false;
-  if (want) won = cas(&hdr<ChunkHdr>(x.m, chunk)->queue_pos, kMarker, kPending) == kMarker;
+  if (want) won = cas(&hdr<ChunkHdr>(x.m, chunk)->queue_pos, kNotQueued, kQueuePending) == kNotQueued;
   if (ballot(won)) enqueue_chunks<G>(x, cq, won, chunk);
 }
 
 // Remove an exhausted chunk from its class queue, wherever it sits (the reference removes with
 // atomicMax(front_, v + 1), ChunkQueue_impl.cuh:432, which can strand live chunks in front of v).
-// `pos` is the queue position the caller found the chunk at: the CAS queue_pos pos -> MARKER admits
+// `pos` is the queue position the caller found the chunk at: the CAS queue_pos pos -> not-queued admits
 // one remover and only for that entry. A caller that took the chunk's last page from an entry that
 // has since been removed must not remove the chunk's next entry: a free may already have re-enqueued
 // it at a position whose slot is still being written. Then the slot is tombstoned; the caller
 // compacts front_ once for the wave afterwards (compact_wave where the lanes reconverge: one
 // wave-wide pass instead of a contended serial one per removing lane). Returns true if removed;
-// `again` reports a free that raced in (the caller re-enqueues: the free saw queue_pos != MARKER and
+// `again` reports a free that raced in (the caller re-enqueues: the free saw the chunk still queued and
 // left it to us).
 template <class I>
 __device__ inline bool cq_remove(Ctx<I>& x, ClassQueue* cq, uint32_t chunk, uint32_t pos, uint32_t& cursor,
@@ -1153,7 +1157,7 @@ __device__ inline bool cq_remove(Ctx<I>& x, ClassQueue* cq, uint32_t chunk, uint
   OURO_DIAG_ADD(7, 1);
   ChunkHdr* h = hdr<ChunkHdr>(x.m, chunk);
   const uint32_t p = pos;
-  if (cas(&h->queue_pos, p, kMarker) != p) return false;
+  if (cas(&h->queue_pos, (uint64_t)p, kNotQueued) != (uint64_t)p) return false;
   OURO_DIAG_ADD(8, 1);
   uint32_t rc = kMarker;
   tomb_slot(x, cq, p, cursor, rc);
@@ -1382,7 +1386,7 @@ __device__ inline void* alloc_chunk_pages(Ctx<I>& x, uint32_t qi, bool want) {
         s[0] = kMarker;
         if (n) peek_slots(x, cq, v, n, cursor, s);  // never walk towards a position at back_
         const uint32_t c = s[0];
-        if (n && chunk_ok(x.m, c) && ld(&hdr<ChunkHdr>(x.m, c)->queue_pos) == v) {
+        if (n && chunk_ok(x.m, c) && ld(&hdr<ChunkHdr>(x.m, c)->queue_pos) == (uint64_t)v) {
           ChunkHdr* h = hdr<ChunkHdr>(x.m, c);
           const AccessMode md = access_alloc_lane(x, h, page);
           if (md == AccessMode::ERROR) break;

@@ -34,6 +34,10 @@ constexpr uint32_t kQCSpots = kBaseChunk / 4;  // QueueChunk::num_spots_ (QueueC
 constexpr uint32_t kOldHold = 10;              // LARGEST_OLD_COUNT_VALUE (Parameters.h:13)
 constexpr uint32_t kMaskWords = 8;             // 512 pages / 64-bit words (ChunkAccess.cuh:13 uses 16 x u32)
 constexpr uint32_t kStateBytes = 8192;         // Ouroboros::size_() for every chain here (Ouroboros.cuh:260)
+// Largest pool in chunks: MemoryIndex keeps 23 chunk bits, and the page indices of chunk 2^23 - 1
+// would run into the queue sentinels (0xFFFFFFFC..0xFFFFFFFF), so the top index stays unused. About
+// 65 GiB of 8 KiB chunks; initialize() rejects larger pools.
+constexpr uint32_t kMaxChunks = (1u << 23) - 1u;
 
 // Queue organisation of one instance (InstanceDefinitions.cuh:1-15).
 enum class Kind : uint32_t { PQ = 0, CQ = 1, VAPQ = 2, VACQ = 3, VLPQ = 4, VLCQ = 5 };
@@ -104,12 +108,17 @@ struct PageChunkHdr {
 };
 // ChunkIndexChunk (ChunkIndexChunk.cuh:5-20) + ChunkAccess (ChunkAccess.cuh:26-28). The
 // availability mask is eight 64-bit words (one wave-wide ballot per word) instead of 16 x u32.
+// queue_pos holds the 32-bit queue position the chunk is queued at, zero-extended to 64 bits, so the
+// two sentinels (not queued, enqueue in flight) can never equal a position, whatever value the
+// wrapping 32-bit positions reach (the reference keeps a 32-bit index_t).
+constexpr uint64_t kNotQueued = ~0ull;
+constexpr uint64_t kQueuePending = ~1ull;
 struct ChunkHdr {
   uint32_t page_size;
   uint32_t size;       // pages on this chunk
   int32_t count;       // free pages
-  uint32_t queue_pos;  // position in the class queue, or marker
-  uint64_t pad;
+  uint32_t pad;
+  uint64_t queue_pos;  // position in the class queue (< 2^32), kNotQueued or kQueuePending
   uint64_t mask[kMaskWords];  // bit set = page free
 };
 static_assert(sizeof(ChunkHdr) <= kMeta, "chunk header exceeds metadata area");

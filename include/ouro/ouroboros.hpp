@@ -478,7 +478,8 @@ struct Ouroboros {
     if (memory.initialized) return hipSuccess;
     static_assert(sizeof(MyType) <= kStateBytes, "manager state exceeds its 8 KiB slot");
     const PoolLayout L = layout(instantiation_size, additionalSizeBeginning, additionalSizeEnd);
-    if (L.max_chunks == 0) return hipErrorInvalidValue;
+    // MemoryIndex holds 23 chunk bits: a larger pool would hand out aliased page handles
+    if (L.max_chunks == 0 || L.max_chunks > kMaxChunks) return hipErrorInvalidValue;
     uint8_t* d = nullptr;
     hipError_t e = hipMalloc(reinterpret_cast<void**>(&d), L.allocation_size);
     if (e != hipSuccess) return e;

@@ -316,6 +316,13 @@ int ouro_serve_expected(const ouro_serve_config* cfg, uint32_t r, uint64_t* thre
 int ouro_sem_share(uint64_t before, int64_t num, int64_t ppc, uint32_t* k);
 uint64_t ouro_sem_make(int64_t count, uint64_t expected);
 
+/* Position origin (test hook, no reference counterpart): on a pool no call has used yet, start every
+ * class queue at position `origin` (a multiple of 2048) instead of 0: front_, back_, the chunk-queue
+ * page cursor and the initial VA / VL queue chunk. Queue positions are 32-bit and wrap; a trace run
+ * from an origin just below 2^32 crosses the wrap in a few rounds and must hand out exactly the
+ * blocks it hands out from origin 0. */
+int ouro_set_origin(ouro_handle h, uint32_t origin);
+
 /* Layout check for host bindings (no reference counterpart): out[i] = sizeof of the i-th public
  * struct, in the order ouro_variant_info, ouro_state, ouro_check_result, ouro_trace_config,
  * ouro_trace_result, ouro_batch_config, ouro_batch_result, ouro_batch_workload, ouro_serve_config,

@@ -24,7 +24,9 @@
 #define QC_RESERVE 32u        /* pool chunks only queue chunks may take */
 #define STATE_BYTES 8192u
 #define BLOCK 4096u
-#define PENDING 0xFFFFFFFEu
+/* a chunk's queue position is a 32-bit value held in 64 bits; the sentinels are never positions */
+#define NOT_QUEUED 0xFFFFFFFFFFFFFFFFull
+#define QUEUE_PENDING 0xFFFFFFFFFFFFFFFEull
 #define NULLPAGE 0xFFFFFFFCu  /* page-queue entry for a page the pool could not supply */
 #define TOMB 0xFFFFFFFDu      /* chunk-queue slot whose chunk was removed */
 #define PEEK 8u               /* slots looked at per scan step */
@@ -49,7 +51,7 @@ typedef struct {
   int type; /* 0 unused, 1 page chunk, 2 chunk-queue chunk, 3 queue chunk */
   uint32_t page_size, size;
   int32_t count;
-  uint32_t queue_pos;
+  uint64_t queue_pos;
   uint64_t mask[8];
   /* queue chunk view */
   uint32_t qc_count, vstart, next;
@@ -289,14 +291,16 @@ static void vl_set_old(ouro_sim* s, cls_t* q, uint32_t free_count) {
 /* QueueChunk::locateQueueChunkForPosition (QueueChunk_impl.cuh:386-414); MARKER when the walk
  * starts past v's queue chunk (that one is gone). */
 static uint32_t vl_locate(ouro_sim* s, uint32_t c, uint32_t v) {
+  const uint32_t vb = v - v % QC_SPOTS;
   for (;;) {
     if (!chunk_ok(s, c)) {
       s->error |= ERR_VSTART;
       return MARKER;
     }
-    const uint32_t vs = ch(s, c)->vstart / QC_SPOTS;
-    if (vs == v / QC_SPOTS) return c;
-    if (vs - v / QC_SPOTS < 0x80000000u) return MARKER;
+    /* full 32-bit virtual starts: differences wrap with the positions */
+    const uint32_t vs = ch(s, c)->vstart;
+    if (vs == vb) return c;
+    if (vs - vb < 0x80000000u) return MARKER;
     c = ch(s, c)->next;
   }
 }
@@ -517,7 +521,7 @@ static uint32_t find_chunk(ouro_sim* s, inst_t* I, cls_t* q, uint32_t v, uint32_
   for (uint32_t i = 0; i < n; ++i) {
     if (!chunk_ok(s, sl[i])) continue;
     const chunk_t* x = ch(s, sl[i]);
-    if (x->count > 0 && x->queue_pos == v + i && x->page_size == q->page_size) {
+    if (x->count > 0 && x->queue_pos == (uint64_t)(v + i) && x->page_size == q->page_size) {
       *chunk = sl[i];
       return i;
     }
@@ -561,10 +565,10 @@ static void compact(ouro_sim* s, inst_t* I, cls_t* q, int wave) {
     } else {
       const uint32_t c = q->front_qc;
       if (!chunk_ok(s, c)) return;
-      const uint32_t vs = ch(s, c)->vstart / QC_SPOTS, fq = f / QC_SPOTS;
-      if (vs != fq) {
-        if (vs - fq >= 0x80000000u) return;
-        nf = vs * QC_SPOTS;
+      const uint32_t vs = ch(s, c)->vstart, fb = f - f % QC_SPOTS;
+      if (vs != fb) {
+        if (vs - fb >= 0x80000000u) return;
+        nf = vs;
       } else if (ch(s, c)->slots[f % QC_SPOTS] != TOMB) {
         return;
       }
@@ -617,9 +621,9 @@ static void finish_reclaim(ouro_sim* s, inst_t* I, cls_t* q, uint32_t reclaim);
  * cq_remove). The caller compacts the front once for the wave. Returns 1 when removed. */
 static int cq_remove(ouro_sim* s, inst_t* I, cls_t* q, uint32_t chunk, uint32_t* cursor, uint32_t* reclaim) {
   chunk_t* x = ch(s, chunk);
-  const uint32_t p = x->queue_pos;
-  if (p == MARKER || p == PENDING) return 0;
-  x->queue_pos = MARKER;
+  if (x->queue_pos == NOT_QUEUED || x->queue_pos == QUEUE_PENDING) return 0;
+  const uint32_t p = (uint32_t)x->queue_pos;
+  x->queue_pos = NOT_QUEUED;
   uint32_t rc = MARKER;
   tomb_slot(s, I, q, p, cursor, &rc);
   if (rc != MARKER) {
@@ -669,7 +673,7 @@ static uint32_t carve(ouro_sim* s, inst_t* I, cls_t* q, uint32_t k) {
       /* ChunkIndexChunk with all pages free (ChunkAccess.cuh:69-89), 64-bit words */
       x->type = 2;
       x->count = (int32_t)q->ppc;
-      x->queue_pos = MARKER;
+      x->queue_pos = NOT_QUEUED;
       for (uint32_t w = 0; w < 8; ++w) {
         const uint32_t lo = w * 64;
         x->mask[w] = q->ppc > lo ? ((q->ppc - lo >= 64) ? ~0ull : ((1ull << (q->ppc - lo)) - 1ull)) : 0ull;
@@ -910,7 +914,7 @@ static void claim_wave(ouro_sim* s, inst_t* I, group_t* g, uint64_t* out) {
       t[i] = 0;
       if (i < nb && chunk_ok(s, sl[i])) {
         const chunk_t* x = ch(s, sl[i]);
-        if (x->count > 0 && x->queue_pos == v + i && x->page_size == q->page_size) cnt[i] = x->count;
+        if (x->count > 0 && x->queue_pos == (uint64_t)(v + i) && x->page_size == q->page_size) cnt[i] = x->count;
       }
       if (cnt[i] > 0 && before < rem) t[i] = (uint32_t)cnt[i] < rem - before ? (uint32_t)cnt[i] : rem - before;
       before += cnt[i] > 0 ? (uint32_t)cnt[i] : 0u;
@@ -927,8 +931,8 @@ static void claim_wave(ouro_sim* s, inst_t* I, group_t* g, uint64_t* out) {
     for (uint32_t i = 0; i < 64; ++i) {
       if (!t[i] || t[i] != (uint32_t)cnt[i]) continue;
       chunk_t* x = ch(s, sl[i]);
-      if (x->queue_pos != v + i) continue;
-      x->queue_pos = MARKER;
+      if (x->queue_pos != (uint64_t)(v + i)) continue;
+      x->queue_pos = NOT_QUEUED;
       uint32_t cur = cursor, rc = MARKER;
       tomb_slot(s, I, q, v + i, &cur, &rc);
       if (rc != MARKER) {
@@ -1186,9 +1190,9 @@ static void inst_free(ouro_sim* s, inst_t* I, const int* lanes, int n, const uin
     chunk_t* x = ch(s, c);
     const int32_t old = x->count;
     x->count += g[j].n;
-    /* count crossed from <= 0 to > 0: the chunk must be in the queue; queue_pos MARKER means it left */
-    if (old <= 0 && old + g[j].n > 0 && x->queue_pos == MARKER) {
-      x->queue_pos = PENDING;
+    /* count crossed from <= 0 to > 0: the chunk must be in the queue; NOT_QUEUED means it left */
+    if (old <= 0 && old + g[j].n > 0 && x->queue_pos == NOT_QUEUED) {
+      x->queue_pos = QUEUE_PENDING;
       ff_chunk[nff] = c;
       ff_q[nff++] = q;
     }
@@ -1348,8 +1352,8 @@ int sim_chunk_info(const ouro_sim* s, uint32_t c, uint32_t* page_size, int32_t* 
   return x->type;
 }
 
-uint32_t sim_chunk_queue_pos(const ouro_sim* s, uint32_t c) {
-  return c < s->max_chunks ? s->chunks[c].queue_pos : MARKER;
+uint64_t sim_chunk_queue_pos(const ouro_sim* s, uint32_t c) {
+  return c < s->max_chunks ? s->chunks[c].queue_pos : NOT_QUEUED;
 }
 
 /* Slot values of class k's queue window [front, back) (k counts the classes of both instances), up to
@@ -1378,11 +1382,11 @@ uint32_t sim_queue_window(const ouro_sim* s, uint32_t k, uint32_t* out, uint32_t
         val = s->chunks[e].slots[v % QC_SPOTS];
     } else {
       uint32_t c = q->front_qc;
-      if (chunk_ok(s, c) && s->chunks[c].vstart / QC_SPOTS - v / QC_SPOTS < 0x80000000u &&
-          s->chunks[c].vstart / QC_SPOTS != v / QC_SPOTS) {
+      const uint32_t vb = v - v % QC_SPOTS;
+      if (chunk_ok(s, c) && s->chunks[c].vstart - vb < 0x80000000u && s->chunks[c].vstart != vb) {
         val = TOMB;
       } else {
-        while (chunk_ok(s, c) && s->chunks[c].vstart / QC_SPOTS != v / QC_SPOTS) c = s->chunks[c].next;
+        while (chunk_ok(s, c) && s->chunks[c].vstart != vb) c = s->chunks[c].next;
         if (chunk_ok(s, c)) val = s->chunks[c].slots[v % QC_SPOTS];
       }
     }
@@ -1392,3 +1396,29 @@ uint32_t sim_queue_window(const ouro_sim* s, uint32_t k, uint32_t* out, uint32_t
 }
 
 void sim_set_block_order(ouro_sim* s, int on) { s->block_order = on; }
+
+/* Start every class queue at position `origin` (a multiple of QC_SPOTS) on a fresh pool: front, back,
+ * the page cursor, and for VA / VL the initial queue chunk's virtual start (VA: its pointer-ring
+ * entry moves to the entry of origin's queue chunk). Positions are 32-bit and wrap; a trace from any
+ * origin hands out the same blocks as from 0. */
+int sim_set_origin(ouro_sim* s, uint32_t origin) {
+  if (origin % QC_SPOTS) return -1;
+  for (int k = 0; k < s->ninst; ++k) {
+    inst_t* I = &s->inst[k];
+    for (uint32_t i = 0; i < I->nq; ++i) {
+      cls_t* q = &I->q[i];
+      if (q->front != 0 || q->back != 0 || q->cnt != 0) return -1;
+      q->front = q->back = origin;
+      q->scan = (uint64_t)origin * q->ppc;
+      if (is_va(I->kind)) {
+        const uint32_t c = q->va[0];
+        q->va[0] = MARKER;
+        q->va[(origin / QC_SPOTS) & (q->ring_size - 1)] = c;
+        ch(s, c)->vstart = origin;
+      } else if (is_vl(I->kind)) {
+        ch(s, q->front_qc)->vstart = origin;
+      }
+    }
+  }
+  return 0;
+}

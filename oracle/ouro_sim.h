@@ -35,6 +35,9 @@ void sim_wave_free(ouro_sim* s, int n, const uint64_t* offsets);
  * (Ouroboros::malloc_tile): n <= 256 requests, request 64 t + l on lane l of sub-tile t; class
  * leaders claim and carve in class order, then the sub-tiles take their pages in order. */
 void sim_set_block_order(ouro_sim* s, int on);
+/* Fresh pool only: every class queue starts at position `origin` (a multiple of 2048) instead of 0,
+ * so a short trace runs its 32-bit positions across the wrap. Returns 0, or -1 if not applicable. */
+int sim_set_origin(ouro_sim* s, uint32_t origin);
 
 typedef struct {
   uint32_t next_free_chunk;
@@ -55,8 +58,9 @@ void sim_get_state(const ouro_sim* s, sim_state* out);
 
 /* Chunk-level views for tests: 0 unused, 1 page chunk, 2 chunk-queue chunk, 3 queue chunk. */
 int sim_chunk_info(const ouro_sim* s, uint32_t chunk, uint32_t* page_size, int32_t* count, uint64_t* mask8);
-/* Chunk queues: the position a chunk is queued at (MARKER when not queued). */
-uint32_t sim_chunk_queue_pos(const ouro_sim* s, uint32_t chunk);
+/* Chunk queues: the position a chunk is queued at (UINT64_MAX when not queued, UINT64_MAX - 1 while
+ * its enqueue is in flight). */
+uint64_t sim_chunk_queue_pos(const ouro_sim* s, uint32_t chunk);
 /* Slot values of class k's live window [front, back) (0xFFFFFFFD = removed entry); returns its length. */
 uint32_t sim_queue_window(const ouro_sim* s, uint32_t k, uint32_t* out, uint32_t max);
 

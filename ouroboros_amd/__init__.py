@@ -275,6 +275,7 @@ SIGNATURES = {
     "ouro_sem_share": (C.c_int, [C.c_uint64, C.c_int64, C.c_int64, C.POINTER(C.c_uint32)]),
     "ouro_sem_make": (C.c_uint64, [C.c_int64, C.c_uint64]),
     "ouro_abi_sizes": (C.c_int, [C.POINTER(C.c_uint32), C.c_uint32]),
+    "ouro_set_origin": (C.c_int, [C.c_void_p, C.c_uint32]),
     "ouro_bench_atomics": (C.c_int, [C.c_uint32, C.c_uint32, C.c_int, C.POINTER(C.c_double)]),
     "ouro_bench_copy": (C.c_int, [C.c_uint64, C.c_uint32, C.POINTER(C.c_double)]),
 }
@@ -406,12 +407,16 @@ class TraceOutput:
 class MemoryManager:
     """One allocator pool on the current HIP device (MultiOuro<variant>)."""
 
-    def __init__(self, variant: str = "PQ", pool_bytes: int = 512 << 20):
+    def __init__(self, variant: str = "PQ", pool_bytes: int = 512 << 20, origin: int = 0):
+        """origin: start every class queue's 32-bit positions there (a multiple of 2048; test hook
+        for the position wrap)."""
         self.variant = variant
         self.pool_bytes = pool_bytes
         h = C.c_void_p()
         _check(lib().ouro_create(VARIANTS[variant], pool_bytes, C.byref(h)), f"ouro_create({variant})")
         self._h = h
+        if origin:
+            _check(lib().ouro_set_origin(self._h, origin), "ouro_set_origin")
 
     # reference host API names
     def getDeviceMemoryManager(self) -> int:  # noqa: N802 - reference name

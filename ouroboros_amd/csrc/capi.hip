@@ -9,6 +9,7 @@
 #include <cmath>
 #include <memory>
 #include <new>
+#include <unordered_map>
 #include <unordered_set>
 #include <vector>
 
@@ -107,44 +108,76 @@ struct DevBuf {
     if (_e != hipSuccess) return OURO_E_HIP; \
   } while (0)
 
-// Host-side check of the live set: alignment to the page grid of the request's class, no overlap,
-// and the fragmentation figures (chunks holding a live block vs. the ideal packing).
-// Live-set invariants: no two live blocks overlap; a block of the first instance starts on its
-// class's page grid inside the chunk (8 KiB instance chunks; other geometries get the 16-byte check).
+// Instances of a variant's chain, from its geometry: the first instance's pages run from the
+// smallest page up to its chunk; a chain whose largest page exceeds that chunk has a second instance
+// whose pages start at twice the first chunk and whose chunk is its largest page (the MultiOuro
+// chains: 16 and 32 KiB pages on 32 KiB chunks of four base chunks, InstanceDefinitions.cuh:17-34).
+struct ChainGeo {
+  uint32_t smallest[2], largest[2], chunk[2], n;
+};
+ChainGeo chain_geo(const ouro_variant_info& g) {
+  ChainGeo c{};
+  c.n = 1;
+  c.smallest[0] = g.smallest_page;
+  c.largest[0] = c.chunk[0] = g.chunk_bytes;
+  if (g.largest_page > g.chunk_bytes) {
+    c.n = 2;
+    c.smallest[1] = 2 * g.chunk_bytes;
+    c.largest[1] = c.chunk[1] = g.largest_page;
+  }
+  return c;
+}
+
+// Run start of a block of page size ps at byte offset `off` from the chunk region, or UINT64_MAX if
+// no run of its instance puts a page there: the block must sit on the page grid of a run of
+// chunk / 8192 base chunks starting at some base chunk c0 (offset = c0 * 8320 + 128 + p * ps with
+// p < chunk / ps), so it never covers the run's header and never reaches past the run's data.
+uint64_t run_start_of(uint64_t off, uint32_t ps, uint32_t chunk) {
+  const uint64_t idx = off / 8320, span = chunk / 8192;
+  for (uint64_t j = 0; j < span && j <= idx; ++j) {
+    const uint64_t c0 = idx - j, in = off - c0 * 8320;
+    if (in >= 128 && (in - 128) % ps == 0 && (in - 128) + ps <= chunk) return c0;
+  }
+  return UINT64_MAX;
+}
+
+// Host-side check of the live set: every block on the page grid of a run of its instance (any
+// geometry), no two live blocks overlapping, and the fragmentation figures (runs holding a live block
+// vs. the ideal packing, per class ceil(live blocks / pages per chunk)).
 void check_live(const std::vector<uint64_t>& offs, const std::vector<uint32_t>& sizes, ouro_trace_result* r,
                 const ouro_variant_info& g) {
   struct Iv {
     uint64_t a, b;
   };
+  const ChainGeo cg = chain_geo(g);
   std::vector<Iv> iv;
   iv.reserve(offs.size());
-  uint64_t live_q[16] = {0};
-  std::unordered_set<uint64_t> chunks;
+  std::unordered_map<uint32_t, uint64_t> live_ps;  // page size -> live blocks
+  std::unordered_set<uint64_t> runs;
   for (size_t i = 0; i < offs.size(); ++i) {
     if (offs[i] == UINT64_MAX) continue;
     const uint32_t ps = page_of(sizes[i], g.smallest_page);
     iv.push_back({offs[i], offs[i] + ps});
-    if (ps <= 8192 && g.chunk_bytes == 8192) {
-      const uint64_t c = offs[i] / 8320, in = offs[i] - c * 8320;
-      if (in < 128 || (in - 128) % ps != 0 || in - 128 + ps > 8192) r->align_errors++;
-      chunks.insert(c);
-      uint32_t q = 0;
-      while ((g.smallest_page << q) < ps) ++q;
-      live_q[q]++;
-    } else if ((offs[i] & 15u) != 0) {
+    const uint32_t k = (cg.n == 2 && ps > cg.largest[0]) ? 1u : 0u;
+    const uint64_t c0 = ps <= cg.largest[k] ? run_start_of(offs[i], ps, cg.chunk[k]) : UINT64_MAX;
+    if (c0 == UINT64_MAX) {
       r->align_errors++;
+      continue;
     }
+    runs.insert(c0);
+    live_ps[ps]++;
   }
   std::sort(iv.begin(), iv.end(), [](const Iv& x, const Iv& y) { return x.a < y.a; });
   for (size_t i = 1; i < iv.size(); ++i)
     if (iv[i].a < iv[i - 1].b) r->overlap_errors++;
   uint64_t ideal = 0;
-  for (uint32_t q = 0; q < 10; ++q) {
-    const uint32_t ppc = std::max<uint32_t>(1u, (g.chunk_bytes / g.smallest_page) >> q);
-    ideal += (live_q[q] + ppc - 1) / ppc;
+  for (const auto& [ps, live] : live_ps) {
+    const uint32_t k = (cg.n == 2 && ps > cg.largest[0]) ? 1u : 0u;
+    const uint64_t ppc = std::max<uint64_t>(1, cg.chunk[k] / ps);
+    ideal += (live + ppc - 1) / ppc;
   }
-  if (chunks.size() > r->peak_live_chunks) {
-    r->peak_live_chunks = chunks.size();
+  if (runs.size() > r->peak_live_chunks) {
+    r->peak_live_chunks = runs.size();
     r->ideal_chunks = ideal;
   }
 }
@@ -304,6 +337,12 @@ int ouro_snapshot(ouro_handle h, void* dst) {
   return hipMemcpy(dst, h->impl->device_manager(), h->impl->allocation_size(), hipMemcpyDefault) == hipSuccess
              ? OURO_OK
              : OURO_E_HIP;
+}
+
+int ouro_set_origin(ouro_handle h, uint32_t origin) {
+  if (!h || origin % ouro::kQCSpots) return OURO_E_INVALID;
+  const hipError_t e = h->impl->set_origin(origin);
+  return e == hipSuccess ? OURO_OK : (e == hipErrorInvalidValue ? OURO_E_INVALID : OURO_E_HIP);
 }
 
 int ouro_variant_geometry(int variant, ouro_variant_info* out) {
@@ -778,8 +817,8 @@ int ouro_check_quiescent(ouro_handle h, ouro_check_result* out) {
       }
       class_pages[k] += ch->count;
       // every page is free again, so the chunk must sit in its class queue inside [front_, back_)
-      if (ch->queue_pos - st.front[k] >= st.back[k] - st.front[k])
-        fail(k, c, "chunk with free pages is not inside its queue window", ch->queue_pos, st.front[k]);
+      if (ch->queue_pos > 0xFFFFFFFFull || (uint32_t)ch->queue_pos - st.front[k] >= st.back[k] - st.front[k])
+        fail(k, c, "chunk with free pages is not inside its queue window", (long long)ch->queue_pos, st.front[k]);
     } else {
       const ouro::PageChunkHdr* ph = reinterpret_cast<const ouro::PageChunkHdr*>(w);
       if (ph->number_pages != ppc) fail(k, c, "page chunk page count differs", ph->number_pages, ppc);
@@ -830,7 +869,8 @@ int ouro_check_quiescent(ouro_handle h, ouro_check_result* out) {
         if (!mem.count(c)) fail(k, c, "queue entry names a chunk outside the class", c, 0);
         if (kind == OURO_CQ && c < nfc) {
           const ouro::ChunkHdr* ch = reinterpret_cast<const ouro::ChunkHdr*>(hdrs.data() + (size_t)c * 128);
-          if (ch->queue_pos != st.front[k] + i) fail(k, c, "chunk queue_pos differs from its queue position", ch->queue_pos, st.front[k] + i);
+          if (ch->queue_pos != (uint64_t)(uint32_t)(st.front[k] + i))
+            fail(k, c, "chunk queue_pos differs from its queue position", (long long)ch->queue_pos, st.front[k] + i);
         }
         if (kind == OURO_PQ && pg >= st.pages_per_chunk[k]) fail(k, c, "queue entry page out of range", pg, st.pages_per_chunk[k]);
         if (!seen.insert(v).second) fail(k, c, "duplicate queue entry", v, 0);

@@ -69,6 +69,8 @@ struct IManager {
   virtual ouro::PoolLayout pool_layout(uint64_t pool_bytes) = 0;
   // immutable queue placement of class k (chain order)
   virtual void class_ring(uint32_t k, uint32_t** ring, uint32_t* ring_size) = 0;
+  // fresh pool only: every class queue starts at position `origin` (a multiple of 2048)
+  virtual hipError_t set_origin(uint32_t origin) = 0;
 #ifdef OURO_DIAG
   virtual hipError_t diag_read(unsigned long long* out16, int reset) = 0;
 #endif
@@ -554,6 +556,49 @@ struct Manager final : IManager {
     *ring = nullptr;
     *ring_size = 0;
     ring_of<0>(mm.chain, k, ring, ring_size);
+  }
+
+  // Moves the fresh class queues of a host copy of the manager state to `origin`: front, back and
+  // the page cursor, and the initial queue chunk of a VA / VL queue (its virtual start; VA: its
+  // pointer-ring entry moves to the entry of origin's queue chunk).
+  template <int N, class... Is>
+  static hipError_t origin_of(ouro::Chain<>&, const ouro::Memory&, uint32_t) { return hipSuccess; }
+  template <int N, class I, class... Rest>
+  static hipError_t origin_of(ouro::Chain<I, Rest...>& ch, const ouro::Memory& mem, uint32_t origin) {
+    using namespace ouro;
+    for (uint32_t i = 0; i < I::kNQ; ++i) {
+      ClassQueue& q = ch.inst.q[i];
+      if (q.front != 0 || q.back != 0 || q.sem != Sem::kInit) return hipErrorInvalidValue;
+      q.front = q.back = origin;
+      q.scan = (uint64_t)origin * q.ppc;
+      uint32_t c = kMarker;
+      hipError_t e = hipSuccess;
+      if (is_va(I::kKind)) {
+        const uint32_t id = (origin / kQCSpots) & (I::kQueueEntries - 1), marker = kMarker;
+        e = hipMemcpy(&c, q.ring, 4, hipMemcpyDeviceToHost);
+        if (e == hipSuccess) e = hipMemcpy(q.ring, &marker, 4, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(q.ring + id, &c, 4, hipMemcpyHostToDevice);
+      } else if (is_vl(I::kKind)) {
+        c = q.front_qc;
+      }
+      if (e == hipSuccess && c != kMarker) {
+        if (c >= mem.max_chunks) return hipErrorInvalidValue;
+        e = hipMemcpy(mem.d_data + (uint64_t)c * kStride + offsetof(QueueChunkHdr, virtual_start), &origin, 4,
+                      hipMemcpyHostToDevice);
+      }
+      if (e != hipSuccess) return e;
+    }
+    return origin_of<N + 1, Rest...>(ch.next, mem, origin);
+  }
+  hipError_t set_origin(uint32_t origin) override {
+    if (origin % ouro::kQCSpots) return hipErrorInvalidValue;
+    std::vector<uint8_t> buf(sizeof(M));
+    hipError_t e = hipMemcpy(buf.data(), mm.memory.d_memory, sizeof(M), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return e;
+    M* h = reinterpret_cast<M*>(buf.data());
+    e = origin_of<0>(h->chain, h->memory, origin);
+    if (e == hipSuccess) e = hipMemcpy(mm.memory.d_memory, buf.data(), sizeof(M), hipMemcpyHostToDevice);
+    return e;
   }
 
 #ifdef OURO_DIAG

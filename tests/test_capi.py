@@ -123,3 +123,14 @@ def test_serve_expected_restatement():
                 q += 1
             words += (16 << q) // 4
         assert ob.serve_expected(cfg, r) == (n, words)
+
+
+def test_pools_beyond_the_chunk_index_range_are_rejected():
+    # MemoryIndex keeps 23 chunk bits: a pool of more than 2^23 - 1 chunks would alias page handles
+    # (and chunk 2^23 - 1 would encode page handles equal to the queue sentinels), so ouro_create
+    # refuses it before touching the device; 64 GiB (8,259,489 VA chunks) stays inside the range
+    assert ob.layout("VAPQ", 64 << 30)[0] < (1 << 23) - 1
+    assert ob.layout("PQ", 72 << 30)[0] > (1 << 23) - 1
+    h = ob.C.c_void_p()
+    assert ob.lib().ouro_create(ob.VARIANTS["PQ"], 72 << 30, ob.C.byref(h)) == 1  # OURO_E_INVALID
+    assert not h.value

@@ -172,3 +172,54 @@ def test_out_of_memory_block_wide(kind, tile):
         assert 0 < served <= 2 * (mc - 1)
         chk = mm.check_quiescent()
         assert chk.violations == 1 and "error word" in chk.message.decode(), chk.message.decode()
+
+
+WRAP = [
+    # (name, sizes, rounds, origin): the class queues' 32-bit positions cross 2^32 during the trace
+    ("class9", lambda: np.full(64, 8192, np.uint32), 40, (1 << 32) - 2048),
+    ("uniform", lambda: (16 << np.random.default_rng(42).integers(0, 10, size=(400, 64))).astype(np.uint32), 400,
+     (1 << 32) - 2048),
+    ("with-instance2", lambda: (16 << (np.arange(64) % 12)).astype(np.uint32), 500, (1 << 32) - 2048),
+]
+
+
+@pytest.mark.parametrize("case", WRAP, ids=[c[0] for c in WRAP])
+@pytest.mark.parametrize("kind,agg", [(k, 64) for k in ALL] + [("VLPQ", 1)])
+def test_position_wrap_single_wave(kind, agg, case):
+    # started just below 2^32 (ouro_set_origin), the device must hand out exactly the blocks the
+    # oracle hands out from position 0: linked queues locate queue chunks across the wrap, chunk
+    # queues take positions 0xFFFFFFFE / 0xFFFFFFFF without confusing them with queue_pos sentinels
+    name, mk, rounds, origin = case
+    sizes = mk()
+    print(f"wrap {kind} agg={agg} {name} origin={origin:#x}", flush=True)
+    with ob.MemoryManager(kind, 512 << 20, origin=origin) as mm:
+        out = mm.run_trace(sizes, rounds, agg=agg, keep_offsets=True, check_rounds=True)
+        st = mm.query()
+        chk = mm.check_quiescent()
+    assert_clean(out.result)
+    want = oracle.Sim(kind, 512 << 20).trace(sizes, rounds)
+    mism = np.argwhere(out.offsets != want)
+    assert len(mism) == 0, f"first mismatch at (round, lane) {mism[0]}"
+    assert any(st.back[k] < origin for k in range(st.num_classes)), "no class crossed the wrap"
+    assert chk.violations == 0, chk.message.decode()
+
+
+@pytest.mark.parametrize("kind", ALL)
+@pytest.mark.parametrize("form", ["agg1", "agg64", "block_t4"])
+def test_position_wrap_concurrent(kind, form):
+    # 65,536 mixed requests per round from positions just below 2^32: every class crosses the wrap in
+    # the first rounds, under full concurrency, with clean invariants and a quiescent pool afterwards
+    print(f"wrap concurrent {kind} {form}", flush=True)
+    sizes = ob.gen_sizes(42, 65536, "uniform")
+    origin = (1 << 32) - 4096
+    with ob.MemoryManager(kind, 1 << 30, origin=origin) as mm:
+        if form == "block_t4":
+            r = mm.run_batch(sizes, rounds=3, warmup=1, agg=0, block=1024, graph=False, check=True, tile=4)
+        else:
+            r = mm.run_trace(sizes, 4, agg=int(form[3:]), check_rounds=True).result
+        assert_clean(r)
+        assert r["null_returns"] == 0
+        st = mm.query()
+        assert all(st.back[k] < origin for k in range(10)), "a class did not cross the wrap"
+        chk = mm.check_quiescent()
+        assert chk.violations == 0, chk.message.decode()

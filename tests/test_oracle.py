@@ -165,8 +165,40 @@ def test_va_first_carve_preallocates_queue_chunk_13():
     assert t == 3 and ps == 0
 
 
+# MultiOuro chains: instance 1 pages 16 B .. 8 KiB on 8 KiB chunks, instance 2 pages 16 and 32 KiB on
+# 32 KiB chunks spanning four base chunks (InstanceDefinitions.cuh:17-34)
+CHAIN = [(16, 8192, 8192), (16384, 32768, 32768)]  # (smallest page, largest page, chunk bytes)
+
+
+def run_start(off, ps, chunk):
+    """Base chunk c0 whose run puts a page of size ps at byte offset `off` (offset = c0 * 8320 + 128 +
+    p * ps, p < chunk / ps), or None: a block never covers its run's header or reaches past its data."""
+    idx = off // STRIDE
+    for j in range(chunk // 8192):
+        c0 = idx - j
+        if c0 < 0:
+            break
+        inside = off - c0 * STRIDE
+        if inside >= 128 and (inside - 128) % ps == 0 and inside - 128 + ps <= chunk:
+            return c0
+    return None
+
+
+def test_page_grid_check_rejects_misplaced_blocks():
+    # the checker itself: blocks on a run's page grid pass; a block over a chunk header, off the grid,
+    # or reaching past a run's data does not
+    assert run_start(5 * STRIDE + 128 + 3 * 64, 64, 8192) == 5
+    assert run_start(5 * STRIDE + 64, 64, 8192) is None            # inside the header
+    assert run_start(5 * STRIDE + 128 + 8, 64, 8192) is None        # off the grid
+    assert run_start(5 * STRIDE + 128 + 8192, 8192, 8192) is None   # the next chunk's header
+    # 32 KiB runs of four base chunks: page 1 of a 16 KiB class lies in the run's third base chunk
+    assert run_start(7 * STRIDE + 128 + 16384, 16384, 32768) == 7
+    assert run_start(7 * STRIDE + 128 + 32768, 16384, 32768) is None  # past the run's data
+    assert run_start(7 * STRIDE + 128 + 8192, 16384, 32768) is None   # not on the 16 KiB grid
+
+
 def _offset_checks(offs, sizes):
-    """alignment to the page grid and no overlap among live blocks"""
+    """every block on the page grid of a run of its instance, and no overlap among live blocks"""
     iv = []
     for o, sz in zip(offs, sizes):
         if o == NONE:
@@ -175,9 +207,8 @@ def _offset_checks(offs, sizes):
         while ps < sz:
             ps <<= 1
         o = int(o)
-        if ps <= 8192:
-            c, inside = divmod(o, STRIDE)
-            assert inside >= 128 and (inside - 128) % ps == 0 and inside - 128 + ps <= 8192
+        inst = [c for c in CHAIN if c[0] <= ps <= c[1]]
+        assert inst and run_start(o, ps, inst[0][2]) is not None, (o, ps)
         iv.append((o, o + ps))
     iv.sort()
     for a, b in zip(iv, iv[1:]):
@@ -361,3 +392,44 @@ def test_block_order_replay_invariants(kind, tile):
         assert st.sem_expected[k] == 0
     if kind in ("CQ", "VACQ", "VLCQ"):
         _check_chunk_queues(s, st)
+
+
+WRAP_CASES = [
+    # (name, sizes, rounds, origin): every case runs its classes' 32-bit positions across 2^32
+    ("class0", np.full(64, 16, np.uint32), 140, (1 << 32) - 8192),
+    ("class9", np.full(64, 8192, np.uint32), 40, (1 << 32) - 2048),
+    ("uniform", None, 400, (1 << 32) - 2048),
+    ("with-instance2", (16 << (np.arange(64) % 12)).astype(np.uint32), 500, (1 << 32) - 2048),
+]
+
+
+def _wrap_sizes(case):
+    name, sizes, rounds, origin = case
+    if sizes is None:
+        rng = np.random.default_rng(42)
+        sizes = (16 << rng.integers(0, 10, size=(rounds, 64))).astype(np.uint32)
+    return sizes
+
+
+@pytest.mark.parametrize("case", WRAP_CASES, ids=[c[0] for c in WRAP_CASES])
+@pytest.mark.parametrize("kind", KINDS)
+def test_position_wrap_is_invisible(kind, case):
+    # queue positions are 32-bit and wrap: started just below 2^32, every class queue crosses the
+    # wrap within a few rounds (chunk queues through the positions 0xFFFFFFFE and 0xFFFFFFFF, which a
+    # 32-bit queue_pos would confuse with its sentinels; linked queues through a queue chunk whose
+    # virtual start is 0), and the trace must hand out exactly the blocks it hands out from 0
+    name, _, rounds, origin = case
+    sizes = _wrap_sizes(case)
+    a = oracle.Sim(kind, 512 << 20)
+    b = oracle.Sim(kind, 512 << 20, origin=origin)
+    ta, tb = a.trace(sizes, rounds), b.trace(sizes, rounds)
+    sa, sb = a.state(), b.state()
+    assert sa.error == 0 and sb.error == 0, (sa.error, sb.error)
+    assert np.array_equal(ta, tb)
+    if not (name == "class0" and kind in ("CQ", "VACQ", "VLCQ")):  # one chunk stays queued there
+        assert any(sb.back[k] < origin for k in range(sb.num_classes)), "no class crossed the wrap"
+    for k in range(sa.num_classes):
+        assert (sb.back[k] - sb.front[k]) % (1 << 32) == (sa.back[k] - sa.front[k]) % (1 << 32)
+        assert sa.sem_count[k] == sb.sem_count[k] and sa.carved[k] == sb.carved[k]
+    if kind in ("CQ", "VACQ", "VLCQ"):
+        _check_chunk_queues(b, sb)
