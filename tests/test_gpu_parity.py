@@ -220,6 +220,9 @@ def test_position_wrap_concurrent(kind, form):
         assert_clean(r)
         assert r["null_returns"] == 0
         st = mm.query()
-        assert all(st.back[k] < origin for k in range(10)), "a class did not cross the wrap"
+        # page queues move every position through the queue; chunk queues only move chunks that empty,
+        # so there the classes of at most four pages per chunk are the ones that cross
+        crossing = range(10) if kind in PAGE_KINDS else range(7, 10)
+        assert all(st.back[k] < origin for k in crossing), "a class did not cross the wrap"
         chk = mm.check_quiescent()
         assert chk.violations == 0, chk.message.decode()
