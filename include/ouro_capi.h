@@ -236,6 +236,17 @@ typedef struct {
 
 int ouro_run_batch(ouro_handle h, const ouro_batch_config* cfg, ouro_batch_result* out);
 
+/* ouro_run_batch without any allocation (no reference counterpart): for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The same rounds as ouro_run_batch -- the observed round (cfg->check; its blocks in
+ * cfg->offsets_out), cfg->warmup rounds, then cfg->rounds rounds whose verify / bounds / null counts
+ * are reported -- on the caller's stream (NULL: the null stream), with the request sizes, block
+ * pointers and error counters staged in a static device scratch of 12 MiB per GPU instead of
+ * buffers the call allocates. At most 2^20 requests (OURO_E_INVALID beyond); graph, latency and
+ * kernel_timing must be 0; no timings and no host overlap / alignment check (those fields stay 0).
+ * Every call on one GPU shares that GPU's scratch: calls that overlap on one GPU race. */
+int ouro_run_batch_static(ouro_handle h, const ouro_batch_config* cfg, void* stream, ouro_batch_result* out);
+
 /* Host trace generator (SplitMix64): dist 0 = fixed size `param`, 1 = uniform over the class
  * sizes 16 << q (q < classes), 2 = Zipf(param) over class ranks. */
 int ouro_gen_sizes(uint64_t seed, uint32_t n, int dist, double param, uint32_t classes, uint32_t* out);

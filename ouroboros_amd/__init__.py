@@ -260,6 +260,7 @@ SIGNATURES = {
     "ouro_gen_batch": (C.c_int, [C.POINTER(BatchWorkload), C.POINTER(C.c_uint32), C.c_uint64,
                                  C.POINTER(C.c_uint64), C.POINTER(C.c_uint32), C.POINTER(C.c_double)]),
     "ouro_run_batch": (C.c_int, [C.c_void_p, C.POINTER(BatchConfig), C.POINTER(BatchResult)]),
+    "ouro_run_batch_static": (C.c_int, [C.c_void_p, C.POINTER(BatchConfig), C.c_void_p, C.POINTER(BatchResult)]),
     "ouro_create": (C.c_int, [C.c_int, C.c_uint64, C.POINTER(C.c_void_p)]),
     "ouro_destroy": (C.c_int, [C.c_void_p]),
     "ouro_device_manager": (C.c_void_p, [C.c_void_p]),
@@ -498,6 +499,25 @@ class MemoryManager:
             cfg.offsets_out = offsets.ctypes.data_as(C.POINTER(C.c_uint64))
         res = BatchResult()
         _check(lib().ouro_run_batch(self._h, C.byref(cfg), C.byref(res)), "ouro_run_batch")
+        return res.as_dict()
+
+    def run_batch_static(self, sizes: np.ndarray, rounds: int, warmup: int = 0, agg: int = 0, block: int = 256,
+                         grid: int = 0, check: bool = False, offsets: np.ndarray | None = None, tile: int = 1,
+                         stream: int = 0) -> dict:
+        """run_batch without allocating: request sizes, block pointers and counters live in the
+        library's static device scratch (at most 2^20 requests), kernels run on `stream`; no timings,
+        no graph, no host overlap check."""
+        sizes = np.ascontiguousarray(sizes, dtype=np.uint32)
+        cfg = BatchConfig()
+        cfg.requests, cfg.rounds, cfg.warmup, cfg.agg, cfg.block, cfg.grid = len(sizes), rounds, warmup, agg, block, grid
+        cfg.check = int(check or offsets is not None)
+        cfg.tile = tile
+        cfg.sizes = sizes.ctypes.data_as(C.POINTER(C.c_uint32))
+        if offsets is not None:
+            assert offsets.dtype == np.uint64 and offsets.shape == sizes.shape and offsets.flags.c_contiguous
+            cfg.offsets_out = offsets.ctypes.data_as(C.POINTER(C.c_uint64))
+        res = BatchResult()
+        _check(lib().ouro_run_batch_static(self._h, C.byref(cfg), stream, C.byref(res)), "ouro_run_batch_static")
         return res.as_dict()
 
     def serve(

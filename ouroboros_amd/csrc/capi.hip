@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <cmath>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <unordered_map>
 #include <unordered_set>
@@ -198,6 +199,49 @@ __global__ void __launch_bounds__(256) k_atomics(uint32_t* buf, uint32_t iters, 
 __global__ void __launch_bounds__(256) k_copy(const uint4* __restrict__ src, uint4* __restrict__ dst, uint64_t n) {
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
     dst[i] = src[i];
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ allocation-free batch rounds
+// Device scratch of ouro_run_batch_static: 256 bytes of error counters, then the request sizes
+// (u32) and the block pointers (u64) of up to kStaticRequests requests, 12 MiB in all. Every GPU the
+// library runs on holds its own copy until the process ends, and every call on a GPU shares it:
+// calls that overlap on one GPU race. Host code never passes `scratch` itself to a kernel or copy
+// (in host code the name stands for a host-side shadow, not the device array): it uses the device
+// address hipGetSymbolAddress returns, looked up once per device.
+constexpr uint32_t kStaticRequests = 1u << 20;
+constexpr size_t kStaticCounterBytes = 256;
+constexpr size_t kStaticBytes = kStaticCounterBytes + (size_t)kStaticRequests * 12;
+static __device__ __align__(256) uint8_t scratch[kStaticBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+uint8_t* g_scratch_dev[kMaxDevices] = {};
+
+// The current device's copy of `scratch` (its device address), looked up on the first call there.
+hipError_t static_scratch(uint8_t** out) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_dev[dev] = reinterpret_cast<uint8_t*>(p);
+  }
+  *out = g_scratch_dev[dev];
+  return hipSuccess;
+}
+
+// Clears the counters with a kernel (a fill kernel rather than a memset node, so the launch sequence
+// stays valid inside a caller's graph capture).
+__global__ void k_zero_counters(unsigned long long* c, uint32_t n) {
+  if (threadIdx.x < n) c[threadIdx.x] = 0ull;
 }
 
 }  // namespace
@@ -533,6 +577,87 @@ int ouro_run_batch(ouro_handle h, const ouro_batch_config* cfg, ouro_batch_resul
   out->peak_chunks = st.next_free_chunk;
   for (auto& e : ev) (void)hipEventDestroy(e);
   (void)hipStreamDestroy(s);
+  return OURO_OK;
+}
+
+int ouro_run_batch_static(ouro_handle h, const ouro_batch_config* cfg, void* stream, ouro_batch_result* out) {
+  if (!h || !cfg || !out || !cfg->sizes) return OURO_E_INVALID;
+  if (cfg->agg != 0 && cfg->agg != 1 && cfg->agg != 16 && cfg->agg != 64) return OURO_E_INVALID;
+  // graph replay, latency samples and kernel-only timing need events or buffers of their own
+  if (cfg->graph || cfg->latency || cfg->kernel_timing) return OURO_E_INVALID;
+  if (cfg->requests > kStaticRequests) return OURO_E_INVALID;  // beyond the static scratch
+  const uint32_t block = cfg->block ? cfg->block : 256;
+  if (block % 64 != 0 || block > 1024) return OURO_E_INVALID;
+  const uint32_t tile = cfg->tile ? cfg->tile : 1;
+  if (tile != 1 && tile != 2 && tile != 4) return OURO_E_INVALID;
+  if (tile != 1 && cfg->agg != 0) return OURO_E_INVALID;
+  memset(out, 0, sizeof(*out));
+  const uint32_t n = cfg->requests;
+  if (n == 0 || (cfg->rounds == 0 && !cfg->check)) return OURO_OK;
+  IManager& m = *h->impl;
+  uint8_t* sc = nullptr;
+  OURO_TRY(static_scratch(&sc));
+  unsigned long long* errs = reinterpret_cast<unsigned long long*>(sc);
+  uint32_t* sizes = reinterpret_cast<uint32_t*>(sc + kStaticCounterBytes);
+  void** ptrs = reinterpret_cast<void**>(sc + kStaticCounterBytes + (size_t)kStaticRequests * 4);
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+
+  int cus = 0;
+  OURO_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, 0));
+  uint32_t grid = (n + block - 1) / block;
+  if (cfg->agg == 0) {
+    grid = (n + block * tile - 1) / (block * tile);
+    const int occ = m.persist_occupancy(block, tile);
+    if (occ <= 0) return OURO_E_INVALID;
+    const uint32_t cap = (uint32_t)occ * (uint32_t)cus;
+    grid = std::min<uint32_t>(grid, cfg->grid ? std::min<uint32_t>(cfg->grid, cap) : cap);
+  }
+  out->grid = grid;
+  const uint8_t* lo = reinterpret_cast<const uint8_t*>(m.data_base());
+  const uint8_t* hi = lo + m.max_chunks() * 8320ull;
+  const uint32_t tgrid = (n + 255) / 256;
+  auto round = [&]() -> hipError_t {
+    hipError_t e = cfg->agg == 0 ? m.malloc_persist(sizes, ptrs, n, s, block, grid, tile)
+                                 : m.malloc_batch(sizes, ptrs, n, cfg->agg, s, nullptr, block);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_touch, dim3(tgrid), dim3(256), 0, s, ptrs, sizes, n, lo, hi, 0, errs);
+    hipLaunchKernelGGL(k_touch, dim3(tgrid), dim3(256), 0, s, ptrs, sizes, n, lo, hi, 1, errs);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    return cfg->agg == 0 ? m.free_persist(ptrs, n, s, block, grid, tile) : m.free_batch(ptrs, n, cfg->agg, s, block);
+  };
+  OURO_TRY(hipMemcpyAsync(sizes, cfg->sizes, (size_t)n * 4, hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, s, errs, 4u);
+  OURO_TRY(hipGetLastError());
+  if (cfg->check) {
+    // the observed round: its blocks (the pointer array keeps them after the free) as offsets
+    OURO_TRY(round());
+    if (cfg->offsets_out) {
+      OURO_TRY(hipMemcpyAsync(cfg->offsets_out, ptrs, (size_t)n * 8, hipMemcpyDeviceToHost, s));
+      OURO_TRY(hipStreamSynchronize(s));
+      for (uint32_t i = 0; i < n; ++i) {
+        const uint64_t p = cfg->offsets_out[i];
+        cfg->offsets_out[i] = p ? p - reinterpret_cast<uint64_t>(lo) : UINT64_MAX;
+      }
+    }
+  }
+  for (uint32_t w = 0; w < cfg->warmup; ++w) OURO_TRY(round());
+  // counters cover the rounds after the observed and warm-up ones, as in ouro_run_batch
+  hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, s, errs, 4u);
+  OURO_TRY(hipGetLastError());
+  for (uint32_t r = 0; r < cfg->rounds; ++r) OURO_TRY(round());
+  unsigned long long e3[3] = {0, 0, 0};
+  uint32_t words[2] = {0, 0};
+  OURO_TRY(hipMemcpyAsync(e3, errs, 24, hipMemcpyDeviceToHost, s));
+  OURO_TRY(hipMemcpyAsync(&words[0], m.device_error_word(), 4, hipMemcpyDeviceToHost, s));
+  OURO_TRY(hipMemcpyAsync(&words[1], m.device_next_free_chunk(), 4, hipMemcpyDeviceToHost, s));
+  OURO_TRY(hipStreamSynchronize(s));
+  out->verify_errors = e3[0];
+  out->bounds_errors = e3[1];
+  out->null_returns = e3[2];
+  out->mallocs = (uint64_t)n * cfg->rounds;
+  for (uint32_t i = 0; i < n; ++i) out->touch_bytes += 2ull * (cfg->sizes[i] & ~3u);
+  out->error_word = words[0];
+  out->peak_chunks = words[1];
   return OURO_OK;
 }
 

@@ -71,6 +71,9 @@ struct IManager {
   virtual void class_ring(uint32_t k, uint32_t** ring, uint32_t* ring_size) = 0;
   // fresh pool only: every class queue starts at position `origin` (a multiple of 2048)
   virtual hipError_t set_origin(uint32_t origin) = 0;
+  // device addresses of the sticky error word and next_free_chunk inside the pool's manager state
+  virtual uint32_t* device_error_word() = 0;
+  virtual uint32_t* device_next_free_chunk() = 0;
 #ifdef OURO_DIAG
   virtual hipError_t diag_read(unsigned long long* out16, int reset) = 0;
 #endif
@@ -590,6 +593,12 @@ struct Manager final : IManager {
     }
     return origin_of<N + 1, Rest...>(ch.next, mem, origin);
   }
+  uint32_t* device_field(const void* host_field) {
+    const size_t off = (size_t)(reinterpret_cast<const uint8_t*>(host_field) - reinterpret_cast<const uint8_t*>(&mm));
+    return reinterpret_cast<uint32_t*>(mm.memory.d_memory + off);
+  }
+  uint32_t* device_error_word() override { return device_field(&mm.memory.error); }
+  uint32_t* device_next_free_chunk() override { return device_field(&mm.memory.next_free_chunk); }
   hipError_t set_origin(uint32_t origin) override {
     if (origin % ouro::kQCSpots) return hipErrorInvalidValue;
     std::vector<uint8_t> buf(sizeof(M));
