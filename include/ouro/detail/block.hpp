@@ -214,19 +214,21 @@ __device__ inline void block_claim_chunks(Memory& m, I& inst, ReuseRing& base, u
     sem_claim_n(x, cq, l == 0, total, carve);
     const uint32_t psize = page_size_of(x, cq);
     uint32_t v = 0;
-    if (l == 0) {
-      const uint64_t p = add(&cq->scan, (uint64_t)total);
-      const uint32_t h = (uint32_t)(p >> __builtin_ctz(ppc_of(x, cq)));
-      const uint32_t f = ld(&cq->front), b = ld(&cq->back);
-      const uint32_t len = b - f;
-      v = (len == 0 || len >= 0x80000000u) ? f : (h - f < len ? h : f + (h - f) % len);
-    }
+    if (l == 0) v = draw_start(x, cq, total);
     v = uniform(v);
-    uint32_t assigned = 0, wraps = 0, cursor = kMarker, reclaim = kMarker;
+    uint32_t assigned = 0, wraps = 0, cursor = kMarker, reclaim = kMarker, miss = 0, redraws = 0;
     Spin it(&m.error);
     while (assigned < total) {
       uint32_t b = 0, nb = 0;
       bool give_up = false;
+      if (miss >= kRedrawAfter && redraws < kMaxRedraws) {
+        // positions passed without a page since the last start: draw a new one (draw_start)
+        if (l == 0) v = draw_start(x, cq, total - assigned);
+        v = uniform(v);
+        cursor = kMarker;
+        miss = 0;
+        ++redraws;
+      }
       if (l == 0) {
         b = ld(&cq->back);
         wraps += scan_wrap<I>(cq, v, cursor, b);
@@ -260,13 +262,24 @@ __device__ inline void block_claim_chunks(Memory& m, I& inst, ReuseRing& base, u
         slot = &ring_of(x, cq)[(v + l) & (I::kQueueEntries - 1)];
         if (l < nb) s = ld(slot);
       } else {
-        uint32_t fill = kMarker;
+        uint32_t fill = kMarker, dead = 0;
         const uint32_t* sl = nullptr;
-        if (l == 0) sl = slots_at(x, cq, v, cursor, fill, &qc);
+        if (l == 0) sl = slots_at(x, cq, v, cursor, fill, &qc, &dead);
         sl = reinterpret_cast<const uint32_t*>(readlane64(reinterpret_cast<uint64_t>(sl), 0));
         cursor = uniform(cursor);
         fill = uniform(fill);
         qc = uniform(qc);
+        dead = uniform(dead);
+        if (dead) {
+          // the queue chunk holds no live entry: pass the rest of it in one step
+          v += dead;
+          miss += dead;
+          if (ballot(it.expired())) {
+            OURO_TIMEOUT(m);
+            break;
+          }
+          continue;
+        }
         slot = const_cast<uint32_t*>(sl) + l;
         if (l < nb) s = sl ? ld(&sl[l]) : fill;
       }
@@ -350,6 +363,7 @@ __device__ inline void block_claim_chunks(Memory& m, I& inst, ReuseRing& base, u
       }
       assigned += took;
       v += nb;
+      miss = took ? 0u : miss + nb;
       if (assigned < total && ballot(it.expired())) {
         OURO_TIMEOUT(m);
         break;

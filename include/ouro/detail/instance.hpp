@@ -1004,19 +1004,32 @@ __device__ inline uint32_t carve_chunks(Ctx<I>& x, ClassQueue* cq, uint32_t k) {
 // Slots of the queue chunk holding virtual position v (VA / VL), from v on: nullptr when there is
 // none to read, with `fill` what its slots read as (a reclaimed or bypassed queue chunk reads as
 // tombstones, an unpublished one as markers). VL walks from `cursor` (front_ptr_ if unset) and
-// leaves it at the queue chunk found.
+// leaves it at the queue chunk found. `dead` (optional) receives the number of positions from v to
+// the end of its queue chunk when that queue chunk holds no live entry (reclaimed, bypassed, or
+// every entry written so far already removed: its count's upper half, enqueued minus dequeued, is
+// zero), else 0: a scan passes such a stretch in one step instead of slot by slot. (A put whose
+// slot is still being written may land behind the scan; the scan finds it on its next pass.)
 template <class I>
 __device__ inline const uint32_t* slots_at(Ctx<I>& x, ClassQueue* cq, uint32_t v, uint32_t& cursor, uint32_t& fill,
-                                          uint32_t* qc_out = nullptr) {
+                                          uint32_t* qc_out = nullptr, uint32_t* dead = nullptr) {
   constexpr Kind K = I::kKind;
   static_assert(!is_std(K), "standard queues index their ring directly");
   fill = kMarker;
+  const uint32_t rest = kQCSpots - v % kQCSpots;
+  if (dead) *dead = 0;
   if constexpr (is_va(K)) {
     const uint32_t e = ld(&ring_of(x, cq)[(v / kQCSpots) & (I::kQueueEntries - 1)]);
-    if (e == qc_tag(v / kQCSpots)) fill = kTomb;
-    else if (chunk_ok(x.m, e) && ld(&hdr<QueueChunkHdr>(x.m, e)->virtual_start) / kQCSpots == v / kQCSpots) {
-      if (qc_out) *qc_out = e;
-      return qc_slots(x.m, e) + v % kQCSpots;
+    if (e == qc_tag(v / kQCSpots)) {
+      fill = kTomb;
+      if (dead) *dead = rest;
+    } else if (chunk_ok(x.m, e)) {
+      const QueueChunkHdr* h = hdr<QueueChunkHdr>(x.m, e);
+      const uint32_t vs = ld(&h->virtual_start), cnt = ld(&h->count);
+      if (vs / kQCSpots == v / kQCSpots) {
+        if (qc_out) *qc_out = e;
+        if (dead && (cnt >> 16) == 0) *dead = rest;
+        return qc_slots(x.m, e) + v % kQCSpots;
+      }
     }
     return nullptr;
   } else {
@@ -1029,9 +1042,11 @@ __device__ inline const uint32_t* slots_at(Ctx<I>& x, ClassQueue* cq, uint32_t v
     if (c != kMarker) {
       cursor = c;
       if (qc_out) *qc_out = c;
+      if (dead && (ld(&hdr<QueueChunkHdr>(x.m, c)->count) >> 16) == 0) *dead = rest;
       return qc_slots(x.m, c) + v % kQCSpots;
     }
     fill = kTomb;
+    if (dead) *dead = rest;
     return nullptr;
   }
 }
@@ -1039,28 +1054,33 @@ __device__ inline const uint32_t* slots_at(Ctx<I>& x, ClassQueue* cq, uint32_t v
 // Load the slots of virtual positions [v, v + n), n <= kPeek, which lie in one queue chunk for
 // VA / VL; entries past n read markers. (ChunkQueue scan, ChunkQueue_impl.cuh:105-145;
 // QueueChunk::access, QueueChunk.cuh:111-114, and the linked walk of accessLinked.)
+// `dead`: as slots_at (always 0 for standard rings).
 template <class I>
 __device__ inline void peek_slots(Ctx<I>& x, ClassQueue* cq, uint32_t v, uint32_t n, uint32_t& cursor,
-                                  uint32_t (&s)[kPeek]) {
+                                  uint32_t (&s)[kPeek], uint32_t& dead) {
+  dead = 0;
   if constexpr (is_std(I::kKind)) {
 #pragma unroll
     for (uint32_t i = 0; i < kPeek; ++i) s[i] = i < n ? ld(&ring_of(x, cq)[(v + i) & (I::kQueueEntries - 1)]) : kMarker;
   } else {
     uint32_t fill;
-    const uint32_t* sl = slots_at(x, cq, v, cursor, fill);
+    const uint32_t* sl = slots_at(x, cq, v, cursor, fill, nullptr, &dead);
 #pragma unroll
     for (uint32_t i = 0; i < kPeek; ++i) s[i] = (i < n && sl) ? ld(&sl[i]) : fill;
   }
 }
 
 // Offset of the first position in [v, v + n) that holds a chunk of this class queued exactly there
-// (queue_pos == position) with free pages; n when there is none. The queue_pos test also rejects a
-// stale slot read out of a queue chunk that was recycled meanwhile.
+// (queue_pos == position) with free pages; n when there is none, with `adv` the positions the scan
+// may pass (n, or the rest of a queue chunk that holds no live entry). The queue_pos test also
+// rejects a stale slot read out of a queue chunk that was recycled meanwhile.
 template <class I>
 __device__ inline uint32_t find_chunk(Ctx<I>& x, ClassQueue* cq, uint32_t v, uint32_t n, uint32_t& cursor,
-                                      uint32_t& chunk, int32_t& cnt) {
-  uint32_t s[kPeek];
-  peek_slots(x, cq, v, n, cursor, s);
+                                      uint32_t& chunk, int32_t& cnt, uint32_t& adv) {
+  uint32_t s[kPeek], dead = 0;
+  peek_slots(x, cq, v, n, cursor, s, dead);
+  adv = n > dead ? n : dead;
+  if (dead) return n;
   int32_t c_[kPeek];
   uint64_t qp[kPeek];
   uint32_t ps[kPeek];
@@ -1233,6 +1253,28 @@ __device__ __forceinline__ uint32_t batch_at(uint32_t v, uint32_t b) {
   return n;
 }
 
+// Where a claim of chunk pages starts scanning: the per-class page cursor (ClassQueue::scan) is
+// advanced by the pages claimed, and its old value read as a queue position (cursor / ppc, one
+// position per chunk of ppc pages) and folded into the live window [front_, back_). Concurrent claims
+// thus begin at disjoint chunks, in queue order. The mapping assumes full chunks; where chunks hold
+// fewer free pages (blocks held across rounds) claims spill over into the chunks the next claims
+// start at, so a claim that passes kRedrawAfter positions without taking a page draws a new start
+// for what it still needs (at most kMaxRedraws times): the extra draws advance the cursor at the rate
+// at which pages are really found, instead of letting the spill cascade from claim to claim.
+constexpr uint32_t kRedrawAfter = 64;
+constexpr uint32_t kMaxRedraws = 4;
+
+__device__ __forceinline__ uint32_t fold_start(ClassQueue* cq, uint32_t h) {
+  const uint32_t f = ld(&cq->front), b = ld(&cq->back);
+  const uint32_t len = b - f;
+  return (len == 0 || len >= 0x80000000u) ? f : (h - f < len ? h : f + (h - f) % len);
+}
+template <class I>
+__device__ __forceinline__ uint32_t draw_start(Ctx<I>& x, ClassQueue* cq, uint32_t n) {
+  const uint64_t p = add(&cq->scan, (uint64_t)n);
+  return fold_start(cq, (uint32_t)(p >> __builtin_ctz(ppc_of(x, cq))));
+}
+
 // One leader's claim of `num` pages from a chunk-based class queue, as a state machine that is
 // stepped once per wave iteration: the leader finds chunks kPeek slots at a time, reserves pages on
 // the chunk count with a CAS and claims them from the chunk's 64-bit mask words. Each step that
@@ -1246,18 +1288,14 @@ __device__ __forceinline__ uint32_t batch_at(uint32_t v, uint32_t b) {
 template <class I>
 struct ChunkClaim {
   uint32_t num = 0, v = 0, cursor = kMarker, need = 0, assigned = 0, t_rem = 0, w = 0, c = kMarker, wraps = 0;
-  uint32_t reclaim = kMarker;
+  uint32_t reclaim = kMarker, miss = 0, redraws = 0;
   bool gave_up = false, removed = false;
   Spin it;
 
   __device__ ChunkClaim(Ctx<I>& x, ClassQueue* cq, bool leader, uint32_t n) : it(&x.m.error) {
     if (!leader) return;
     num = need = n;
-    const uint64_t p = add(&cq->scan, (uint64_t)n);
-    const uint32_t h = (uint32_t)(p >> __builtin_ctz(ppc_of(x, cq)));
-    const uint32_t f = ld(&cq->front), b = ld(&cq->back);
-    const uint32_t len = b - f;
-    v = (len == 0 || len >= 0x80000000u) ? f : (h - f < len ? h : f + (h - f) % len);
+    v = draw_start(x, cq, n);
   }
   __device__ bool finished() const { return assigned >= num || gave_up; }
 
@@ -1270,6 +1308,12 @@ struct ChunkClaim {
     uint32_t n = 0;
     if (t_rem == 0) {
       OURO_DIAG_ADD(1, 1);
+      if (miss >= kRedrawAfter && redraws < kMaxRedraws) {
+        v = draw_start(x, cq, need);
+        cursor = kMarker;
+        miss = 0;
+        ++redraws;
+      }
       const uint32_t b = ld(&cq->back);
       wraps += scan_wrap<I>(cq, v, cursor, b);
       n = batch_at<I>(v, b);
@@ -1279,9 +1323,9 @@ struct ChunkClaim {
         gave_up = true;
         return;
       }
-      uint32_t s = kMarker;
+      uint32_t s = kMarker, adv = n;
       int32_t cnt = 0;
-      const uint32_t i = n ? find_chunk(x, cq, v, n, cursor, s, cnt) : 0u;
+      const uint32_t i = n ? find_chunk(x, cq, v, n, cursor, s, cnt, adv) : 0u;
       if (i < n) {
         ChunkHdr* h = hdr<ChunkHdr>(x.m, s);
         uint32_t t = 0;
@@ -1293,6 +1337,7 @@ struct ChunkClaim {
           cnt = seen;
         }
         if (cnt > 0) {
+          miss = 0;
           c = s;
           t_rem = t;
           need -= t;
@@ -1307,10 +1352,12 @@ struct ChunkClaim {
         } else {
           OURO_DIAG_ADD(4, 1);
           v += i + 1u;  // drained meanwhile; its last taker removes it
+          miss += i + 1u;
         }
       } else {
-        OURO_DIAG_ADD(2, n);
-        v += n;
+        OURO_DIAG_ADD(2, adv);
+        v += adv;
+        miss += adv;
         if (n == 0) backoff(it.it);
       }
     }
@@ -1364,29 +1411,31 @@ __device__ inline void* alloc_chunk_pages(Ctx<I>& x, uint32_t qi, bool want) {
     if (ok && (int)lane() == L) p0 = add(&cq->scan, (uint64_t)popc(grp));
     p0 = shfl64(p0, L);
     if (ok) {
-      uint32_t v;
-      {
-        const uint32_t h = (uint32_t)((p0 + rank_in(grp)) >> __builtin_ctz(ppc_of(x, cq)));
-        const uint32_t f = ld(&cq->front), b = ld(&cq->back);
-        const uint32_t len = b - f;
-        v = (len == 0 || len >= 0x80000000u) ? f : (h - f < len ? h : f + (h - f) % len);
-      }
-      uint32_t cursor = kMarker, wraps = 0;
+      uint32_t v = fold_start(cq, (uint32_t)((p0 + rank_in(grp)) >> __builtin_ctz(ppc_of(x, cq))));
+      uint32_t cursor = kMarker, wraps = 0, miss = 0, redraws = 0;
       Spin it(&x.m.error);
       while (true) {
         OURO_DIAG_ADD(0, 1);
+        if (miss >= kRedrawAfter && redraws < kMaxRedraws) {
+          v = draw_start(x, cq, 1u);
+          cursor = kMarker;
+          miss = 0;
+          ++redraws;
+        }
         const uint32_t b = ld(&cq->back);
         wraps += scan_wrap<I>(cq, v, cursor, b);
-        uint32_t s[kPeek];
-        const uint32_t n = batch_at<I>(v, b) ? 1u : 0u;
+        const uint32_t n = batch_at<I>(v, b);  // never walk towards a position at back_
         if ((wraps >= 2 || n == 0) && pool_exhausted(x.m)) {
           sem_signal(cq, 1);  // the pool ran out: give the claim back, return nullptr
           break;
         }
-        s[0] = kMarker;
-        if (n) peek_slots(x, cq, v, n, cursor, s);  // never walk towards a position at back_
-        const uint32_t c = s[0];
-        if (n && chunk_ok(x.m, c) && ld(&hdr<ChunkHdr>(x.m, c)->queue_pos) == (uint64_t)v) {
+        // the first chunk queued in [v, v + n) with free pages (kPeek slots per round trip; the
+        // chunks are tried in queue order, as the reference's one-by-one walk does)
+        uint32_t c = kMarker, adv = n;
+        int32_t cnt = 0;
+        const uint32_t i = n ? find_chunk(x, cq, v, n, cursor, c, cnt, adv) : 0u;
+        if (i < n) {
+          v += i;
           ChunkHdr* h = hdr<ChunkHdr>(x.m, c);
           const AccessMode md = access_alloc_lane(x, h, page);
           if (md == AccessMode::ERROR) break;
@@ -1401,8 +1450,10 @@ __device__ inline void* alloc_chunk_pages(Ctx<I>& x, uint32_t qi, bool want) {
             }
             break;
           }
+          adv = 1;  // drained meanwhile: go on after it
         }
-        v += n;
+        v += adv;
+        miss += adv;
         if (it.expired()) { OURO_TIMEOUT(x.m); break; }
         backoff(it.it > 64 ? it.it : 0);
       }

@@ -34,6 +34,8 @@
                                  thread of a 64-thread workgroup */
 #define COMPACT_STEPS 256u    /* CAS steps per call of the per-lane compaction */
 #define WAVE_COMPACT_STEPS 8192u /* front moves per call of the wave-wide compaction */
+#define REDRAW_AFTER 64u      /* positions a claim passes without a page before it draws a new start */
+#define MAX_REDRAWS 4u
 
 enum { ERR_OOM = 1, ERR_QUEUE_FULL = 2, ERR_VSTART = 4, ERR_TIMEOUT = 8, ERR_BAD_FREE = 16, ERR_TOO_LARGE = 32,
        ERR_REUSE_FULL = 64, ERR_CORRUPT = 256 };
@@ -481,9 +483,13 @@ static void take_lanes(ouro_sim* s, inst_t* I, cls_t** qs, int n, const uint32_t
 /* ------------------------------------------------------------------ chunk-queue scanning */
 /* Slots of positions [v, v + n), n <= PEEK, inside one queue chunk for VA / VL: a reclaimed or
  * bypassed queue chunk reads as tombstones, an unpublished one as markers (the device's peek_slots;
- * ChunkQueue_impl.cuh:105-145, QueueChunk.cuh:111-114). */
+ * ChunkQueue_impl.cuh:105-145, QueueChunk.cuh:111-114). *dead (if given): the positions from v to
+ * the end of its queue chunk when that queue chunk holds no live entry (reclaimed, bypassed, or its
+ * count's enqueued-minus-dequeued half zero), else 0; a scan passes such a stretch in one step. */
 static void peek_slots(ouro_sim* s, inst_t* I, cls_t* q, uint32_t v, uint32_t n, uint32_t* cursor,
-                       uint32_t* out) {
+                       uint32_t* out, uint32_t* dead) {
+  const uint32_t rest = QC_SPOTS - v % QC_SPOTS;
+  if (dead) *dead = 0;
   if (is_std(I->kind)) {
     for (uint32_t i = 0; i < PEEK; ++i) out[i] = i < n ? *std_slot(q, v + i) : MARKER;
     return;
@@ -492,9 +498,13 @@ static void peek_slots(ouro_sim* s, inst_t* I, cls_t* q, uint32_t v, uint32_t n,
   uint32_t fill = MARKER;
   if (is_va(I->kind)) {
     const uint32_t e = q->va[(v / QC_SPOTS) & (q->ring_size - 1)];
-    if (e == qtag(v / QC_SPOTS)) fill = TOMB;
-    else if (chunk_ok(s, e) && ch(s, e)->slots && ch(s, e)->vstart / QC_SPOTS == v / QC_SPOTS)
+    if (e == qtag(v / QC_SPOTS)) {
+      fill = TOMB;
+      if (dead) *dead = rest;
+    } else if (chunk_ok(s, e) && ch(s, e)->slots && ch(s, e)->vstart / QC_SPOTS == v / QC_SPOTS) {
       sl = ch(s, e)->slots + v % QC_SPOTS;
+      if (dead && (ch(s, e)->qc_count >> 16) == 0) *dead = rest;
+    }
   } else {
     if (*cursor == MARKER) *cursor = q->front_qc;
     uint32_t c = vl_locate(s, *cursor, v);
@@ -505,19 +515,24 @@ static void peek_slots(ouro_sim* s, inst_t* I, cls_t* q, uint32_t v, uint32_t n,
     if (c != MARKER) {
       *cursor = c;
       sl = ch(s, c)->slots + v % QC_SPOTS;
+      if (dead && (ch(s, c)->qc_count >> 16) == 0) *dead = rest;
     } else {
       fill = TOMB;
+      if (dead) *dead = rest;
     }
   }
   for (uint32_t i = 0; i < PEEK; ++i) out[i] = (i < n && sl) ? sl[i] : fill;
 }
 
 /* First offset in [v, v + n) that holds a chunk of this class queued exactly there with free pages;
- * n when there is none. */
+ * n when there is none, with *adv the positions the scan passes (n, or the rest of a queue chunk
+ * that holds no live entry). */
 static uint32_t find_chunk(ouro_sim* s, inst_t* I, cls_t* q, uint32_t v, uint32_t n, uint32_t* cursor,
-                           uint32_t* chunk) {
-  uint32_t sl[PEEK];
-  peek_slots(s, I, q, v, n, cursor, sl);
+                           uint32_t* chunk, uint32_t* adv) {
+  uint32_t sl[PEEK], dead = 0;
+  peek_slots(s, I, q, v, n, cursor, sl, &dead);
+  *adv = n > dead ? n : dead;
+  if (dead) return n;
   for (uint32_t i = 0; i < n; ++i) {
     if (!chunk_ok(s, sl[i])) continue;
     const chunk_t* x = ch(s, sl[i]);
@@ -768,6 +783,20 @@ static uint32_t mask_words(uint32_t ppc) {
   return w == 0 ? 1 : (w > 8 ? 8 : w);
 }
 
+/* Start of a claim of n chunk pages (the device's draw_start): the class's page cursor advances by n
+ * and its old value, read as a position (cursor / ppc), is folded into the window [front, back). A
+ * claim that passes REDRAW_AFTER positions without a page draws again for what it still needs, at
+ * most MAX_REDRAWS times. */
+static uint32_t fold_start(const cls_t* q, uint32_t h) {
+  const uint32_t f = q->front, len = q->back - q->front;
+  return (len == 0 || len >= 0x80000000u) ? f : (h - f < len ? h : f + (h - f) % len);
+}
+static uint32_t draw_start(cls_t* q, uint32_t n) {
+  const uint32_t h = (uint32_t)(q->scan / q->ppc);
+  q->scan += (uint64_t)n;
+  return fold_start(q, h);
+}
+
 /* Group claim for chunk-based queues: the leader starts at a spread position of the live queue
  * window (per-class page cursor: position = cursor / ppc) and looks at PEEK slots per step for a chunk queued exactly there
  * with free pages; it reserves pages on the chunk count and takes them from the 64-bit mask words,
@@ -776,17 +805,20 @@ static uint32_t mask_words(uint32_t ppc) {
  * ChunkAccess::allocPage ChunkAccess_impl.cuh:38-135, as the device's W = 64 rule.) */
 static void claim_group(ouro_sim* s, inst_t* I, group_t* g, uint64_t* out, uint32_t* reclaim) {
   cls_t* q = &I->q[g->qi];
-  const uint32_t h = (uint32_t)(q->scan / q->ppc);
-  q->scan += (uint64_t)g->n;
-  const uint32_t f = q->front, len = q->back - q->front;
-  uint32_t v = (len == 0 || len >= 0x80000000u) ? f : (h - f < len ? h : f + (h - f) % len);
-  uint32_t cursor = MARKER, need = (uint32_t)g->n, assigned = 0, wraps = 0;
+  uint32_t v = draw_start(q, (uint32_t)g->n);
+  uint32_t cursor = MARKER, need = (uint32_t)g->n, assigned = 0, wraps = 0, miss = 0, redraws = 0;
   int guard = 0;
   *reclaim = MARKER;
   while (assigned < (uint32_t)g->n) {
     if (++guard > 10000000) {
       s->error |= ERR_TIMEOUT;
       return;
+    }
+    if (miss >= REDRAW_AFTER && redraws < MAX_REDRAWS) {
+      v = draw_start(q, need);
+      cursor = MARKER;
+      miss = 0;
+      ++redraws;
     }
     const uint32_t b = q->back;
     if (v - b < 0x80000000u || v - q->front >= 0x80000000u) { /* at or past back, or behind front */
@@ -806,12 +838,14 @@ static void claim_group(ouro_sim* s, inst_t* I, group_t* g, uint64_t* out, uint3
       s->error |= ERR_TIMEOUT;
       return;
     }
-    uint32_t sl = MARKER;
-    const uint32_t i = find_chunk(s, I, q, v, n, &cursor, &sl);
+    uint32_t sl = MARKER, adv = n;
+    const uint32_t i = find_chunk(s, I, q, v, n, &cursor, &sl, &adv);
     if (i == n) {
-      v += n;
+      v += adv;
+      miss += adv;
       continue;
     }
+    miss = 0;
     chunk_t* x = ch(s, sl);
     const int32_t cnt = x->count;
     const uint32_t t = (uint32_t)cnt < need ? (uint32_t)cnt : need;
@@ -863,16 +897,19 @@ static void finish_reclaim(ouro_sim* s, inst_t* I, cls_t* q, uint32_t reclaim) {
 static void claim_wave(ouro_sim* s, inst_t* I, group_t* g, uint64_t* out) {
   cls_t* q = &I->q[g->qi];
   const uint32_t total = (uint32_t)g->n;
-  const uint32_t h = (uint32_t)(q->scan / q->ppc);
-  q->scan += (uint64_t)total;
-  const uint32_t f = q->front, len = q->back - q->front;
-  uint32_t v = (len == 0 || len >= 0x80000000u) ? f : (h - f < len ? h : f + (h - f) % len);
-  uint32_t cursor = MARKER, assigned = 0, wraps = 0, rec = MARKER;
+  uint32_t v = draw_start(q, total);
+  uint32_t cursor = MARKER, assigned = 0, wraps = 0, rec = MARKER, miss = 0, redraws = 0;
   int guard = 0;
   while (assigned < total) {
     if (++guard > 10000000) {
       s->error |= ERR_TIMEOUT;
       return;
+    }
+    if (miss >= REDRAW_AFTER && redraws < MAX_REDRAWS) {
+      v = draw_start(q, total - assigned);
+      cursor = MARKER;
+      miss = 0;
+      ++redraws;
     }
     const uint32_t b = q->back;
     if (v - b < 0x80000000u || v - q->front >= 0x80000000u) {
@@ -898,11 +935,17 @@ static void claim_wave(ouro_sim* s, inst_t* I, group_t* g, uint64_t* out) {
     if (is_std(I->kind)) {
       for (uint32_t i = 0; i < nb; ++i) sl[i] = *std_slot(q, v + i);
     } else {
-      uint32_t tmp[PEEK];
+      uint32_t tmp[PEEK], dead = 0;
       for (uint32_t i0 = 0; i0 < nb; i0 += PEEK) {
         const uint32_t m = nb - i0 < PEEK ? nb - i0 : PEEK;
-        peek_slots(s, I, q, v + i0, m, &cursor, tmp);
+        peek_slots(s, I, q, v + i0, m, &cursor, tmp, i0 == 0 ? &dead : NULL);
+        if (dead) break;
         for (uint32_t i = 0; i < m; ++i) sl[i0 + i] = tmp[i];
+      }
+      if (dead) { /* the queue chunk holds no live entry: pass the rest of it in one step */
+        v += dead;
+        miss += dead;
+        continue;
       }
     }
     int32_t cnt[64];
@@ -973,6 +1016,7 @@ static void claim_wave(ouro_sim* s, inst_t* I, group_t* g, uint64_t* out) {
     }
     assigned += took;
     v += nb;
+    miss = took ? 0u : miss + nb;
   }
   if (rec != MARKER) finish_reclaim(s, I, q, rec);
 }
