@@ -216,19 +216,13 @@ __device__ inline void block_claim_chunks(Memory& m, I& inst, ReuseRing& base, u
     uint32_t v = 0;
     if (l == 0) v = draw_start(x, cq, total);
     v = uniform(v);
-    uint32_t assigned = 0, wraps = 0, cursor = kMarker, reclaim = kMarker, miss = 0, redraws = 0;
+    // (no redraws here: a whole wave passes 64 positions a step, and measured, drawing new starts
+    // scattered the block claims' consumption order and doubled their kernel time)
+    uint32_t assigned = 0, wraps = 0, cursor = kMarker, reclaim = kMarker;
     Spin it(&m.error);
     while (assigned < total) {
       uint32_t b = 0, nb = 0;
       bool give_up = false;
-      if (miss >= kRedrawAfter && redraws < kMaxRedraws) {
-        // positions passed without a page since the last start: draw a new one (draw_start)
-        if (l == 0) v = draw_start(x, cq, total - assigned);
-        v = uniform(v);
-        cursor = kMarker;
-        miss = 0;
-        ++redraws;
-      }
       if (l == 0) {
         b = ld(&cq->back);
         wraps += scan_wrap<I>(cq, v, cursor, b);
@@ -255,6 +249,10 @@ __device__ inline void block_claim_chunks(Memory& m, I& inst, ReuseRing& base, u
         backoff(it.it);
         continue;
       }
+      if (l == 0) {
+        OURO_DIAG_ADD(0, 1);
+        OURO_DIAG_ADD(2, nb);
+      }
       // the slots of positions v .. v + nb (one queue chunk for VA / VL)
       uint32_t s = kMarker, qc = kMarker;
       uint32_t* slot = nullptr;
@@ -273,7 +271,6 @@ __device__ inline void block_claim_chunks(Memory& m, I& inst, ReuseRing& base, u
         if (dead) {
           // the queue chunk holds no live entry: pass the rest of it in one step
           v += dead;
-          miss += dead;
           if (ballot(it.expired())) {
             OURO_TIMEOUT(m);
             break;
@@ -307,6 +304,7 @@ __device__ inline void block_claim_chunks(Memory& m, I& inst, ReuseRing& base, u
       bool removed = false;
       if (t && t == c) removed = cas(&h->queue_pos, (uint64_t)(v + l), kNotQueued) == (uint64_t)(v + l);
       const uint64_t rm = ballot(removed);
+      if (l == 0) OURO_DIAG_ADD(8, popc(rm));
       if (rm) {
         if (removed) xchg(slot, kTomb);
         const uint32_t nrm = popc(rm);
@@ -363,7 +361,6 @@ __device__ inline void block_claim_chunks(Memory& m, I& inst, ReuseRing& base, u
       }
       assigned += took;
       v += nb;
-      miss = took ? 0u : miss + nb;
       if (assigned < total && ballot(it.expired())) {
         OURO_TIMEOUT(m);
         break;

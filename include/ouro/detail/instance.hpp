@@ -456,7 +456,8 @@ __device__ inline void compact(Ctx<I>& x, ClassQueue* cq) {
 // compact() for the active lanes of a wave together (all of them must call): each round looks at up
 // to 16 positions per active lane from front_ and moves front_ over the leading dead ones with one CAS
 // (a reclaimed VA queue chunk or a VL queue chunk front_ptr_ has left is a single move). It stops at
-// the first position it cannot prove dead or after kWaveCompactSteps moves. Every remover of a class
+// the first position it cannot prove dead or after kWaveCompactSteps moves (then it returns true).
+// Every remover of a class
 // compacts, so the front_ CAS is contended and its rate bounds how fast front_ advances: each
 // successful CAS should pass as many positions as it can. A wave whose CAS lost to a compactor that
 // already passed every position this wave saw dead leaves the rest to that winner; otherwise it
@@ -464,7 +465,7 @@ __device__ inline void compact(Ctx<I>& x, ClassQueue* cq) {
 constexpr uint32_t kWaveCompactSteps = 8192;
 
 template <class I>
-__device__ inline void compact_wave(Ctx<I>& x, ClassQueue* cq) {
+__device__ inline bool compact_wave_pass(Ctx<I>& x, ClassQueue* cq) {
   constexpr Kind K = I::kKind;
   static_assert(is_chunk_kind(K), "page queues dequeue strictly at front_");
   constexpr uint32_t kPer = 16;  // positions per lane and round
@@ -530,7 +531,11 @@ __device__ inline void compact_wave(Ctx<I>& x, ClassQueue* cq) {
       nf = f + k;
     }
     uint32_t seen = 0;
-    if (lead) seen = cas(&cq->front, f, nf);
+    if (lead) {
+      OURO_DIAG_ADD(14, 1);
+      seen = cas(&cq->front, f, nf);
+      if (seen != f) OURO_DIAG_ADD(15, 1);
+    }
     seen = readlane(seen, L);
     if (seen != f) {
       // another compactor moved front_. If it passed every position this wave saw dead (and this
@@ -556,6 +561,7 @@ __device__ inline void compact_wave(Ctx<I>& x, ClassQueue* cq) {
       if (nf - f < n) break;  // stopped at a live position
     }
   }
+  return moves >= kWaveCompactSteps;  // stopped at the step limit, not at a live position
 }
 
 // ------------------------------------------------------------------ slot sequences
@@ -1191,6 +1197,70 @@ __device__ inline bool cq_remove(Ctx<I>& x, ClassQueue* cq, uint32_t chunk, uint
   return true;
 }
 
+// Front relocation. Claims start at spread positions, so a live chunk near front_ can be passed
+// over while the chunks behind it are consumed and re-queued at back_; front_ cannot move past it
+// and the window [front_, back_) grows round after round (blocks held across rounds, many
+// concurrent block claims), lengthening every scan and, left alone, running into the queue's
+// capacity. When the window is more than twice the class's queued chunks plus kRelocSlack
+// positions, compaction moves the live chunk that blocks front_ to back_: it removes the entry
+// where it sits (queue_pos CAS, tombstone, count_, as cq_remove) and queues the chunk again at back_
+// if it still has free pages (a chunk whose last page a claim took meanwhile stays out of the queue
+// until a free gives it pages, as after any removal). The chunk's pages stay counted by the
+// semaphore throughout. At most kMaxRelocate moves per call.
+constexpr uint32_t kRelocSlack = 4096;
+constexpr uint32_t kMaxRelocate = 16;
+
+template <class I>
+__device__ inline bool relocate_front(Ctx<I>& x, ClassQueue* cq) {
+  constexpr Kind K = I::kKind;
+  const int L = first(active());
+  const bool lead = (int)lane() == L;
+  uint32_t chunk = kMarker, f = 0;
+  if (lead) {
+    f = ld(&cq->front);
+    const uint32_t b = ld(&cq->back), len = b - f;
+    const int32_t live = ld(&cq->count);
+    if (len < 0x80000000u && len > 2u * (uint32_t)(live > 0 ? live : 0) + kRelocSlack) {
+      OURO_DIAG_ADD(9, 1);
+      uint32_t e = kMarker;
+      if constexpr (is_std(K)) {
+        e = ld(&ring_of(x, cq)[f & (I::kQueueEntries - 1)]);
+      } else {
+        uint32_t cursor = kMarker, fill = kMarker;
+        const uint32_t* sl = slots_at(x, cq, f, cursor, fill);
+        e = sl ? ld(sl) : fill;
+      }
+      if (chunk_ok(x.m, e) && ld(&hdr<ChunkHdr>(x.m, e)->queue_pos) == (uint64_t)f &&
+          ld(&hdr<ChunkHdr>(x.m, e)->page_size) == page_size_of(x, cq))
+        chunk = e;
+      if (chunk != kMarker) OURO_DIAG_ADD(10, 1);
+      else OURO_DIAG_LOG(((unsigned long long)f << 32) | e);
+    }
+  }
+  chunk = readlane(chunk, L);
+  f = readlane(f, L);
+  if (chunk == kMarker) return false;
+  uint32_t cursor = kMarker, reclaim = kMarker;
+  bool again = false, removed = false;
+  if (lead) removed = cq_remove(x, cq, chunk, f, cursor, reclaim, again);
+  reclaim = readlane(reclaim, L);
+  finish_reclaim(x, cq, lead && reclaim != kMarker, reclaim);
+  ensure_in_queue<1>(x, cq, lead && removed && again, chunk);
+  return readlane((uint32_t)removed, L) != 0;
+}
+
+// Wave-wide compaction of a chunk queue's front_ (compact_wave_pass), with front relocation; a pass
+// that stopped at its step limit inside a long dead stretch is followed by another (the same budget
+// of kMaxRelocate more passes).
+template <class I>
+__device__ inline void compact_wave(Ctx<I>& x, ClassQueue* cq) {
+  bool more = compact_wave_pass(x, cq);
+  for (uint32_t r = 0; r < kMaxRelocate; ++r) {
+    if (!more && !relocate_front(x, cq)) break;
+    more = compact_wave_pass(x, cq);
+  }
+}
+
 // ChunkAccess::allocPage (ChunkAccess_impl.cuh:38-135), per lane, with 64-bit mask words.
 enum class AccessMode { SUCCESS, DEQUEUE, RE_ENQUEUE, CONTINUE, ERROR };
 
@@ -1261,6 +1331,13 @@ __device__ __forceinline__ uint32_t batch_at(uint32_t v, uint32_t b) {
 // start at, so a claim that passes kRedrawAfter positions without taking a page draws a new start
 // for what it still needs (at most kMaxRedraws times): the extra draws advance the cursor at the rate
 // at which pages are really found, instead of letting the spill cascade from claim to claim.
+// A cursor that front_ has passed (chunks held fewer free pages than the cursor counts on, so
+// consumption outran it) restarts at front_: the claim starts there and the cursor is set to front_
+// plus its pages, so consumption resumes at the oldest live chunks and front_ can be compacted; a
+// cursor folded back into the window at some arbitrary position would leave old live chunks behind
+// and let the window grow round after round. A cursor that ran past back_ (claims ahead of the
+// chunks being queued) folds back into the window and keeps its value: moving it back would send
+// the claims after it over positions already consumed.
 constexpr uint32_t kRedrawAfter = 64;
 constexpr uint32_t kMaxRedraws = 4;
 
@@ -1269,10 +1346,27 @@ __device__ __forceinline__ uint32_t fold_start(ClassQueue* cq, uint32_t h) {
   const uint32_t len = b - f;
   return (len == 0 || len >= 0x80000000u) ? f : (h - f < len ? h : f + (h - f) % len);
 }
+// Page cursor value p0 (the old value of a fetch-add of n pages): its start position, or front_
+// after restarting the cursor there (p0 is then front_'s cursor value).
+template <class I>
+__device__ __forceinline__ uint32_t start_of(Ctx<I>& x, ClassQueue* cq, uint64_t& p0, uint32_t n) {
+  const uint32_t sh = __builtin_ctz(ppc_of(x, cq));
+  const uint32_t h = (uint32_t)(p0 >> sh);
+  const uint32_t f = ld(&cq->front), b = ld(&cq->back);
+  const uint32_t len = b - f;
+  if (len == 0 || len >= 0x80000000u) return f;
+  const uint32_t d = h - f;
+  if (d < len) return h;
+  if (d < 0x80000000u) return f + d % len;  // past back_
+  const uint64_t pf = (uint64_t)f << sh;
+  cas(&cq->scan, p0 + n, pf + n);  // a claim that moved the cursor meanwhile keeps its value
+  p0 = pf;
+  return f;
+}
 template <class I>
 __device__ __forceinline__ uint32_t draw_start(Ctx<I>& x, ClassQueue* cq, uint32_t n) {
-  const uint64_t p = add(&cq->scan, (uint64_t)n);
-  return fold_start(cq, (uint32_t)(p >> __builtin_ctz(ppc_of(x, cq))));
+  uint64_t p = add(&cq->scan, (uint64_t)n);
+  return start_of(x, cq, p, n);
 }
 
 // One leader's claim of `num` pages from a chunk-based class queue, as a state machine that is
@@ -1408,7 +1502,10 @@ __device__ inline void* alloc_chunk_pages(Ctx<I>& x, uint32_t qi, bool want) {
     // long contended walk.
     const int L = first(grp);
     uint64_t p0 = 0;
-    if (ok && (int)lane() == L) p0 = add(&cq->scan, (uint64_t)popc(grp));
+    if (ok && (int)lane() == L) {
+      p0 = add(&cq->scan, (uint64_t)popc(grp));
+      (void)start_of(x, cq, p0, popc(grp));  // restarts a cursor that has left the window
+    }
     p0 = shfl64(p0, L);
     if (ok) {
       uint32_t v = fold_start(cq, (uint32_t)((p0 + rank_in(grp)) >> __builtin_ctz(ppc_of(x, cq))));

@@ -36,6 +36,8 @@
 #define WAVE_COMPACT_STEPS 8192u /* front moves per call of the wave-wide compaction */
 #define REDRAW_AFTER 64u      /* positions a claim passes without a page before it draws a new start */
 #define MAX_REDRAWS 4u
+#define RELOC_SLACK 4096u     /* front relocation: window longer than 2 x queued chunks + this */
+#define MAX_RELOCATE 16u
 
 enum { ERR_OOM = 1, ERR_QUEUE_FULL = 2, ERR_VSTART = 4, ERR_TIMEOUT = 8, ERR_BAD_FREE = 16, ERR_TOO_LARGE = 32,
        ERR_REUSE_FULL = 64, ERR_CORRUPT = 256 };
@@ -549,25 +551,27 @@ static uint32_t find_chunk(ouro_sim* s, inst_t* I, cls_t* q, uint32_t v, uint32_
  * most COMPACT_STEPS steps, a step on a standard ring passing up to PEEK leading tombstones at once;
  * the wave-wide form makes at most WAVE_COMPACT_STEPS moves, a move being one position or one jump
  * over a whole queue chunk. */
-static void compact(ouro_sim* s, inst_t* I, cls_t* q, int wave) {
+static int compact_pass(ouro_sim* s, inst_t* I, cls_t* q, int wave) {
   uint32_t f = q->front;
   const uint32_t b = q->back;
   if (is_std(I->kind)) {
     const uint32_t steps = wave ? WAVE_COMPACT_STEPS : COMPACT_STEPS, per = wave ? 1u : PEEK;
-    for (uint32_t i = 0; i < steps && f != b && b - f < 0x80000000u; ++i) {
+    uint32_t i = 0;
+    for (; i < steps && f != b && b - f < 0x80000000u; ++i) {
       const uint32_t n = b - f < per ? b - f : per;
       uint32_t k = 0;
       while (k < n && *std_slot(q, f + k) == TOMB) ++k;
-      if (k == 0) return;
+      if (k == 0) return 0;
       for (uint32_t j = 0; j < k; ++j) *std_slot(q, f + j) = MARKER;
       f += k;
       q->front = f;
-      if (k < n) return;
+      if (k < n) return 0;
     }
-    return;
+    return wave && i == steps; /* stopped at the step limit */
   }
   const uint32_t steps = wave ? WAVE_COMPACT_STEPS : COMPACT_STEPS;
-  for (uint32_t i = 0; i < steps && f != b && b - f < 0x80000000u; ++i) {
+  uint32_t i = 0;
+  for (; i < steps && f != b && b - f < 0x80000000u; ++i) {
     uint32_t nf = f + 1u;
     if (is_va(I->kind)) {
       const uint32_t e = q->va[(f / QC_SPOTS) & (q->ring_size - 1)];
@@ -575,22 +579,23 @@ static void compact(ouro_sim* s, inst_t* I, cls_t* q, int wave) {
         nf = (f / QC_SPOTS + 1u) * QC_SPOTS;
       } else if (!chunk_ok(s, e) || !ch(s, e)->slots || ch(s, e)->vstart / QC_SPOTS != f / QC_SPOTS ||
                  ch(s, e)->slots[f % QC_SPOTS] != TOMB) {
-        return;
+        return 0;
       }
     } else {
       const uint32_t c = q->front_qc;
-      if (!chunk_ok(s, c)) return;
+      if (!chunk_ok(s, c)) return 0;
       const uint32_t vs = ch(s, c)->vstart, fb = f - f % QC_SPOTS;
       if (vs != fb) {
-        if (vs - fb >= 0x80000000u) return;
+        if (vs - fb >= 0x80000000u) return 0;
         nf = vs;
       } else if (ch(s, c)->slots[f % QC_SPOTS] != TOMB) {
-        return;
+        return 0;
       }
     }
     q->front = nf;
     f = nf;
   }
+  return wave && i == steps; /* stopped at the step limit */
 }
 
 /* Tombstone the slot of a removed chunk at p (ChunkQueue_impl.cuh:434, QueueChunk::deleteElement
@@ -784,7 +789,9 @@ static uint32_t mask_words(uint32_t ppc) {
 }
 
 /* Start of a claim of n chunk pages (the device's draw_start): the class's page cursor advances by n
- * and its old value, read as a position (cursor / ppc), is folded into the window [front, back). A
+ * and its old value, read as a position (cursor / ppc), is the start if it lies in the window
+ * [front, back); one past back is folded into the window, one that front has passed restarts at
+ * front (fold_start folds the in-group offsets of the per-lane path). A
  * claim that passes REDRAW_AFTER positions without a page draws again for what it still needs, at
  * most MAX_REDRAWS times. */
 static uint32_t fold_start(const cls_t* q, uint32_t h) {
@@ -792,9 +799,15 @@ static uint32_t fold_start(const cls_t* q, uint32_t h) {
   return (len == 0 || len >= 0x80000000u) ? f : (h - f < len ? h : f + (h - f) % len);
 }
 static uint32_t draw_start(cls_t* q, uint32_t n) {
-  const uint32_t h = (uint32_t)(q->scan / q->ppc);
+  const uint64_t p0 = q->scan;
+  const uint32_t h = (uint32_t)(p0 / q->ppc), f = q->front, len = q->back - q->front;
   q->scan += (uint64_t)n;
-  return fold_start(q, h);
+  if (len == 0 || len >= 0x80000000u) return f;
+  if (h - f < len) return h;
+  if (h - f < 0x80000000u) return f + (h - f) % len; /* ran past back: folded, the cursor keeps its value */
+  /* front passed the cursor: it restarts at front (front's cursor value plus this claim) */
+  q->scan = (uint64_t)f * q->ppc + n;
+  return f;
 }
 
 /* Group claim for chunk-based queues: the leader starts at a spread position of the live queue
@@ -889,6 +902,46 @@ static void finish_reclaim(ouro_sim* s, inst_t* I, cls_t* q, uint32_t reclaim) {
   else if (is_vl(I->kind)) vl_set_old(s, q, reclaim & 0x7FFFFFFFu);
 }
 
+/* Front relocation (the device's relocate_front): when the window is longer than twice the class's
+ * queued chunks plus RELOC_SLACK, the live chunk that blocks front is removed where it sits and, if it
+ * still has free pages, queued again at back. */
+static int relocate_front(ouro_sim* s, inst_t* I, cls_t* q) {
+  const uint32_t f = q->front, len = q->back - q->front;
+  const int32_t live = q->count;
+  if (len >= 0x80000000u || len <= 2u * (uint32_t)(live > 0 ? live : 0) + RELOC_SLACK) return 0;
+  uint32_t e = MARKER;
+  if (is_std(I->kind)) {
+    e = *std_slot(q, f);
+  } else {
+    uint32_t cur = MARKER, tmp[PEEK];
+    peek_slots(s, I, q, f, 1, &cur, tmp, NULL);
+    e = tmp[0];
+  }
+  if (!chunk_ok(s, e)) return 0;
+  chunk_t* x = ch(s, e);
+  if (x->queue_pos != (uint64_t)f || x->page_size != q->page_size) return 0;
+  uint32_t cursor = MARKER, reclaim = MARKER;
+  if (!cq_remove(s, I, q, e, &cursor, &reclaim)) return 0;
+  finish_reclaim(s, I, q, reclaim);
+  if (x->count > 0) {
+    x->queue_pos = QUEUE_PENDING;
+    cls_t* qs[1] = {q};
+    enqueue_chunks(s, I, qs, 1, &e);
+  }
+  return 1;
+}
+
+/* Compaction of a chunk queue's front (the per-lane form, wave = 0, or the wave-wide form, which
+ * also relocates a live chunk blocking a too-long window, at most MAX_RELOCATE times). */
+static void compact(ouro_sim* s, inst_t* I, cls_t* q, int wave) {
+  int more = compact_pass(s, I, q, wave);
+  if (!wave) return;
+  for (uint32_t r = 0; r < MAX_RELOCATE; ++r) {
+    if (!more && !relocate_front(s, I, q)) break;
+    more = compact_pass(s, I, q, wave);
+  }
+}
+
 /* Block-wide claim of one class by a whole wave (the device's block_claim_chunks): up to 64 queue
  * positions at a time, one chunk per lane; a prefix sum over the chunks' free counts divides the
  * pages in lane order; chunks emptied are removed at the position they were found (slots tombed in
@@ -898,18 +951,12 @@ static void claim_wave(ouro_sim* s, inst_t* I, group_t* g, uint64_t* out) {
   cls_t* q = &I->q[g->qi];
   const uint32_t total = (uint32_t)g->n;
   uint32_t v = draw_start(q, total);
-  uint32_t cursor = MARKER, assigned = 0, wraps = 0, rec = MARKER, miss = 0, redraws = 0;
+  uint32_t cursor = MARKER, assigned = 0, wraps = 0, rec = MARKER;
   int guard = 0;
-  while (assigned < total) {
+  while (assigned < total) { /* (block claims draw no new starts) */
     if (++guard > 10000000) {
       s->error |= ERR_TIMEOUT;
       return;
-    }
-    if (miss >= REDRAW_AFTER && redraws < MAX_REDRAWS) {
-      v = draw_start(q, total - assigned);
-      cursor = MARKER;
-      miss = 0;
-      ++redraws;
     }
     const uint32_t b = q->back;
     if (v - b < 0x80000000u || v - q->front >= 0x80000000u) {
@@ -944,7 +991,6 @@ static void claim_wave(ouro_sim* s, inst_t* I, group_t* g, uint64_t* out) {
       }
       if (dead) { /* the queue chunk holds no live entry: pass the rest of it in one step */
         v += dead;
-        miss += dead;
         continue;
       }
     }
@@ -1016,7 +1062,6 @@ static void claim_wave(ouro_sim* s, inst_t* I, group_t* g, uint64_t* out) {
     }
     assigned += took;
     v += nb;
-    miss = took ? 0u : miss + nb;
   }
   if (rec != MARKER) finish_reclaim(s, I, q, rec);
 }

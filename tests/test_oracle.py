@@ -433,3 +433,30 @@ def test_position_wrap_is_invisible(kind, case):
         assert sa.sem_count[k] == sb.sem_count[k] and sa.carved[k] == sb.carved[k]
     if kind in ("CQ", "VACQ", "VLCQ"):
         _check_chunk_queues(b, sb)
+
+
+@pytest.mark.parametrize("kind", ["CQ", "VLCQ"])
+def test_chunk_queue_window_stays_bounded_under_held_blocks(kind):
+    # skewed sizes and blocks held for a random number of waves: claims start at spread positions,
+    # so live chunks near front_ get passed over while the chunks behind them cycle through back_;
+    # front relocation keeps the window within twice the queued chunks plus the 4096-position slack
+    # (without it the window grows without bound: about 8,400 dead positions after 20,000 waves)
+    rng = np.random.default_rng(5)
+    s = oracle.Sim(kind, 512 << 20)
+    live = []
+    worst = 0
+    for step in range(14000):
+        sizes = (16 << np.minimum(rng.zipf(1.3, size=64) - 1, 9)).astype(np.uint32)
+        live.append(s.wave_malloc(sizes))
+        while len(live) > 40:
+            s.wave_free(live.pop(int(rng.integers(0, len(live)))))
+        if step % 100 == 0:
+            st = s.state()
+            assert st.error == 0
+            for k in range(10):
+                length = (st.back[k] - st.front[k]) % (1 << 32)
+                worst = max(worst, length - 2 * st.queue_count[k])
+    assert 2048 < worst <= 4096 + 64, worst
+    for o in live:
+        s.wave_free(o)
+    _check_chunk_queues(s, s.state())
