@@ -67,6 +67,8 @@ def _load_sim() -> C.CDLL:
         lib.sim_chunk_queue_pos.restype = C.c_uint64
         lib.sim_set_origin.argtypes = [C.c_void_p, C.c_uint32]
         lib.sim_set_origin.restype = C.c_int
+        lib.sim_set_agg.argtypes = [C.c_void_p, C.c_int]
+        lib.sim_set_agg.restype = C.c_int
         lib.sim_queue_window.argtypes = [C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32), C.c_uint32]
         lib.sim_queue_window.restype = C.c_uint32
         lib.sim_class_of.argtypes = [C.c_uint32, C.c_uint32]
@@ -117,13 +119,15 @@ def sem_share(count: int, expected: int, num: int, ppc: int) -> int:
 class Sim:
     """One simulated pool (MultiOuro<variant>)."""
 
-    def __init__(self, variant: str, pool_bytes: int, origin: int = 0):
+    def __init__(self, variant: str, pool_bytes: int, origin: int = 0, agg: int = 64):
         self._lib = _load_sim()
         self._h = self._lib.sim_create(VARIANTS[variant], pool_bytes)
         if not self._h:
             raise ValueError(variant)
         if origin and self._lib.sim_set_origin(self._h, origin) != 0:
             raise ValueError(f"origin {origin:#x}")
+        if self._lib.sim_set_agg(self._h, agg) != 0:
+            raise ValueError(f"aggregation width {agg}")
 
     def close(self) -> None:
         if self._h:

@@ -91,6 +91,7 @@ struct ouro_sim {
   uint64_t max_chunks, alloc_size;
   uint32_t next_free, error, qc_reserve, qc_next;
   int block_order; /* 1: groups in class order (block-wide calls of one wavefront) */
+  int agg;         /* aggregation width of the per-thread chunk claims: 64 (groups) or 1 (per lane) */
   uint32_t* locator;
   chunk_t* chunks;
   inst_t inst[2];
@@ -1066,6 +1067,83 @@ static void claim_wave(ouro_sim* s, inst_t* I, group_t* g, uint64_t* out) {
   if (rec != MARKER) finish_reclaim(s, I, q, rec);
 }
 
+/* The per-lane claim of width 1 (the device's W = 1 path of alloc_chunk_pages: the reference's
+ * per-thread ChunkQueue::allocPage, ChunkQueue_impl.cuh:88-151, with ChunkAccess::allocPage,
+ * ChunkAccess_impl.cuh:38-135): the lane starts at (p0 + rank) / ppc folded into the window, looks at
+ * PEEK positions per step for the first chunk queued there with free pages, takes one page off its
+ * count (its last page: the lane removes the chunk where it found it) and claims a bit of the mask:
+ * from word (tid + 1) mod words, the first free bit at or after bit tid mod 64, cyclically (the
+ * reference starts at word (threadIdx.x + 1) mod 16 of its 32-bit words and rotates by threadIdx.x +
+ * blockIdx.x; here words are 64 bits and the wave is its own block). Restated for a wave whose lanes
+ * request pairwise different classes: lanes of one class race on the chunk count in an order the
+ * hardware does not define. */
+static uint32_t fold_start(const cls_t* q, uint32_t h);
+static uint32_t draw_start(cls_t* q, uint32_t n);
+static int claim_lane_w1(ouro_sim* s, inst_t* I, cls_t* q, uint32_t tid, uint32_t h, uint64_t* out,
+                         uint32_t* reclaim) {
+  uint32_t v = fold_start(q, h), cursor = MARKER, wraps = 0, miss = 0, redraws = 0;
+  int guard = 0;
+  for (;;) {
+    if (++guard > 10000000) {
+      s->error |= ERR_TIMEOUT;
+      return 0;
+    }
+    if (miss >= REDRAW_AFTER && redraws < MAX_REDRAWS) {
+      v = draw_start(q, 1);
+      cursor = MARKER;
+      miss = 0;
+      ++redraws;
+    }
+    const uint32_t b = q->back;
+    if (v - b < 0x80000000u || v - q->front >= 0x80000000u) {
+      v = q->front;
+      cursor = MARKER;
+      ++wraps;
+    }
+    uint32_t n = b - v;
+    if (n >= 0x80000000u) n = 0;
+    if (n > PEEK) n = PEEK;
+    if (!is_std(I->kind) && n > QC_SPOTS - v % QC_SPOTS) n = QC_SPOTS - v % QC_SPOTS;
+    if ((wraps >= 2 || n == 0) && (s->error & (ERR_OOM | ERR_QUEUE_FULL))) {
+      q->cnt += 1; /* the pool ran out: the claim goes back, the lane gets nullptr */
+      return 0;
+    }
+    if (n == 0) {
+      s->error |= ERR_TIMEOUT;
+      return 0;
+    }
+    uint32_t c = MARKER, adv = n;
+    const uint32_t i = find_chunk(s, I, q, v, n, &cursor, &c, &adv);
+    if (i < n) {
+      v += i;
+      chunk_t* x = ch(s, c);
+      const int32_t old = x->count--;
+      const uint32_t nw = mask_words(x->size), offset = tid % 64u;
+      uint32_t wi = tid, page = MARKER;
+      for (int wg = 0; wg < 16 && page == MARKER; ++wg) {
+        ++wi;
+        const uint64_t cm = x->mask[wi % nw];
+        for (uint32_t j = 0; j < 64; ++j) {
+          const uint32_t bit = (j + offset) % 64u;
+          if ((cm >> bit) & 1ull) {
+            x->mask[wi % nw] &= ~(1ull << bit);
+            page = (wi % nw) * 64u + bit;
+            break;
+          }
+        }
+      }
+      if (page == MARKER) {
+        s->error |= ERR_TIMEOUT;
+        return 0;
+      }
+      *out = page_offset(s, c, page, q->page_size);
+      return old == 1 && cq_remove(s, I, q, c, &cursor, reclaim);
+    }
+    v += adv;
+    miss += adv;
+  }
+}
+
 /* nact: active lanes of the whole wave (the device enters each instance's block with all of them). */
 static void inst_malloc(ouro_sim* s, inst_t* I, const int* lanes, int n, const uint32_t* sizes, uint64_t* out,
                         int nact) {
@@ -1114,7 +1192,33 @@ static void inst_malloc(ouro_sim* s, inst_t* I, const int* lanes, int n, const u
   uint32_t rec[64];
   for (int j = 0; j < ng; ++j) {
     rec[j] = MARKER;
-    if (ok[j]) claim_group(s, I, &g[j], out, &rec[j]);
+    if (!ok[j]) continue;
+    if (s->agg != 1) {
+      claim_group(s, I, &g[j], out, &rec[j]);
+      continue;
+    }
+    /* width 1: the group's cursor draw (one fetch-add, restarted at front when front passed it),
+       then each lane on its own; a lane's pending queue-chunk reclaim is finished right away (with
+       pairwise different classes the order of lanes does not matter) */
+    cls_t* q = &I->q[g[j].qi];
+    const uint64_t p0 = q->scan;
+    const uint32_t h0 = (uint32_t)(p0 / q->ppc), f = q->front, len = q->back - q->front;
+    q->scan += (uint64_t)g[j].n;
+    uint64_t base = p0;
+    if (len != 0 && len < 0x80000000u && h0 - f >= len && h0 - f >= 0x80000000u) {
+      base = (uint64_t)f * q->ppc;
+      q->scan = base + (uint64_t)g[j].n;
+    }
+    for (int r = 0; r < g[j].n; ++r) {
+      const int lane = g[j].lanes[r];
+      uint32_t rc = MARKER;
+      if (claim_lane_w1(s, I, q, (uint32_t)lane, (uint32_t)((base + (uint64_t)r) / q->ppc), &out[lane], &rc))
+        g[j].removed = 1;
+      if (rc != MARKER) {
+        if (rec[j] != MARKER) finish_reclaim(s, I, q, rec[j]);
+        rec[j] = rc;
+      }
+    }
   }
   /* one wave-wide compaction per class queue the claims removed chunks from */
   for (int j = 0; j < ng; ++j) {
@@ -1344,6 +1448,7 @@ ouro_sim* sim_create(int variant, uint64_t pool_bytes) {
   if (variant < 0 || variant > 5) return NULL;
   ouro_sim* s = (ouro_sim*)calloc(1, sizeof(ouro_sim));
   s->variant = variant;
+  s->agg = 64;
   sim_layout(variant, pool_bytes, &s->max_chunks, &s->alloc_size);
   /* a virtualized pool keeps its top chunks for queue chunks (at most a quarter of a tiny pool) */
   s->qc_reserve = is_std(variant) ? 0u : (uint32_t)(QC_RESERVE < s->max_chunks / 4 ? QC_RESERVE : s->max_chunks / 4);
@@ -1485,6 +1590,11 @@ uint32_t sim_queue_window(const ouro_sim* s, uint32_t k, uint32_t* out, uint32_t
 }
 
 void sim_set_block_order(ouro_sim* s, int on) { s->block_order = on; }
+int sim_set_agg(ouro_sim* s, int w) {
+  if (w != 1 && w != 64) return -1;
+  s->agg = w;
+  return 0;
+}
 
 /* Start every class queue at position `origin` (a multiple of QC_SPOTS) on a fresh pool: front, back,
  * the page cursor, and for VA / VL the initial queue chunk's virtual start (VA: its pointer-ring

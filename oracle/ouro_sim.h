@@ -38,6 +38,10 @@ void sim_set_block_order(ouro_sim* s, int on);
 /* Fresh pool only: every class queue starts at position `origin` (a multiple of 2048) instead of 0,
  * so a short trace runs its 32-bit positions across the wrap. Returns 0, or -1 if not applicable. */
 int sim_set_origin(ouro_sim* s, uint32_t origin);
+/* Aggregation width of later chunk-queue wave mallocs: 64 (default; the group claims of widths 16
+ * and 64) or 1 (the per-lane claims of the plain path; defined for waves whose lanes request
+ * pairwise different classes, lane l being thread l of its workgroup). Returns 0, or -1. */
+int sim_set_agg(ouro_sim* s, int w);
 
 typedef struct {
   uint32_t next_free_chunk;

@@ -246,3 +246,54 @@ def test_position_wrap_concurrent(kind, form):
         assert all(st.back[k] < origin for k in crossing), "a class did not cross the wrap"
         chk = mm.check_quiescent()
         assert chk.violations == 0, chk.message.decode()
+
+
+def _distinct_class_rounds(seed, rounds, lanes=12):
+    # a wave of `lanes` threads; every round a new assignment of pairwise different classes of both
+    # instances to the lanes (the harness has no idle lane: a size of 0 is a request for 16 bytes)
+    rng = np.random.default_rng(seed)
+    return np.stack([(16 << rng.permutation(12)[:lanes]).astype(np.uint32) for _ in range(rounds)])
+
+
+def _segment_class_rounds(seed, rounds):
+    # every round: all 64 lanes, one class per 16-lane segment, the four classes pairwise different
+    rng = np.random.default_rng(seed)
+    sizes = np.zeros((rounds, 64), np.uint32)
+    for r in range(rounds):
+        cls = rng.permutation(12)[:4]
+        sizes[r] = np.repeat(16 << cls, 16).astype(np.uint32)
+    return sizes
+
+
+@pytest.mark.parametrize("kind", CHUNK_KINDS)
+@pytest.mark.parametrize("seed,lanes", [(1, 12), (2, 10)])
+def test_per_lane_chunk_claims_bit_exact(kind, seed, lanes):
+    # the plain path of the chunk kinds (W = 1: every lane walks its class queue and claims its page
+    # bit on its own, the reference's per-thread ChunkAccess::allocPage): with pairwise different
+    # classes in a wave the claims touch disjoint queues, so the trace has one answer, the oracle's
+    sizes = _distinct_class_rounds(seed, 30, lanes)
+    print(f"per-lane chunk claims {kind} seed={seed} lanes={lanes}", flush=True)
+    with ob.MemoryManager(kind, 512 << 20) as mm:
+        out = mm.run_trace(sizes, 30, agg=1, keep_offsets=True, check_rounds=True)
+        chk = mm.check_quiescent()
+    assert_clean(out.result)
+    want = oracle.Sim(kind, 512 << 20, agg=1).trace(sizes, 30)
+    mism = np.argwhere(out.offsets != want)
+    assert len(mism) == 0, f"first mismatch at (round, lane) {mism[0]}"
+    assert chk.violations == 0, chk.message.decode()
+
+
+@pytest.mark.parametrize("kind", ALL)
+def test_width16_segments_bit_exact(kind):
+    # W = 16: groups inside 16-lane segments. One class per segment, pairwise different, gives groups
+    # that touch disjoint class queues, so the trace is the oracle's group replay
+    sizes = _segment_class_rounds(3, 24)
+    print(f"width-16 segments {kind}", flush=True)
+    with ob.MemoryManager(kind, 512 << 20) as mm:
+        out = mm.run_trace(sizes, 24, agg=16, keep_offsets=True, check_rounds=True)
+        chk = mm.check_quiescent()
+    assert_clean(out.result)
+    want = oracle.Sim(kind, 512 << 20).trace(sizes, 24)
+    mism = np.argwhere(out.offsets != want)
+    assert len(mism) == 0, f"first mismatch at (round, lane) {mism[0]}"
+    assert chk.violations == 0, chk.message.decode()

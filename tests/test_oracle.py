@@ -460,3 +460,53 @@ def test_chunk_queue_window_stays_bounded_under_held_blocks(kind):
     for o in live:
         s.wave_free(o)
     _check_chunk_queues(s, s.state())
+
+
+def _w1_first_page(lane, q):
+    # the per-lane claim's first page in a fresh chunk of class q (ChunkAccess_impl.cuh:85-125 as
+    # restated on 64-bit words): word (lane + 1) mod words, the first free bit at or after lane mod 64,
+    # cyclically; a fresh chunk's words hold min(ppc, 64) free bits from bit 0
+    ppc = 512 >> q
+    words = max(1, ppc // 64)
+    bits = min(ppc, 64)
+    off = lane % 64
+    return ((lane + 1) % words) * 64 + (off if off < bits else 0)
+
+
+@pytest.mark.parametrize("kind,first", [("CQ", 0), ("VACQ", 12), ("VLCQ", 12)])
+def test_per_lane_chunk_claim_closed_form(kind, first):
+    # one lane alone in its wave takes the first page of a freshly carved chunk by the per-lane rule
+    for lane in (0, 5, 37, 63):
+        for q in (0, 2, 3, 5, 9):
+            s = oracle.Sim(kind, 512 << 20, agg=1)
+            sizes = np.zeros(64, np.uint32)
+            sizes[lane] = 16 << q
+            off = s.wave_malloc(sizes)
+            want = first * STRIDE + 128 + _w1_first_page(lane, q) * (16 << q)
+            assert off[lane] == want, (lane, q, off[lane], want)
+            assert (off[np.arange(64) != lane] == NONE).all()
+
+
+@pytest.mark.parametrize("kind", ["CQ", "VACQ", "VLCQ"])
+def test_per_lane_chunk_claims_distinct_classes(kind):
+    # waves whose lanes request pairwise different classes (both instances), blocks held across
+    # waves: the per-lane claims keep the live set disjoint and on the page grid, and every chunk
+    # queue ends holding exactly its chunks with free pages
+    rng = np.random.default_rng(17)
+    s = oracle.Sim(kind, 512 << 20, agg=1)
+    live = []
+    for step in range(400):
+        sizes = np.zeros(64, np.uint32)
+        lanes = rng.choice(64, size=12, replace=False)
+        sizes[lanes] = (16 << rng.permutation(12)).astype(np.uint32)
+        offs = s.wave_malloc(sizes)  # size 0: an idle lane of the replay
+        assert (offs[sizes > 0] != NONE).all()
+        live.append((offs, sizes))
+        _offset_checks(np.concatenate([o for o, _ in live]), np.concatenate([z for _, z in live]))
+        if len(live) > 5 or rng.random() < 0.4:
+            s.wave_free(live.pop(int(rng.integers(0, len(live))))[0])
+    for o, _ in live:
+        s.wave_free(o)
+    st = s.state()
+    assert st.error == 0
+    _check_chunk_queues(s, st)
