@@ -248,6 +248,78 @@ __device__ inline void chain_block_release(Memory& m, Chain<I, Rest...>& ch, Reu
   if constexpr (sizeof...(Rest) > 0) chain_block_release(m, ch.next, base, sm, gc, r, chunk, page, off + I::kNQ);
 }
 
+// enqueueInitialChunk for class qi of instance I (single lane).
+template <class I>
+__device__ inline void inst_preload(Memory& m, I& inst, ReuseRing& base, uint32_t qi, uint32_t chunk, int available) {
+  Ctx<I> x{m, inst, base};
+  ClassQueue* cq = &inst.q[qi];
+  const uint32_t ppc = ppc_of(x, cq), psize = page_size_of(x, cq);
+  const uint32_t avail = available < 0 ? 0u : ((uint32_t)available > ppc ? ppc : (uint32_t)available);
+  const uint32_t taken = ppc - avail;
+  mark_chunk_class(m, chunk, I::kAdd, psize);
+  if constexpr (I::kChunks) {
+    ChunkHdr* h = hdr<ChunkHdr>(m, chunk);
+    dev::st(&h->page_size, psize);
+    dev::st(&h->size, ppc);
+    dev::st(&h->count, (int32_t)avail);
+    dev::st(&h->queue_pos, kNotQueued);
+    for (uint32_t w = 0; w < kMaskWords; ++w) {
+      uint64_t bits = 0;
+      for (uint32_t b = 0; b < 64; ++b) {
+        const uint32_t pg = w * 64 + b;
+        if (pg >= taken && pg < ppc) bits |= 1ull << b;
+      }
+      dev::st(&h->mask[w], bits);
+    }
+    dev::order();
+    dev::add(&cq->carved, 1u);
+    if (avail) enqueue_chunks<1>(x, cq, true, chunk);  // ChunkQueue_impl.cuh:74-75
+  } else {
+    PageChunkHdr* h = hdr<PageChunkHdr>(m, chunk);
+    dev::st(&h->page_size, psize);
+    dev::st(&h->number_pages, ppc);
+    dev::order();
+    // the whole chunk belongs to the class: the caller's pages come back through free()
+    dev::add(&cq->carved, ppc);
+    const uint32_t v0 = avail ? dev::add(&cq->back, avail) : 0u;
+    for (uint32_t p = taken; p < ppc; ++p) put_lanes<1>(x, cq, true, v0 + (p - taken), make_index(chunk, p));
+  }
+  dev::order();
+  sem_signal(cq, avail);  // publish, then signal
+}
+
+// Chain routing for enqueueInitialChunk / preloadChunk: class indices run over the whole chain.
+template <class I, class... Rest>
+__device__ inline void chain_preload(Memory& m, Chain<I, Rest...>& ch, ReuseRing& base, uint32_t k, uint32_t chunk,
+                                     int available, uint32_t ppc) {
+  if (k < I::kNQ) {
+    if (ppc != ((I::kChunkSize >> I::kSmallLog2) >> k) || !chunk_ok(m, chunk)) {
+      set_error(m, ERR_CORRUPT);
+      return;
+    }
+    inst_preload(m, ch.inst, base, k, chunk, available);
+    return;
+  }
+  if constexpr (sizeof...(Rest) > 0) chain_preload(m, ch.next, base, k - I::kNQ, chunk, available, ppc);
+  else set_error(m, ERR_CORRUPT);
+}
+
+template <class I, class... Rest>
+__device__ inline uint32_t chain_preload_take(Memory& m, Chain<I, Rest...>& ch, ReuseRing& base, uint32_t k,
+                                              int available) {
+  if (k < I::kNQ) {
+    uint32_t chunk = kMarker;
+    if (alloc_chunks(m, ch.inst.reuse, I::kAdd, 1, chunk) != 1) return kMarker;
+    inst_preload(m, ch.inst, base, k, chunk, available);
+    return chunk;
+  }
+  if constexpr (sizeof...(Rest) > 0) return chain_preload_take(m, ch.next, base, k - I::kNQ, available);
+  else {
+    set_error(m, ERR_CORRUPT);
+    return kMarker;
+  }
+}
+
 template <class I>
 inline void host_instance_geometry(InstanceGeometry& g) {
   g.kind = (uint32_t)I::kKind;
@@ -264,6 +336,15 @@ struct Ouroboros {
   static constexpr bool kVirtualized = ((!is_std(Is::kKind)) || ...);
   // base chunks per chunk of the first instance; queue chunks take runs of this length too
   static constexpr uint32_t kBaseAdd = std::tuple_element_t<0, std::tuple<Is...>>::kAdd;
+  // Geometry tag of the chain (queue kinds, page ranges, queue capacities), kept in Memory::variant so
+  // a restored copy can be checked against the pool it goes into.
+  static constexpr uint32_t kChainTag = [] {
+    uint32_t t = 2166136261u;
+    ((t = (t ^ ((uint32_t)Is::kKind | (Is::kSmallLog2 << 4) | (Is::kNQ << 12))) * 16777619u,
+      t = (t ^ Is::kQueueEntries) * 16777619u),
+     ...);
+    return t;
+  }();
 
   Memory memory;
   Chain<Is...> chain;
@@ -419,43 +500,22 @@ struct Ouroboros {
     }
   }
 
-  // Bulk preload of a chunk's pages into a class queue (Ouroboros.cuh:291-295). Single thread,
-  // before any concurrent use, like the reference.
+  // Bulk preload of a chunk's pages into a class queue (Ouroboros.cuh:291-295, PageQueue_impl.cuh:40-53,
+  // ChunkQueue_impl.cuh:72-78): pages [ppc - available_pages, ppc) of chunk `chunk_index` become free
+  // pages of class `queue_index`; the first ppc - available_pages pages stay with the caller, who may
+  // free them later. Single thread, before any concurrent use, like the reference. The class index
+  // runs over the whole chain (the reference reaches only its first instance); pages_per_chunk must
+  // be the class's. The chunk must belong to the caller (preloadChunk takes one from the pool).
   __device__ void enqueueInitialChunk(uint32_t queue_index, uint32_t chunk_index, int available_pages,
                                       uint32_t pages_per_chunk) {
-    auto& inst = chain.inst;
-    ClassQueue* cq = &inst.q[queue_index];
-    using I = decltype(chain.inst);
-    detail::Ctx<I> x{memory, inst, inst.reuse};
-    detail::mark_chunk_class(memory, chunk_index, I::kAdd, cq->page_size);
-    if constexpr (I::kChunks) {
-      ChunkHdr* h = detail::hdr<ChunkHdr>(memory, chunk_index);
-      h->page_size = cq->page_size;
-      h->size = pages_per_chunk;
-      h->count = available_pages;
-      const uint32_t taken = pages_per_chunk - (uint32_t)available_pages;
-      for (uint32_t w = 0; w < kMaskWords; ++w) {
-        uint64_t bits = 0;
-        for (uint32_t b = 0; b < 64; ++b) {
-          const uint32_t pg = w * 64 + b;
-          if (pg >= taken && pg < pages_per_chunk) bits |= 1ull << b;
-        }
-        h->mask[w] = bits;
-      }
-      detail::enqueue_chunks<1>(x, cq, true, chunk_index);
-      dev::add(&cq->carved, 1u);
-    } else {
-      PageChunkHdr* h = detail::hdr<PageChunkHdr>(memory, chunk_index);
-      h->page_size = cq->page_size;
-      h->number_pages = pages_per_chunk;
-      for (uint32_t p = pages_per_chunk - (uint32_t)available_pages; p < pages_per_chunk; ++p) {
-        const uint32_t v = dev::add(&cq->back, 1u);
-        detail::put_lanes<1>(x, cq, true, v, make_index(chunk_index, p));
-      }
-      dev::add(&cq->carved, (uint32_t)available_pages);
-    }
-    dev::order();
-    detail::sem_signal(cq, (uint32_t)available_pages);
+    detail::chain_preload(memory, chain, chain.inst.reuse, queue_index, chunk_index, available_pages,
+                          pages_per_chunk);
+  }
+
+  // Takes a chunk (run) for class `queue_index` from the pool, as a carve would, and preloads it with
+  // enqueueInitialChunk. Returns the chunk index, or kMarker (ERR_OOM set) when the pool has none.
+  __device__ uint32_t preloadChunk(uint32_t queue_index, int available_pages) {
+    return detail::chain_preload_take(memory, chain, chain.inst.reuse, queue_index, available_pages);
   }
 
   // ---------------------------------------------------------------- host API
@@ -480,8 +540,10 @@ struct Ouroboros {
     const PoolLayout L = layout(instantiation_size, additionalSizeBeginning, additionalSizeEnd);
     // MemoryIndex holds 23 chunk bits: a larger pool would hand out aliased page handles
     if (L.max_chunks == 0 || L.max_chunks > kMaxChunks) return hipErrorInvalidValue;
+    // one allocation holds the whole state: the reference's partition of allocation_size bytes,
+    // then the chunk class bytes (so a copy of snapshot_bytes() bytes is the complete allocator)
     uint8_t* d = nullptr;
-    hipError_t e = hipMalloc(reinterpret_cast<void**>(&d), L.allocation_size);
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&d), L.allocation_size + class_bytes(L.max_chunks));
     if (e != hipSuccess) return e;
     memory.d_memory = d;
     memory.d_data = d + L.data_offset;
@@ -489,12 +551,13 @@ struct Ouroboros {
     memory.max_chunks = (uint32_t)L.max_chunks;
     memory.locator_bytes = L.locator_bytes;
     memory.locator = reinterpret_cast<uint32_t*>(d + L.allocation_size - L.locator_bytes);
-    e = hipMalloc(reinterpret_cast<void**>(&memory.chunk_log2), L.max_chunks);
-    if (e == hipSuccess) e = hipMemset(memory.chunk_log2, 0, L.max_chunks);
+    memory.chunk_log2 = d + L.allocation_size;
+    e = hipMemset(memory.chunk_log2, 0, class_bytes(L.max_chunks));
     if (e != hipSuccess) {
       release();
       return e;
     }
+    memory.variant = kChainTag;
     memory.next_free_chunk = 0;
     memory.qc_next = 0;
     memory.qc_add = kBaseAdd;
@@ -543,13 +606,56 @@ struct Ouroboros {
 
   void release() {
     if (memory.d_memory) (void)hipFree(memory.d_memory);
-    if (memory.chunk_log2) (void)hipFree(memory.chunk_log2);
     memory.d_memory = nullptr;
     memory.chunk_log2 = nullptr;
     memory.initialized = 0;
   }
 
+  // Bytes of the chunk class array behind the pool partition, and of the whole allocation (the
+  // reference's pool plus that array): a copy of snapshot_bytes() from d_memory on is the complete
+  // allocator state.
+  static constexpr uint64_t class_bytes(uint64_t max_chunks) { return align_up(max_chunks, 256); }
+  uint64_t snapshot_bytes() const { return memory.allocation_size + class_bytes(memory.max_chunks); }
+
+  // Restore a copy taken with snapshot_bytes() from the pool of an identical instantiation (same
+  // chain, same pool size) into this one: the copy replaces the pool, then every address inside the
+  // manager state (pool, data, locator, class bytes, queue arrays and rings) is set to this pool's
+  // own, since queue contents and links hold chunk indices, never addresses. `src` may be host or
+  // device memory. hipErrorInvalidValue if the copy belongs to another geometry.
+  hipError_t restore(const void* src) {
+    if (!memory.initialized || !src) return hipErrorInvalidValue;
+    std::vector<uint8_t> head(sizeof(MyType));
+    hipError_t e = hipMemcpy(head.data(), src, sizeof(MyType), hipMemcpyDefault);
+    if (e != hipSuccess) return e;
+    MyType* h = reinterpret_cast<MyType*>(head.data());
+    if (!h->memory.initialized || h->memory.allocation_size != memory.allocation_size ||
+        h->memory.max_chunks != memory.max_chunks || h->memory.variant != memory.variant ||
+        (uint64_t)(h->memory.d_data - h->memory.d_memory) != (uint64_t)(memory.d_data - memory.d_memory))
+      return hipErrorInvalidValue;
+    e = hipMemcpy(memory.d_memory, src, snapshot_bytes(), hipMemcpyDefault);
+    if (e != hipSuccess) return e;
+    h->memory.d_memory = memory.d_memory;
+    h->memory.d_data = memory.d_data;
+    h->memory.locator = memory.locator;
+    h->memory.chunk_log2 = memory.chunk_log2;
+    rebase<0, Is...>(h->chain, chain);
+    e = hipMemcpy(memory.d_memory, head.data(), sizeof(MyType), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    return e;
+  }
+
  private:
+  // queue-array and ring addresses of `to` taken from this pool's host mirror `from`
+  template <int N>
+  static void rebase(Chain<>&, const Chain<>&) {}
+  template <int N, class I, class... Rest>
+  static void rebase(Chain<I, Rest...>& to, const Chain<I, Rest...>& from) {
+    for (uint32_t i = 0; i < 16; ++i) to.inst.tab.ring[i] = from.inst.tab.ring[i];
+    to.inst.reuse.ring = from.inst.reuse.ring;
+    for (uint32_t i = 0; i < I::kNQ; ++i) to.inst.q[i].ring = from.inst.q[i].ring;
+    rebase<N + 1, Rest...>(to.next, from.next);
+  }
+
   template <int N>
   static void fill_geometry(InstanceGeometry*) {}
   template <int N, class I, class... Rest>

@@ -175,8 +175,27 @@ int ouro_query(ouro_handle h, ouro_state* out);
 /* Invariant checker of a quiescent pool (no reference counterpart): queues, masks and counts back at
  * what the carved chunks account for, no error bit set. */
 int ouro_check_quiescent(ouro_handle h, ouro_check_result* out);
-/* Copy the whole pool (the complete allocator state) to dst (device or host memory, allocation_size bytes). */
+/* enqueueInitialChunk (Ouroboros.cuh:291-295; PageQueue_impl.cuh:40-53, ChunkQueue_impl.cuh:72-78)
+ * for bulk-loaded consumers: takes one chunk for size class `class_index` (counted over the whole
+ * chain) from the pool and makes its last `available_pages` pages free pages of the class; the first
+ * pages_per_chunk - available_pages pages belong to the caller, who may hand them back with free().
+ * *chunk_out = the chunk index (page p is at data + chunk * 8320 + 128 + p * page size).
+ * OURO_E_NOMEM when the pool has no chunk left. Call before any kernel uses the pool concurrently. */
+int ouro_preload(ouro_handle h, uint32_t class_index, uint32_t available_pages, uint32_t* chunk_out);
+
+/* Bytes of a snapshot: the reference's pool partition (allocation_size) plus the chunk class bytes
+ * that follow it in the same allocation. */
+int ouro_snapshot_size(ouro_handle h, uint64_t* bytes);
+/* Copy the complete allocator state (the whole allocation, ouro_snapshot_size bytes) to dst, device
+ * or host memory. Take it while no kernel uses the pool. (Ouroboros keeps all state in the pool and
+ * mirrors the manager struct into it, Ouroboros_impl.cuh:137-155.) */
 int ouro_snapshot(ouro_handle h, void* dst);
+/* Replace the state of h with a snapshot taken from a pool of the same variant and pool size (device
+ * or host memory): the allocator continues exactly where the snapshot was taken. The addresses
+ * inside the manager state are rebased onto h's pool. OURO_E_INVALID for a snapshot of another
+ * variant or size. No reference counterpart (the reference's updateMemoryManagerDevice,
+ * Ouroboros_impl.cuh:148-155, copies only the manager struct of the same pool). */
+int ouro_restore(ouro_handle h, const void* src);
 
 /* ---- harness -------------------------------------------------------------------------------- */
 /* The harness loop of src/main.cu:146-172: alloc / write / verify / free rounds, kernels timed apart. */

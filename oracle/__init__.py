@@ -69,6 +69,8 @@ def _load_sim() -> C.CDLL:
         lib.sim_set_origin.restype = C.c_int
         lib.sim_set_agg.argtypes = [C.c_void_p, C.c_int]
         lib.sim_set_agg.restype = C.c_int
+        lib.sim_preload.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32]
+        lib.sim_preload.restype = C.c_uint32
         lib.sim_queue_window.argtypes = [C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32), C.c_uint32]
         lib.sim_queue_window.restype = C.c_uint32
         lib.sim_class_of.argtypes = [C.c_uint32, C.c_uint32]
@@ -151,6 +153,10 @@ class Sim:
         offsets = np.ascontiguousarray(offsets, dtype=np.uint64)
         assert len(offsets) <= 64
         self._lib.sim_wave_free(self._h, len(offsets), offsets.ctypes.data_as(C.POINTER(C.c_uint64)))
+
+    def preload(self, k: int, available: int) -> int:
+        """enqueueInitialChunk on a chunk taken from the pool; returns the chunk index."""
+        return self._lib.sim_preload(self._h, k, available)
 
     def state(self) -> SimState:
         st = SimState()

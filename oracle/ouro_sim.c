@@ -1590,6 +1590,56 @@ uint32_t sim_queue_window(const ouro_sim* s, uint32_t k, uint32_t* out, uint32_t
 }
 
 void sim_set_block_order(ouro_sim* s, int on) { s->block_order = on; }
+/* enqueueInitialChunk on a chunk taken from the pool (the device's preloadChunk; PageQueue_impl.cuh:40-53,
+ * ChunkQueue_impl.cuh:72-78): pages [ppc - available, ppc) become free pages of chain class k, the
+ * others stay with the caller. Returns the chunk, or MARKER when the pool has none. */
+uint32_t sim_preload(ouro_sim* s, uint32_t k, uint32_t available) {
+  int i = 0;
+  while (i < s->ninst && k >= s->inst[i].nq) k -= s->inst[i++].nq;
+  if (i == s->ninst) return MARKER;
+  inst_t* I = &s->inst[i];
+  cls_t* q = &I->q[k];
+  uint32_t chunk = MARKER;
+  if (alloc_chunks(s, &I->reuse, I->add, 1, &chunk, 0) != 1) return MARKER;
+  const uint32_t avail = available > q->ppc ? q->ppc : available, taken = q->ppc - avail;
+  chunk_t* x = ch(s, chunk);
+  x->page_size = q->page_size;
+  x->size = q->ppc;
+  if (is_chunk_kind(I->kind)) {
+    x->type = 2;
+    x->count = (int32_t)avail;
+    x->queue_pos = NOT_QUEUED;
+    for (uint32_t w = 0; w < 8; ++w) {
+      x->mask[w] = 0;
+      for (uint32_t b = 0; b < 64; ++b)
+        if (w * 64 + b >= taken && w * 64 + b < q->ppc) x->mask[w] |= 1ull << b;
+    }
+    q->carved += 1;
+    if (avail) {
+      cls_t* qs[1] = {q};
+      enqueue_chunks(s, I, qs, 1, &chunk);
+    }
+  } else {
+    x->type = 1;
+    q->carved += q->ppc;
+    const uint32_t v0 = q->back;
+    q->back += avail;
+    for (uint32_t e0 = 0; e0 < avail; e0 += 64) {
+      cls_t* qs[64];
+      uint32_t v[64], val[64];
+      int m = 0;
+      for (uint32_t e = e0; e < avail && e < e0 + 64; ++e, ++m) {
+        qs[m] = q;
+        v[m] = v0 + e;
+        val[m] = make_index(chunk, taken + e);
+      }
+      put_lanes(s, I, qs, m, v, val);
+    }
+  }
+  q->cnt += avail;
+  return chunk;
+}
+
 int sim_set_agg(ouro_sim* s, int w) {
   if (w != 1 && w != 64) return -1;
   s->agg = w;

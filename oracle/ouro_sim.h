@@ -42,6 +42,9 @@ int sim_set_origin(ouro_sim* s, uint32_t origin);
  * and 64) or 1 (the per-lane claims of the plain path; defined for waves whose lanes request
  * pairwise different classes, lane l being thread l of its workgroup). Returns 0, or -1. */
 int sim_set_agg(ouro_sim* s, int w);
+/* enqueueInitialChunk on a chunk taken from the pool: its last `available` pages become free pages of
+ * class k (counted over the chain). Returns the chunk, or 0xFFFFFFFF when the pool has none. */
+uint32_t sim_preload(ouro_sim* s, uint32_t k, uint32_t available);
 
 typedef struct {
   uint32_t next_free_chunk;

@@ -271,6 +271,9 @@ SIGNATURES = {
     "ouro_query": (C.c_int, [C.c_void_p, C.POINTER(State)]),
     "ouro_check_quiescent": (C.c_int, [C.c_void_p, C.POINTER(CheckResult)]),
     "ouro_snapshot": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "ouro_snapshot_size": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64)]),
+    "ouro_restore": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "ouro_preload": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.POINTER(C.c_uint32)]),
     "ouro_run_trace": (C.c_int, [C.c_void_p, C.POINTER(TraceConfig), C.POINTER(TraceResult)]),
     "ouro_gen_sizes": (C.c_int, [C.c_uint64, C.c_uint32, C.c_int, C.c_double, C.c_uint32, C.POINTER(C.c_uint32)]),
     "ouro_sem_share": (C.c_int, [C.c_uint64, C.c_int64, C.c_int64, C.POINTER(C.c_uint32)]),
@@ -452,6 +455,26 @@ class MemoryManager:
         r = CheckResult()
         _check(lib().ouro_check_quiescent(self._h, C.byref(r)), "ouro_check_quiescent")
         return r
+
+    def preload(self, class_index: int, available_pages: int) -> int:
+        """enqueueInitialChunk on a chunk taken from the pool: its last `available_pages` pages become
+        free pages of the class (counted over the whole chain); returns the chunk index."""
+        c = C.c_uint32()
+        _check(lib().ouro_preload(self._h, class_index, available_pages, C.byref(c)), "ouro_preload")
+        return c.value
+
+    def snapshot(self) -> np.ndarray:
+        """Host copy of the complete allocator state (the pool and its chunk class bytes)."""
+        n = C.c_uint64()
+        _check(lib().ouro_snapshot_size(self._h, C.byref(n)), "ouro_snapshot_size")
+        buf = np.empty(n.value, dtype=np.uint8)
+        _check(lib().ouro_snapshot(self._h, buf.ctypes.data_as(C.c_void_p)), "ouro_snapshot")
+        return buf
+
+    def restore(self, snap: np.ndarray) -> None:
+        """Continue from a snapshot of a pool of the same variant and size."""
+        snap = np.ascontiguousarray(snap, dtype=np.uint8)
+        _check(lib().ouro_restore(self._h, snap.ctypes.data_as(C.c_void_p)), "ouro_restore")
 
     def malloc_batch(self, d_sizes: int, d_out: int, n: int, agg: int = 64, stream: int = 0) -> None:
         _check(lib().ouro_malloc_batch(self._h, d_sizes, d_out, n, agg, stream), "ouro_malloc_batch")

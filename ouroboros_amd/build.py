@@ -55,7 +55,21 @@ def build(jobs: int | None = None, verbose: bool = False, extra_flags: list[str]
     tmp.replace(lib)
     if verbose:
         print(f"built {lib}", file=sys.stderr)
+    if not extra_flags and out is None:
+        build_dropin()
     return lib
+
+
+DROPIN = ROOT / "tests" / "dropin"
+
+
+def build_dropin() -> Path:
+    """The test kernels written only against include/ouro/instances.hpp (tests/dropin/dropin.hip)."""
+    so = DROPIN / "libdropin.so"
+    tmp = so.with_suffix(".so.tmp")
+    _run([HIPCC, *FLAGS, "-shared", "-o", str(tmp), str(DROPIN / "dropin.hip")])
+    tmp.replace(so)
+    return so
 
 
 if __name__ == "__main__":

@@ -376,11 +376,40 @@ int ouro_query(ouro_handle h, ouro_state* out) {
   return h->impl->query(out) == hipSuccess ? OURO_OK : OURO_E_HIP;
 }
 
+int ouro_preload(ouro_handle h, uint32_t class_index, uint32_t available_pages, uint32_t* chunk_out) {
+  if (!h || !chunk_out || class_index >= h->impl->num_classes()) return OURO_E_INVALID;
+  ouro_variant_info g;
+  h->impl->geometry(&g);
+  // pages per chunk of the class (a class of the second instance spans two of its classes' range)
+  const uint32_t page = g.smallest_page << class_index;
+  const uint32_t chunk = page <= g.chunk_bytes ? g.chunk_bytes : g.largest_page;
+  if (available_pages > chunk / page) return OURO_E_INVALID;
+  uint32_t c = 0;
+  const hipError_t e = h->impl->preload(class_index, (int)available_pages, &c);
+  if (e != hipSuccess) return OURO_E_HIP;
+  *chunk_out = c;
+  return c == 0xFFFFFFFFu ? OURO_E_NOMEM : OURO_OK;
+}
+
+int ouro_snapshot_size(ouro_handle h, uint64_t* bytes) {
+  if (!h || !bytes) return OURO_E_INVALID;
+  *bytes = h->impl->snapshot_bytes();
+  return OURO_OK;
+}
+
 int ouro_snapshot(ouro_handle h, void* dst) {
   if (!h || !dst) return OURO_E_INVALID;
-  return hipMemcpy(dst, h->impl->device_manager(), h->impl->allocation_size(), hipMemcpyDefault) == hipSuccess
+  if (hipDeviceSynchronize() != hipSuccess) return OURO_E_HIP;
+  return hipMemcpy(dst, h->impl->device_manager(), h->impl->snapshot_bytes(), hipMemcpyDefault) == hipSuccess
              ? OURO_OK
              : OURO_E_HIP;
+}
+
+int ouro_restore(ouro_handle h, const void* src) {
+  if (!h || !src) return OURO_E_INVALID;
+  if (hipDeviceSynchronize() != hipSuccess) return OURO_E_HIP;
+  const hipError_t e = h->impl->restore(src);
+  return e == hipSuccess ? OURO_OK : (e == hipErrorInvalidValue ? OURO_E_INVALID : OURO_E_HIP);
 }
 
 int ouro_set_origin(ouro_handle h, uint32_t origin) {

@@ -36,6 +36,8 @@ struct IManager {
   virtual void* device_manager() = 0;
   virtual void* data_base() = 0;
   virtual uint64_t allocation_size() = 0;
+  virtual uint64_t snapshot_bytes() = 0;
+  virtual hipError_t restore(const void* src) = 0;
   virtual uint64_t max_chunks() = 0;
   virtual int variant() = 0;
   virtual hipError_t malloc_batch(const uint32_t* d_sizes, void** d_out, uint32_t n, int agg, hipStream_t s,
@@ -71,6 +73,9 @@ struct IManager {
   virtual void class_ring(uint32_t k, uint32_t** ring, uint32_t* ring_size) = 0;
   // fresh pool only: every class queue starts at position `origin` (a multiple of 2048)
   virtual hipError_t set_origin(uint32_t origin) = 0;
+  // Ouroboros::preloadChunk(k, available) on one device thread; *chunk = the chunk, or kMarker
+  virtual hipError_t preload(uint32_t k, int available, uint32_t* chunk) = 0;
+  virtual uint32_t num_classes() = 0;
   // device addresses of the sticky error word and next_free_chunk inside the pool's manager state
   virtual uint32_t* device_error_word() = 0;
   virtual uint32_t* device_next_free_chunk() = 0;
@@ -232,6 +237,12 @@ __global__ void __launch_bounds__(1024) k_free_persist(M* mm, void* const* __res
 
 inline uint32_t grid_for(uint32_t n, uint32_t block) { return (n + block - 1) / block; }
 
+// One preload (Ouroboros::preloadChunk) by a single device thread.
+template <class M>
+__global__ void k_preload(M* mm, uint32_t k, int available, uint32_t* out) {
+  if (threadIdx.x == 0) out[0] = mm->preloadChunk(k, available);
+}
+
 // ------------------------------------------------------------------ serving requests
 // Zipf draw over the class ranks: class q is the first with r < thr[q] (r a 32-bit random number).
 struct ZipfTable {
@@ -372,6 +383,8 @@ struct Manager final : IManager {
   void* device_manager() override { return mm.getDeviceMemoryManager(); }
   void* data_base() override { return mm.memory.d_data; }
   uint64_t allocation_size() override { return mm.memory.allocation_size; }
+  uint64_t snapshot_bytes() override { return mm.snapshot_bytes(); }
+  hipError_t restore(const void* src) override { return mm.restore(src); }
   uint64_t max_chunks() override { return mm.memory.max_chunks; }
   int variant() override { return var; }
 
@@ -598,6 +611,17 @@ struct Manager final : IManager {
     return reinterpret_cast<uint32_t*>(mm.memory.d_memory + off);
   }
   uint32_t* device_error_word() override { return device_field(&mm.memory.error); }
+  uint32_t num_classes() override { return M::kNumClasses; }
+  hipError_t preload(uint32_t k, int available, uint32_t* chunk) override {
+    uint32_t* d = nullptr;
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&d), 4);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_preload<M>), dim3(1), dim3(1), 0, 0, mm.getDeviceMemoryManager(), k, available, d);
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpy(chunk, d, 4, hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    return e;
+  }
   uint32_t* device_next_free_chunk() override { return device_field(&mm.memory.next_free_chunk); }
   hipError_t set_origin(uint32_t origin) override {
     if (origin % ouro::kQCSpots) return hipErrorInvalidValue;

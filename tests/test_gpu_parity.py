@@ -297,3 +297,76 @@ def test_width16_segments_bit_exact(kind):
     mism = np.argwhere(out.offsets != want)
     assert len(mism) == 0, f"first mismatch at (round, lane) {mism[0]}"
     assert chk.violations == 0, chk.message.decode()
+
+
+@pytest.mark.parametrize("kind", ALL)
+def test_snapshot_restore_continues_bit_exact(kind):
+    # a snapshot taken after round k of a single-wave trace, restored into a fresh pool, continues with
+    # exactly the blocks the uninterrupted run hands out (the oracle's rounds k..R); a snapshot of
+    # another variant or pool size is refused
+    sizes = (16 << (np.arange(64) % 12)).astype(np.uint32)
+    rounds, k = 12, 5
+    print(f"snapshot/restore {kind}", flush=True)
+    want = oracle.Sim(kind, 512 << 20).trace(sizes, rounds)
+    with ob.MemoryManager(kind, 512 << 20) as a:
+        first = a.run_trace(sizes, k, agg=64, keep_offsets=True).offsets
+        snap = a.snapshot()
+    assert np.array_equal(first, want[:k])
+    with ob.MemoryManager(kind, 512 << 20) as b:
+        b.restore(snap)
+        rest = b.run_trace(sizes, rounds - k, agg=64, keep_offsets=True, check_rounds=True)
+        assert_clean(rest.result)
+        mism = np.argwhere(rest.offsets != want[k:])
+        assert len(mism) == 0, f"first mismatch at (round, lane) {mism[0]}"
+        assert b.check_quiescent().violations == 0
+    other = "VACQ" if kind != "VACQ" else "PQ"
+    with ob.MemoryManager(other, 512 << 20) as c, pytest.raises(ob.OuroError):
+        c.restore(snap)
+    with ob.MemoryManager(kind, 640 << 20) as d, pytest.raises(ob.OuroError):
+        d.restore(snap)
+
+
+def _free_host_pointers(mm, ptrs):
+    # hand blocks back through ouro_free_batch from a device copy of their addresses (HIP runtime)
+    import ctypes as C
+
+    hip = C.CDLL("libamdhip64.so")
+    d = C.c_void_p()
+    arr = np.ascontiguousarray(np.asarray(ptrs, np.uint64))
+    assert hip.hipMalloc(C.byref(d), C.c_size_t(max(8, arr.nbytes))) == 0
+    try:
+        assert hip.hipMemcpy(d, arr.ctypes.data_as(C.c_void_p), C.c_size_t(arr.nbytes), 1) == 0  # host -> device
+        mm.free_batch(d.value, len(arr), 64)
+        assert hip.hipDeviceSynchronize() == 0
+    finally:
+        hip.hipFree(d)
+
+
+@pytest.mark.parametrize("kind", ALL)
+def test_preload_matches_oracle(kind):
+    # enqueueInitialChunk through ouro_preload: chunks taken from the pool for four classes (one of
+    # the second instance), only their last pages free; the single-wave trace that follows must hand
+    # out exactly the oracle's blocks, never one of the caller's pages, and once the caller frees its
+    # pages the pool is quiescent
+    pre = [(0, 100), (3, 40), (9, 0), (10, 1)]  # (class over the chain, available pages)
+    sizes = (16 << (np.arange(64) % 12)).astype(np.uint32)
+    sim = oracle.Sim(kind, 512 << 20)
+    chunks = [sim.preload(k, a) for k, a in pre]
+    want = sim.trace(sizes, 6)
+    assert sim.state().error == 0
+    print(f"preload {kind}", flush=True)
+    with ob.MemoryManager(kind, 512 << 20) as mm:
+        assert [mm.preload(k, a) for k, a in pre] == chunks
+        out = mm.run_trace(sizes, 6, agg=64, keep_offsets=True, check_rounds=True)
+        assert_clean(out.result)
+        mism = np.argwhere(out.offsets != want)
+        assert len(mism) == 0, f"first mismatch at (round, lane) {mism[0]}"
+        taken = []
+        for (k, a), c in zip(pre, chunks):
+            ps = 16 << k
+            ppc = (8192 if k < 10 else 32768) // ps
+            taken += [c * 8320 + 128 + p * ps for p in range(ppc - a)]
+        assert not set(taken) & set(int(o) for o in out.offsets.ravel())
+        _free_host_pointers(mm, [mm.data_base() + t for t in taken])
+        chk = mm.check_quiescent()
+        assert chk.violations == 0, chk.message.decode()

@@ -510,3 +510,41 @@ def test_per_lane_chunk_claims_distinct_classes(kind):
     st = s.state()
     assert st.error == 0
     _check_chunk_queues(s, st)
+
+
+@pytest.mark.parametrize("kind,first", [("PQ", 0), ("VAPQ", 12), ("VLPQ", 12)])
+def test_preload_closed_form_page_kinds(kind, first):
+    # enqueueInitialChunk (PageQueue_impl.cuh:40-53): the last `available` pages of the preloaded chunk
+    # are the class's free pages in page order, so a wave takes them first, FIFO; the caller's pages
+    # come back through free() and are handed out after them
+    s = oracle.Sim(kind, 512 << 20)
+    avail = 100
+    assert s.preload(0, avail) == first
+    st = s.state()
+    assert st.sem_count[0] == avail and st.back[0] - st.front[0] == avail and st.carved[0] == 512
+    offs = s.wave_malloc(np.full(64, 16))
+    assert (offs == np.array([first * STRIDE + 128 + 16 * (512 - avail + l) for l in range(64)], np.uint64)).all()
+    s.wave_free(offs)
+    taken = np.array([first * STRIDE + 128 + 16 * p for p in range(512 - avail)], np.uint64)
+    for i in range(0, len(taken), 64):
+        s.wave_free(taken[i:i + 64])
+    st = s.state()
+    assert st.error == 0 and st.sem_count[0] == 512 and st.back[0] - st.front[0] == 512
+    # FIFO: the 36 untouched preloaded pages, then the wave's own 64, then the caller's in order
+    offs = s.wave_malloc(np.full(64, 16))
+    want = [512 - avail + 64 + l for l in range(36)] + [512 - avail + l for l in range(28)]
+    assert (offs == np.array([first * STRIDE + 128 + 16 * p for p in want], np.uint64)).all()
+
+
+@pytest.mark.parametrize("kind,first", [("CQ", 0), ("VACQ", 12), ("VLCQ", 12)])
+def test_preload_chunk_kinds(kind, first):
+    # ChunkQueue_impl.cuh:72-78: the chunk joins the queue with `available` free pages; its mask holds
+    # exactly pages [ppc - available, ppc), which the group claim takes lowest bit first
+    s = oracle.Sim(kind, 512 << 20)
+    assert s.preload(3, 40) == first  # class 3: 64 pages per chunk
+    t, ps, cnt, mask = s.chunk(first)
+    assert (t, ps, cnt) == (2, 128, 40) and int(mask[0]) == ((1 << 64) - 1) ^ ((1 << 24) - 1)
+    offs = s.wave_malloc(np.full(40, 128))
+    assert (offs == np.array([first * STRIDE + 128 + 128 * (24 + l) for l in range(40)], np.uint64)).all()
+    st = s.state()
+    assert st.error == 0 and st.sem_count[3] == 0 and st.queue_count[3] == 0
