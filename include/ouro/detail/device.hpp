@@ -41,6 +41,13 @@ constexpr uint32_t kWave = 64;             // CDNA wavefront width (WARP_SIZE=32
 // MI355X, so about 2 s) gives up and records ERR_TIMEOUT; nested waits are bounded by depth x budget.
 constexpr uint64_t kSpinTicks = 200000000ull;
 
+// Statistics counters (Memory::stats) are compiled in unless OURO_STATISTICS is 0, and counted while
+// the pool's stats_on flag is set: one atomic per wave (or workgroup) and call, never per request.
+#ifndef OURO_STATISTICS
+#define OURO_STATISTICS 1
+#endif
+enum : uint32_t { STAT_PAGE_ALLOC = 0, STAT_PAGE_FREE = 1, STAT_CHUNK_ALLOC = 2, STAT_CHUNK_REUSE = 3 };
+
 #ifdef OURO_DIAG
 // Diagnostic event counters (diagnostic builds only).
 // [0, 16) counters, [16] events logged, [17, 96) the first 79 event words.

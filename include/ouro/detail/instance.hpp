@@ -54,6 +54,25 @@ using namespace dev;
 
 __device__ __forceinline__ void set_error(Memory& m, uint32_t bits) { or_(&m.error, bits); }
 
+// Adds n to statistics counter i (the caller's lane issues it once for its wave or group).
+__device__ __forceinline__ void stat_add(Memory& m, uint32_t i, uint64_t n) {
+#if OURO_STATISTICS
+  if (n && m.stats_on) add(&m.stats[i], (unsigned long long)n);
+#else
+  (void)m, (void)i, (void)n;
+#endif
+}
+// Statistics for the active lanes of a wave with `want` set: one atomic by the lowest such lane.
+__device__ __forceinline__ void stat_wave(Memory& m, uint32_t i, bool want) {
+#if OURO_STATISTICS
+  if (!m.stats_on) return;
+  const uint64_t b = ballot(want);
+  if (b && (int)lane() == first(b)) add(&m.stats[i], (unsigned long long)popc(b));
+#else
+  (void)m, (void)i, (void)want;
+#endif
+}
+
 // Class byte of every base chunk of a freshly carved run (Memory::chunk_log2), published with the
 // chunk's header before any of its pages.
 __device__ __forceinline__ void mark_chunk_class(Memory& m, uint32_t chunk, uint32_t run, uint32_t page_size) {
@@ -102,6 +121,7 @@ __device__ inline void ring_put(Memory& m, ReuseRing& r, bool want, uint32_t val
     }
     fill = acc;
     pos = acc ? add(&r.back, acc) : 0u;
+    stat_add(m, STAT_CHUNK_REUSE, acc);
   }
   fill = readlane(fill, l);
   pos = readlane(pos, l);
@@ -191,6 +211,7 @@ __device__ inline uint32_t alloc_chunks(Memory& m, ReuseRing& r, uint32_t add_fa
   // valid chunks form a prefix in active-rank order
   const uint32_t k_ok = popc(okm);
   if (k_ok < k && (int)lane() == l) set_error(m, ERR_OOM);
+  if ((int)lane() == l) stat_add(m, STAT_CHUNK_ALLOC, k_ok);
   return k_ok;
 }
 

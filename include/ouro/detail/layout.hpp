@@ -148,10 +148,15 @@ struct alignas(128) Memory {
   uint32_t initialized;
   uint32_t qc_reserve;      // chunks at the top of the pool that only queue chunks may take
   uint32_t qc_add;          // base chunks per queue chunk: the first instance's chunk run length
+  uint32_t stats_on;        // 1: count the statistics below (set by the host)
+  uint64_t pool_bytes;      // the whole allocation: the partition above plus the chunk class bytes
   alignas(128) uint32_t next_free_chunk;
   uint32_t qc_next;         // bump pointer inside the queue-chunk reserve
   alignas(128) uint32_t error;
-  alignas(128) uint32_t stats[4];  // pageAlloc, pageFree, chunkAlloc, chunkReuse (Statistics.h:5-17)
+  // Statistics (Statistics.h:5-17): pageAllocCount, pageFreeCount (requests served by / returned to
+  // an instance), chunkAllocationCount (chunks handed out, fresh or reused, queue chunks included),
+  // chunkReuseCount (chunks returned to a reuse ring)
+  alignas(128) unsigned long long stats[4];
 };
 
 // IndexQueue (queues/Queue.h:5-24): chunk reuse ring, 65,536 entries per instance.

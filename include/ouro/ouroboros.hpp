@@ -76,12 +76,30 @@ __device__ inline void inst_free(Memory& m, I& inst, ReuseRing& base, uint32_t q
   else free_pages<W>(x, qi, want, chunk, page);
 }
 
+// AllocationHelper::getNextPow2 (Helper.cuh:12-18).
+__device__ __forceinline__ uint64_t next_pow2(uint64_t v) { return v <= 1 ? 1 : 1ull << (64 - __builtin_clzll(v - 1)); }
+
+// Above the largest page of the chain: the device heap at the next power of two (the chain's end,
+// Ouroboros<>::malloc, Ouroboros.cuh:398-408). nullptr and ERR_TOO_LARGE when the heap has none.
+__device__ inline void* heap_malloc(Memory& m, uint64_t size) {
+  void* p = ::malloc(next_pow2(size));
+  if (p == nullptr) set_error(m, ERR_TOO_LARGE);
+  return p;
+}
+
+// Pointers outside the whole pool allocation came from the device heap (Ouroboros_impl.cuh:98-104).
+__device__ __forceinline__ bool in_pool(const Memory& m, const void* p) {
+  const uint8_t* q = reinterpret_cast<const uint8_t*>(p);
+  return q >= m.d_memory && q < m.d_memory + m.pool_bytes;
+}
+
 // Walk the chain: lanes whose size fits instance I allocate there, the rest move on.
 template <int W, class I, class... Rest>
 __device__ inline void* chain_malloc(Memory& m, Chain<I, Rest...>& ch, ReuseRing& base, uint64_t size, bool want) {
   const bool here = want && size <= I::kLargest;
   void* r = nullptr;
   if (dev::ballot(here)) {
+    stat_wave(m, STAT_PAGE_ALLOC, here);
     void* p = inst_malloc<W>(m, ch.inst, base, size, here);
     if (here) r = p;
   }
@@ -92,7 +110,7 @@ __device__ inline void* chain_malloc(Memory& m, Chain<I, Rest...>& ch, ReuseRing
       if (later) r = p;
     }
   } else {
-    if (want && !here) set_error(m, ERR_TOO_LARGE);  // the reference falls back to the device heap
+    if (want && !here) r = heap_malloc(m, size);
   }
   return r;
 }
@@ -115,6 +133,7 @@ __device__ inline void chain_free(Memory& m, Chain<I, Rest...>& ch, ReuseRing& b
       }
       if (!ok) set_error(m, ERR_BAD_FREE);
     }
+    stat_wave(m, STAT_PAGE_FREE, ok);
     inst_free<W>(m, ch.inst, base, qi, ok, chunk, page);
   }
   if constexpr (sizeof...(Rest) > 0) {
@@ -126,11 +145,16 @@ __device__ inline void chain_free(Memory& m, Chain<I, Rest...>& ch, ReuseRing& b
 }
 
 // validOuroborosPointer (Ouroboros.cuh:350-356), tightened to the chunk region, and the chunk /
-// page-size lookup of Ouroboros::free (Ouroboros_impl.cuh:95-113). Sets ERR_BAD_FREE on a pointer
-// that is not a block of this pool.
+// page-size lookup of Ouroboros::free (Ouroboros_impl.cuh:95-113). A pointer outside the pool goes
+// back to the device heap (Ouroboros_impl.cuh:98-104); one inside the pool that is not a block of it
+// sets ERR_BAD_FREE.
 __device__ inline bool route_free(Memory& m, const void* ptr, uint32_t& chunk, uint32_t& page_size, uint64_t& offset) {
   const uint8_t* p = reinterpret_cast<const uint8_t*>(ptr);
   if (p == nullptr) return false;
+  if (!in_pool(m, p)) {
+    ::free(const_cast<uint8_t*>(p));
+    return false;
+  }
   bool want = p >= m.d_data && p < m.d_data + (uint64_t)m.max_chunks * kStride;
   if (want) {
     const uint64_t diff = (uint64_t)(p - m.d_data);
@@ -392,14 +416,22 @@ struct Ouroboros {
   template <uint32_t T>
   __device__ void malloc_tile(const size_t (&size)[T], void* (&out)[T], TileScratch<T>& sm) {
     uint32_t gc[T], r[T];
+    bool any = false;
 #pragma unroll
     for (uint32_t t = 0; t < T; ++t) {
-      gc[t] = kMarker;
-      if (size[t]) {
-        gc[t] = detail::chain_alloc_class<Is...>((uint64_t)size[t]);
-        if (gc[t] == kMarker) detail::set_error(memory, ERR_TOO_LARGE);
-      }
+      gc[t] = detail::chain_alloc_class<Is...>((uint64_t)size[t]);
+      if (!size[t]) gc[t] = kMarker;
+      any |= gc[t] != kMarker;
     }
+#if OURO_STATISTICS
+    if (memory.stats_on) {
+      uint32_t n = 0;
+#pragma unroll
+      for (uint32_t t = 0; t < T; ++t) n += dev::popc(dev::ballot(gc[t] != kMarker));
+      if (dev::lane() == 0) detail::stat_add(memory, STAT_PAGE_ALLOC, n);
+    }
+#endif
+    (void)any;
     detail::block_rank_tile<T>(sm, gc, r, kNumClasses);
     detail::chain_block_claim(memory, chain, chain.inst.reuse, sm, kNumClasses);
     __syncthreads();
@@ -412,6 +444,10 @@ struct Ouroboros {
       for (uint32_t t = 0; t < T; ++t)
         out[t] = detail::chain_block_take(memory, chain, chain.inst.reuse, sm, gc[t], r[t]);
     }
+    // requests above the largest page: the device heap, per request
+#pragma unroll
+    for (uint32_t t = 0; t < T; ++t)
+      if (size[t] && gc[t] == kMarker) out[t] = detail::heap_malloc(memory, (uint64_t)size[t]);
   }
 
   template <uint32_t T>
@@ -448,6 +484,14 @@ struct Ouroboros {
         if (gc[t] == kMarker) detail::set_error(memory, ERR_BAD_FREE);
       }
     }
+#if OURO_STATISTICS
+    if (memory.stats_on) {
+      uint32_t nf = 0;
+#pragma unroll
+      for (uint32_t t = 0; t < T; ++t) nf += dev::popc(dev::ballot(gc[t] != kMarker));
+      if (dev::lane() == 0) detail::stat_add(memory, STAT_PAGE_FREE, nf);
+    }
+#endif
     // Each wave's stores to the blocks it frees complete at its XCD's L2 before the block barrier
     // inside block_rank_tile; after it, one agent-scope release by wave 0 writes those lines back for
     // the whole workgroup (buffer_wbl2 acts on the entire L2 of the XCD), instead of one per wave.
@@ -476,7 +520,10 @@ struct Ouroboros {
     bool want = p != nullptr;
     uint32_t chunk = 0, page_size = 0;
     uint64_t offset = 0;
-    if (want) {
+    if (want && !detail::in_pool(memory, p)) {
+      ::free(ptr);  // a block of the device heap (Ouroboros_impl.cuh:98-104)
+      want = false;
+    } else if (want) {
       // validOuroborosPointer (Ouroboros.cuh:350-356), tightened to the chunk region
       const uint64_t span = (uint64_t)memory.max_chunks * kStride;
       want = p >= memory.d_data && p < memory.d_data + span;
@@ -552,6 +599,16 @@ struct Ouroboros {
     memory.locator_bytes = L.locator_bytes;
     memory.locator = reinterpret_cast<uint32_t*>(d + L.allocation_size - L.locator_bytes);
     memory.chunk_log2 = d + L.allocation_size;
+    memory.pool_bytes = L.allocation_size + class_bytes(L.max_chunks);
+    memory.stats_on = 0;
+    // requests above the largest page go to the device heap: give it the reference's 500 MB
+    // (cuda_heap_size, Parameters.h:48; MemoryInitialization.cuh:130-134) unless it is larger already
+    {
+      size_t heap = 0;
+      if (hipDeviceGetLimit(&heap, hipLimitMallocHeapSize) == hipSuccess && heap < kHeapBytes)
+        (void)hipDeviceSetLimit(hipLimitMallocHeapSize, kHeapBytes);
+      (void)hipGetLastError();
+    }
     e = hipMemset(memory.chunk_log2, 0, class_bytes(L.max_chunks));
     if (e != hipSuccess) {
       release();
@@ -615,6 +672,7 @@ struct Ouroboros {
   // reference's pool plus that array): a copy of snapshot_bytes() from d_memory on is the complete
   // allocator state.
   static constexpr uint64_t class_bytes(uint64_t max_chunks) { return align_up(max_chunks, 256); }
+  static constexpr size_t kHeapBytes = 500ull * 1024 * 1024;
   uint64_t snapshot_bytes() const { return memory.allocation_size + class_bytes(memory.max_chunks); }
 
   // Restore a copy taken with snapshot_bytes() from the pool of an identical instantiation (same

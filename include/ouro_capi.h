@@ -183,6 +183,18 @@ int ouro_check_quiescent(ouro_handle h, ouro_check_result* out);
  * OURO_E_NOMEM when the pool has no chunk left. Call before any kernel uses the pool concurrently. */
 int ouro_preload(ouro_handle h, uint32_t class_index, uint32_t available_pages, uint32_t* chunk_out);
 
+/* Statistics counters (Statistics.h:5-17, incremented as Ouroboros_impl.cuh:20-21,33-34,53-54,66-67
+ * and Ouroboros.cuh:113-114 do), compiled in unless the library is built with OURO_STATISTICS=0 and
+ * counted only while switched on, one atomic per wave or workgroup and call:
+ * out4[0] pageAllocCount   requests an instance served or refused (not the device-heap ones),
+ * out4[1] pageFreeCount    blocks returned to an instance,
+ * out4[2] chunkAllocationCount  chunks handed out for data or queue chunks, fresh or reused,
+ * out4[3] chunkReuseCount  chunks returned to a reuse ring.
+ * ouro_set_statistics(h, 1) zeroes the counters and starts counting, (h, 0) stops; both wait for the
+ * device. OURO_E_INVALID when the library has no statistics. */
+int ouro_set_statistics(ouro_handle h, int on);
+int ouro_statistics(ouro_handle h, uint64_t* out4);
+
 /* Bytes of a snapshot: the reference's pool partition (allocation_size) plus the chunk class bytes
  * that follow it in the same allocation. */
 int ouro_snapshot_size(ouro_handle h, uint64_t* bytes);

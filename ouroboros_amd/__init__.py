@@ -274,6 +274,8 @@ SIGNATURES = {
     "ouro_snapshot_size": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64)]),
     "ouro_restore": (C.c_int, [C.c_void_p, C.c_void_p]),
     "ouro_preload": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.POINTER(C.c_uint32)]),
+    "ouro_set_statistics": (C.c_int, [C.c_void_p, C.c_int]),
+    "ouro_statistics": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64)]),
     "ouro_run_trace": (C.c_int, [C.c_void_p, C.POINTER(TraceConfig), C.POINTER(TraceResult)]),
     "ouro_gen_sizes": (C.c_int, [C.c_uint64, C.c_uint32, C.c_int, C.c_double, C.c_uint32, C.POINTER(C.c_uint32)]),
     "ouro_sem_share": (C.c_int, [C.c_uint64, C.c_int64, C.c_int64, C.POINTER(C.c_uint32)]),
@@ -455,6 +457,16 @@ class MemoryManager:
         r = CheckResult()
         _check(lib().ouro_check_quiescent(self._h, C.byref(r)), "ouro_check_quiescent")
         return r
+
+    def set_statistics(self, on: bool = True) -> None:
+        """Zero the statistics counters and start counting (on), or stop."""
+        _check(lib().ouro_set_statistics(self._h, int(on)), "ouro_set_statistics")
+
+    def statistics(self) -> dict:
+        """pageAllocCount, pageFreeCount, chunkAllocationCount, chunkReuseCount (Statistics.h:5-17)."""
+        out = (C.c_uint64 * 4)()
+        _check(lib().ouro_statistics(self._h, out), "ouro_statistics")
+        return dict(zip(("page_alloc", "page_free", "chunk_alloc", "chunk_reuse"), list(out)))
 
     def preload(self, class_index: int, available_pages: int) -> int:
         """enqueueInitialChunk on a chunk taken from the pool: its last `available_pages` pages become

@@ -48,17 +48,19 @@ inline uint32_t page_of(uint32_t size, uint32_t smallest = 16) {
 // ------------------------------------------------------------------ touch pass (write / verify)
 // Writes (or checks) the owning thread's id into every 32-bit word of its block, as
 // d_testWriteToMemory / d_testReadFromMemory (src/main.cu:20-53). Blocks up to 256 B are handled by
-// their own lane; larger blocks by the whole wavefront with coalesced 16-byte accesses. Pointers
-// outside [lo, hi) are counted, never dereferenced.
+// their own lane; larger blocks by the whole wavefront with coalesced 16-byte accesses. Pool blocks
+// outside [lo, hi) are counted, never dereferenced; blocks of more than `heap_above` bytes come from
+// the device heap and need only be 16-byte aligned.
 __global__ void __launch_bounds__(256) k_touch(void* const* __restrict__ ptrs, const uint32_t* __restrict__ sizes,
                                                uint32_t n, const uint8_t* lo, const uint8_t* hi, int verify,
-                                               unsigned long long* __restrict__ errs) {
+                                               unsigned long long* __restrict__ errs, uint32_t heap_above) {
   const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
   const bool in = tid < n;
   const uint8_t* p = in ? reinterpret_cast<const uint8_t*>(ptrs[tid]) : nullptr;
   const uint32_t size = in ? sizes[tid] : 0u;
   const uint32_t bytes = size & ~3u;
-  const bool ok = p != nullptr && p >= lo && p + bytes <= hi && (reinterpret_cast<uintptr_t>(p) & 15u) == 0;
+  const bool ok = p != nullptr && (size > heap_above || (p >= lo && p + bytes <= hi)) &&
+                  (reinterpret_cast<uintptr_t>(p) & 15u) == 0;
   if (p != nullptr && !ok) atomicAdd(&errs[1], 1ull);
   if (!verify && in && size && p == nullptr) atomicAdd(&errs[2], 1ull);  // requests answered with nullptr
   unsigned long long bad = 0;
@@ -156,7 +158,7 @@ void check_live(const std::vector<uint64_t>& offs, const std::vector<uint32_t>& 
   std::unordered_map<uint32_t, uint64_t> live_ps;  // page size -> live blocks
   std::unordered_set<uint64_t> runs;
   for (size_t i = 0; i < offs.size(); ++i) {
-    if (offs[i] == UINT64_MAX) continue;
+    if (offs[i] == UINT64_MAX || sizes[i] > g.largest_page) continue;  // null, or a device-heap block
     const uint32_t ps = page_of(sizes[i], g.smallest_page);
     iv.push_back({offs[i], offs[i] + ps});
     const uint32_t k = (cg.n == 2 && ps > cg.largest[0]) ? 1u : 0u;
@@ -391,6 +393,31 @@ int ouro_preload(ouro_handle h, uint32_t class_index, uint32_t available_pages, 
   return c == 0xFFFFFFFFu ? OURO_E_NOMEM : OURO_OK;
 }
 
+int ouro_set_statistics(ouro_handle h, int on) {
+  if (!h) return OURO_E_INVALID;
+#if OURO_STATISTICS
+  if (hipDeviceSynchronize() != hipSuccess) return OURO_E_HIP;
+  const uint32_t flag = on ? 1u : 0u;
+  const unsigned long long zero[4] = {0, 0, 0, 0};
+  if (on && hipMemcpy(h->impl->device_stats(), zero, sizeof(zero), hipMemcpyHostToDevice) != hipSuccess)
+    return OURO_E_HIP;
+  return hipMemcpy(h->impl->device_stats_on(), &flag, 4, hipMemcpyHostToDevice) == hipSuccess ? OURO_OK : OURO_E_HIP;
+#else
+  (void)on;
+  return OURO_E_INVALID;  // built without statistics
+#endif
+}
+
+int ouro_statistics(ouro_handle h, uint64_t* out4) {
+  if (!h || !out4) return OURO_E_INVALID;
+#if OURO_STATISTICS
+  if (hipDeviceSynchronize() != hipSuccess) return OURO_E_HIP;
+  return hipMemcpy(out4, h->impl->device_stats(), 32, hipMemcpyDeviceToHost) == hipSuccess ? OURO_OK : OURO_E_HIP;
+#else
+  return OURO_E_INVALID;
+#endif
+}
+
 int ouro_snapshot_size(ouro_handle h, uint64_t* bytes) {
   if (!h || !bytes) return OURO_E_INVALID;
   *bytes = h->impl->snapshot_bytes();
@@ -461,6 +488,9 @@ int ouro_run_batch(ouro_handle h, const ouro_batch_config* cfg, ouro_batch_resul
   out->grid = grid;
   const uint8_t* lo = reinterpret_cast<const uint8_t*>(m.data_base());
   const uint8_t* hi = lo + m.max_chunks() * 8320ull;
+  ouro_variant_info geo;
+  m.geometry(&geo);
+  const uint32_t heap_above = geo.largest_page;  // larger requests are served by the device heap
   const uint32_t tgrid = (n + 255) / 256;
 
   hipStream_t s;
@@ -478,8 +508,8 @@ int ouro_run_batch(ouro_handle h, const ouro_batch_config* cfg, ouro_batch_resul
     return cfg->agg == 0 ? m.free_persist(ptrs, n, s, block, grid, tile) : m.free_batch(ptrs, n, cfg->agg, s, block);
   };
   auto touch = [&]() -> hipError_t {
-    hipLaunchKernelGGL(k_touch, dim3(tgrid), dim3(256), 0, s, ptrs, sizes, n, lo, hi, 0, errs);
-    hipLaunchKernelGGL(k_touch, dim3(tgrid), dim3(256), 0, s, ptrs, sizes, n, lo, hi, 1, errs);
+    hipLaunchKernelGGL(k_touch, dim3(tgrid), dim3(256), 0, s, ptrs, sizes, n, lo, hi, 0, errs, heap_above);
+    hipLaunchKernelGGL(k_touch, dim3(tgrid), dim3(256), 0, s, ptrs, sizes, n, lo, hi, 1, errs, heap_above);
     return hipGetLastError();
   };
   auto round = [&]() -> hipError_t {
@@ -521,6 +551,10 @@ int ouro_run_batch(ouro_handle h, const ouro_batch_config* cfg, ouro_batch_resul
   hipGraph_t g = nullptr;
   hipGraphExec_t ge = nullptr;
   if (cfg->graph && cfg->rounds) {
+    // every event is recorded once outside the capture first: an event whose first record is a
+    // captured node fails the later elapsed-time queries (seen with no warm-up round before it)
+    for (auto& e : ev) OURO_TRY(hipEventRecord(e, s));
+    OURO_TRY(hipStreamSynchronize(s));
     OURO_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     const hipError_t ce = round();
     hipGraph_t cg = nullptr;
@@ -644,13 +678,16 @@ int ouro_run_batch_static(ouro_handle h, const ouro_batch_config* cfg, void* str
   out->grid = grid;
   const uint8_t* lo = reinterpret_cast<const uint8_t*>(m.data_base());
   const uint8_t* hi = lo + m.max_chunks() * 8320ull;
+  ouro_variant_info geo;
+  m.geometry(&geo);
+  const uint32_t heap_above = geo.largest_page;  // larger requests are served by the device heap
   const uint32_t tgrid = (n + 255) / 256;
   auto round = [&]() -> hipError_t {
     hipError_t e = cfg->agg == 0 ? m.malloc_persist(sizes, ptrs, n, s, block, grid, tile)
                                  : m.malloc_batch(sizes, ptrs, n, cfg->agg, s, nullptr, block);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_touch, dim3(tgrid), dim3(256), 0, s, ptrs, sizes, n, lo, hi, 0, errs);
-    hipLaunchKernelGGL(k_touch, dim3(tgrid), dim3(256), 0, s, ptrs, sizes, n, lo, hi, 1, errs);
+    hipLaunchKernelGGL(k_touch, dim3(tgrid), dim3(256), 0, s, ptrs, sizes, n, lo, hi, 0, errs, heap_above);
+    hipLaunchKernelGGL(k_touch, dim3(tgrid), dim3(256), 0, s, ptrs, sizes, n, lo, hi, 1, errs, heap_above);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     return cfg->agg == 0 ? m.free_persist(ptrs, n, s, block, grid, tile) : m.free_batch(ptrs, n, cfg->agg, s, block);
   };
@@ -785,6 +822,9 @@ int ouro_run_trace(ouro_handle h, const ouro_trace_config* cfg, ouro_trace_resul
 
   const uint8_t* lo = reinterpret_cast<const uint8_t*>(m.data_base());
   const uint8_t* hi = lo + m.max_chunks() * 8320ull;
+  ouro_variant_info geo;
+  m.geometry(&geo);
+  const uint32_t heap_above = geo.largest_page;  // larger requests are served by the device heap
   const uint32_t grid = (n + block - 1) / block;
   std::vector<uint32_t> hsizes(n), psizes(n);
   std::vector<uint8_t> hold(n, 0), phold(n, 0), inv(n, 0);
@@ -814,10 +854,10 @@ int ouro_run_trace(ouro_handle h, const ouro_trace_config* cfg, ouro_trace_resul
                             cfg->latency ? d_lat.as<uint64_t>() : nullptr, block));
     OURO_TRY(hipEventRecord(ev[1], s));
     hipLaunchKernelGGL(k_touch, dim3(grid), dim3(block), 0, s, d_ptrs[cur].as<void*>(), d_sizes[cur].as<uint32_t>(), n,
-                       lo, hi, 0, d_errs.as<unsigned long long>());
+                       lo, hi, 0, d_errs.as<unsigned long long>(), heap_above);
     OURO_TRY(hipEventRecord(ev[2], s));
     hipLaunchKernelGGL(k_touch, dim3(grid), dim3(block), 0, s, d_ptrs[cur].as<void*>(), d_sizes[cur].as<uint32_t>(), n,
-                       lo, hi, 1, d_errs.as<unsigned long long>());
+                       lo, hi, 1, d_errs.as<unsigned long long>(), heap_above);
     OURO_TRY(hipEventRecord(ev[3], s));
     OURO_TRY(hipStreamSynchronize(s));
     {

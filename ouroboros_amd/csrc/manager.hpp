@@ -78,6 +78,8 @@ struct IManager {
   virtual uint32_t num_classes() = 0;
   // device addresses of the sticky error word and next_free_chunk inside the pool's manager state
   virtual uint32_t* device_error_word() = 0;
+  virtual uint32_t* device_stats_on() = 0;
+  virtual unsigned long long* device_stats() = 0;
   virtual uint32_t* device_next_free_chunk() = 0;
 #ifdef OURO_DIAG
   virtual hipError_t diag_read(unsigned long long* out16, int reset) = 0;
@@ -611,6 +613,10 @@ struct Manager final : IManager {
     return reinterpret_cast<uint32_t*>(mm.memory.d_memory + off);
   }
   uint32_t* device_error_word() override { return device_field(&mm.memory.error); }
+  uint32_t* device_stats_on() override { return device_field(&mm.memory.stats_on); }
+  unsigned long long* device_stats() override {
+    return reinterpret_cast<unsigned long long*>(device_field(&mm.memory.stats[0]));
+  }
   uint32_t num_classes() override { return M::kNumClasses; }
   hipError_t preload(uint32_t k, int available, uint32_t* chunk) override {
     uint32_t* d = nullptr;

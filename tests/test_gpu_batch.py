@@ -108,3 +108,15 @@ def test_batch_latency_round(agg, tile):
         assert r["lat_samples"] == len(sizes) // 64
         assert 0 < r["lat_p50_ns"] <= r["lat_p99_ns"] <= r["lat_max_ns"] < 1e9
         assert mm.check_quiescent().violations == 0
+
+
+def test_graph_replay_without_warmup():
+    # a captured round with no untimed round before it (its events are first recorded inside the
+    # capture unless the runner records them once beforehand)
+    print("graph without warm-up", flush=True)
+    sizes = ob.gen_sizes(9, 65536, "uniform")
+    with ob.MemoryManager("PQ", 1 << 30) as mm:
+        r = mm.run_batch(sizes, rounds=2, warmup=0, agg=0, block=1024, graph=True, tile=4)
+        assert r["graph_used"] == 1 and r["mallocs"] == 2 * len(sizes)
+        assert_clean(r)
+        assert mm.check_quiescent().violations == 0
