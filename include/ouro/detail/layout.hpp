@@ -150,6 +150,8 @@ struct alignas(128) Memory {
   uint32_t qc_add;          // base chunks per queue chunk: the first instance's chunk run length
   uint32_t stats_on;        // 1: count the statistics below (set by the host)
   uint64_t pool_bytes;      // the whole allocation: the partition above plus the chunk class bytes
+  uint64_t instantiation_size;  // initialize()'s arguments, for reinitialize()
+  uint32_t add_beg, add_end;
   alignas(128) uint32_t next_free_chunk;
   uint32_t qc_next;         // bump pointer inside the queue-chunk reserve
   alignas(128) uint32_t error;

@@ -600,6 +600,9 @@ struct Ouroboros {
     memory.locator = reinterpret_cast<uint32_t*>(d + L.allocation_size - L.locator_bytes);
     memory.chunk_log2 = d + L.allocation_size;
     memory.pool_bytes = L.allocation_size + class_bytes(L.max_chunks);
+    memory.instantiation_size = instantiation_size;
+    memory.add_beg = (uint32_t)additionalSizeBeginning;
+    memory.add_end = (uint32_t)additionalSizeEnd;
     memory.stats_on = 0;
     // requests above the largest page go to the device heap: give it the reference's 500 MB
     // (cuda_heap_size, Parameters.h:48; MemoryInitialization.cuh:130-134) unless it is larger already
@@ -654,6 +657,20 @@ struct Ouroboros {
     if (e == hipSuccess) e = hipDeviceSynchronize();
     if (e != hipSuccess) release();
     return e;
+  }
+
+  // reinitialize(overallocation_factor) (declared, never defined, in Ouroboros.cuh:83,179,283): a fresh
+  // pool of overallocation_factor times the size given to initialize(), replacing this one. Every
+  // block handed out before becomes invalid and the device manager moves: call it with no block live
+  // and pass getDeviceMemoryManager() anew. On failure the manager is left uninitialized.
+  hipError_t reinitialize(float overallocation_factor) {
+    if (!memory.initialized || !(overallocation_factor > 0.0f)) return hipErrorInvalidValue;
+    const size_t size = (size_t)((double)memory.instantiation_size * (double)overallocation_factor);
+    const size_t beg = memory.add_beg, end = memory.add_end;
+    hipError_t e = hipDeviceSynchronize();
+    if (e != hipSuccess) return e;
+    release();
+    return initialize(size, beg, end);
   }
 
   // Device -> host copy of the manager state (updateMemoryManagerHost, Ouroboros_impl.cuh:137-144).

@@ -154,6 +154,12 @@ typedef struct {
 /* ---- lifetime ------------------------------------------------------------------------------ */
 /* Ouroboros::initialize(pool_bytes) (device/MemoryInitialization.cuh:123-219) on the current device. */
 int ouro_create(int variant, uint64_t pool_bytes, ouro_handle* out);
+/* reinitialize(overallocation_factor) (declared, never defined, in device/Ouroboros.cuh:83,179,283):
+ * replaces the pool with a fresh one of overallocation_factor times the size given to ouro_create
+ * (grow with a factor above 1). Every block handed out before becomes invalid and the device
+ * manager moves (call ouro_device_manager again); call it with no block live. If the new pool
+ * cannot be allocated the handle has no pool left (OURO_E_NOMEM) and can only be destroyed. */
+int ouro_reinitialize(ouro_handle h, double overallocation_factor);
 /* Memory::~Memory (device/Ouroboros.cuh:11-15): frees the pool. */
 int ouro_destroy(ouro_handle h);
 /* getDeviceMemoryManager() (device/Ouroboros.cuh:297): pass to kernels built on include/ouro. */

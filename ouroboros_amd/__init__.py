@@ -273,6 +273,7 @@ SIGNATURES = {
     "ouro_snapshot": (C.c_int, [C.c_void_p, C.c_void_p]),
     "ouro_snapshot_size": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64)]),
     "ouro_restore": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "ouro_reinitialize": (C.c_int, [C.c_void_p, C.c_double]),
     "ouro_preload": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.POINTER(C.c_uint32)]),
     "ouro_set_statistics": (C.c_int, [C.c_void_p, C.c_int]),
     "ouro_statistics": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64)]),
@@ -474,6 +475,11 @@ class MemoryManager:
         c = C.c_uint32()
         _check(lib().ouro_preload(self._h, class_index, available_pages, C.byref(c)), "ouro_preload")
         return c.value
+
+    def reinitialize(self, overallocation_factor: float) -> None:
+        """A fresh pool of overallocation_factor times the original size (blocks handed out before
+        become invalid; the device manager moves)."""
+        _check(lib().ouro_reinitialize(self._h, overallocation_factor), "ouro_reinitialize")
 
     def snapshot(self) -> np.ndarray:
         """Host copy of the complete allocator state (the pool and its chunk class bytes)."""

@@ -343,6 +343,14 @@ int ouro_create(int variant, uint64_t pool_bytes, ouro_handle* out) {
   return OURO_OK;
 }
 
+int ouro_reinitialize(ouro_handle h, double overallocation_factor) {
+  if (!h || !(overallocation_factor > 0.0)) return OURO_E_INVALID;
+  const hipError_t e = h->impl->reinitialize(overallocation_factor);
+  if (e == hipErrorOutOfMemory) return OURO_E_NOMEM;
+  if (e == hipErrorInvalidValue) return OURO_E_INVALID;
+  return e == hipSuccess ? OURO_OK : OURO_E_HIP;
+}
+
 int ouro_destroy(ouro_handle h) {
   delete h;
   return OURO_OK;

@@ -38,6 +38,7 @@ struct IManager {
   virtual uint64_t allocation_size() = 0;
   virtual uint64_t snapshot_bytes() = 0;
   virtual hipError_t restore(const void* src) = 0;
+  virtual hipError_t reinitialize(double factor) = 0;
   virtual uint64_t max_chunks() = 0;
   virtual int variant() = 0;
   virtual hipError_t malloc_batch(const uint32_t* d_sizes, void** d_out, uint32_t n, int agg, hipStream_t s,
@@ -387,6 +388,7 @@ struct Manager final : IManager {
   uint64_t allocation_size() override { return mm.memory.allocation_size; }
   uint64_t snapshot_bytes() override { return mm.snapshot_bytes(); }
   hipError_t restore(const void* src) override { return mm.restore(src); }
+  hipError_t reinitialize(double factor) override { return mm.reinitialize((float)factor); }
   uint64_t max_chunks() override { return mm.memory.max_chunks; }
   int variant() override { return var; }
 

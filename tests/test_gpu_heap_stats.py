@@ -71,3 +71,21 @@ def test_statistics_count_exactly(kind, form):
         mm.set_statistics(False)
         mm.run_trace(sizes, 1, agg=64)
         assert mm.statistics() == st  # switched off: nothing counted
+
+
+@pytest.mark.parametrize("kind", ["PQ", "VAPQ", "VLCQ"])
+def test_reinitialize_grows_the_pool(kind):
+    # a pool that runs out, reinitialized at eight times its size, serves the same trace in full
+    pool = (400 << 20) if kind == "PQ" else (24 << 20)
+    sizes = np.full(16384, 8192, np.uint32)
+    print(f"reinitialize {kind}", flush=True)
+    with ob.MemoryManager(kind, pool) as mm:
+        small = mm.query().max_chunks
+        r = mm.run_trace(sizes, 1, agg=64).result
+        assert r["null_returns"] > 0 and "OOM" in r["errors"]
+        mm.reinitialize(8.0)
+        st = mm.query()
+        assert st.max_chunks > 4 * small and st.error == 0 and st.next_free_chunk == (0 if kind == "PQ" else 12)
+        r = mm.run_trace(sizes, 2, agg=64, check_rounds=True).result
+        assert r["null_returns"] == 0 and r["error_word"] == 0 and r["verify_errors"] == 0, r
+        assert mm.check_quiescent().violations == 0
