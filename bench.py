@@ -72,19 +72,32 @@ def latency(variant: str, agg: int, sizes: np.ndarray) -> dict:
     return {"p50_ns": r["lat_p50_ns"], "p99_ns": r["lat_p99_ns"], "max_ns": r["lat_max_ns"], "samples": r["lat_samples"]}
 
 
+# host threads one GPU's share of the box may use (its CPU share is 16 threads per GPU, whatever
+# nproc reports for the whole machine)
+CPU_SHARE = 16
+
+
 def cpu_baseline(sizes: np.ndarray, rounds: int) -> dict:
     import oracle
 
-    cores = min(16, os.cpu_count() or 1)
+    nproc = os.cpu_count() or 1
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        affinity = nproc
+    cores = max(1, min(CPU_SHARE, affinity))
     r = oracle.glibc_trace(sizes, rounds, cores)
     t = r["alloc_s"] + r["free_s"]
     return {
         "value": round(2.0 * len(sizes) * rounds / t / 1e6, 3) if t > 0 else None,
         "unit": "Mops/s",
         "cores": cores,
+        "nproc": nproc,
+        "affinity_cpus": affinity,
         "kind": "port",
-        "sample": f"glibc malloc/free over the identical trace: {len(sizes):,} requests x {rounds} rounds, "
-        f"one thread per core, malloc and free phases timed separately (errors={r['errors']})",
+        "sample": f"glibc malloc/free over the identical trace: {len(sizes):,} requests x {rounds} rounds (the "
+        f"GPU's timed round count), one thread per core on the {cores} host threads of one GPU's CPU share "
+        f"(nproc {nproc}), malloc and free phases timed separately (errors={r['errors']})",
     }
 
 
@@ -225,13 +238,23 @@ def probe(args, sizes: np.ndarray) -> int:
     plain launches, each kernel timed by its own dispatch's events), nothing else."""
     variant = args.variant or "PQ"
     gbs = ob.bench_copy(COPY_BYTES, 1)
+    if args.agg:
+        # the per-thread kernels (k_malloc / k_free, one request per kernel thread) at this width
+        with ob.MemoryManager(variant, POOL) as mm:
+            mm.run_trace(sizes, max(1, args.warmup), agg=args.agg)
+            r = mm.run_trace(sizes, args.steps, agg=args.agg).result
+        print(json.dumps({"probe": variant, "agg": args.agg, "copy_gbs": round(gbs, 1), "mops": round(mops(r, len(sizes), args.steps), 1),
+                          "alloc_ms": r["alloc_ms"] / args.steps, "free_ms": r["free_ms"] / args.steps}), flush=True)
+        return 0
     with ob.MemoryManager(variant, POOL) as mm:
         r = mm.run_batch(sizes, rounds=args.steps, warmup=args.warmup, agg=0, block=args.block,
                          graph=bool(args.graph), tile=args.tile, kernel_timing=True)
+        floor = mm.bench_floor(sizes, block=args.block, tile=args.tile, rounds=args.steps)
     kr = max(1, r["kernel_rounds"])
     print(json.dumps({"probe": variant, "block": args.block, "tile": args.tile, "copy_gbs": round(gbs, 1),
                       "mops": round(r["mops"], 1), "alloc_kernel_ms": r["alloc_kernel_ms"] / kr,
                       "free_kernel_ms": r["free_kernel_ms"] / kr, "kernel_rounds": r["kernel_rounds"],
+                      "floor_kernel_ms": floor["kernel_ms"], "floor_grid": floor["grid"],
                       "alloc_ms": r["alloc_ms"] / args.steps, "free_ms": r["free_ms"] / args.steps}), flush=True)
     return 0
 
@@ -262,6 +285,8 @@ def main() -> int:
     ap.add_argument("--tile", type=int, default=1, help="requests per thread and call with --variant (1, 2, 4)")
     ap.add_argument("--probe", action="store_true",
                     help="profiling run: a 1 GiB device copy (counter calibration), then the headline rounds only")
+    ap.add_argument("--agg", type=int, default=0, help="with --probe: 1, 16 or 64 = the per-thread kernels at that "
+                    "aggregation width instead of the batched form")
     ap.add_argument("--graph", type=int, default=1, help="with --probe: 1 = graph-replayed rounds as the headline, "
                     "0 = plain launches (counter passes)")
     ap.add_argument("--no-table", action="store_true", help="skip the per-kind plain / wave / batched table")
@@ -269,7 +294,7 @@ def main() -> int:
     ap.add_argument("--no-config4", action="store_true", help="skip the 256-workload batch set")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving form")
     ap.add_argument("--no-config3", action="store_true", help="skip the held-block stress run")
-    ap.add_argument("--serve-requests", type=int, default=400)
+    ap.add_argument("--serve-requests", type=int, default=2000, help="requests per offered load and mode")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one MI355X only; --gpus {args.gpus} is not supported (the allocator "
@@ -299,6 +324,8 @@ def main() -> int:
         # allocation-call latency of the same form: one more round, sampled on lane 0 of every wave
         lat_head = mm.run_batch(sizes, rounds=1, warmup=0, agg=best.agg, block=best.block, grid=best.grid,
                                 graph=False, latency=True, tile=best.tile)
+        # the floor under the allocation kernel: the same launch, loads, LDS rank and barriers, no allocator
+        floor = mm.bench_floor(sizes, block=best.block, tile=best.tile, rounds=steps) if best.agg == 0 else None
         # the same geometry with one request per thread and call: shorter calls, more of them
         lat_t1 = None
         if best.agg == 0 and best.tile != 1:
@@ -357,7 +384,7 @@ def main() -> int:
     alg_bytes = THREADS * (ALG_BYTES_MALLOC if dominant == "alloc" else ALG_BYTES_FREE)
     achieved_gbs = alg_bytes / dom_s / 1e9
     touch_gbs = head["touch_bytes"] / (head["touch_ms"] / steps * 1e-3) / 1e9 if head["touch_ms"] else None
-    cpu = None if args.no_cpu else cpu_baseline(sizes, 3)
+    cpu = None if args.no_cpu else cpu_baseline(sizes, steps)
     kern = "k_malloc_persist" if dominant == "alloc" else "k_free_persist"
     if best.agg != 0:
         kern = "k_malloc" if dominant == "alloc" else "k_free"
@@ -386,7 +413,9 @@ def main() -> int:
             "block": best.block,
             "tile": best.tile,
             "grid": head["grid"],
-            "threads": THREADS,
+            "requests": THREADS,
+            "gpu_threads": head["grid"] * best.block,
+            "requests_per_thread_per_call": best.tile if best.agg == 0 else 1,
             "pool_bytes": POOL,
         },
         "plain_mops": round(plain["mops"], 3),
@@ -396,9 +425,12 @@ def main() -> int:
         "wave64_mops": round(wave["mops"], 3),
         "p99_alloc_latency_ns": round(lat_head["lat_p99_ns"], 1),
         "p50_alloc_latency_ns": round(lat_head["lat_p50_ns"], 1),
+        "latency_unit": "per allocation call of the headline form, which is also the latency of every request "
+        "it serves: a call returns all of its requests' blocks at once",
         "latency_note": f"headline form ({best.label()}): wall_clock64 around each malloc_tile call on lane 0 "
-        f"of every wave (one call serves {best.tile} request(s) per thread), {lat_head['lat_samples']} samples, "
-        "the whole 1M-request round in flight",
+        f"of every wave (one call serves {best.tile} request(s) per thread, {best.block * best.tile} per "
+        f"workgroup), {lat_head['lat_samples']} samples, the whole 1M-request round in flight; the per-thread "
+        "device malloc of the wave-aggregated path is p99_alloc_latency_ns_wave64",
         "p99_alloc_latency_ns_tile1": round(lat_t1["lat_p99_ns"], 1) if lat_t1 else None,
         "p50_alloc_latency_ns_tile1": round(lat_t1["lat_p50_ns"], 1) if lat_t1 else None,
         "p99_alloc_latency_ns_wave64": round(lat["p99_ns"], 1),
@@ -429,6 +461,12 @@ def main() -> int:
             "launch_timing": launch_timing,
             "avg_launch_ms_graph_node": round((head["alloc_ms"] if dominant == "alloc" else head["free_ms"]) / steps, 4),
             "algorithmic_bytes_per_launch": alg_bytes,
+            "floor_kernel_ms": round(floor["kernel_ms"], 4) if floor else None,
+            "alloc_kernel_ms": round(alloc_avg_s * 1e3, 4),
+            "free_kernel_ms": round(free_avg_s * 1e3, 4),
+            "floor_note": "k_null_persist at the allocation kernel's launch shape: loads the sizes, ranks them "
+            "per class in LDS with the same barriers and writes a made-up pointer per request, touching no "
+            "allocator state; alloc_kernel_ms / floor_kernel_ms is how far the allocator sits above it",
         },
         "atomic_roofline": atomic_roofline(best, kind, sizes, head, steps, atom_distinct, atom_same_wave),
         "cpu_baseline": cpu,

@@ -282,10 +282,15 @@ __device__ inline void block_claim_chunks(Memory& m, I& inst, ReuseRing& base, u
       }
       ChunkHdr* h = nullptr;
       int32_t cnt = 0;
+      uint32_t g = 0;
       if (l < nb && chunk_ok(m, s)) {
         h = hdr<ChunkHdr>(m, s);
-        cnt = ld(&h->count);
-        if (cnt <= 0 || ld(&h->queue_pos) != (uint64_t)(v + l) || ld(&h->page_size) != psize) cnt = 0;
+        const uint64_t cg = ld(count_gen(h));
+        cnt = cg_count(cg);
+        g = cg_gen(cg);
+        if (cnt <= 0 || ld(&h->queue_pos) != qp_at(g, v + l) || ld(&h->page_size) != psize ||
+            gen_class(g) != gc - off)
+          cnt = 0;
       }
       const uint32_t c = cnt > 0 ? (uint32_t)cnt : 0u;
       uint32_t offered = 0;
@@ -295,14 +300,15 @@ __device__ inline void block_claim_chunks(Memory& m, I& inst, ReuseRing& base, u
       // ROCm 7.2 compiler hoisted the min out of the branch and dropped the before < rem guard
       const uint32_t owed = before < rem ? rem - before : 0u;
       uint32_t t = owed < c ? owed : c;
-      if (t && cas(&h->count, cnt, cnt - (int32_t)t) != cnt) t = 0;  // changed meanwhile: skip it
+      // changed (or recycled) meanwhile: skip it
+      if (t && cas(count_gen(h), make_cg(cnt, g), make_cg(cnt - (int32_t)t, g)) != make_cg(cnt, g)) t = 0;
       uint32_t took = 0;
       const uint32_t r0 = assigned + wave_exclusive_sum(t, &took);
       // chunks this claim emptied leave the queue at the position they were found at (cq_remove for
       // the whole batch: slots tombed in parallel, one count update per queue and queue chunk, one
       // wave-wide compaction of front_)
       bool removed = false;
-      if (t && t == c) removed = cas(&h->queue_pos, (uint64_t)(v + l), kNotQueued) == (uint64_t)(v + l);
+      if (t && t == c) removed = cas(&h->queue_pos, qp_at(g, v + l), qp_not(g)) == qp_at(g, v + l);
       const uint64_t rm = ballot(removed);
       if (l == 0) OURO_DIAG_ADD(8, popc(rm));
       if (rm) {
@@ -329,7 +335,7 @@ __device__ inline void block_claim_chunks(Memory& m, I& inst, ReuseRing& base, u
           reclaim = rc;
         }
         order();
-        if (removed && ld(&h->count) > 0) ensure_in_queue<1>(x, cq, true, s);  // a free raced in
+        if (removed && ld(&h->count) > 0) ensure_in_queue<1>(x, cq, true, s, g);  // a free raced in
         compact_wave(x, cq);
       }
       if (t) {
@@ -385,10 +391,12 @@ __device__ inline void* block_page_of(Memory& m, I& inst, const S& sm, bool here
 }
 
 // Chunk-queue frees of one instance: bit, then the count per chunk (one add per chunk group of the
-// wave), re-enqueue on the 0 -> >0 crossing; the class signal follows at block level.
+// wave), recycling of chunks made full, re-enqueue on the 0 -> >0 crossing; the class signal follows
+// at block level, less the pages that left with recycled chunks (taken off the block's class total
+// in LDS, `total`).
 template <class I>
 __device__ inline void block_release_chunks(Memory& m, I& inst, ReuseRing& base, bool here, uint32_t qi,
-                                            uint32_t chunk, uint32_t page) {
+                                            uint32_t chunk, uint32_t page, uint32_t* total) {
   Ctx<I> x{m, inst, base};
   ClassQueue* cq = &inst.q[here ? qi : 0u];
   ChunkHdr* h = hdr<ChunkHdr>(m, here ? chunk : 0u);
@@ -397,12 +405,22 @@ __device__ inline void block_release_chunks(Memory& m, I& inst, ReuseRing& base,
   const uint64_t grp = match<64>(here ? (uint64_t)chunk : ~0ull);
   const int L = first(grp);
   const uint32_t n = popc(grp);
-  bool crossing = false;
+  const bool recycle = m.recycle_min != kRecycleOff;
+  bool crossing = false, full = false;
+  uint32_t gen = 0, size = 0;
   if (here && (int)lane() == L) {
+    if (recycle) {  // read while the group's pages still pin the chunk's life
+      gen = ld(&h->gen);
+      size = ld(&h->size);
+    }
     const int32_t old = add(&h->count, (int32_t)n);
     crossing = old <= 0 && old + (int32_t)n > 0;
+    if (!recycle && crossing) gen = ld(&h->gen);
+    full = recycle && old + (int32_t)n == (int32_t)size;
   }
-  ensure_in_queue<64>(x, cq, crossing, chunk);
+  const bool gone = flash_chunks(x, cq, full, chunk, gen, size, size - n, crossing ? 1u : 0u);
+  ensure_in_queue<64>(x, cq, crossing && !gone, chunk, gen);
+  if (gone) __hip_atomic_fetch_sub(total, n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
 }  // namespace detail

@@ -79,6 +79,9 @@ __device__ __forceinline__ void mark_chunk_class(Memory& m, uint32_t chunk, uint
   const uint32_t lg = 31u - (uint32_t)__builtin_clz(page_size);
   for (uint32_t j = 0; j < run; ++j) st(&m.chunk_log2[chunk + j], (uint8_t)(lg | (j << 5)));
 }
+__device__ __forceinline__ void clear_chunk_class(Memory& m, uint32_t chunk, uint32_t run) {
+  for (uint32_t j = 0; j < run; ++j) st(&m.chunk_log2[chunk + j], (uint8_t)0);
+}
 // a bounded wait gave up; diagnostic builds also log the source line of the wait
 #define OURO_TIMEOUT(mem)                                         \
   do {                                                            \
@@ -105,20 +108,30 @@ __device__ __forceinline__ int leader_of(uint64_t grp) { return first(grp); }
 
 // ------------------------------------------------------------------ reuse ring (Queue_impl.cuh)
 // IndexQueue::enqueue (Queue_impl.cuh:20-38) for every lane with want=true, positions in lane order.
-__device__ inline void ring_put(Memory& m, ReuseRing& r, bool want, uint32_t value) {
+// Returns whether this lane's value went in; values beyond the ring's room are refused, which sets
+// ERR_REUSE_FULL unless the caller takes them back itself (strict = false). `headroom` entries of the
+// ring are left free (recycled data chunks leave room for the queue chunks that share the ring).
+__device__ inline bool ring_put(Memory& m, ReuseRing& r, bool want, uint32_t value, bool strict = true,
+                                uint32_t headroom = 0) {
   const uint64_t mask = ballot(want);
-  if (!mask) return;
+  if (!mask) return false;
   const int l = first(mask);
   const uint32_t n = popc(mask);
   uint32_t fill = 0, pos = 0;
   if ((int)lane() == l) {
-    const int32_t f = add(&r.count, (int32_t)n);
-    const int32_t room = (int32_t)r.size - f;
-    const uint32_t acc = room <= 0 ? 0u : (room >= (int32_t)n ? n : (uint32_t)room);
-    if (acc < n) {
-      sub(&r.count, (int32_t)(n - acc));
-      set_error(m, ERR_REUSE_FULL);
+    // reserve only what fits, with a CAS: a count that overshot and was taken back would let a
+    // concurrent take claim entries that are never written
+    int32_t c = ld(&r.count);
+    uint32_t acc = 0;
+    while (true) {
+      const int32_t room = (int32_t)(r.size - headroom) - c;
+      acc = room <= 0 ? 0u : (room >= (int32_t)n ? n : (uint32_t)room);
+      if (acc == 0) break;
+      const int32_t seen = cas(&r.count, c, c + (int32_t)acc);
+      if (seen == c) break;
+      c = seen;
     }
+    if (acc < n && strict) set_error(m, ERR_REUSE_FULL);
     fill = acc;
     pos = acc ? add(&r.back, acc) : 0u;
     stat_add(m, STAT_CHUNK_REUSE, acc);
@@ -134,8 +147,10 @@ __device__ inline void ring_put(Memory& m, ReuseRing& r, bool want, uint32_t val
         if (it.expired()) { OURO_TIMEOUT(m); break; }
         backoff(it.it);
       }
+      return true;
     }
   }
+  return false;
 }
 
 // IndexQueue::dequeue (Queue_impl.cuh:54-67) for up to k entries, issued by lane `l` (uniform),
@@ -145,9 +160,17 @@ __device__ inline uint32_t ring_take(Memory& m, ReuseRing& r, uint32_t k, int l,
                                      uint32_t keep = 0) {
   uint32_t t = 0, pos = 0;
   if ((int)lane() == l) {
-    const int32_t old = sub(&r.count, (int32_t)k) - (int32_t)keep;
-    t = old <= 0 ? 0u : ((uint32_t)old >= k ? k : (uint32_t)old);
-    if (t < k) add(&r.count, (int32_t)(k - t));
+    // take only what is there, with a CAS: a count that dipped and was given back would let a
+    // concurrent put write past entries not yet taken
+    int32_t c = ld(&r.count);
+    while (true) {
+      const int32_t avail = c - (int32_t)keep;
+      t = avail <= 0 ? 0u : ((uint32_t)avail >= k ? k : (uint32_t)avail);
+      if (t == 0) break;
+      const int32_t seen = cas(&r.count, c, c - (int32_t)t);
+      if (seen == c) break;
+      c = seen;
+    }
     pos = t ? add(&r.front, t) : 0u;
   }
   t = readlane(t, l);
@@ -727,20 +750,26 @@ __device__ inline void put_lanes(Ctx<I>& x, ClassQueue* cq, bool want, uint32_t 
             todo = false;
           }
         }
+        // one step per round for every lane: a lane whose next link is not set yet waits at the
+        // round, not in a loop of its own, so the wave's other lanes (which may set the links that
+        // another wave is waiting for, in another class queue) keep moving
+        bool waiting = false;
         if (todo) {
-          uint32_t nx;
-          while ((nx = ld(&hdr<QueueChunkHdr>(x.m, cur)->next)) == kMarker) {
-            if (it.expired()) break;
-            backoff(it.it);
-          }
-          if (!chunk_ok(x.m, nx)) {
-            if (nx == kMarker) OURO_TIMEOUT(x.m);
-            else set_error(x.m, ERR_VSTART);
+          const uint32_t nx = ld(&hdr<QueueChunkHdr>(x.m, cur)->next);
+          if (nx == kMarker) {
+            waiting = true;
+            if (it.expired()) {
+              OURO_TIMEOUT(x.m);
+              todo = false;
+            }
+          } else if (!chunk_ok(x.m, nx)) {
+            set_error(x.m, ERR_VSTART);
             todo = false;
           } else {
             cur = nx;
           }
         }
+        if (ballot(waiting) == ballot(todo)) backoff(it.it);  // nobody moved: back off
       }
       // every slot and link of the wave is written: count per queue chunk, then move back_ptr_ /
       // front_ptr_ over a queue chunk this completed
@@ -967,8 +996,11 @@ __device__ __forceinline__ uint32_t mask_words(uint32_t ppc) {
 
 // Enqueue chunk indices (ChunkQueue::enqueue, ChunkQueue_impl.cuh:20-38; ChunkQueueVA::enqueue,
 // :344-376; ChunkQueueVL::enqueue, :264-273): count_ / back_ per queue group, queue_pos recorded.
+// queue_pos names the position only once the put is complete (slot written, queue-chunk count
+// updated): until then it reads enqueue-in-flight, so whoever removes the chunk by its queue_pos
+// (a flash, a front relocation) never tombstones a slot before the put has written it.
 template <int G, class I>
-__device__ inline void enqueue_chunks(Ctx<I>& x, ClassQueue* cq, bool want, uint32_t chunk) {
+__device__ inline void enqueue_chunks(Ctx<I>& x, ClassQueue* cq, bool want, uint32_t chunk, uint32_t gen) {
   uint32_t n;
   // count_ (VL keeps no count_ in the reference, but tracking it costs nothing here)
   const uint64_t key = want ? reinterpret_cast<uint64_t>(cq) : ~0ull;
@@ -976,9 +1008,9 @@ __device__ inline void enqueue_chunks(Ctx<I>& x, ClassQueue* cq, bool want, uint
   const int L = first(grp);
   if (want && (int)lane() == L) add(&cq->count, (int32_t)popc(grp));
   const uint32_t v = fetch_positions<G>(&cq->back, want, n);
-  if (want) st(&hdr<ChunkHdr>(x.m, chunk)->queue_pos, (uint64_t)v);
-  order();
   put_lanes<G>(x, cq, want, v, chunk);
+  order();
+  if (want) st(&hdr<ChunkHdr>(x.m, chunk)->queue_pos, qp_at(gen, v));
 }
 
 template <class I>
@@ -1002,14 +1034,18 @@ __device__ inline uint32_t carve_chunks(Ctx<I>& x, ClassQueue* cq, uint32_t k) {
     }
   }
   if (!readlane(cap_ok, l0)) k_ok = 0;
+  // one generation per chunk carved (the carve counter advances by the chunks of this carve)
+  uint32_t g0 = 0;
+  if ((int)lane() == l0 && k_ok) g0 = add(&x.m.carves, k_ok);
+  const uint32_t gen = make_gen(readlane(g0, l0) + me, (uint32_t)(cq - x.inst.q));
   if (me < k_ok) {
     // ChunkIndexChunk::initializeChunk with all pages free (ChunkAccess ctor, ChunkAccess.cuh:69-89)
     ChunkHdr* h = hdr<ChunkHdr>(x.m, chunk);
     mark_chunk_class(x.m, chunk, I::kAdd, psize);
     st(&h->page_size, psize);
     st(&h->size, ppc);
-    st(&h->count, (int32_t)ppc);
-    st(&h->queue_pos, kNotQueued);
+    st(count_gen(h), make_cg((int32_t)ppc, gen));
+    st(&h->queue_pos, qp_pending(gen));  // enqueued below
     for (uint32_t w = 0; w < kMaskWords; ++w) {
       const uint32_t lo = w * 64u;
       uint64_t bits = 0;
@@ -1019,7 +1055,7 @@ __device__ inline uint32_t carve_chunks(Ctx<I>& x, ClassQueue* cq, uint32_t k) {
   }
   order();
   // chunks enqueue in lane order: one group, positions = active ranks
-  enqueue_chunks<64>(x, cq, me < k_ok, chunk);
+  enqueue_chunks<64>(x, cq, me < k_ok, chunk, gen);
   if ((int)lane() == l0) sem_signal_expected(cq, k_ok * ppc, k * ppc);
   return k_ok;
 }
@@ -1098,32 +1134,36 @@ __device__ inline void peek_slots(Ctx<I>& x, ClassQueue* cq, uint32_t v, uint32_
 }
 
 // Offset of the first position in [v, v + n) that holds a chunk of this class queued exactly there
-// (queue_pos == position) with free pages; n when there is none, with `adv` the positions the scan
-// may pass (n, or the rest of a queue chunk that holds no live entry). The queue_pos test also
-// rejects a stale slot read out of a queue chunk that was recycled meanwhile.
+// (queue_pos == position, in the generation read with its count) with free pages; n when there is
+// none, with `adv` the positions the scan may pass (n, or the rest of a queue chunk that holds no
+// live entry). The queue_pos test also rejects a stale slot read out of a queue chunk that was
+// recycled meanwhile. `cnt` and `gen` are the chunk's {count, gen} word as read.
 template <class I>
 __device__ inline uint32_t find_chunk(Ctx<I>& x, ClassQueue* cq, uint32_t v, uint32_t n, uint32_t& cursor,
-                                      uint32_t& chunk, int32_t& cnt, uint32_t& adv) {
+                                      uint32_t& chunk, int32_t& cnt, uint32_t& gen, uint32_t& adv) {
   uint32_t s[kPeek], dead = 0;
   peek_slots(x, cq, v, n, cursor, s, dead);
   adv = n > dead ? n : dead;
   if (dead) return n;
-  int32_t c_[kPeek];
+  uint64_t cg[kPeek];
   uint64_t qp[kPeek];
   uint32_t ps[kPeek];
 #pragma unroll
   for (uint32_t i = 0; i < kPeek; ++i) {
-    const ChunkHdr* h = hdr<ChunkHdr>(x.m, chunk_ok(x.m, s[i]) ? s[i] : 0u);
-    c_[i] = ld(&h->count);
+    ChunkHdr* h = hdr<ChunkHdr>(x.m, chunk_ok(x.m, s[i]) ? s[i] : 0u);
+    cg[i] = ld(count_gen(h));
     qp[i] = ld(&h->queue_pos);
     ps[i] = ld(&h->page_size);
   }
-  const uint32_t psize = page_size_of(x, cq);
+  const uint32_t psize = page_size_of(x, cq), qi = (uint32_t)(cq - x.inst.q);
 #pragma unroll
   for (uint32_t i = 0; i < kPeek; ++i) {
-    if (i < n && chunk_ok(x.m, s[i]) && c_[i] > 0 && qp[i] == (uint64_t)(v + i) && ps[i] == psize) {
+    const uint32_t g = cg_gen(cg[i]);
+    if (i < n && chunk_ok(x.m, s[i]) && cg_count(cg[i]) > 0 && qp[i] == qp_at(g, v + i) && ps[i] == psize &&
+        gen_class(g) == qi) {
       chunk = s[i];
-      cnt = c_[i];
+      cnt = cg_count(cg[i]);
+      gen = g;
       return i;
     }
   }
@@ -1181,10 +1221,10 @@ __device__ inline void finish_reclaim(Ctx<I>& x, ClassQueue* cq, bool want, uint
 // (FIRST_FREE, ChunkQueue_impl.cuh:166-170, and RE_ENQUEUE_CHUNK, :423-428). The queue_pos
 // not-queued -> pending CAS admits exactly one enqueue however many lanes or removers race.
 template <int G, class I>
-__device__ inline void ensure_in_queue(Ctx<I>& x, ClassQueue* cq, bool want, uint32_t chunk) {
+__device__ inline void ensure_in_queue(Ctx<I>& x, ClassQueue* cq, bool want, uint32_t chunk, uint32_t gen) {
   bool won = false;
-  if (want) won = cas(&hdr<ChunkHdr>(x.m, chunk)->queue_pos, kNotQueued, kQueuePending) == kNotQueued;
-  if (ballot(won)) enqueue_chunks<G>(x, cq, won, chunk);
+  if (want) won = cas(&hdr<ChunkHdr>(x.m, chunk)->queue_pos, qp_not(gen), qp_pending(gen)) == qp_not(gen);
+  if (ballot(won)) enqueue_chunks<G>(x, cq, won, chunk, gen);
 }
 
 // Remove an exhausted chunk from its class queue, wherever it sits (the reference removes with
@@ -1198,13 +1238,13 @@ __device__ inline void ensure_in_queue(Ctx<I>& x, ClassQueue* cq, bool want, uin
 // `again` reports a free that raced in (the caller re-enqueues: the free saw the chunk still queued and
 // left it to us).
 template <class I>
-__device__ inline bool cq_remove(Ctx<I>& x, ClassQueue* cq, uint32_t chunk, uint32_t pos, uint32_t& cursor,
-                                 uint32_t& reclaim, bool& again) {
+__device__ inline bool cq_remove(Ctx<I>& x, ClassQueue* cq, uint32_t chunk, uint32_t gen, uint32_t pos,
+                                 uint32_t& cursor, uint32_t& reclaim, bool& again) {
   again = false;
   OURO_DIAG_ADD(7, 1);
   ChunkHdr* h = hdr<ChunkHdr>(x.m, chunk);
   const uint32_t p = pos;
-  if (cas(&h->queue_pos, (uint64_t)p, kNotQueued) != (uint64_t)p) return false;
+  if (cas(&h->queue_pos, qp_at(gen, p), qp_not(gen)) != qp_at(gen, p)) return false;
   OURO_DIAG_ADD(8, 1);
   uint32_t rc = kMarker;
   tomb_slot(x, cq, p, cursor, rc);
@@ -1236,7 +1276,7 @@ __device__ inline bool relocate_front(Ctx<I>& x, ClassQueue* cq) {
   constexpr Kind K = I::kKind;
   const int L = first(active());
   const bool lead = (int)lane() == L;
-  uint32_t chunk = kMarker, f = 0;
+  uint32_t chunk = kMarker, f = 0, gen = 0;
   if (lead) {
     f = ld(&cq->front);
     const uint32_t b = ld(&cq->back), len = b - f;
@@ -1251,22 +1291,27 @@ __device__ inline bool relocate_front(Ctx<I>& x, ClassQueue* cq) {
         const uint32_t* sl = slots_at(x, cq, f, cursor, fill);
         e = sl ? ld(sl) : fill;
       }
-      if (chunk_ok(x.m, e) && ld(&hdr<ChunkHdr>(x.m, e)->queue_pos) == (uint64_t)f &&
-          ld(&hdr<ChunkHdr>(x.m, e)->page_size) == page_size_of(x, cq))
-        chunk = e;
+      if (chunk_ok(x.m, e)) {
+        ChunkHdr* h = hdr<ChunkHdr>(x.m, e);
+        gen = cg_gen(ld(count_gen(h)));
+        if (ld(&h->queue_pos) == qp_at(gen, f) && ld(&h->page_size) == page_size_of(x, cq) &&
+            gen_class(gen) == (uint32_t)(cq - x.inst.q))
+          chunk = e;
+      }
       if (chunk != kMarker) OURO_DIAG_ADD(10, 1);
       else OURO_DIAG_LOG(((unsigned long long)f << 32) | e);
     }
   }
   chunk = readlane(chunk, L);
   f = readlane(f, L);
+  gen = readlane(gen, L);
   if (chunk == kMarker) return false;
   uint32_t cursor = kMarker, reclaim = kMarker;
   bool again = false, removed = false;
-  if (lead) removed = cq_remove(x, cq, chunk, f, cursor, reclaim, again);
+  if (lead) removed = cq_remove(x, cq, chunk, gen, f, cursor, reclaim, again);
   reclaim = readlane(reclaim, L);
   finish_reclaim(x, cq, lead && reclaim != kMarker, reclaim);
-  ensure_in_queue<1>(x, cq, lead && removed && again, chunk);
+  ensure_in_queue<1>(x, cq, lead && removed && again, chunk, gen);
   return readlane((uint32_t)removed, L) != 0;
 }
 
@@ -1285,15 +1330,29 @@ __device__ inline void compact_wave(Ctx<I>& x, ClassQueue* cq) {
 // ChunkAccess::allocPage (ChunkAccess_impl.cuh:38-135), per lane, with 64-bit mask words.
 enum class AccessMode { SUCCESS, DEQUEUE, RE_ENQUEUE, CONTINUE, ERROR };
 
+// With recycling on, a chunk can be recycled and carved again between the scan that found it and
+// this claim: the page is then reserved with a CAS on {count, gen}, which fails on a later life.
 template <class I>
-__device__ inline AccessMode access_alloc_lane(Ctx<I>& x, ChunkHdr* h, uint32_t& page) {
+__device__ inline AccessMode access_alloc_lane(Ctx<I>& x, ChunkHdr* h, uint32_t gen, uint32_t& page) {
   int32_t cur = 0;
   AccessMode mode = AccessMode::SUCCESS;
   Spin it(&x.m.error);
-  while ((cur = sub(&h->count, 1)) <= 0) {
-    if ((cur = add(&h->count, 1)) < 0) return AccessMode::CONTINUE;
-    if (cur == 0) mode = AccessMode::RE_ENQUEUE;
-    if (it.expired()) { OURO_TIMEOUT(x.m); return AccessMode::ERROR; }
+  if (x.m.recycle_min != kRecycleOff) {
+    uint64_t w = ld(count_gen(h));
+    while (true) {
+      if (cg_gen(w) != gen || cg_count(w) <= 0) return AccessMode::CONTINUE;
+      const uint64_t seen = cas(count_gen(h), w, w - 1u);  // count > 0: no borrow into gen
+      if (seen == w) break;
+      w = seen;
+      if (it.expired()) { OURO_TIMEOUT(x.m); return AccessMode::ERROR; }
+    }
+    cur = cg_count(w);
+  } else {
+    while ((cur = sub(&h->count, 1)) <= 0) {
+      if ((cur = add(&h->count, 1)) < 0) return AccessMode::CONTINUE;
+      if (cur == 0) mode = AccessMode::RE_ENQUEUE;
+      if (it.expired()) { OURO_TIMEOUT(x.m); return AccessMode::ERROR; }
+    }
   }
   if (cur == 1) mode = AccessMode::DEQUEUE;
   const uint32_t nw = mask_words(ld(&h->size));
@@ -1438,18 +1497,18 @@ struct ChunkClaim {
         gave_up = true;
         return;
       }
-      uint32_t s = kMarker, adv = n;
+      uint32_t s = kMarker, adv = n, g = 0;
       int32_t cnt = 0;
-      const uint32_t i = n ? find_chunk(x, cq, v, n, cursor, s, cnt, adv) : 0u;
+      const uint32_t i = n ? find_chunk(x, cq, v, n, cursor, s, cnt, g, adv) : 0u;
       if (i < n) {
         ChunkHdr* h = hdr<ChunkHdr>(x.m, s);
         uint32_t t = 0;
         while (cnt > 0) {
           t = (uint32_t)cnt < need ? (uint32_t)cnt : need;
-          const int32_t seen = cas(&h->count, cnt, cnt - (int32_t)t);
-          if (seen == cnt) break;
+          const uint64_t seen = cas(count_gen(h), make_cg(cnt, g), make_cg(cnt - (int32_t)t, g));
+          if (seen == make_cg(cnt, g)) break;
           OURO_DIAG_ADD(3, 1);
-          cnt = seen;
+          cnt = cg_gen(seen) == g ? cg_count(seen) : 0;  // recycled meanwhile: not this chunk any more
         }
         if (cnt > 0) {
           miss = 0;
@@ -1459,9 +1518,9 @@ struct ChunkClaim {
           const uint32_t size = ld(&h->size);
           w = ((size - (uint32_t)cnt) / 64u) % mask_words(size);
           bool again = false;
-          if (cnt == (int32_t)t && cq_remove(x, cq, s, v + i, cursor, reclaim, again)) {
+          if (cnt == (int32_t)t && cq_remove(x, cq, s, g, v + i, cursor, reclaim, again)) {
             removed = true;
-            if (again) ensure_in_queue<1>(x, cq, true, s);
+            if (again) ensure_in_queue<1>(x, cq, true, s, g);
           }
           v += cnt == (int32_t)t ? i + 1u : i;
         } else {
@@ -1549,22 +1608,22 @@ __device__ inline void* alloc_chunk_pages(Ctx<I>& x, uint32_t qi, bool want) {
         }
         // the first chunk queued in [v, v + n) with free pages (kPeek slots per round trip; the
         // chunks are tried in queue order, as the reference's one-by-one walk does)
-        uint32_t c = kMarker, adv = n;
+        uint32_t c = kMarker, adv = n, g = 0;
         int32_t cnt = 0;
-        const uint32_t i = n ? find_chunk(x, cq, v, n, cursor, c, cnt, adv) : 0u;
+        const uint32_t i = n ? find_chunk(x, cq, v, n, cursor, c, cnt, g, adv) : 0u;
         if (i < n) {
           v += i;
           ChunkHdr* h = hdr<ChunkHdr>(x.m, c);
-          const AccessMode md = access_alloc_lane(x, h, page);
+          const AccessMode md = access_alloc_lane(x, h, g, page);
           if (md == AccessMode::ERROR) break;
           if (md != AccessMode::CONTINUE) {
             chunk = c;
             got = true;
             bool again = false;
-            if (md == AccessMode::RE_ENQUEUE) ensure_in_queue<1>(x, cq, true, c);
-            if (md == AccessMode::DEQUEUE && cq_remove(x, cq, c, v, cursor, reclaim, again)) {
+            if (md == AccessMode::RE_ENQUEUE) ensure_in_queue<1>(x, cq, true, c, g);
+            if (md == AccessMode::DEQUEUE && cq_remove(x, cq, c, g, v, cursor, reclaim, again)) {
               removed = true;
-              if (again) ensure_in_queue<1>(x, cq, true, c);
+              if (again) ensure_in_queue<1>(x, cq, true, c, g);
             }
             break;
           }
@@ -1617,6 +1676,109 @@ __device__ inline void* alloc_chunk_pages(Ctx<I>& x, uint32_t qi, bool want) {
   return chunk_addr(x.m, chunk) + kMeta + (uint64_t)page * page_size_of(x, cq);
 }
 
+// Take n published pages out of a class semaphore, only if that many are unclaimed (the semaphore
+// never owes claims more than it held before): BulkSemaphore::tryReduce (BulkSemaphore_impl.cuh:7-17),
+// with a CAS loop instead of a fetch-add and undo, so concurrent claims never see the dip.
+__device__ inline bool sem_try_take(ClassQueue* cq, uint32_t n) {
+  uint64_t v = ld(&cq->sem);
+  while (Sem::count(v) >= (int64_t)n) {
+    const uint64_t seen = cas(&cq->sem, v, v - (uint64_t)n);
+    if (seen == v) return true;
+    v = seen;
+  }
+  return false;
+}
+
+// Chunk recycling (ChunkQueue::freePage's flash, ChunkQueue_impl.cuh:163-199). A lane whose free made
+// every page of its chunk free again (`want`) tries it when recycling is on and the class queue holds
+// more than recycle_min chunks, counting its own chunk (`unqueued` = 1 when the free found the chunk
+// exhausted, hence out of the queue: it is recycled before it would be queued again, where the
+// reference's FIRST_FREE enqueue wins and never recycles a chunk of one page, ChunkAccess_impl.cuh:
+// 19-23): take `reduce` pages (those of the chunk already signalled) out of the
+// class semaphore, only if that many are unclaimed; move the chunk's count from full to 0 with a CAS
+// on {count, gen} (a claim that reserved a page meanwhile makes it fail, and the pages go back); take
+// the chunk out of its queue (queue_pos, tombstone, count_; an enqueue in flight is waited for); clear
+// its class; hand it to the instance's reuse ring, where any class of the instance (or a queue chunk)
+// can take it. A chunk the ring has no room for is put back as it was. The lanes go one at a time in
+// lane order (flashes are rare, and a single wave then acts exactly as the oracle's sequential model);
+// the wave compacts each queue it removed chunks from afterwards. Returns true for a recycled chunk.
+// (The reference flashes only above a fill level of 838,860 queued chunks, which its configurations
+// never reach, and disables it in the VA and VL chunk queues, ChunkQueueVA_impl.cuh:229 and
+// ChunkQueueVL_impl.cuh:160-192; here one switch covers all three kinds.)
+template <class I>
+__device__ inline bool flash_chunks(Ctx<I>& x, ClassQueue* cq, bool want, uint32_t chunk, uint32_t gen, uint32_t size,
+                                    uint32_t reduce, uint32_t unqueued) {
+  const uint32_t lim = x.m.recycle_min;
+  if (lim == kRecycleOff) return false;
+  uint64_t todo = ballot(want);
+  if (!todo) return false;
+  ChunkHdr* h = hdr<ChunkHdr>(x.m, want ? chunk : 0u);
+  bool flashed = false, removed = false;
+  uint32_t reclaim = kMarker;
+  while (todo) {
+    const int l = first(todo);
+    todo &= todo - 1;
+    if ((int)lane() != l) continue;
+    if ((int64_t)ld(&cq->count) + unqueued <= (int64_t)lim || !sem_try_take(cq, reduce)) continue;
+    if (cas(count_gen(h), make_cg((int32_t)size, gen), make_cg(0, gen)) != make_cg((int32_t)size, gen)) {
+      sem_signal(cq, reduce);  // a claim reserved a page meanwhile: tryFlashChunk failed
+      continue;
+    }
+    flashed = true;
+    Spin it(&x.m.error);
+    while (true) {
+      const uint64_t qp = ld(&h->queue_pos);
+      if (qp_queued(qp)) {
+        if (cas(&h->queue_pos, qp, qp_recycled(gen)) == qp) {
+          uint32_t cursor = kMarker, rc = kMarker;
+          tomb_slot(x, cq, (uint32_t)qp, cursor, rc);
+          if (rc != kMarker) {
+            if (reclaim != kMarker) finish_reclaim(x, cq, true, reclaim);
+            reclaim = rc;
+          }
+          sub(&cq->count, 1);
+          removed = true;
+          break;
+        }
+      } else if (qp == qp_not(gen)) {
+        if (cas(&h->queue_pos, qp, qp_recycled(gen)) == qp) break;
+      }
+      // qp_pending: an enqueue is in flight; it publishes the position next
+      if (it.expired()) {
+        OURO_TIMEOUT(x.m);
+        break;
+      }
+      backoff(it.it);
+    }
+    sub(&cq->carved, 1u);
+    clear_chunk_class(x.m, chunk, I::kAdd);
+    st(&h->page_size, 0u);
+  }
+  order();
+  // a quarter of the ring stays free for queue chunks, which share it and cannot be put back
+  const bool put = ring_put(x.m, x.inst.reuse, flashed, chunk, false, x.inst.reuse.size / 4);
+  const bool back = flashed && !put;
+  if (back) {
+    // no room in the ring: the chunk returns to its class with every page free
+    const uint32_t psize = page_size_of(x, cq);
+    mark_chunk_class(x.m, chunk, I::kAdd, psize);
+    st(&h->page_size, psize);
+    add(&cq->carved, 1u);
+    st(count_gen(h), make_cg((int32_t)size, gen));
+    st(&h->queue_pos, qp_not(gen));  // ensure_in_queue takes it from here
+    order();
+  }
+  ensure_in_queue<1>(x, cq, back, chunk, gen);
+  if (back) sem_signal(cq, reduce);
+  for (uint64_t pend = ballot(removed); pend;) {
+    ClassQueue* q = reinterpret_cast<ClassQueue*>(readlane64(reinterpret_cast<uint64_t>(cq), first(pend)));
+    compact_wave(x, q);
+    pend &= ~ballot(cq == q);
+  }
+  finish_reclaim(x, cq, reclaim != kMarker, reclaim);
+  return put;
+}
+
 template <int W, class I>
 __device__ inline void free_chunk_pages(Ctx<I>& x, uint32_t qi, bool want, uint32_t chunk, uint32_t page) {
   ClassQueue* cq = &x.inst.q[want ? qi : 0];
@@ -1628,14 +1790,25 @@ __device__ inline void free_chunk_pages(Ctx<I>& x, uint32_t qi, bool want, uint3
   const uint64_t grp = match<W>(key);  // W = 1: every lane on its own
   const int L = first(grp);
   const uint32_t n = popc(grp);
-  bool crossing = false;
-  if (want && (int)lane() == L) {
+  const bool lead = want && (int)lane() == L;
+  const bool recycle = x.m.recycle_min != kRecycleOff;
+  bool crossing = false, full = false;
+  uint32_t gen = 0, size = 0;
+  if (lead) {
+    // the generation is read while the group's pages still pin the chunk's life
+    if (recycle) {
+      gen = ld(&h->gen);
+      size = ld(&h->size);
+    }
     const int32_t old = add(&h->count, (int32_t)n);
     crossing = old <= 0 && old + (int32_t)n > 0;
+    if (!recycle && crossing) gen = ld(&h->gen);
+    full = recycle && old + (int32_t)n == (int32_t)size;
   }
+  const bool gone = flash_chunks(x, cq, full, chunk, gen, size, size - n, crossing ? 1u : 0u);
   // back into the queue before the pages are signalled
-  ensure_in_queue<W>(x, cq, crossing, chunk);
-  if (want && (int)lane() == L) sem_signal(cq, n);
+  ensure_in_queue<W>(x, cq, crossing && !gone, chunk, gen);
+  if (lead && !gone) sem_signal(cq, n);
 }
 
 }  // namespace detail

@@ -108,20 +108,41 @@ struct PageChunkHdr {
 };
 // ChunkIndexChunk (ChunkIndexChunk.cuh:5-20) + ChunkAccess (ChunkAccess.cuh:26-28). The
 // availability mask is eight 64-bit words (one wave-wide ballot per word) instead of 16 x u32.
-// queue_pos holds the 32-bit queue position the chunk is queued at, zero-extended to 64 bits, so the
-// two sentinels (not queued, enqueue in flight) can never equal a position, whatever value the
-// wrapping 32-bit positions reach (the reference keeps a 32-bit index_t).
-constexpr uint64_t kNotQueued = ~0ull;
-constexpr uint64_t kQueuePending = ~1ull;
+//
+// A chunk carries a generation, set when it is carved for a class: a carve counter (26 bits) and the
+// class index (low 4 bits), with the top two bits set. A chunk recycled to the reuse ring and carved
+// again gets a new one, so an operation that read a chunk's state before the recycle cannot act on
+// its next life: the free count and the generation share one 64-bit word that claims change with a
+// 64-bit CAS, and queue_pos carries the generation too. (The top bits keep a generation apart from
+// every value the same word of a queue chunk's header holds: chunk indices below 2^23 and markers.)
+// queue_pos: bit 63 set = queued at the 32-bit position in bits 0-31 (positions wrap, so the states
+// below can never equal one); clear = not queued (0), enqueue in flight (1) or recycled (2) in bits
+// 0-31. Bits 32-62 hold the generation's low 31 bits in every state (the reference keeps a 32-bit
+// index_t and no generation).
+constexpr uint32_t kGenBits = 0xC0000000u;
+OURO_HD constexpr uint32_t make_gen(uint32_t carve, uint32_t qi) { return kGenBits | ((carve & 0x3FFFFFFu) << 4) | (qi & 15u); }
+OURO_HD constexpr uint32_t gen_class(uint32_t gen) { return gen & 15u; }
+OURO_HD constexpr uint64_t qp_state(uint32_t gen, uint32_t s) { return ((uint64_t)(gen & 0x7FFFFFFFu) << 32) | s; }
+OURO_HD constexpr uint64_t qp_at(uint32_t gen, uint32_t pos) { return (1ull << 63) | qp_state(gen, pos); }
+OURO_HD constexpr uint64_t qp_not(uint32_t gen) { return qp_state(gen, 0); }
+OURO_HD constexpr uint64_t qp_pending(uint32_t gen) { return qp_state(gen, 1); }
+OURO_HD constexpr uint64_t qp_recycled(uint32_t gen) { return qp_state(gen, 2); }
+OURO_HD constexpr bool qp_queued(uint64_t qp) { return (qp >> 63) != 0; }
 struct ChunkHdr {
   uint32_t page_size;
   uint32_t size;       // pages on this chunk
   int32_t count;       // free pages
-  uint32_t pad;
-  uint64_t queue_pos;  // position in the class queue (< 2^32), kNotQueued or kQueuePending
+  uint32_t gen;        // generation; {count, gen} form one 64-bit word (count_gen)
+  uint64_t queue_pos;  // qp_at(gen, position), qp_not / qp_pending / qp_recycled(gen)
   uint64_t mask[kMaskWords];  // bit set = page free
 };
 static_assert(sizeof(ChunkHdr) <= kMeta, "chunk header exceeds metadata area");
+static_assert(offsetof(ChunkHdr, count) % 8 == 0 && offsetof(ChunkHdr, gen) == offsetof(ChunkHdr, count) + 4,
+              "count and gen must form one aligned 64-bit word");
+OURO_HD inline uint64_t* count_gen(ChunkHdr* h) { return reinterpret_cast<uint64_t*>(&h->count); }
+OURO_HD constexpr uint64_t make_cg(int32_t count, uint32_t gen) { return (uint64_t)(uint32_t)count | ((uint64_t)gen << 32); }
+OURO_HD constexpr int32_t cg_count(uint64_t w) { return (int32_t)(uint32_t)w; }
+OURO_HD constexpr uint32_t cg_gen(uint64_t w) { return (uint32_t)(w >> 32); }
 // QueueChunk (QueueChunk.cuh:37-42): slots live in the 8 KiB data area.
 struct QueueChunkHdr {
   uint32_t page_size;      // always 0 (CommonChunk)
@@ -132,6 +153,7 @@ struct QueueChunkHdr {
 };
 
 // ------------------------------------------------------------------ device-resident state
+constexpr uint32_t kRecycleOff = 0xFFFFFFFFu;
 // Memory (Ouroboros.cuh:8-29): shared by every instance of a chain.
 struct alignas(128) Memory {
   uint8_t* d_memory;
@@ -152,8 +174,13 @@ struct alignas(128) Memory {
   uint64_t pool_bytes;      // the whole allocation: the partition above plus the chunk class bytes
   uint64_t instantiation_size;  // initialize()'s arguments, for reinitialize()
   uint32_t add_beg, add_end;
+  // chunk-queue recycling (ChunkQueue_impl.cuh:163-199): a chunk whose pages are all free again goes
+  // back to the reuse ring when its class queue holds more than recycle_min chunks; kRecycleOff
+  // (the default) never recycles, as the reference whose fill level is never reached
+  uint32_t recycle_min;
   alignas(128) uint32_t next_free_chunk;
   uint32_t qc_next;         // bump pointer inside the queue-chunk reserve
+  uint32_t carves;          // chunk-queue carve counter (generations)
   alignas(128) uint32_t error;
   // Statistics (Statistics.h:5-17): pageAllocCount, pageFreeCount (requests served by / returned to
   // an instance), chunkAllocationCount (chunks handed out, fresh or reused, queue chunks included),

@@ -43,23 +43,6 @@ struct Chain<I, Rest...> {
 
 namespace detail {
 
-// ChunkLocator::getChunkIndex (ChunkLocator.cuh:29-51): highest run-start bit <= idx. Bits are set
-// once and never cleared, so a set bit seen through a plain (possibly stale) load is final; only a
-// miss needs the device-coherent walk.
-__device__ inline uint32_t locate_run(Memory& m, uint32_t idx) {
-  const uint32_t* loc = m.locator;
-  if ((loc[idx >> 5] >> (idx & 31u)) & 1u) return idx;
-  int64_t w = idx >> 5;
-  uint32_t mask = (uint32_t)((2ull << (idx & 31u)) - 1ull);
-  while (true) {
-    const uint32_t bits = dev::ld(&loc[w]) & mask;
-    if (bits) return (uint32_t)(w * 32 + (31 - __builtin_clz(bits)));
-    if (w == 0) return kMarker;
-    --w;
-    mask = 0xFFFFFFFFu;
-  }
-}
-
 template <int W, class I>
 __device__ inline void* inst_malloc(Memory& m, I& inst, ReuseRing& base, uint64_t size, bool want) {
   Ctx<I> x{m, inst, base};
@@ -159,10 +142,11 @@ __device__ inline bool route_free(Memory& m, const void* ptr, uint32_t& chunk, u
   if (want) {
     const uint64_t diff = (uint64_t)(p - m.d_data);
     const uint32_t idx = (uint32_t)(diff / kStride);
-    // the chunk's class byte, in place of ChunkLocator::getChunkIndex and CommonChunk::page_size: a
-    // byte only ever changes from 0 to its final value, so a plain (L2-cached) load is exact unless
-    // it reads 0, which is confirmed with a device-coherent load
-    uint32_t b = m.chunk_log2[idx];
+    // the chunk's class byte, in place of ChunkLocator::getChunkIndex and CommonChunk::page_size:
+    // without recycling a byte only ever changes from 0 to its final value, so a plain (L2-cached)
+    // load is exact unless it reads 0, which is confirmed with a device-coherent load; a recycled
+    // chunk changes class, so with recycling on the byte is always read coherently
+    uint32_t b = m.recycle_min == kRecycleOff ? m.chunk_log2[idx] : 0u;
     if (b == 0) b = dev::ld(&m.chunk_log2[idx]);
     want = b != 0 && (b >> 5) <= idx;
     if (want) {
@@ -263,7 +247,7 @@ __device__ inline void chain_block_release(Memory& m, Chain<I, Rest...>& ch, Reu
   if (dev::ballot(here)) {
     const uint32_t qi = here ? gc - off : 0u;
     if constexpr (I::kChunks) {
-      block_release_chunks(m, ch.inst, base, here, qi, chunk, page);
+      block_release_chunks(m, ch.inst, base, here, qi, chunk, page, &sm.total[here ? gc : off]);
     } else {
       Ctx<I> x{m, ch.inst, base};
       put_lanes<64>(x, &ch.inst.q[qi], here, here ? sm.base[gc] + r : 0u, make_index(chunk, page));
@@ -283,10 +267,11 @@ __device__ inline void inst_preload(Memory& m, I& inst, ReuseRing& base, uint32_
   mark_chunk_class(m, chunk, I::kAdd, psize);
   if constexpr (I::kChunks) {
     ChunkHdr* h = hdr<ChunkHdr>(m, chunk);
+    const uint32_t gen = make_gen(dev::add(&m.carves, 1u), qi);
     dev::st(&h->page_size, psize);
     dev::st(&h->size, ppc);
-    dev::st(&h->count, (int32_t)avail);
-    dev::st(&h->queue_pos, kNotQueued);
+    dev::st(count_gen(h), make_cg((int32_t)avail, gen));
+    dev::st(&h->queue_pos, avail ? qp_pending(gen) : qp_not(gen));  // enqueued below when it has pages
     for (uint32_t w = 0; w < kMaskWords; ++w) {
       uint64_t bits = 0;
       for (uint32_t b = 0; b < 64; ++b) {
@@ -297,7 +282,7 @@ __device__ inline void inst_preload(Memory& m, I& inst, ReuseRing& base, uint32_
     }
     dev::order();
     dev::add(&cq->carved, 1u);
-    if (avail) enqueue_chunks<1>(x, cq, true, chunk);  // ChunkQueue_impl.cuh:74-75
+    if (avail) enqueue_chunks<1>(x, cq, true, chunk, gen);  // ChunkQueue_impl.cuh:74-75
   } else {
     PageChunkHdr* h = hdr<PageChunkHdr>(m, chunk);
     dev::st(&h->page_size, psize);
@@ -450,6 +435,24 @@ struct Ouroboros {
       if (size[t] && gc[t] == kMarker) out[t] = detail::heap_malloc(memory, (uint64_t)size[t]);
   }
 
+  // The floor under malloc_tile, for measurement: the same class lookup, block rank in LDS and
+  // barriers, and one pointer written per request, with no allocator state touched (the pointer is
+  // made up from the class and rank and never dereferenced).
+  template <uint32_t T>
+  __device__ void null_tile(const size_t (&size)[T], void* (&out)[T], TileScratch<T>& sm) {
+    uint32_t gc[T], r[T];
+#pragma unroll
+    for (uint32_t t = 0; t < T; ++t) {
+      gc[t] = detail::chain_alloc_class<Is...>((uint64_t)size[t]);
+      if (!size[t]) gc[t] = kMarker;
+    }
+    detail::block_rank_tile<T>(sm, gc, r, kNumClasses);
+    __syncthreads();
+#pragma unroll
+    for (uint32_t t = 0; t < T; ++t)
+      out[t] = gc[t] == kMarker ? nullptr : memory.d_data + ((uint64_t)gc[t] << 27) + (uint64_t)r[t] * 16u;
+  }
+
   template <uint32_t T>
   __device__ void free_tile(void* const (&ptr)[T], TileScratch<T>& sm) {
     if constexpr (T > 1 && kVirtualized && !kAnyChunks) {
@@ -516,31 +519,12 @@ struct Ouroboros {
   template <int W = OURO_DEFAULT_AGG>
   __device__ void free(void* ptr) {
     static_assert(W == 1 || W == 16 || W == 64, "aggregation width must be 1, 16 or 64");
-    const uint8_t* p = reinterpret_cast<const uint8_t*>(ptr);
-    bool want = p != nullptr;
+    // one lookup for every form: the dense class byte (route_free), in place of the locator walk and
+    // chunk header read of Ouroboros::free (Ouroboros_impl.cuh:95-113); device-heap blocks go back to
+    // the heap there, and anything else outside the chunk grid sets ERR_BAD_FREE
     uint32_t chunk = 0, page_size = 0;
     uint64_t offset = 0;
-    if (want && !detail::in_pool(memory, p)) {
-      ::free(ptr);  // a block of the device heap (Ouroboros_impl.cuh:98-104)
-      want = false;
-    } else if (want) {
-      // validOuroborosPointer (Ouroboros.cuh:350-356), tightened to the chunk region
-      const uint64_t span = (uint64_t)memory.max_chunks * kStride;
-      want = p >= memory.d_data && p < memory.d_data + span;
-      if (want) {
-        const uint64_t diff = (uint64_t)(p - memory.d_data);
-        const uint32_t c = detail::locate_run(memory, (uint32_t)(diff / kStride));
-        want = c != kMarker;
-        if (want) {
-          chunk = c;
-          page_size = dev::ld(detail::hdr<uint32_t>(memory, c));  // CommonChunk::page_size
-          const uint64_t base = (uint64_t)c * kStride + kMeta;
-          want = diff >= base && page_size != 0;
-          offset = diff - base;
-        }
-      }
-      if (!want) detail::set_error(memory, ERR_BAD_FREE);
-    }
+    const bool want = detail::route_free(memory, ptr, chunk, page_size, offset);
     if (dev::ballot(want)) {
       dev::release_block_contents();
       detail::chain_free<W>(memory, chain, chain.inst.reuse, page_size, want, chunk, offset);
@@ -604,6 +588,8 @@ struct Ouroboros {
     memory.add_beg = (uint32_t)additionalSizeBeginning;
     memory.add_end = (uint32_t)additionalSizeEnd;
     memory.stats_on = 0;
+    memory.recycle_min = kRecycleOff;
+    memory.carves = 0;
     // requests above the largest page go to the device heap: give it the reference's 500 MB
     // (cuda_heap_size, Parameters.h:48; MemoryInitialization.cuh:130-134) unless it is larger already
     {

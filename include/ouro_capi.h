@@ -199,6 +199,16 @@ int ouro_preload(ouro_handle h, uint32_t class_index, uint32_t available_pages, 
  * ouro_set_statistics(h, 1) zeroes the counters and starts counting, (h, 0) stops; both wait for the
  * device. OURO_E_INVALID when the library has no statistics. */
 int ouro_set_statistics(ouro_handle h, int on);
+/* Chunk recycling (the flash of ChunkQueue::freePage, device/queues/chunk/ChunkQueue_impl.cuh:163-199)
+ * for the chunk-queue kinds (CQ, VACQ, VLCQ): a free that makes every page of its chunk free again
+ * returns the chunk to the instance's reuse ring, where any class can carve it, whenever its class
+ * queue holds more than min_queued chunks (the reference's lower_fill_level, ChunkQueue.cuh:30).
+ * OURO_RECYCLE_OFF (the default after ouro_create) switches it off: the reference's own fill level
+ * (838,860 queued chunks) is never reached, and its VA and VL chunk queues never recycle
+ * (ChunkQueueVA_impl.cuh:229, ChunkQueueVL_impl.cuh:160-192). Waits for the device; OURO_E_INVALID
+ * for page-queue kinds, whose chunks are never recycled. */
+#define OURO_RECYCLE_OFF 0xFFFFFFFFu
+int ouro_set_recycling(ouro_handle h, uint32_t min_queued);
 int ouro_statistics(ouro_handle h, uint64_t* out4);
 
 /* Bytes of a snapshot: the reference's pool partition (allocation_size) plus the chunk class bytes
@@ -385,6 +395,13 @@ int ouro_bench_atomics(uint32_t threads, uint32_t iters, int pattern, double* op
 /* Plain device copy of `bytes` bytes (16-byte vector loads/stores), `reps` times; returns GB/s.
  * Used to calibrate the FETCH_SIZE / WRITE_SIZE counters against known traffic. */
 int ouro_bench_copy(uint64_t bytes, uint32_t reps, double* gb_per_s);
+/* The floor under the batched malloc: a persistent kernel at the launch shape of the allocator's
+ * (block, tile, and the grid ouro_run_batch would use, or `grid` if smaller) that loads the n sizes,
+ * ranks them per class in LDS with the same barriers, and writes a made-up pointer per request,
+ * touching no allocator state. Average kernel time over `rounds` launches (after one warm-up),
+ * each timed by its own dispatch's events, in *kernel_ms; the grid used in *grid_out (may be NULL). */
+int ouro_bench_floor(ouro_handle h, const uint32_t* sizes, uint32_t n, uint32_t block, uint32_t grid, uint32_t tile,
+                     uint32_t rounds, double* kernel_ms, uint32_t* grid_out);
 
 #ifdef __cplusplus
 }

@@ -69,6 +69,8 @@ def _load_sim() -> C.CDLL:
         lib.sim_set_origin.restype = C.c_int
         lib.sim_set_agg.argtypes = [C.c_void_p, C.c_int]
         lib.sim_set_agg.restype = C.c_int
+        lib.sim_set_recycling.argtypes = [C.c_void_p, C.c_uint32]
+        lib.sim_set_recycling.restype = C.c_int
         lib.sim_preload.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32]
         lib.sim_preload.restype = C.c_uint32
         lib.sim_queue_window.argtypes = [C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32), C.c_uint32]
@@ -123,6 +125,7 @@ class Sim:
 
     def __init__(self, variant: str, pool_bytes: int, origin: int = 0, agg: int = 64):
         self._lib = _load_sim()
+        self.kind = variant
         self._h = self._lib.sim_create(VARIANTS[variant], pool_bytes)
         if not self._h:
             raise ValueError(variant)
@@ -130,6 +133,12 @@ class Sim:
             raise ValueError(f"origin {origin:#x}")
         if self._lib.sim_set_agg(self._h, agg) != 0:
             raise ValueError(f"aggregation width {agg}")
+
+    def set_recycling(self, min_queued: int | None = 0) -> None:
+        """Chunk recycling (chunk-queue kinds): None switches it off."""
+        v = 0xFFFFFFFF if min_queued is None else int(min_queued)
+        if self._lib.sim_set_recycling(self._h, v) != 0:
+            raise ValueError("recycling applies to chunk-queue kinds only")
 
     def close(self) -> None:
         if self._h:

@@ -27,6 +27,7 @@
 /* a chunk's queue position is a 32-bit value held in 64 bits; the sentinels are never positions */
 #define NOT_QUEUED 0xFFFFFFFFFFFFFFFFull
 #define QUEUE_PENDING 0xFFFFFFFFFFFFFFFEull
+#define RECYCLED 0xFFFFFFFFFFFFFFFDull /* a chunk handed back to the reuse ring */
 #define NULLPAGE 0xFFFFFFFCu  /* page-queue entry for a page the pool could not supply */
 #define TOMB 0xFFFFFFFDu      /* chunk-queue slot whose chunk was removed */
 #define PEEK 8u               /* slots looked at per scan step */
@@ -92,6 +93,7 @@ struct ouro_sim {
   uint32_t next_free, error, qc_reserve, qc_next;
   int block_order; /* 1: groups in class order (block-wide calls of one wavefront) */
   int agg;         /* aggregation width of the per-thread chunk claims: 64 (groups) or 1 (per lane) */
+  uint32_t recycle_min; /* chunk recycling: MARKER = off, else the queued-chunk threshold */
   uint32_t* locator;
   chunk_t* chunks;
   inst_t inst[2];
@@ -642,7 +644,7 @@ static void finish_reclaim(ouro_sim* s, inst_t* I, cls_t* q, uint32_t reclaim);
  * cq_remove). The caller compacts the front once for the wave. Returns 1 when removed. */
 static int cq_remove(ouro_sim* s, inst_t* I, cls_t* q, uint32_t chunk, uint32_t* cursor, uint32_t* reclaim) {
   chunk_t* x = ch(s, chunk);
-  if (x->queue_pos == NOT_QUEUED || x->queue_pos == QUEUE_PENDING) return 0;
+  if (x->queue_pos == NOT_QUEUED || x->queue_pos == QUEUE_PENDING || x->queue_pos == RECYCLED) return 0;
   const uint32_t p = (uint32_t)x->queue_pos;
   x->queue_pos = NOT_QUEUED;
   uint32_t rc = MARKER;
@@ -1339,6 +1341,85 @@ void sim_wave_malloc(ouro_sim* s, int n, const uint32_t* sizes, uint64_t* out) {
 }
 
 /* ---- free */
+/* Chunk recycling (the device's flash_chunks; ChunkQueue::freePage's flash, ChunkQueue_impl.cuh:163-199).
+ * Groups whose free made their chunk full, in leader-lane order and before any enqueue or signal of
+ * this call: when the class queue holds more than recycle_min chunks (a chunk that the free found
+ * exhausted, so out of the queue, counting itself) and the semaphore has the chunk's other
+ * size - n pages unclaimed, they leave the semaphore, the chunk leaves its queue (tombstone, count_)
+ * and its class (carved, page size 0). Then, as the wave does: the flashed chunks go to the
+ * instance's reuse ring in that order (those the ring has no room for return to their class with
+ * every page free and are queued again), the queues that lost entries are compacted (wave form) in
+ * the order of their first flash, and queue chunks emptied by the tombstones are reclaimed.
+ * gone[j] marks the groups whose pages left with their chunk. */
+static void flash(ouro_sim* s, inst_t* I, const int* lanes, int n, const uint32_t* qi, const group_t* g, int ng,
+                  const int* unqueued, int* gone) {
+  cls_t* rq[64];
+  cls_t* fq[64];
+  cls_t* recq[64];
+  uint32_t rec[64], fc[64];
+  int64_t fred[64];
+  int fj[64];
+  int nrq = 0, nrec = 0, nf = 0;
+  for (int j = 0; j < ng; ++j) {
+    const uint32_t c = g[j].qi;
+    chunk_t* x = ch(s, c);
+    int idx = 0;
+    for (int i = 0; i < n; ++i)
+      if (lanes[i] == g[j].lanes[0]) idx = i;
+    cls_t* q = &I->q[qi[idx]];
+    if (x->count != (int32_t)x->size || (int64_t)q->count + unqueued[j] <= (int64_t)s->recycle_min) continue;
+    const int64_t reduce = (int64_t)x->size - g[j].n;
+    if (q->cnt < reduce) continue;
+    q->cnt -= reduce;
+    x->count = 0;
+    if (x->queue_pos < 0x100000000ull) {
+      uint32_t cursor = MARKER, rc = MARKER;
+      tomb_slot(s, I, q, (uint32_t)x->queue_pos, &cursor, &rc);
+      if (rc != MARKER) {
+        recq[nrec] = q;
+        rec[nrec++] = rc;
+      }
+      q->count--;
+      int seen = 0;
+      for (int r = 0; r < nrq; ++r) seen |= rq[r] == q;
+      if (!seen) rq[nrq++] = q;
+    } else if (x->queue_pos != NOT_QUEUED) {
+      s->error |= ERR_TIMEOUT; /* an enqueue in flight cannot exist in the sequential model */
+    }
+    x->queue_pos = RECYCLED;
+    q->carved--;
+    x->page_size = 0;
+    fq[nf] = q;
+    fc[nf] = c;
+    fred[nf] = reduce;
+    fj[nf++] = j;
+  }
+  cls_t* bq[64];
+  uint32_t bc[64];
+  int nb = 0;
+  for (int f = 0; f < nf; ++f) {
+    chunk_t* x = ch(s, fc[f]);
+    if (I->reuse.count < (int32_t)(I->reuse.size - I->reuse.size / 4)) { /* a quarter stays free for queue chunks */
+      x->type = 0;
+      ring_put(s, &I->reuse, fc[f]);
+      gone[fj[f]] = 1;
+    } else {
+      /* no room in the ring: back to its class, every page free */
+      x->page_size = fq[f]->page_size;
+      fq[f]->carved++;
+      x->count = (int32_t)x->size;
+      x->queue_pos = QUEUE_PENDING;
+      bq[nb] = fq[f];
+      bc[nb++] = fc[f];
+    }
+  }
+  if (nb) enqueue_chunks(s, I, bq, nb, bc);
+  for (int f = 0; f < nf; ++f)
+    if (!gone[fj[f]]) fq[f]->cnt += fred[f];
+  for (int r = 0; r < nrq; ++r) compact(s, I, rq[r], 1);
+  for (int r = 0; r < nrec; ++r) finish_reclaim(s, I, recq[r], rec[r]);
+}
+
 static void inst_free(ouro_sim* s, inst_t* I, const int* lanes, int n, const uint32_t* qi, const uint32_t* chunk,
                       const uint32_t* page) {
   if (!is_chunk_kind(I->kind)) {
@@ -1373,6 +1454,7 @@ static void inst_free(ouro_sim* s, inst_t* I, const int* lanes, int n, const uin
   const int ng = make_groups(lanes, n, chunk, g); /* groups keyed by chunk index */
   uint32_t ff_chunk[64];
   cls_t* ff_q[64];
+  int ff_j[64], unqueued[64] = {0};
   int nff = 0;
   for (int j = 0; j < ng; ++j) {
     const uint32_t c = g[j].qi;
@@ -1384,18 +1466,29 @@ static void inst_free(ouro_sim* s, inst_t* I, const int* lanes, int n, const uin
     const int32_t old = x->count;
     x->count += g[j].n;
     /* count crossed from <= 0 to > 0: the chunk must be in the queue; NOT_QUEUED means it left */
-    if (old <= 0 && old + g[j].n > 0 && x->queue_pos == NOT_QUEUED) {
-      x->queue_pos = QUEUE_PENDING;
+    unqueued[j] = old <= 0 && old + g[j].n > 0;
+    if (unqueued[j] && x->queue_pos == NOT_QUEUED) {
+      ff_j[nff] = j;
       ff_chunk[nff] = c;
       ff_q[nff++] = q;
     }
   }
-  if (nff) enqueue_chunks(s, I, ff_q, nff, ff_chunk);
+  /* recycling first: a chunk recycled here is not queued again */
+  int gone[64] = {0};
+  if (s->recycle_min != MARKER) flash(s, I, lanes, n, qi, g, ng, unqueued, gone);
+  int m = 0;
+  for (int f = 0; f < nff; ++f)
+    if (!gone[ff_j[f]]) {
+      ch(s, ff_chunk[f])->queue_pos = QUEUE_PENDING;
+      ff_chunk[m] = ff_chunk[f];
+      ff_q[m++] = ff_q[f];
+    }
+  if (m) enqueue_chunks(s, I, ff_q, m, ff_chunk);
   for (int j = 0; j < ng; ++j) {
     int idx = 0;
     for (int i = 0; i < n; ++i)
       if (lanes[i] == g[j].lanes[0]) idx = i;
-    I->q[qi[idx]].cnt += g[j].n;
+    if (!gone[j]) I->q[qi[idx]].cnt += g[j].n;
   }
 }
 
@@ -1449,6 +1542,7 @@ ouro_sim* sim_create(int variant, uint64_t pool_bytes) {
   ouro_sim* s = (ouro_sim*)calloc(1, sizeof(ouro_sim));
   s->variant = variant;
   s->agg = 64;
+  s->recycle_min = MARKER;
   sim_layout(variant, pool_bytes, &s->max_chunks, &s->alloc_size);
   /* a virtualized pool keeps its top chunks for queue chunks (at most a quarter of a tiny pool) */
   s->qc_reserve = is_std(variant) ? 0u : (uint32_t)(QC_RESERVE < s->max_chunks / 4 ? QC_RESERVE : s->max_chunks / 4);
@@ -1593,6 +1687,12 @@ void sim_set_block_order(ouro_sim* s, int on) { s->block_order = on; }
 /* enqueueInitialChunk on a chunk taken from the pool (the device's preloadChunk; PageQueue_impl.cuh:40-53,
  * ChunkQueue_impl.cuh:72-78): pages [ppc - available, ppc) become free pages of chain class k, the
  * others stay with the caller. Returns the chunk, or MARKER when the pool has none. */
+int sim_set_recycling(ouro_sim* s, uint32_t min_queued) {
+  if (!is_chunk_kind(s->inst[0].kind)) return -1;
+  s->recycle_min = min_queued;
+  return 0;
+}
+
 uint32_t sim_preload(ouro_sim* s, uint32_t k, uint32_t available) {
   int i = 0;
   while (i < s->ninst && k >= s->inst[i].nq) k -= s->inst[i++].nq;

@@ -42,6 +42,10 @@ int sim_set_origin(ouro_sim* s, uint32_t origin);
  * and 64) or 1 (the per-lane claims of the plain path; defined for waves whose lanes request
  * pairwise different classes, lane l being thread l of its workgroup). Returns 0, or -1. */
 int sim_set_agg(ouro_sim* s, int w);
+/* Chunk recycling for the chunk-queue kinds (the device's flash_chunks): a free that makes its chunk
+ * full returns it to the reuse ring while its class queue holds more than min_queued chunks;
+ * 0xFFFFFFFF switches it off (the default). Returns 0, or -1 for page-queue kinds. */
+int sim_set_recycling(ouro_sim* s, uint32_t min_queued);
 /* enqueueInitialChunk on a chunk taken from the pool: its last `available` pages become free pages of
  * class k (counted over the chain). Returns the chunk, or 0xFFFFFFFF when the pool has none. */
 uint32_t sim_preload(ouro_sim* s, uint32_t k, uint32_t available);

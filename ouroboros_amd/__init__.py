@@ -274,6 +274,7 @@ SIGNATURES = {
     "ouro_snapshot_size": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64)]),
     "ouro_restore": (C.c_int, [C.c_void_p, C.c_void_p]),
     "ouro_reinitialize": (C.c_int, [C.c_void_p, C.c_double]),
+    "ouro_set_recycling": (C.c_int, [C.c_void_p, C.c_uint32]),
     "ouro_preload": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.POINTER(C.c_uint32)]),
     "ouro_set_statistics": (C.c_int, [C.c_void_p, C.c_int]),
     "ouro_statistics": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64)]),
@@ -285,6 +286,8 @@ SIGNATURES = {
     "ouro_set_origin": (C.c_int, [C.c_void_p, C.c_uint32]),
     "ouro_bench_atomics": (C.c_int, [C.c_uint32, C.c_uint32, C.c_int, C.POINTER(C.c_double)]),
     "ouro_bench_copy": (C.c_int, [C.c_uint64, C.c_uint32, C.POINTER(C.c_double)]),
+    "ouro_bench_floor": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32), C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                                   C.c_uint32, C.POINTER(C.c_double), C.POINTER(C.c_uint32)]),
 }
 
 _lib = None
@@ -462,6 +465,21 @@ class MemoryManager:
     def set_statistics(self, on: bool = True) -> None:
         """Zero the statistics counters and start counting (on), or stop."""
         _check(lib().ouro_set_statistics(self._h, int(on)), "ouro_set_statistics")
+
+    def bench_floor(self, sizes: np.ndarray, block: int = 1024, tile: int = 1, rounds: int = 10,
+                    grid: int = 0) -> dict:
+        """The floor kernel under the batched malloc at the allocator's launch shape: average kernel ms."""
+        sizes = np.ascontiguousarray(sizes, dtype=np.uint32)
+        ms, g = C.c_double(), C.c_uint32()
+        _check(lib().ouro_bench_floor(self._h, sizes.ctypes.data_as(C.POINTER(C.c_uint32)), len(sizes), block, grid,
+                                      tile, rounds, C.byref(ms), C.byref(g)), "ouro_bench_floor")
+        return {"kernel_ms": ms.value, "grid": g.value}
+
+    def set_recycling(self, min_queued: int | None = 0) -> None:
+        """Chunk recycling for the chunk-queue kinds: fully free chunks go back to the reuse ring while
+        their class queue holds more than min_queued chunks; None switches it off (the default)."""
+        v = 0xFFFFFFFF if min_queued is None else int(min_queued)
+        _check(lib().ouro_set_recycling(self._h, v), "ouro_set_recycling")
 
     def statistics(self) -> dict:
         """pageAllocCount, pageFreeCount, chunkAllocationCount, chunkReuseCount (Statistics.h:5-17)."""

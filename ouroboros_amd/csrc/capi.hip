@@ -298,6 +298,42 @@ int ouro_bench_atomics(uint32_t threads, uint32_t iters, int pattern, double* op
   return OURO_OK;
 }
 
+int ouro_bench_floor(ouro_handle h, const uint32_t* sizes, uint32_t n, uint32_t block, uint32_t grid, uint32_t tile,
+                     uint32_t rounds, double* kernel_ms, uint32_t* grid_out) {
+  if (!h || !sizes || !kernel_ms || n == 0 || rounds == 0 || block % 64 != 0 || block == 0 || block > 1024)
+    return OURO_E_INVALID;
+  if (tile != 1 && tile != 2 && tile != 4) return OURO_E_INVALID;
+  IManager& m = *h->impl;
+  int cus = 0;
+  OURO_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, 0));
+  const int occ = m.persist_occupancy(block, tile);  // the allocator kernels' limit: the same launch shape
+  if (occ <= 0) return OURO_E_INVALID;
+  const uint32_t cap = (uint32_t)occ * (uint32_t)cus;
+  const uint32_t g = std::min<uint32_t>((n + block * tile - 1) / (block * tile), grid ? std::min(grid, cap) : cap);
+  DevBuf d_sizes, d_out;
+  OURO_TRY(d_sizes.alloc((size_t)n * 4));
+  OURO_TRY(d_out.alloc((size_t)n * 8));
+  OURO_TRY(hipMemcpy(d_sizes.p, sizes, (size_t)n * 4, hipMemcpyHostToDevice));
+  hipEvent_t e0, e1;
+  OURO_TRY(hipEventCreate(&e0));
+  OURO_TRY(hipEventCreate(&e1));
+  double total = 0;
+  hipError_t e = hipSuccess;
+  for (uint32_t r = 0; r <= rounds && e == hipSuccess; ++r) {  // launch 0 warms up
+    e = m.null_persist(d_sizes.as<uint32_t>(), d_out.as<void*>(), n, 0, block, g, tile, e0, e1);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (r) total += ms;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (e != hipSuccess) return OURO_E_HIP;
+  *kernel_ms = total / rounds;
+  if (grid_out) *grid_out = g;
+  return OURO_OK;
+}
+
 int ouro_bench_copy(uint64_t bytes, uint32_t reps, double* gb_per_s) {
   if (!gb_per_s || bytes < 16 || reps == 0) return OURO_E_INVALID;
   const uint64_t n = bytes / 16;
@@ -414,6 +450,16 @@ int ouro_set_statistics(ouro_handle h, int on) {
   (void)on;
   return OURO_E_INVALID;  // built without statistics
 #endif
+}
+
+int ouro_set_recycling(ouro_handle h, uint32_t min_queued) {
+  if (!h) return OURO_E_INVALID;
+  ouro_state st;
+  if (h->impl->query(&st) != hipSuccess) return OURO_E_HIP;
+  if (st.kind[0] != OURO_CQ && st.kind[0] != OURO_VACQ && st.kind[0] != OURO_VLCQ) return OURO_E_INVALID;
+  if (hipDeviceSynchronize() != hipSuccess) return OURO_E_HIP;
+  return hipMemcpy(h->impl->device_recycle_min(), &min_queued, 4, hipMemcpyHostToDevice) == hipSuccess ? OURO_OK
+                                                                                                    : OURO_E_HIP;
 }
 
 int ouro_statistics(ouro_handle h, uint64_t* out4) {
@@ -1019,7 +1065,8 @@ int ouro_check_quiescent(ouro_handle h, ouro_check_result* out) {
       }
       class_pages[k] += ch->count;
       // every page is free again, so the chunk must sit in its class queue inside [front_, back_)
-      if (ch->queue_pos > 0xFFFFFFFFull || (uint32_t)ch->queue_pos - st.front[k] >= st.back[k] - st.front[k])
+      if (!ouro::qp_queued(ch->queue_pos) || ch->queue_pos != ouro::qp_at(ch->gen, (uint32_t)ch->queue_pos) ||
+          (uint32_t)ch->queue_pos - st.front[k] >= st.back[k] - st.front[k])
         fail(k, c, "chunk with free pages is not inside its queue window", (long long)ch->queue_pos, st.front[k]);
     } else {
       const ouro::PageChunkHdr* ph = reinterpret_cast<const ouro::PageChunkHdr*>(w);
@@ -1071,7 +1118,7 @@ int ouro_check_quiescent(ouro_handle h, ouro_check_result* out) {
         if (!mem.count(c)) fail(k, c, "queue entry names a chunk outside the class", c, 0);
         if (kind == OURO_CQ && c < nfc) {
           const ouro::ChunkHdr* ch = reinterpret_cast<const ouro::ChunkHdr*>(hdrs.data() + (size_t)c * 128);
-          if (ch->queue_pos != (uint64_t)(uint32_t)(st.front[k] + i))
+          if (ch->queue_pos != ouro::qp_at(ch->gen, st.front[k] + i))
             fail(k, c, "chunk queue_pos differs from its queue position", (long long)ch->queue_pos, st.front[k] + i);
         }
         if (kind == OURO_PQ && pg >= st.pages_per_chunk[k]) fail(k, c, "queue entry page out of range", pg, st.pages_per_chunk[k]);

@@ -56,6 +56,9 @@ struct IManager {
                                     hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) = 0;
   virtual hipError_t free_persist(void* const* d_ptrs, uint32_t n, hipStream_t s, uint32_t block, uint32_t grid,
                                   uint32_t tile, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) = 0;
+  // the floor kernel (k_null_persist) at the same launch shape, timed by its own dispatch's events
+  virtual hipError_t null_persist(const uint32_t* d_sizes, void** d_out, uint32_t n, hipStream_t s, uint32_t block,
+                                  uint32_t grid, uint32_t tile, hipEvent_t e0, hipEvent_t e1) = 0;
   // workgroups of `block` threads one CU holds at once for the persistent kernels of `tile` (the
   // smallest over the three)
   virtual int persist_occupancy(uint32_t block, uint32_t tile) = 0;
@@ -80,6 +83,7 @@ struct IManager {
   // device addresses of the sticky error word and next_free_chunk inside the pool's manager state
   virtual uint32_t* device_error_word() = 0;
   virtual uint32_t* device_stats_on() = 0;
+  virtual uint32_t* device_recycle_min() = 0;
   virtual unsigned long long* device_stats() = 0;
   virtual uint32_t* device_next_free_chunk() = 0;
 #ifdef OURO_DIAG
@@ -174,6 +178,38 @@ __global__ void __launch_bounds__(1024) k_malloc_persist(M* mm, const uint32_t* 
     }
     void* p[T];
     mm->template malloc_tile<T>(size, p, sm);
+#pragma unroll
+    for (uint32_t t = 0; t < T; ++t) {
+      const uint32_t i = base + t * blockDim.x + threadIdx.x;
+      if (i < n) out[i] = p[t];
+      size[t] = next[t];
+    }
+  }
+}
+
+// k_malloc_persist's launch, loop and loads around null_tile: the floor an allocator kernel of this
+// shape cannot go below.
+template <class M, uint32_t T>
+__global__ void __launch_bounds__(1024) k_null_persist(M* mm, const uint32_t* __restrict__ sizes,
+                                                       void** __restrict__ out, uint32_t n) {
+  __shared__ typename M::template TileScratch<T> sm;
+  const uint32_t tile = T * blockDim.x, step = gridDim.x * tile;
+  uint32_t base = blockIdx.x * tile;
+  size_t size[T];
+#pragma unroll
+  for (uint32_t t = 0; t < T; ++t) {
+    const uint32_t i = base + t * blockDim.x + threadIdx.x;
+    size[t] = i < n ? sizes[i] : 0u;
+  }
+  for (; base < n; base += step) {
+    uint32_t next[T];
+#pragma unroll
+    for (uint32_t t = 0; t < T; ++t) {
+      const uint32_t j = base + step + t * blockDim.x + threadIdx.x;
+      next[t] = j < n ? sizes[j] : 0u;
+    }
+    void* p[T];
+    mm->template null_tile<T>(size, p, sm);
 #pragma unroll
     for (uint32_t t = 0; t < T; ++t) {
       const uint32_t i = base + t * blockDim.x + threadIdx.x;
@@ -444,6 +480,17 @@ struct Manager final : IManager {
       default: return hipErrorInvalidValue;
     }
   }
+  hipError_t null_persist(const uint32_t* d_sizes, void** d_out, uint32_t n, hipStream_t s, uint32_t block,
+                          uint32_t grid, uint32_t tile, hipEvent_t e0, hipEvent_t e1) override {
+    M* dm = mm.getDeviceMemoryManager();
+    switch (tile) {
+      case 1: hipExtLaunchKernelGGL((k_null_persist<M, 1>), dim3(grid), dim3(block), 0, s, e0, e1, 0, dm, d_sizes, d_out, n); break;
+      case 2: hipExtLaunchKernelGGL((k_null_persist<M, 2>), dim3(grid), dim3(block), 0, s, e0, e1, 0, dm, d_sizes, d_out, n); break;
+      case 4: hipExtLaunchKernelGGL((k_null_persist<M, 4>), dim3(grid), dim3(block), 0, s, e0, e1, 0, dm, d_sizes, d_out, n); break;
+      default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+  }
   template <uint32_t T>
   hipError_t launch_free_persist(void* const* d_ptrs, uint32_t n, hipStream_t s, uint32_t block, uint32_t grid,
                                  hipEvent_t e0, hipEvent_t e1) {
@@ -616,6 +663,7 @@ struct Manager final : IManager {
   }
   uint32_t* device_error_word() override { return device_field(&mm.memory.error); }
   uint32_t* device_stats_on() override { return device_field(&mm.memory.stats_on); }
+  uint32_t* device_recycle_min() override { return device_field(&mm.memory.recycle_min); }
   unsigned long long* device_stats() override {
     return reinterpret_cast<unsigned long long*>(device_field(&mm.memory.stats[0]));
   }
