@@ -119,17 +119,18 @@ __device__ inline bool ring_put(Memory& m, ReuseRing& r, bool want, uint32_t val
   const uint32_t n = popc(mask);
   uint32_t fill = 0, pos = 0;
   if ((int)lane() == l) {
-    // reserve only what fits, with a CAS: a count that overshot and was taken back would let a
-    // concurrent take claim entries that are never written
-    int32_t c = ld(&r.count);
     uint32_t acc = 0;
-    while (true) {
-      const int32_t room = (int32_t)(r.size - headroom) - c;
+    if (headroom) {
+      // a put that keeps headroom (recycled data chunks) goes in whole or not at all, decided on a
+      // plain read: the headroom absorbs puts racing past the check, and nothing is ever taken back
+      // from the count, where a concurrent take could have claimed it as an entry to come
+      acc = ld(&r.count) + (int32_t)n <= (int32_t)(r.size - headroom) ? n : 0u;
+      if (acc) add(&r.count, (int32_t)acc);
+    } else {
+      const int32_t f = add(&r.count, (int32_t)n);
+      const int32_t room = (int32_t)r.size - f;
       acc = room <= 0 ? 0u : (room >= (int32_t)n ? n : (uint32_t)room);
-      if (acc == 0) break;
-      const int32_t seen = cas(&r.count, c, c + (int32_t)acc);
-      if (seen == c) break;
-      c = seen;
+      if (acc < n) sub(&r.count, (int32_t)(n - acc));  // the ring is full: an error either way
     }
     if (acc < n && strict) set_error(m, ERR_REUSE_FULL);
     fill = acc;
@@ -160,17 +161,9 @@ __device__ inline uint32_t ring_take(Memory& m, ReuseRing& r, uint32_t k, int l,
                                      uint32_t keep = 0) {
   uint32_t t = 0, pos = 0;
   if ((int)lane() == l) {
-    // take only what is there, with a CAS: a count that dipped and was given back would let a
-    // concurrent put write past entries not yet taken
-    int32_t c = ld(&r.count);
-    while (true) {
-      const int32_t avail = c - (int32_t)keep;
-      t = avail <= 0 ? 0u : ((uint32_t)avail >= k ? k : (uint32_t)avail);
-      if (t == 0) break;
-      const int32_t seen = cas(&r.count, c, c - (int32_t)t);
-      if (seen == c) break;
-      c = seen;
-    }
+    const int32_t old = sub(&r.count, (int32_t)k) - (int32_t)keep;
+    t = old <= 0 ? 0u : ((uint32_t)old >= k ? k : (uint32_t)old);
+    if (t < k) add(&r.count, (int32_t)(k - t));
     pos = t ? add(&r.front, t) : 0u;
   }
   t = readlane(t, l);
@@ -1699,9 +1692,10 @@ __device__ inline bool sem_try_take(ClassQueue* cq, uint32_t n) {
 // on {count, gen} (a claim that reserved a page meanwhile makes it fail, and the pages go back); take
 // the chunk out of its queue (queue_pos, tombstone, count_; an enqueue in flight is waited for); clear
 // its class; hand it to the instance's reuse ring, where any class of the instance (or a queue chunk)
-// can take it. A chunk the ring has no room for is put back as it was. The lanes go one at a time in
-// lane order (flashes are rare, and a single wave then acts exactly as the oracle's sequential model);
-// the wave compacts each queue it removed chunks from afterwards. Returns true for a recycled chunk.
+// can take it. The wave's recycled chunks enter the ring together, and only while a quarter of the
+// ring stays free for the queue chunks that share it; otherwise they are put back as they were. The
+// lanes go one at a time in lane order (flashes are rare, and a single wave then acts exactly as the
+// oracle's sequential model); the wave compacts each queue it removed chunks from afterwards. Returns true for a recycled chunk.
 // (The reference flashes only above a fill level of 838,860 queued chunks, which its configurations
 // never reach, and disables it in the VA and VL chunk queues, ChunkQueueVA_impl.cuh:229 and
 // ChunkQueueVL_impl.cuh:160-192; here one switch covers all three kinds.)

@@ -1397,9 +1397,11 @@ static void flash(ouro_sim* s, inst_t* I, const int* lanes, int n, const uint32_
   cls_t* bq[64];
   uint32_t bc[64];
   int nb = 0;
+  /* the wave's flashed chunks go into the ring together or not at all (a quarter stays free) */
+  const int fits = I->reuse.count + nf <= (int32_t)(I->reuse.size - I->reuse.size / 4);
   for (int f = 0; f < nf; ++f) {
     chunk_t* x = ch(s, fc[f]);
-    if (I->reuse.count < (int32_t)(I->reuse.size - I->reuse.size / 4)) { /* a quarter stays free for queue chunks */
+    if (fits) {
       x->type = 0;
       ring_put(s, &I->reuse, fc[f]);
       gone[fj[f]] = 1;

@@ -537,7 +537,7 @@ class MemoryManager:
         kernels (agg 0, `tile` = 1, 2 or 4 requests per thread and call) or the per-thread kernels
         (agg 1, 16, 64); rounds replay a captured graph. `offsets`, if given, receives the validation
         round's blocks (byte offsets, UINT64_MAX = null); `latency` adds an untimed round that samples
-        the wall-clock time of each allocation call; `kernel_timing` (agg 0) adds up to 20 rounds that
+        the wall-clock time of each allocation call; `kernel_timing` (agg 0) adds as many rounds again that
         time each persistent kernel by its own dispatch's events (alloc_kernel_ms, free_kernel_ms)."""
         sizes = np.ascontiguousarray(sizes, dtype=np.uint32)
         cfg = BatchConfig()

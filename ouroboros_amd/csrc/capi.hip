@@ -602,29 +602,41 @@ int ouro_run_batch(ouro_handle h, const ouro_batch_config* cfg, ouro_batch_resul
   OURO_TRY(hipMemsetAsync(d_errs.p, 0, 32, s));
   OURO_TRY(hipStreamSynchronize(s));
 
-  hipGraph_t g = nullptr;
-  hipGraphExec_t ge = nullptr;
+  // graph form: the alloc, touch and free phases are captured as three graphs and replayed one after
+  // the other, with the timing events recorded between the launches on the stream (event records
+  // captured inside a graph do not time the nodes between them on this runtime)
+  hipGraph_t g[3] = {nullptr, nullptr, nullptr};
+  hipGraphExec_t ge[3] = {nullptr, nullptr, nullptr};
+  auto drop = [&]() {
+    for (auto& x : ge)
+      if (x) (void)hipGraphExecDestroy(x);
+    for (auto& x : g)
+      if (x) (void)hipGraphDestroy(x);
+  };
   if (cfg->graph && cfg->rounds) {
-    // every event is recorded once outside the capture first: an event whose first record is a
-    // captured node fails the later elapsed-time queries (seen with no warm-up round before it)
-    for (auto& e : ev) OURO_TRY(hipEventRecord(e, s));
-    OURO_TRY(hipStreamSynchronize(s));
-    OURO_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    const hipError_t ce = round();
-    hipGraph_t cg = nullptr;
-    const hipError_t ee = hipStreamEndCapture(s, &cg);
-    if (ce != hipSuccess || ee != hipSuccess) {
-      if (cg) (void)hipGraphDestroy(cg);
-      return OURO_E_HIP;
+    for (int k = 0; k < 3; ++k) {
+      OURO_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      const hipError_t ce = k == 0 ? alloc() : k == 1 ? touch() : release();
+      const hipError_t ee = hipStreamEndCapture(s, &g[k]);
+      if (ce != hipSuccess || ee != hipSuccess || hipGraphInstantiate(&ge[k], g[k], nullptr, nullptr, 0) != hipSuccess) {
+        drop();
+        return OURO_E_HIP;
+      }
     }
-    g = cg;
-    OURO_TRY(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
     out->graph_used = 1;
   }
+  auto graph_round = [&]() -> hipError_t {
+    hipError_t e;
+    for (int k = 0; k < 3; ++k)
+      if ((e = hipEventRecord(ev[k], s)) != hipSuccess || (e = hipGraphLaunch(ge[k], s)) != hipSuccess) return e;
+    return hipEventRecord(ev[3], s);
+  };
   for (uint32_t r = 0; r < cfg->rounds; ++r) {
-    if (ge) OURO_TRY(hipGraphLaunch(ge, s));
-    else OURO_TRY(round());
-    OURO_TRY(hipEventSynchronize(ev[3]));
+    const hipError_t re = ge[0] ? graph_round() : round();
+    if (re != hipSuccess || hipEventSynchronize(ev[3]) != hipSuccess) {
+      drop();
+      return OURO_E_HIP;
+    }
     float t = 0;
     OURO_TRY(hipEventElapsedTime(&t, ev[0], ev[1]));
     out->alloc_ms += t;
@@ -636,8 +648,7 @@ int ouro_run_batch(ouro_handle h, const ouro_batch_config* cfg, ouro_batch_resul
     out->round_ms += t;
   }
   OURO_TRY(hipStreamSynchronize(s));
-  if (ge) (void)hipGraphExecDestroy(ge);
-  if (g) (void)hipGraphDestroy(g);
+  drop();
   unsigned long long e3[3];
   OURO_TRY(hipMemcpy(e3, d_errs.p, 24, hipMemcpyDeviceToHost));
   if (cfg->kernel_timing && cfg->agg == 0 && cfg->rounds) {
@@ -645,7 +656,7 @@ int ouro_run_batch(ouro_handle h, const ouro_batch_config* cfg, ouro_batch_resul
     // the start / end events its own dispatch records
     hipEvent_t ek[4];
     for (auto& e : ek) OURO_TRY(hipEventCreate(&e));
-    const uint32_t kr = std::min<uint32_t>(cfg->rounds, 20);
+    const uint32_t kr = cfg->rounds;
     for (uint32_t r = 0; r < kr; ++r) {
       OURO_TRY(m.malloc_persist(sizes, ptrs, n, s, block, grid, tile, nullptr, ek[0], ek[1]));
       OURO_TRY(touch());

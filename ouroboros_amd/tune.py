@@ -148,12 +148,18 @@ def device_runner(pool_bytes: int, rounds: int = 3, warmup: int = 2) -> Runner:
             except RuntimeError:
                 pass
         try:
-            r = mm.run_batch(sizes, rounds=rounds, warmup=warmup, agg=c.agg, block=c.block, grid=c.grid, graph=True,
-                             tile=c.tile)
+            # persistent candidates are ranked by their kernels' own durations (the events of each
+            # dispatch), the per-thread ones by stream events around their launches
+            r = mm.run_batch(sizes, rounds=rounds, warmup=warmup, agg=c.agg, block=c.block, grid=c.grid, graph=False,
+                             tile=c.tile, kernel_timing=c.agg == 0)
         except RuntimeError as e:  # e.g. a block size the persistent kernels cannot launch
             return 0.0, False, str(e)
         ok = r["error_word"] == 0 and r["null_returns"] == 0 and r["verify_errors"] == 0
-        return r["mops"], ok, ",".join(r["errors"])
+        mops = r["mops"]
+        if r.get("kernel_rounds"):
+            t = (r["alloc_kernel_ms"] + r["free_kernel_ms"]) / r["kernel_rounds"] * 1e-3
+            mops = 2.0 * len(sizes) / t / 1e6 if t > 0 else 0.0
+        return mops, ok, ",".join(r["errors"])
 
     run.close = lambda: [p.close() for p in pools.values()]  # type: ignore[attr-defined]
     return run
