@@ -292,6 +292,46 @@ __device__ __forceinline__ uint32_t* ring_of(Ctx<I>& x, const ClassQueue* cq) {
   return x.inst.tab.ring[class_index(x, cq)];
 }
 
+// ---- VL queue-chunk directory
+// A linked queue finds the queue chunk of a position by walking `next` links from back_ptr_ or
+// front_ptr_, and the link to queue chunk j + 1 appears only once the put at chunk j's slot 0 has
+// itself walked to chunk j: with a million puts in flight, a class's ~50 new links per kernel form
+// one chain and each put walks tens of links. The directory caches, per class, the (virtual start,
+// queue chunk) pair of the kVLDir most recent queue chunks, entry (virtual start / 2048) mod kVLDir,
+// stored when the queue chunk has been initialised (before it is linked). While a put, take or
+// removal of position v is pending, the queue chunk holding v cannot be reclaimed (fewer than 2048
+// of its positions are dequeued), so an entry whose virtual start equals v's names exactly that
+// queue chunk. Any other entry (not published yet, or overwritten by a queue chunk kVLDir * 2048
+// positions later) sends the caller to the walk: the links stay the queue's truth. Scans, which may
+// read positions already dequeued, check the queue chunk's own virtual start as well.
+constexpr uint32_t kVLDir = 4096;
+constexpr uint64_t kVLDirEmpty = ~0ull;
+__device__ __forceinline__ uint64_t vl_dir_entry(uint32_t vstart, uint32_t qc) { return ((uint64_t)vstart << 32) | qc; }
+template <class I>
+__device__ __forceinline__ uint64_t* vl_dir_slot(Ctx<I>& x, const ClassQueue* cq, uint32_t vstart) {
+  return reinterpret_cast<uint64_t*>(x.inst.tab.ring[class_index(x, cq)]) + ((vstart / kQCSpots) & (kVLDir - 1));
+}
+// The queue chunk with virtual start vb from the directory, else kMarker. With `wait`, an entry that
+// still holds an older queue chunk (vb's is not published yet) is waited for.
+template <class I>
+__device__ inline uint32_t vl_dir_find(Ctx<I>& x, const ClassQueue* cq, uint32_t vb, bool wait) {
+  const uint64_t* e = vl_dir_slot(x, cq, vb);
+  Spin it(&x.m.error);
+  while (true) {
+    const uint64_t w = ld(e);
+    const uint32_t vs = (uint32_t)(w >> 32);
+    if (w != kVLDirEmpty && vs == vb) return (uint32_t)w;
+    // overwritten by a later queue chunk: only the walk can find vb's
+    if (!wait || (w != kVLDirEmpty && vs - vb < 0x80000000u)) {
+      OURO_DIAG_ADD(1, 1);
+      return kMarker;
+    }
+    OURO_DIAG_ADD(2, 1);
+    if (it.expired()) return kMarker;  // the walk waits on the links with its own bound
+    backoff(it.it);
+  }
+}
+
 // ---- VL pointer maintenance (QueueChunk_impl.cuh:440-551)
 template <class I>
 __device__ inline void vl_set_back(Ctx<I>& x, ClassQueue* cq, uint32_t c) {
@@ -306,22 +346,65 @@ __device__ inline void vl_set_back(Ctx<I>& x, ClassQueue* cq, uint32_t c) {
     if (it.expired()) { OURO_TIMEOUT(x.m); break; }
   }
 }
+// front_ptr_ moves from an emptied queue chunk c over every queue chunk after it that is emptied too
+// (all 2048 positions dequeued), one CAS per move of up to kVLAhead queue chunks: the queue chunks
+// after c and their counts come from the directory, kVLAhead loads in flight at a time, so the lane
+// that empties the oldest queue chunk of a long emptied stretch does not walk it link by link. A queue
+// chunk before the new front_ptr_ has all its positions dequeued, which is final; the queue chunk the
+// move stops at is checked again after the move (its last dequeue may have seen the old front_ptr_ and
+// left the move to us). Returns the number of queue chunks passed.
+constexpr uint32_t kVLAhead = 8;
 template <class I>
 __device__ inline uint32_t vl_set_front(Ctx<I>& x, ClassQueue* cq, uint32_t c) {
   uint32_t moved = 0;
+  uint32_t vs = ld(&hdr<QueueChunkHdr>(x.m, c)->virtual_start);
   while (true) {
-    QueueChunkHdr* h = hdr<QueueChunkHdr>(x.m, c);
-    const uint32_t nx = ld(&h->next);
-    if (!chunk_ok(x.m, nx)) break;
+    uint32_t e[kVLAhead], cnt[kVLAhead];
+#pragma unroll
+    for (uint32_t j = 0; j < kVLAhead; ++j) {
+      const uint32_t want = vs + (j + 1u) * kQCSpots;
+      const uint64_t w = ld(vl_dir_slot(x, cq, want));
+      e[j] = (w != kVLDirEmpty && (uint32_t)(w >> 32) == want && chunk_ok(x.m, (uint32_t)w)) ? (uint32_t)w : kMarker;
+    }
+    uint32_t p = 0;  // directory entries found, in a row
+#pragma unroll
+    for (uint32_t j = 0; j < kVLAhead; ++j)
+      if (p == j && e[j] != kMarker) ++p;
+#pragma unroll
+    for (uint32_t j = 0; j < kVLAhead; ++j) cnt[j] = j < p ? ld(&hdr<QueueChunkHdr>(x.m, e[j])->count) : 0u;
+    uint32_t k = 0;  // of those, emptied, in a row
+#pragma unroll
+    for (uint32_t j = 0; j < kVLAhead; ++j)
+      if (k == j && j < p && cnt[j] == kQCSpots) ++k;
+    uint32_t nx, step;
+    bool final = false;
+    if (p == 0) {  // not in the directory: follow the link
+      nx = ld(&hdr<QueueChunkHdr>(x.m, c)->next);
+      if (!chunk_ok(x.m, nx)) break;
+      step = 1;
+    } else if (k == p) {
+      nx = e[p - 1];
+      step = p;
+      final = true;
+    } else {
+      nx = e[k];
+      step = k + 1;
+    }
     if (cas(&cq->front_qc, c, nx) != c) break;
-    ++moved;
-    if (ld(&hdr<QueueChunkHdr>(x.m, nx)->count) == kQCSpots) c = nx;
-    else break;
+    moved += step;
+    if (!final && ld(&hdr<QueueChunkHdr>(x.m, nx)->count) != kQCSpots) break;
+    c = nx;
+    vs += step * kQCSpots;
   }
   return moved;
 }
 // setOldPointer: hold back the LARGEST_OLD_COUNT_VALUE most recently emptied queue chunks, recycle
-// the rest through the base reuse ring. Wave-uniform loop so ring_put stays lane ordered.
+// the rest through the base reuse ring. Requesting lanes go one at a time in lane order, each
+// recycling all of its queue chunks in list order, so the base ring receives them in an order a
+// sequential replay reproduces. The active lanes recycle together: the lane of rank j finds the j-th
+// held queue chunk in the directory, one CAS moves old_ptr_ over the batch, and one ring put (lane
+// order = list order) returns it; a held queue chunk missing from the directory is taken by the link
+// walk, one at a time.
 template <class I>
 __device__ inline void vl_set_old(Ctx<I>& x, ClassQueue* cq, bool want, uint32_t free_count) {
   if (want && free_count) {
@@ -340,26 +423,54 @@ __device__ inline void vl_set_old(Ctx<I>& x, ClassQueue* cq, bool want, uint32_t
   } else {
     free_count = 0;
   }
-  // one lane at a time in lane order, each recycling all of its queue chunks, so the base ring
-  // receives them in an order a sequential replay reproduces
+  const uint64_t act = active();
+  const uint32_t me = rank_in(act), nact = popc(act);
   uint64_t todo = ballot(free_count > 0);
   while (todo) {
     const int l = first(todo);
     todo &= todo - 1;
-    if ((int)lane() == l) {
-      uint32_t cur = ld(&cq->old_qc);
-      Spin it(&x.m.error);
-      while (free_count > 0) {
-        const uint32_t nx = chunk_ok(x.m, cur) ? ld(&hdr<QueueChunkHdr>(x.m, cur)->next) : kMarker;
-        const uint32_t seen = !chunk_ok(x.m, nx) ? ~cur : cas(&cq->old_qc, cur, nx);
-        if (seen == cur) {
-          ring_put(x.m, x.base_ring, true, cur);
-          cur = nx;
-          --free_count;
-        } else if (chunk_ok(x.m, nx)) {
-          cur = seen;
+    ClassQueue* lq = reinterpret_cast<ClassQueue*>(readlane64(reinterpret_cast<uint64_t>(cq), l));
+    uint32_t left = readlane(free_count, l);
+    Spin it(&x.m.error);
+    while (left > 0) {
+      uint32_t cur = 0, vs = 0;
+      if ((int)lane() == l) {
+        cur = ld(&lq->old_qc);
+        vs = chunk_ok(x.m, cur) ? ld(&hdr<QueueChunkHdr>(x.m, cur)->virtual_start) : 0u;
+      }
+      cur = readlane(cur, l);
+      vs = readlane(vs, l);
+      // ranks 1..b look up queue chunks 1..b of the held list (chunk b becomes old_ptr_)
+      const uint32_t b = chunk_ok(x.m, cur) ? (left < nact - 1u ? left : nact - 1u) : 0u;
+      uint32_t mine = me == 0 ? cur : kMarker;
+      bool miss = false;
+      if (me >= 1 && me <= b) {
+        const uint32_t wvs = vs + me * kQCSpots;
+        const uint64_t w = ld(vl_dir_slot(x, lq, wvs));
+        mine = (w != kVLDirEmpty && (uint32_t)(w >> 32) == wvs && chunk_ok(x.m, (uint32_t)w)) ? (uint32_t)w : kMarker;
+        miss = mine == kMarker;
+      }
+      uint32_t moved = 0;
+      if (b > 0 && !ballot(miss)) {
+        const uint32_t nx = readlane(mine, lane_of_rank(act, b));
+        if ((int)lane() == l) moved = cas(&lq->old_qc, cur, nx) == cur ? b : 0u;
+        moved = readlane(moved, l);
+        ring_put(x.m, x.base_ring, moved && me < b, mine);
+      } else {
+        // one queue chunk by its link
+        if ((int)lane() == l) {
+          const uint32_t nx = chunk_ok(x.m, cur) ? ld(&hdr<QueueChunkHdr>(x.m, cur)->next) : kMarker;
+          moved = chunk_ok(x.m, nx) && cas(&lq->old_qc, cur, nx) == cur ? 1u : 0u;
         }
-        if (it.expired()) { OURO_TIMEOUT(x.m); free_count = 0; }
+        moved = readlane(moved, l);
+        ring_put(x.m, x.base_ring, moved && (int)lane() == l, cur);
+      }
+      left -= moved;
+      uint32_t gone = 0;
+      if ((int)lane() == l && left > 0) gone = it.expired();
+      if (readlane(gone, l)) {
+        OURO_TIMEOUT(x.m);
+        break;
       }
     }
   }
@@ -676,6 +787,10 @@ __device__ inline void put_lanes(Ctx<I>& x, ClassQueue* cq, bool want, uint32_t 
       }
       const bool have = need && qc != kMarker;
       qc_init_lanes(x.m, have, qc, v + kQCSpots);
+      // the initialised queue chunk goes into the directory before it is linked
+      if constexpr (is_vl(K)) {
+        if (have) st(vl_dir_slot(x, cq, v + kQCSpots), vl_dir_entry(v + kQCSpots, qc));
+      }
       if constexpr (is_va(K)) {
         if (have) {
           // the entry still names the queue chunk ring_size positions back: publish only over a
@@ -724,8 +839,10 @@ __device__ inline void put_lanes(Ctx<I>& x, ClassQueue* cq, bool want, uint32_t 
       }
       ring_put(x.m, x.base_ring, reclaim, victim);
     } else {
-      // VL lockstep walk from back_ptr_
-      uint32_t cur = want ? ld(&cq->back_qc) : 0u;
+      // the directory, else the VL lockstep walk from back_ptr_
+      uint32_t cur = want ? vl_dir_find(x, cq, v - v % kQCSpots, true) : kMarker;
+      const bool direct = want && cur != kMarker;
+      if (want && !direct) cur = ld(&cq->back_qc);
       bool todo = want && chunk_ok(x.m, cur);
       if (want && !todo) set_error(x.m, ERR_VSTART);
       bool old_want = false;
@@ -735,7 +852,7 @@ __device__ inline void put_lanes(Ctx<I>& x, ClassQueue* cq, bool want, uint32_t 
         if (todo) {
           OURO_DIAG_ADD(9, 1);
           QueueChunkHdr* h = hdr<QueueChunkHdr>(x.m, cur);
-          if (ld(&h->virtual_start) / kQCSpots == v / kQCSpots) {
+          if (direct || ld(&h->virtual_start) / kQCSpots == v / kQCSpots) {
             if (v % kQCSpots == 0 && qc != kMarker) xchg(&h->next, qc);
             if constexpr (is_chunk_kind(K)) xchg(&qc_slots(x.m, cur)[v % kQCSpots], val);
             else st(&qc_slots(x.m, cur)[v % kQCSpots], val);  // a fresh queue chunk: the slot is MARKER
@@ -794,8 +911,12 @@ __device__ inline uint32_t take_lanes(Ctx<I>& x, ClassQueue* cq, bool want, uint
   } else {
     uint32_t c = kMarker;
     if (want) {
-      if constexpr (is_va(K)) c = va_wait_qc(x, cq, v);
-      else c = vl_locate(x, ld(&cq->front_qc), v);
+      if constexpr (is_va(K)) {
+        c = va_wait_qc(x, cq, v);
+      } else {
+        c = vl_dir_find(x, cq, v - v % kQCSpots, true);
+        if (c == kMarker) c = vl_locate(x, ld(&cq->front_qc), v);
+      }
       if (c != kMarker) val = ring_slot_take(x.m, &qc_slots(x.m, c)[v % kQCSpots]);
     }
     // the slots are emptied before the queue-chunk count, whose last update may recycle it
@@ -1089,6 +1210,19 @@ __device__ inline const uint32_t* slots_at(Ctx<I>& x, ClassQueue* cq, uint32_t v
     }
     return nullptr;
   } else {
+    // the directory (its queue chunk may have been reclaimed and reused since: check the virtual
+    // start the queue chunk itself holds), else the walk from the cursor
+    const uint32_t vb = v - v % kQCSpots;
+    if (const uint32_t d = vl_dir_find(x, cq, vb, false); d != kMarker) {
+      const QueueChunkHdr* h = hdr<QueueChunkHdr>(x.m, d);
+      const uint32_t vs = ld(&h->virtual_start), cnt = ld(&h->count);
+      if (vs == vb) {
+        cursor = d;
+        if (qc_out) *qc_out = d;
+        if (dead && (cnt >> 16) == 0) *dead = rest;
+        return qc_slots(x.m, d) + v % kQCSpots;
+      }
+    }
     if (cursor == kMarker) cursor = ld(&cq->front_qc);
     uint32_t c = vl_locate(x, cursor, v);
     if (c == kMarker) {  // the cursor's queue chunk is already past v: retry from front_ptr_
@@ -1186,7 +1320,8 @@ __device__ inline void tomb_slot(Ctx<I>& x, ClassQueue* cq, uint32_t p, uint32_t
       if (chunk_ok(x.m, victim)) reclaim = victim;
     }
   } else {
-    uint32_t c = cursor != kMarker ? vl_locate(x, cursor, p) : kMarker;
+    uint32_t c = vl_dir_find(x, cq, p - p % kQCSpots, false);
+    if (c == kMarker && cursor != kMarker) c = vl_locate(x, cursor, p);
     if (c == kMarker) c = vl_locate(x, ld(&cq->front_qc), p);
     if (c == kMarker) {
       set_error(x.m, ERR_CORRUPT);

@@ -574,7 +574,7 @@ struct Ouroboros {
     // one allocation holds the whole state: the reference's partition of allocation_size bytes,
     // then the chunk class bytes (so a copy of snapshot_bytes() bytes is the complete allocator)
     uint8_t* d = nullptr;
-    hipError_t e = hipMalloc(reinterpret_cast<void**>(&d), L.allocation_size + class_bytes(L.max_chunks));
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&d), L.allocation_size + class_bytes(L.max_chunks) + kDirBytes);
     if (e != hipSuccess) return e;
     memory.d_memory = d;
     memory.d_data = d + L.data_offset;
@@ -583,7 +583,7 @@ struct Ouroboros {
     memory.locator_bytes = L.locator_bytes;
     memory.locator = reinterpret_cast<uint32_t*>(d + L.allocation_size - L.locator_bytes);
     memory.chunk_log2 = d + L.allocation_size;
-    memory.pool_bytes = L.allocation_size + class_bytes(L.max_chunks);
+    memory.pool_bytes = L.allocation_size + class_bytes(L.max_chunks) + kDirBytes;
     memory.instantiation_size = instantiation_size;
     memory.add_beg = (uint32_t)additionalSizeBeginning;
     memory.add_end = (uint32_t)additionalSizeEnd;
@@ -614,10 +614,15 @@ struct Ouroboros {
     uint8_t* end = d + L.allocation_size - L.locator_bytes;
     std::vector<uint32_t> qcs;  // initial queue chunks of VA / VL queues, in allocation order
     std::vector<std::pair<uint32_t*, uint32_t>> va_heads;  // ring entry 0 of each VA queue
-    place<0, Is...>(chain, end, qcs, va_heads);
-    // every queue array and ring to the empty marker, the locator to zero
+    std::vector<std::pair<uint64_t*, uint64_t>> vl_heads;  // directory entry 0 of each VL queue
+    uint8_t* dir = memory.chunk_log2 + class_bytes(L.max_chunks);
+    place<0, Is...>(chain, end, dir, qcs, va_heads, vl_heads);
+    // every queue array and ring to the empty marker, the locator to zero, the VL directories empty
     const uint64_t qbytes = (uint64_t)(d + L.allocation_size - L.locator_bytes - end);
     e = hipMemsetD32(reinterpret_cast<hipDeviceptr_t>(end), kMarker, qbytes / 4);
+    if (e == hipSuccess && kDirBytes)
+      e = hipMemsetD32(reinterpret_cast<hipDeviceptr_t>(memory.chunk_log2 + class_bytes(L.max_chunks)), kMarker,
+                       kDirBytes / 4);
     if (e == hipSuccess) e = hipMemset(memory.locator, 0, L.locator_bytes);
     // initial queue chunks (PageQueueVA::init etc. take chunks 0, 1, 2, ... via allocateChunk<true>)
     if (e == hipSuccess && !qcs.empty()) {
@@ -636,6 +641,8 @@ struct Ouroboros {
       if (e == hipSuccess) e = hipMemcpy(memory.locator, loc.data(), ((used + 31) / 32) * 4, hipMemcpyHostToDevice);
       for (size_t i = 0; i < va_heads.size() && e == hipSuccess; ++i)
         e = hipMemcpy(va_heads[i].first, &va_heads[i].second, 4, hipMemcpyHostToDevice);
+      for (size_t i = 0; i < vl_heads.size() && e == hipSuccess; ++i)
+        e = hipMemcpy(vl_heads[i].first, &vl_heads[i].second, 8, hipMemcpyHostToDevice);
       memory.next_free_chunk = used;
     }
     memory.initialized = 1;
@@ -675,8 +682,12 @@ struct Ouroboros {
   // reference's pool plus that array): a copy of snapshot_bytes() from d_memory on is the complete
   // allocator state.
   static constexpr uint64_t class_bytes(uint64_t max_chunks) { return align_up(max_chunks, 256); }
+  // The VL queue-chunk directories (kVLDir entries of 8 bytes per class of every linked instance)
+  // follow the chunk class bytes in the same allocation.
+  static constexpr uint64_t kDirBytes =
+      ((is_vl(Is::kKind) ? (uint64_t)Is::kNQ * detail::kVLDir * 8ull : 0ull) + ... + 0ull);
   static constexpr size_t kHeapBytes = 500ull * 1024 * 1024;
-  uint64_t snapshot_bytes() const { return memory.allocation_size + class_bytes(memory.max_chunks); }
+  uint64_t snapshot_bytes() const { return memory.allocation_size + class_bytes(memory.max_chunks) + kDirBytes; }
 
   // Restore a copy taken with snapshot_bytes() from the pool of an identical instantiation (same
   // chain, same pool size) into this one: the copy replaces the pool, then every address inside the
@@ -726,10 +737,12 @@ struct Ouroboros {
   }
 
   template <int N>
-  static void place(Chain<>&, uint8_t*&, std::vector<uint32_t>&, std::vector<std::pair<uint32_t*, uint32_t>>&) {}
+  static void place(Chain<>&, uint8_t*&, uint8_t*&, std::vector<uint32_t>&, std::vector<std::pair<uint32_t*, uint32_t>>&,
+                    std::vector<std::pair<uint64_t*, uint64_t>>&) {}
   template <int N, class I, class... Rest>
-  static void place(Chain<I, Rest...>& ch, uint8_t*& end, std::vector<uint32_t>& qcs,
-                    std::vector<std::pair<uint32_t*, uint32_t>>& va_heads) {
+  static void place(Chain<I, Rest...>& ch, uint8_t*& end, uint8_t*& dir, std::vector<uint32_t>& qcs,
+                    std::vector<std::pair<uint32_t*, uint32_t>>& va_heads,
+                    std::vector<std::pair<uint64_t*, uint64_t>>& vl_heads) {
     InstanceGeometry g;
     detail::host_instance_geometry<I>(g);
     I& inst = ch.inst;
@@ -759,11 +772,18 @@ struct Ouroboros {
         // queue chunks take whole runs of the first instance's chunk length
         const uint32_t c = (uint32_t)qcs.size() * kBaseAdd;
         qcs.push_back(c);
-        if (is_vl(I::kKind)) q.front_qc = q.back_qc = q.old_qc = c;
+        if (is_vl(I::kKind)) {
+          q.front_qc = q.back_qc = q.old_qc = c;
+          // the directory's table pointer takes the unused queue-array entry of a linked queue
+          uint64_t* t = reinterpret_cast<uint64_t*>(dir);
+          dir += detail::kVLDir * 8ull;
+          inst.tab.ring[i] = reinterpret_cast<uint32_t*>(t);
+          vl_heads.emplace_back(t, ((uint64_t)0 << 32) | c);  // queue chunk 0 starts at position 0
+        }
         if (is_va(I::kKind)) va_heads.emplace_back(q.ring, c);
       }
     }
-    place<N + 1, Rest...>(ch.next, end, qcs, va_heads);
+    place<N + 1, Rest...>(ch.next, end, dir, qcs, va_heads, vl_heads);
   }
 };
 

@@ -646,7 +646,13 @@ struct Manager final : IManager {
         if (e == hipSuccess) e = hipMemcpy(q.ring, &marker, 4, hipMemcpyHostToDevice);
         if (e == hipSuccess) e = hipMemcpy(q.ring + id, &c, 4, hipMemcpyHostToDevice);
       } else if (is_vl(I::kKind)) {
+        // the queue chunk's directory entry moves with its virtual start
         c = q.front_qc;
+        uint64_t* dir = reinterpret_cast<uint64_t*>(ch.inst.tab.ring[i]);
+        const uint64_t empty = ouro::detail::kVLDirEmpty, entry = ((uint64_t)origin << 32) | c;
+        e = hipMemcpy(dir, &empty, 8, hipMemcpyHostToDevice);
+        if (e == hipSuccess)
+          e = hipMemcpy(dir + ((origin / kQCSpots) & (ouro::detail::kVLDir - 1)), &entry, 8, hipMemcpyHostToDevice);
       }
       if (e == hipSuccess && c != kMarker) {
         if (c >= mem.max_chunks) return hipErrorInvalidValue;
