@@ -394,9 +394,15 @@ __device__ inline void* block_page_of(Memory& m, I& inst, const S& sm, bool here
 // wave), recycling of chunks made full, re-enqueue on the 0 -> >0 crossing; the class signal follows
 // at block level, less the pages that left with recycled chunks (taken off the block's class total
 // in LDS, `total`).
+//
+// With `defer` (recycling off) a chunk whose count this release moved from <= 0 to > 0 is not enqueued
+// here: the lane claims the enqueue (queue_pos not queued -> pending) and returns true with the chunk's
+// generation in gen_out, and the workgroup enqueues every such chunk of the tile together afterwards
+// (block_enqueue_back, block_enqueue_chunks).
 template <class I>
-__device__ inline void block_release_chunks(Memory& m, I& inst, ReuseRing& base, bool here, uint32_t qi,
-                                            uint32_t chunk, uint32_t page, uint32_t* total) {
+__device__ inline bool block_release_chunks(Memory& m, I& inst, ReuseRing& base, bool here, uint32_t qi,
+                                            uint32_t chunk, uint32_t page, uint32_t* total, bool defer,
+                                            uint32_t& gen_out) {
   Ctx<I> x{m, inst, base};
   ClassQueue* cq = &inst.q[here ? qi : 0u];
   ChunkHdr* h = hdr<ChunkHdr>(m, here ? chunk : 0u);
@@ -419,8 +425,57 @@ __device__ inline void block_release_chunks(Memory& m, I& inst, ReuseRing& base,
     full = recycle && old + (int32_t)n == (int32_t)size;
   }
   const bool gone = flash_chunks(x, cq, full, chunk, gen, size, size - n, crossing ? 1u : 0u);
-  ensure_in_queue<64>(x, cq, crossing && !gone, chunk, gen);
   if (gone) __hip_atomic_fetch_sub(total, n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  if (defer) {
+    const bool won = crossing && !gone && cas(&h->queue_pos, qp_not(gen), qp_pending(gen)) == qp_not(gen);
+    if (won) gen_out = gen;  // lanes of other instances keep theirs
+    return won;
+  }
+  ensure_in_queue<64>(x, cq, crossing && !gone, chunk, gen);
+  return false;
+}
+
+// Wave 0, once the tile's deferred enqueues are ranked per class (sm.total, sm.off): one count_ and
+// one back_ fetch-add per class (ChunkQueue::enqueue, ChunkQueue_impl.cuh:20-38, for the whole tile).
+template <class I, class S>
+__device__ inline void block_enqueue_back(I& inst, uint32_t off, S& sm) {
+  static_assert(I::kChunks, "chunk queues only");
+  const uint32_t l = lane();
+  const bool mine = l >= off && l < off + I::kNQ;
+  const uint32_t total = mine ? sm.total[l] : 0u;
+  if (total) {
+    add(&inst.q[l - off].count, (int32_t)total);
+    sm.base[l] = add(&inst.q[l - off].back, total);
+  }
+}
+
+// The tile's deferred enqueues of one instance: chunk[t] goes to position sm.base[ec[t]] + er[t]. Every
+// queue-chunk pre-allocation of the T positions runs before any put waits (a put may wait for a queue
+// chunk that a later position of another workgroup's tile pre-allocates); queue_pos names the position
+// once the put is complete.
+template <uint32_t T, class I, class S>
+__device__ inline void block_enqueue_chunks(Memory& m, I& inst, ReuseRing& base, uint32_t off, const S& sm,
+                                            const uint32_t (&ec)[T], const uint32_t (&er)[T],
+                                            const uint32_t (&chunk)[T], const uint32_t (&gen)[T]) {
+  static_assert(I::kChunks, "chunk queues only");
+  Ctx<I> x{m, inst, base};
+  bool here[T];
+  uint32_t pos[T], qc[T];
+  ClassQueue* cq[T];
+#pragma unroll
+  for (uint32_t t = 0; t < T; ++t) {
+    here[t] = ec[t] != kMarker && ec[t] >= off && ec[t] < off + I::kNQ;
+    cq[t] = &inst.q[here[t] ? ec[t] - off : 0u];
+    pos[t] = here[t] ? sm.base[ec[t]] + er[t] : 0u;
+  }
+#pragma unroll
+  for (uint32_t t = 0; t < T; ++t) qc[t] = put_prealloc(x, cq[t], here[t], pos[t]);
+#pragma unroll
+  for (uint32_t t = 0; t < T; ++t) put_lanes_after<64>(x, cq[t], here[t], pos[t], chunk[t], qc[t]);
+  order();
+#pragma unroll
+  for (uint32_t t = 0; t < T; ++t)
+    if (here[t]) st(&hdr<ChunkHdr>(m, chunk[t])->queue_pos, qp_at(gen[t], pos[t]));
 }
 
 }  // namespace detail

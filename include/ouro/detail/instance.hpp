@@ -753,27 +753,18 @@ __device__ inline uint32_t ring_slot_take(Memory& m, uint32_t* slot) {
 // in the same order a sequential replay would. A queue chunk's counter (QueueChunk count_) takes one
 // atomic per queue chunk of a G-lane group (G = 1: one per lane, as the reference), led by the
 // group's lowest lane.
-template <int G, class I>
-__device__ inline void put_lanes(Ctx<I>& x, ClassQueue* cq, bool want, uint32_t v, uint32_t val) {
+//
+// put_prealloc is the pre-phase alone (it waits for nothing); put_lanes_after the rest, taking the
+// queue chunk the pre-phase allocated (qc, VL links it). A caller with several positions per lane
+// runs every pre-phase before any put, so that no put waits on a pre-allocation its own wave has
+// still to make.
+template <class I>
+__device__ inline uint32_t put_prealloc(Ctx<I>& x, ClassQueue* cq, bool want, uint32_t v) {
   constexpr Kind K = I::kKind;
-  if constexpr (is_std(K) && !is_chunk_kind(K)) {
-    if (want) ring_slot_put(x.m, &ring_of(x, cq)[v & (I::kQueueEntries - 1)], val);
-    order();  // the slots are written before the caller signals the pages
-  } else if constexpr (is_std(K)) {
-    if (want) {
-      uint32_t* slot = &ring_of(x, cq)[v & (I::kQueueEntries - 1)];
-      Spin it(&x.m.error);
-      while (cas(slot, kMarker, val) != kMarker) {
-        // a chunk queue's slot may still hold the tombstone of position v - ring_size
-        if constexpr (is_chunk_kind(K)) compact(x, cq);
-        if (it.expired()) { OURO_TIMEOUT(x.m); break; }
-        backoff(it.it);
-      }
-    }
-  } else {
-    // pre-phase: position 0 of a queue chunk allocates the next queue chunk
+  uint32_t qc = kMarker;
+  if constexpr (!is_std(K)) {
+    // position 0 of a queue chunk allocates the next queue chunk
     const bool need = want && (v % kQCSpots) == 0;
-    uint32_t qc = kMarker;
     if (ballot(need)) {
       uint32_t got = kMarker;
       // lanes that need one form a sub-wave; alloc_chunks hands them out in lane order
@@ -807,6 +798,37 @@ __device__ inline void put_lanes(Ctx<I>& x, ClassQueue* cq, bool want, uint32_t 
         }
       }
     }
+  }
+  return qc;
+}
+
+template <int G, class I>
+__device__ inline void put_lanes_after(Ctx<I>& x, ClassQueue* cq, bool want, uint32_t v, uint32_t val, uint32_t qc);
+
+template <int G, class I>
+__device__ inline void put_lanes(Ctx<I>& x, ClassQueue* cq, bool want, uint32_t v, uint32_t val) {
+  const uint32_t qc = put_prealloc(x, cq, want, v);
+  put_lanes_after<G>(x, cq, want, v, val, qc);
+}
+
+template <int G, class I>
+__device__ inline void put_lanes_after(Ctx<I>& x, ClassQueue* cq, bool want, uint32_t v, uint32_t val, uint32_t qc) {
+  constexpr Kind K = I::kKind;
+  if constexpr (is_std(K) && !is_chunk_kind(K)) {
+    if (want) ring_slot_put(x.m, &ring_of(x, cq)[v & (I::kQueueEntries - 1)], val);
+    order();  // the slots are written before the caller signals the pages
+  } else if constexpr (is_std(K)) {
+    if (want) {
+      uint32_t* slot = &ring_of(x, cq)[v & (I::kQueueEntries - 1)];
+      Spin it(&x.m.error);
+      while (cas(slot, kMarker, val) != kMarker) {
+        // a chunk queue's slot may still hold the tombstone of position v - ring_size
+        if constexpr (is_chunk_kind(K)) compact(x, cq);
+        if (it.expired()) { OURO_TIMEOUT(x.m); break; }
+        backoff(it.it);
+      }
+    }
+  } else {
     if constexpr (is_va(K)) {
       bool reclaim = false;
       uint32_t victim = 0;

@@ -240,20 +240,47 @@ __device__ inline void chain_block_signal(Chain<I, Rest...>& ch, S& sm, uint32_t
   if constexpr (sizeof...(Rest) > 0) chain_block_signal(ch.next, sm, off + I::kNQ);
 }
 
+// Returns true when this lane's chunk is to be enqueued by the tile's deferred enqueue (`defer`).
 template <class S, class I, class... Rest>
-__device__ inline void chain_block_release(Memory& m, Chain<I, Rest...>& ch, ReuseRing& base, S& sm, uint32_t gc,
-                                           uint32_t r, uint32_t chunk, uint32_t page, uint32_t off = 0) {
+__device__ inline bool chain_block_release(Memory& m, Chain<I, Rest...>& ch, ReuseRing& base, S& sm, uint32_t gc,
+                                           uint32_t r, uint32_t chunk, uint32_t page, bool defer, uint32_t& gen,
+                                           uint32_t off = 0) {
   const bool here = gc != kMarker && gc >= off && gc < off + I::kNQ;
+  bool won = false;
   if (dev::ballot(here)) {
     const uint32_t qi = here ? gc - off : 0u;
     if constexpr (I::kChunks) {
-      block_release_chunks(m, ch.inst, base, here, qi, chunk, page, &sm.total[here ? gc : off]);
+      const bool w = block_release_chunks(m, ch.inst, base, here, qi, chunk, page, &sm.total[here ? gc : off], defer, gen);
+      if (here) won = w;
     } else {
       Ctx<I> x{m, ch.inst, base};
       put_lanes<64>(x, &ch.inst.q[qi], here, here ? sm.base[gc] + r : 0u, make_index(chunk, page));
     }
   }
-  if constexpr (sizeof...(Rest) > 0) chain_block_release(m, ch.next, base, sm, gc, r, chunk, page, off + I::kNQ);
+  if constexpr (sizeof...(Rest) > 0) {
+    const bool w = chain_block_release(m, ch.next, base, sm, gc, r, chunk, page, defer, gen, off + I::kNQ);
+    if (!here) won = w;
+  }
+  return won;
+}
+
+template <class S, class I, class... Rest>
+__device__ inline void chain_block_enqueue_back(Chain<I, Rest...>& ch, S& sm, uint32_t off = 0) {
+  if constexpr (I::kChunks) block_enqueue_back(ch.inst, off, sm);
+  if constexpr (sizeof...(Rest) > 0) chain_block_enqueue_back(ch.next, sm, off + I::kNQ);
+}
+
+template <uint32_t T, class S, class I, class... Rest>
+__device__ inline void chain_block_enqueue(Memory& m, Chain<I, Rest...>& ch, ReuseRing& base, const S& sm,
+                                           const uint32_t (&ec)[T], const uint32_t (&er)[T], const uint32_t (&chunk)[T],
+                                           const uint32_t (&gen)[T], uint32_t off = 0) {
+  if constexpr (I::kChunks) {
+    bool any = false;
+#pragma unroll
+    for (uint32_t t = 0; t < T; ++t) any |= ec[t] != kMarker && ec[t] >= off && ec[t] < off + I::kNQ;
+    if (dev::ballot(any)) block_enqueue_chunks<T>(m, ch.inst, base, off, sm, ec, er, chunk, gen);
+  }
+  if constexpr (sizeof...(Rest) > 0) chain_block_enqueue<T>(m, ch.next, base, sm, ec, er, chunk, gen, off + I::kNQ);
 }
 
 // enqueueInitialChunk for class qi of instance I (single lane).
@@ -508,9 +535,32 @@ struct Ouroboros {
     if constexpr (kRingPages) {
       detail::chain_put_ring_tile<T>(memory, chain, sm, gc, r, chunk, page);
     } else {
+      // chunk queues without recycling defer the enqueues of the chunks the tile's frees bring back
+      // to the queue, and enqueue them together: one count_ and one back_ fetch-add per class and
+      // tile instead of one per wave and sub-tile (same-address atomics serialise device-wide)
+      const bool defer = kAnyChunks && memory.recycle_min == kRecycleOff;
+      uint32_t ec[T], gen[T];
 #pragma unroll
-      for (uint32_t t = 0; t < T; ++t)
-        detail::chain_block_release(memory, chain, chain.inst.reuse, sm, gc[t], r[t], chunk[t], page[t]);
+      for (uint32_t t = 0; t < T; ++t) {
+        gen[t] = 0;
+        const bool won =
+            detail::chain_block_release(memory, chain, chain.inst.reuse, sm, gc[t], r[t], chunk[t], page[t], defer, gen[t]);
+        ec[t] = won ? gc[t] : kMarker;
+      }
+      if constexpr (kAnyChunks) {
+        if (defer) {
+          // the class totals of the frees wait in sm.served for the signal; the enqueues are ranked
+          // per class like requests
+          if (threadIdx.x < kNumClasses) sm.served[threadIdx.x] = sm.total[threadIdx.x];
+          __syncthreads();
+          uint32_t er[T];
+          detail::block_rank_tile<T>(sm, ec, er, kNumClasses);
+          if (threadIdx.x < kWave) detail::chain_block_enqueue_back(chain, sm);
+          __syncthreads();
+          detail::chain_block_enqueue<T>(memory, chain, chain.inst.reuse, sm, ec, er, chunk, gen);
+          if (threadIdx.x < kNumClasses) sm.total[threadIdx.x] = sm.served[threadIdx.x];
+        }
+      }
     }
     __syncthreads();  // every put / release of the block is done: publish, then signal
     if (threadIdx.x < kWave) detail::chain_block_signal(chain, sm);
