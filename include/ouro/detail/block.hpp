@@ -188,17 +188,23 @@ __device__ inline void block_signal(I& inst, uint32_t off, S& sm) {
 
 // ---- chunk queues
 // A class's block total is claimed by one whole wave (class gc goes to wave gc mod waves, so the
-// classes of a block proceed in parallel). The wave looks at up to 64 queue positions at once, one
-// chunk per lane: a prefix sum over the chunks' free counts divides the pages in lane order, each
-// lane takes its share with one count CAS, removes a chunk it emptied at the position it found it
-// at, and claims the bits from the chunk's 64-bit mask words, writing the page of each rank it
-// serves into sm.pidx. The reference walks the queue one chunk at a time per thread
-// (ChunkQueue_impl.cuh:88-151); a serial walk per class leader leaves a block waiting on its
-// largest class, where every chunk holds a single page.
+// classes of a block proceed in parallel). Each step looks at up to 64 * kClaimPer queue positions,
+// kClaimPer consecutive ones per lane, all slot and header loads in flight together (measured: with 2
+// or 4 per lane the persistent kernels spill registers and the claims got slower, so 1): a prefix sum over
+// the chunks' free counts (in position order) divides the pages, each lane takes its share of each of
+// its chunks with one count CAS, removes a chunk it emptied at the position it found it at, and claims
+// the bits from the chunks' 64-bit mask words (the first word of every chunk in flight together),
+// writing the page of each rank it serves into sm.pidx. front_ is compacted once, after the claim, if
+// it removed anything. The reference walks the queue one chunk at a time per thread
+// (ChunkQueue_impl.cuh:88-151); a serial walk per class leader leaves a block waiting on its largest
+// class, where every chunk holds a single page.
+constexpr uint32_t kClaimPer = 1;
+
 template <class I, class S>
 __device__ inline void block_claim_chunks(Memory& m, I& inst, ReuseRing& base, uint32_t off, S& sm,
                                           uint32_t nclasses) {
   constexpr Kind K = I::kKind;
+  constexpr uint32_t P = kClaimPer;
   const uint32_t wave = threadIdx.x >> 6, nwaves = (blockDim.x + 63u) >> 6;
   const uint32_t l = lane();
   Ctx<I> x{m, inst, base};
@@ -216,9 +222,10 @@ __device__ inline void block_claim_chunks(Memory& m, I& inst, ReuseRing& base, u
     uint32_t v = 0;
     if (l == 0) v = draw_start(x, cq, total);
     v = uniform(v);
-    // (no redraws here: a whole wave passes 64 positions a step, and measured, drawing new starts
-    // scattered the block claims' consumption order and doubled their kernel time)
+    // (no redraws here: a whole wave passes 64 * kClaimPer positions a step, and measured, drawing
+    // new starts scattered the block claims' consumption order and doubled their kernel time)
     uint32_t assigned = 0, wraps = 0, cursor = kMarker, reclaim = kMarker;
+    bool any_removed = false;
     Spin it(&m.error);
     while (assigned < total) {
       uint32_t b = 0, nb = 0;
@@ -226,9 +233,9 @@ __device__ inline void block_claim_chunks(Memory& m, I& inst, ReuseRing& base, u
       if (l == 0) {
         b = ld(&cq->back);
         wraps += scan_wrap<I>(cq, v, cursor, b);
-        nb = b - v;  // 64 positions at a time, not past back_, inside one queue chunk for VA / VL
+        nb = b - v;  // 64 * kClaimPer positions at a time, not past back_, inside one queue chunk for VA / VL
         if (nb >= 0x80000000u) nb = 0;
-        if (nb > kWave) nb = kWave;
+        if (nb > kWave * P) nb = kWave * P;
         if constexpr (!is_std(K)) nb = nb < kQCSpots - v % kQCSpots ? nb : kQCSpots - v % kQCSpots;
         give_up = (wraps >= 2 || nb == 0) && pool_exhausted(m);
       }
@@ -253,15 +260,12 @@ __device__ inline void block_claim_chunks(Memory& m, I& inst, ReuseRing& base, u
         OURO_DIAG_ADD(0, 1);
         OURO_DIAG_ADD(2, nb);
       }
-      // the slots of positions v .. v + nb (one queue chunk for VA / VL)
-      uint32_t s = kMarker, qc = kMarker;
-      uint32_t* slot = nullptr;
-      if constexpr (is_std(K)) {
-        slot = &ring_of(x, cq)[(v + l) & (I::kQueueEntries - 1)];
-        if (l < nb) s = ld(slot);
-      } else {
+      // the slots of positions v .. v + nb (one queue chunk for VA / VL): lane l holds v + l * P + j
+      uint32_t s[P], qc = kMarker;
+      uint32_t* slot[P];
+      const uint32_t* sl = nullptr;
+      if constexpr (!is_std(K)) {
         uint32_t fill = kMarker, dead = 0;
-        const uint32_t* sl = nullptr;
         if (l == 0) sl = slots_at(x, cq, v, cursor, fill, &qc, &dead);
         sl = reinterpret_cast<const uint32_t*>(readlane64(reinterpret_cast<uint64_t>(sl), 0));
         cursor = uniform(cursor);
@@ -277,43 +281,85 @@ __device__ inline void block_claim_chunks(Memory& m, I& inst, ReuseRing& base, u
           }
           continue;
         }
-        slot = const_cast<uint32_t*>(sl) + l;
-        if (l < nb) s = sl ? ld(&sl[l]) : fill;
+#pragma unroll
+        for (uint32_t j = 0; j < P; ++j) {
+          const uint32_t i = l * P + j;
+          slot[j] = const_cast<uint32_t*>(sl) + i;
+          s[j] = i < nb ? (sl ? ld(&sl[i]) : fill) : kMarker;
+        }
+      } else {
+#pragma unroll
+        for (uint32_t j = 0; j < P; ++j) {
+          const uint32_t i = l * P + j;
+          slot[j] = &ring_of(x, cq)[(v + i) & (I::kQueueEntries - 1)];
+          s[j] = i < nb ? ld(slot[j]) : kMarker;
+        }
       }
-      ChunkHdr* h = nullptr;
-      int32_t cnt = 0;
-      uint32_t g = 0;
-      if (l < nb && chunk_ok(m, s)) {
-        h = hdr<ChunkHdr>(m, s);
-        const uint64_t cg = ld(count_gen(h));
-        cnt = cg_count(cg);
-        g = cg_gen(cg);
-        if (cnt <= 0 || ld(&h->queue_pos) != qp_at(g, v + l) || ld(&h->page_size) != psize ||
-            gen_class(g) != gc - off)
-          cnt = 0;
+      // the chunks' headers, all in flight
+      uint64_t cg[P], qp[P];
+      uint32_t ps[P], sz[P];
+#pragma unroll
+      for (uint32_t j = 0; j < P; ++j) {
+        ChunkHdr* h = hdr<ChunkHdr>(m, chunk_ok(m, s[j]) ? s[j] : 0u);
+        cg[j] = ld(count_gen(h));
+        qp[j] = ld(&h->queue_pos);
+        ps[j] = ld(&h->page_size);
+        sz[j] = ld(&h->size);
       }
-      const uint32_t c = cnt > 0 ? (uint32_t)cnt : 0u;
+      int32_t cnt[P];
+      uint32_t g[P], c[P], lane_sum = 0;
+#pragma unroll
+      for (uint32_t j = 0; j < P; ++j) {
+        cnt[j] = cg_count(cg[j]);
+        g[j] = cg_gen(cg[j]);
+        const bool ok = chunk_ok(m, s[j]) && cnt[j] > 0 && qp[j] == qp_at(g[j], v + l * P + j) && ps[j] == psize &&
+                        gen_class(g[j]) == gc - off;
+        c[j] = ok ? (uint32_t)cnt[j] : 0u;
+        lane_sum += c[j];
+      }
       uint32_t offered = 0;
-      const uint32_t before = wave_exclusive_sum(c, &offered);
+      uint32_t before = wave_exclusive_sum(lane_sum, &offered);
       const uint32_t rem = total - assigned;
-      // share = clamp(rem - before, 0, c), written branch-free: with the bound inside an `if` the
-      // ROCm 7.2 compiler hoisted the min out of the branch and dropped the before < rem guard
-      const uint32_t owed = before < rem ? rem - before : 0u;
-      uint32_t t = owed < c ? owed : c;
-      // changed (or recycled) meanwhile: skip it
-      if (t && cas(count_gen(h), make_cg(cnt, g), make_cg(cnt - (int32_t)t, g)) != make_cg(cnt, g)) t = 0;
+      uint32_t t[P], lane_took = 0;
+#pragma unroll
+      for (uint32_t j = 0; j < P; ++j) {
+        // share = clamp(rem - before, 0, c), written branch-free: with the bound inside an `if` the
+        // ROCm 7.2 compiler hoisted the min out of the branch and dropped the before < rem guard
+        const uint32_t owed = before < rem ? rem - before : 0u;
+        t[j] = owed < c[j] ? owed : c[j];
+        before += c[j];
+      }
+      // changed (or recycled) meanwhile: skip it (the CASes in flight together)
+      uint64_t seen[P];
+#pragma unroll
+      for (uint32_t j = 0; j < P; ++j)
+        seen[j] = t[j] ? cas(count_gen(hdr<ChunkHdr>(m, s[j])), make_cg(cnt[j], g[j]), make_cg(cnt[j] - (int32_t)t[j], g[j]))
+                       : 0ull;
+#pragma unroll
+      for (uint32_t j = 0; j < P; ++j) {
+        if (t[j] && seen[j] != make_cg(cnt[j], g[j])) t[j] = 0;
+        lane_took += t[j];
+      }
       uint32_t took = 0;
-      const uint32_t r0 = assigned + wave_exclusive_sum(t, &took);
+      const uint32_t r0 = assigned + wave_exclusive_sum(lane_took, &took);
       // chunks this claim emptied leave the queue at the position they were found at (cq_remove for
-      // the whole batch: slots tombed in parallel, one count update per queue and queue chunk, one
-      // wave-wide compaction of front_)
-      bool removed = false;
-      if (t && t == c) removed = cas(&h->queue_pos, qp_at(g, v + l), qp_not(g)) == qp_at(g, v + l);
-      const uint64_t rm = ballot(removed);
-      if (l == 0) OURO_DIAG_ADD(8, popc(rm));
-      if (rm) {
-        if (removed) xchg(slot, kTomb);
-        const uint32_t nrm = popc(rm);
+      // the whole batch: slots tombed in parallel, one count update per queue and queue chunk)
+      bool removed[P];
+      uint32_t lane_rm = 0;
+#pragma unroll
+      for (uint32_t j = 0; j < P; ++j) {
+        const uint64_t want_qp = qp_at(g[j], v + l * P + j);
+        removed[j] = t[j] && t[j] == c[j] && cas(&hdr<ChunkHdr>(m, s[j])->queue_pos, want_qp, qp_not(g[j])) == want_qp;
+        lane_rm += removed[j] ? 1u : 0u;
+      }
+      uint32_t nrm = 0;
+      (void)wave_exclusive_sum(lane_rm, &nrm);
+      if (l == 0) OURO_DIAG_ADD(8, nrm);
+      if (nrm) {
+        any_removed = true;
+#pragma unroll
+        for (uint32_t j = 0; j < P; ++j)
+          if (removed[j]) xchg(slot[j], kTomb);
         uint32_t rc = kMarker;
         if (l == 0) {
           sub(&cq->count, (int32_t)nrm);
@@ -335,34 +381,61 @@ __device__ inline void block_claim_chunks(Memory& m, I& inst, ReuseRing& base, u
           reclaim = rc;
         }
         order();
-        if (removed && ld(&h->count) > 0) ensure_in_queue<1>(x, cq, true, s, g);  // a free raced in
-        compact_wave(x, cq);
+#pragma unroll
+        for (uint32_t j = 0; j < P; ++j) {
+          // a free raced in
+          const bool again = removed[j] && ld(&hdr<ChunkHdr>(m, s[j])->count) > 0;
+          if (ballot(again)) ensure_in_queue<1>(x, cq, again, s[j], g[j]);
+        }
       }
-      if (t) {
-        const uint32_t size = ld(&h->size);
-        const uint32_t nw = mask_words(size);
-        uint32_t w = ((size - (uint32_t)cnt) / 64u) % nw;
-        uint32_t left = t, r = start + r0;
+      // mask claims: the first word of every chunk with pages to take, all in flight; a chunk whose
+      // pages run over into further words continues word by word
+      uint32_t w[P], left[P], r = start + r0;
+      uint64_t mw[P];
+#pragma unroll
+      for (uint32_t j = 0; j < P; ++j) {
+        const uint32_t nw = mask_words(sz[j]);
+        w[j] = t[j] ? ((sz[j] - (uint32_t)cnt[j]) / 64u) % nw : 0u;
+        left[j] = t[j];
+        mw[j] = t[j] ? ld(&hdr<ChunkHdr>(m, s[j])->mask[w[j]]) : 0ull;
+      }
+      uint64_t take[P], got[P];
+#pragma unroll
+      for (uint32_t j = 0; j < P; ++j) {
+        take[j] = left[j] ? lowest_bits(mw[j], left[j]) : 0ull;
+        got[j] = take[j] ? and_(&hdr<ChunkHdr>(m, s[j])->mask[w[j]], ~take[j]) & take[j] : 0ull;
+      }
+#pragma unroll
+      for (uint32_t j = 0; j < P; ++j) {
+        if (!t[j]) continue;
+        const uint32_t nw = mask_words(sz[j]);
+        ChunkHdr* h = hdr<ChunkHdr>(m, s[j]);
+        uint64_t gb = got[j];
+        bool whole = take[j] != 0 && gb == take[j];
+        if (take[j] == 0) w[j] = (w[j] + 1) % nw;
         Spin bits_wait(&m.error);
-        while (left > 0) {
-          const uint64_t take = lowest_bits(ld(&h->mask[w]), left);
-          if (take == 0) {
-            w = (w + 1) % nw;
-          } else {
-            uint64_t got = and_(&h->mask[w], ~take) & take;
-            left -= popc(got);
-            const bool whole = got == take;
-            while (got) {
-              const uint32_t bit = (uint32_t)__builtin_ctzll(got);
-              got &= got - 1;
-              sm.pidx[r++] = make_index(s, w * 64u + bit);
-            }
-            if (whole && left > 0) w = (w + 1) % nw;
+        while (true) {
+          left[j] -= popc(gb);
+          while (gb) {
+            const uint32_t bit = (uint32_t)__builtin_ctzll(gb);
+            gb &= gb - 1;
+            sm.pidx[r++] = make_index(s[j], w[j] * 64u + bit);
           }
-          if (left > 0 && bits_wait.expired()) {
+          if (left[j] == 0) break;
+          if (whole) w[j] = (w[j] + 1) % nw;
+          if (bits_wait.expired()) {
             OURO_TIMEOUT(m);
             break;
           }
+          const uint64_t tk = lowest_bits(ld(&h->mask[w[j]]), left[j]);
+          if (tk == 0) {
+            w[j] = (w[j] + 1) % nw;
+            gb = 0;
+            whole = false;
+            continue;
+          }
+          gb = and_(&h->mask[w[j]], ~tk) & tk;
+          whole = gb == tk;
         }
       }
       assigned += took;
@@ -372,6 +445,8 @@ __device__ inline void block_claim_chunks(Memory& m, I& inst, ReuseRing& base, u
         break;
       }
     }
+    // front_ over the leading positions this claim emptied (one wave-wide compaction per claim)
+    if (ballot(any_removed)) compact_wave(x, cq);
     if (l == 0) {
       sm.base[gc] = start;
       sm.served[gc] = assigned;

@@ -945,17 +945,18 @@ static void compact(ouro_sim* s, inst_t* I, cls_t* q, int wave) {
   }
 }
 
-/* Block-wide claim of one class by a whole wave (the device's block_claim_chunks): up to 64 queue
- * positions at a time, one chunk per lane; a prefix sum over the chunks' free counts divides the
- * pages in lane order; chunks emptied are removed at the position they were found (slots tombed in
- * lane order, then one compaction of front_, as the lockstep wave does); each lane claims its pages
- * from the chunk's mask words, lowest bits first. */
+/* Block-wide claim of one class by a whole wave (the device's block_claim_chunks): up to CLAIM_STEP
+ * queue positions at a time (one per lane); a prefix sum over the chunks' free counts
+ * divides the pages in position order; chunks emptied are removed at the position they were found
+ * (slots tombed in position order); each chunk's pages come from its mask words, lowest bits first;
+ * front_ is compacted once, after the claim, if it removed anything. */
+#define CLAIM_STEP 64u
 static void claim_wave(ouro_sim* s, inst_t* I, group_t* g, uint64_t* out) {
   cls_t* q = &I->q[g->qi];
   const uint32_t total = (uint32_t)g->n;
   uint32_t v = draw_start(q, total);
   uint32_t cursor = MARKER, assigned = 0, wraps = 0, rec = MARKER;
-  int guard = 0;
+  int guard = 0, any_removed = 0;
   while (assigned < total) { /* (block claims draw no new starts) */
     if (++guard > 10000000) {
       s->error |= ERR_TIMEOUT;
@@ -967,10 +968,10 @@ static void claim_wave(ouro_sim* s, inst_t* I, group_t* g, uint64_t* out) {
       cursor = MARKER;
       ++wraps;
     }
-    /* 64 positions at a time (a single leader peeks PEEK), inside one queue chunk for VA / VL */
+    /* CLAIM_STEP positions at a time (a single leader peeks PEEK), inside one queue chunk for VA / VL */
     uint32_t nb = b - v;
     if (nb >= 0x80000000u) nb = 0;
-    if (nb > 64) nb = 64;
+    if (nb > CLAIM_STEP) nb = CLAIM_STEP;
     if (!is_std(I->kind) && nb > QC_SPOTS - v % QC_SPOTS) nb = QC_SPOTS - v % QC_SPOTS;
     if ((wraps >= 2 || nb == 0) && (s->error & (ERR_OOM | ERR_QUEUE_FULL))) {
       q->cnt += total - assigned;
@@ -980,8 +981,8 @@ static void claim_wave(ouro_sim* s, inst_t* I, group_t* g, uint64_t* out) {
       s->error |= ERR_TIMEOUT;
       return;
     }
-    uint32_t sl[64];
-    for (uint32_t i = 0; i < 64; ++i) sl[i] = MARKER;
+    uint32_t sl[CLAIM_STEP];
+    for (uint32_t i = 0; i < CLAIM_STEP; ++i) sl[i] = MARKER;
     if (is_std(I->kind)) {
       for (uint32_t i = 0; i < nb; ++i) sl[i] = *std_slot(q, v + i);
     } else {
@@ -997,11 +998,11 @@ static void claim_wave(ouro_sim* s, inst_t* I, group_t* g, uint64_t* out) {
         continue;
       }
     }
-    int32_t cnt[64];
-    uint32_t t[64];
+    int32_t cnt[CLAIM_STEP];
+    uint32_t t[CLAIM_STEP];
     uint32_t before = 0, took = 0;
     const uint32_t rem = total - assigned;
-    for (uint32_t i = 0; i < 64; ++i) {
+    for (uint32_t i = 0; i < CLAIM_STEP; ++i) {
       cnt[i] = 0;
       t[i] = 0;
       if (i < nb && chunk_ok(s, sl[i])) {
@@ -1011,16 +1012,15 @@ static void claim_wave(ouro_sim* s, inst_t* I, group_t* g, uint64_t* out) {
       if (cnt[i] > 0 && before < rem) t[i] = (uint32_t)cnt[i] < rem - before ? (uint32_t)cnt[i] : rem - before;
       before += cnt[i] > 0 ? (uint32_t)cnt[i] : 0u;
     }
-    uint32_t r0[64];
-    for (uint32_t i = 0; i < 64; ++i) {
+    uint32_t r0[CLAIM_STEP];
+    for (uint32_t i = 0; i < CLAIM_STEP; ++i) {
       r0[i] = assigned + took;
       took += t[i];
       if (t[i]) ch(s, sl[i])->count -= (int32_t)t[i];
     }
-    /* removals: the batch's emptied chunks are tombed (lane order), then front_ is compacted once;
-     * a queue chunk the batch empties is reclaimed once, held as the claim's pending reclaim */
-    int removed = 0;
-    for (uint32_t i = 0; i < 64; ++i) {
+    /* removals: the batch's emptied chunks are tombed (position order); a queue chunk the batch
+     * empties is reclaimed once, held as the claim's pending reclaim */
+    for (uint32_t i = 0; i < CLAIM_STEP; ++i) {
       if (!t[i] || t[i] != (uint32_t)cnt[i]) continue;
       chunk_t* x = ch(s, sl[i]);
       if (x->queue_pos != (uint64_t)(v + i)) continue;
@@ -1032,11 +1032,10 @@ static void claim_wave(ouro_sim* s, inst_t* I, group_t* g, uint64_t* out) {
         rec = rc;
       }
       q->count--;
-      removed = 1;
+      any_removed = 1;
     }
-    if (removed) compact(s, I, q, 1);
     /* mask claims */
-    for (uint32_t i = 0; i < 64; ++i) {
+    for (uint32_t i = 0; i < CLAIM_STEP; ++i) {
       if (!t[i]) continue;
       chunk_t* x = ch(s, sl[i]);
       const uint32_t nw = mask_words(x->size);
@@ -1066,6 +1065,7 @@ static void claim_wave(ouro_sim* s, inst_t* I, group_t* g, uint64_t* out) {
     assigned += took;
     v += nb;
   }
+  if (any_removed) compact(s, I, q, 1);
   if (rec != MARKER) finish_reclaim(s, I, q, rec);
 }
 
