@@ -49,6 +49,8 @@ __device__ inline void block_rank_tile(S& sm, const uint32_t (&gc)[T], uint32_t 
     if (valid && (int)lane() == first(grp)) row[gc[t]] = popc(grp);
   }
   __syncthreads();
+  // (measured: one exclusive wave scan per class over the rows was no faster at four requests per
+  // thread and 10% slower at one than this per-class walk by the class's lane of wave 0)
   if (wave == 0 && lane() < nclasses) {
     uint32_t run = 0;
     for (uint32_t t = 0; t < T; ++t)

@@ -157,14 +157,18 @@ struct Spin {
 // ------------------------------------------------------------------ wave primitives
 __device__ __forceinline__ uint32_t lane() { return __lane_id(); }
 __device__ __forceinline__ uint32_t uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
-// Exclusive prefix sum over the 64 lanes of a fully active wave; *total gets the wave's sum.
+// Exclusive prefix sum over the 64 lanes of a fully active wave; *total gets the wave's sum. An
+// inclusive scan with DPP moves (row shifts by 1, 2, 4, 8 inside each 16-lane row, then the row
+// broadcasts of lanes 15 and 31 into the rows above), which stay in the VALU instead of going
+// through the LDS crossbar as lane shuffles do.
 __device__ __forceinline__ uint32_t wave_exclusive_sum(uint32_t v, uint32_t* total) {
   uint32_t x = v;
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    const uint32_t y = __shfl_up(x, d, 64);
-    if ((int)__lane_id() >= d) x += y;
-  }
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xf, 0xf, false);  // row_shr:1
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xf, 0xf, false);  // row_shr:2
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xf, 0xf, false);  // row_shr:4
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xf, 0xf, false);  // row_shr:8
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xa, 0xf, false);  // row_bcast:15
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xc, 0xf, false);  // row_bcast:31
   *total = __builtin_amdgcn_readlane(x, 63);
   return x - v;
 }
