@@ -1417,8 +1417,12 @@ __device__ inline bool cq_remove(Ctx<I>& x, ClassQueue* cq, uint32_t chunk, uint
 // where it sits (queue_pos CAS, tombstone, count_, as cq_remove) and queues the chunk again at back_
 // if it still has free pages (a chunk whose last page a claim took meanwhile stays out of the queue
 // until a free gives it pages, as after any removal). The chunk's pages stay counted by the
-// semaphore throughout. At most kMaxRelocate moves per call.
-constexpr uint32_t kRelocSlack = 4096;
+// semaphore throughout. At most kMaxRelocate moves per call. The slack is small: with 4096 positions,
+// a class of few chunks (512 pages each) let the chunk its claims had left partly used at front_
+// block compaction until the window held thousands of dead positions, the page cursor ran past
+// back_ and folded into them, and the per-thread claims scanned them every round (measured: the
+// 64-lane malloc of a 1M-request round grew from 3 ms to 80 ms over 13 rounds).
+constexpr uint32_t kRelocSlack = 64;
 constexpr uint32_t kMaxRelocate = 16;
 
 template <class I>

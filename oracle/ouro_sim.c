@@ -37,7 +37,7 @@
 #define WAVE_COMPACT_STEPS 8192u /* front moves per call of the wave-wide compaction */
 #define REDRAW_AFTER 64u      /* positions a claim passes without a page before it draws a new start */
 #define MAX_REDRAWS 4u
-#define RELOC_SLACK 4096u     /* front relocation: window longer than 2 x queued chunks + this */
+#define RELOC_SLACK 64u       /* front relocation: window longer than 2 x queued chunks + this */
 #define MAX_RELOCATE 16u
 
 enum { ERR_OOM = 1, ERR_QUEUE_FULL = 2, ERR_VSTART = 4, ERR_TIMEOUT = 8, ERR_BAD_FREE = 16, ERR_TOO_LARGE = 32,

@@ -439,7 +439,7 @@ def test_position_wrap_is_invisible(kind, case):
 def test_chunk_queue_window_stays_bounded_under_held_blocks(kind):
     # skewed sizes and blocks held for a random number of waves: claims start at spread positions,
     # so live chunks near front_ get passed over while the chunks behind them cycle through back_;
-    # front relocation keeps the window within twice the queued chunks plus the 4096-position slack
+    # front relocation keeps the window within twice the queued chunks plus the 64-position slack
     # (without it the window grows without bound: about 8,400 dead positions after 20,000 waves)
     rng = np.random.default_rng(5)
     s = oracle.Sim(kind, 512 << 20)
@@ -456,7 +456,7 @@ def test_chunk_queue_window_stays_bounded_under_held_blocks(kind):
             for k in range(10):
                 length = (st.back[k] - st.front[k]) % (1 << 32)
                 worst = max(worst, length - 2 * st.queue_count[k])
-    assert 2048 < worst <= 4096 + 64, worst
+    assert 0 < worst <= 64 + 64, worst  # reaches the slack (relocation is exercised), never far past it
     for o in live:
         s.wave_free(o)
     _check_chunk_queues(s, s.state())
