@@ -83,12 +83,14 @@ def test_concurrent_mixed_invariants(kind, agg):
 @pytest.mark.parametrize("form", ["agg1", "agg64", "block_t4"])
 def test_chunk_queue_front_keeps_up(kind, form):
     # chunk queues remove emptied chunks wherever they sit and compact front_ over the tombstones;
-    # across rounds the window back_ - front_ must stay within a bounded distance of the live entry
-    # count (a compaction that falls behind lets it grow by a fixed amount every round until the
-    # ring wraps onto uncompacted slots)
+    # across rounds every class window back_ - front_ must stay within twice its live entries plus the
+    # 64-position relocation slack (and a margin for compaction in flight). A compaction that falls
+    # behind lets it grow by a fixed amount every round; a partly used chunk left at front_ did so in
+    # the classes of 256 and 512 pages per chunk, by 200-400 positions a round, until the per-thread
+    # claims scanned thousands of dead positions (3 -> 80 ms per 1M-request round)
     print(f"front keeps up {kind} {form}", flush=True)
     sizes = ob.gen_sizes(42, 262144, "uniform")
-    gaps = []
+    excess = []
     with ob.MemoryManager(kind, 2 << 30) as mm:
         for _ in range(16):
             if form == "block_t4":
@@ -97,9 +99,9 @@ def test_chunk_queue_front_keeps_up(kind, form):
                 r = mm.run_trace(sizes, 1, agg=int(form[3:])).result
             assert r["error_word"] == 0, r["errors"]
             st = mm.query()
-            gaps.append(max((st.back[k] - st.front[k]) % (1 << 32) - st.queue_count[k] for k in range(10)))
+            excess.append(max((st.back[k] - st.front[k]) % (1 << 32) - 2 * st.queue_count[k] for k in range(10)))
         assert mm.check_quiescent().violations == 0
-    assert max(gaps) < 16384, gaps
+    assert max(excess) <= 64 + 512, excess
 
 
 @pytest.mark.parametrize("kind", ["CQ", "VACQ", "VLCQ"])
@@ -107,7 +109,7 @@ def test_chunk_queue_front_keeps_up(kind, form):
 def test_chunk_queue_window_bounded_with_held_blocks(kind, agg):
     # blocks held across rounds leave chunks with few free pages; claims that start at spread
     # positions pass some of them over, and front relocation must keep each class window within
-    # twice its queued chunks plus the 4096-position slack (before it, the window of the 8 KiB class
+    # twice its queued chunks plus the 64-position slack (before it, the window of the 8 KiB class
     # grew by some 50,000 positions a round)
     print(f"held-block window {kind} agg={agg}", flush=True)
     sizes = ob.gen_sizes(42, 1 << 20, "zipf", 1.2)
@@ -117,7 +119,7 @@ def test_chunk_queue_window_bounded_with_held_blocks(kind, agg):
         st = mm.query()
         for k in range(10):
             length = (st.back[k] - st.front[k]) % (1 << 32)
-            assert length <= 2 * st.queue_count[k] + 4096 + 8192, (k, length, st.queue_count[k])
+            assert length <= 2 * st.queue_count[k] + 64 + 8192, (k, length, st.queue_count[k])
         chk = mm.check_quiescent()
         assert chk.violations == 0, chk.message.decode()
 
