@@ -208,13 +208,29 @@ def serve_config5(kind: str, requests: int) -> dict:
             "quiescent_violations": int(chk.violations), "loads": rows}
 
 
+def counted_atomics(variant: str, block: int, tile: int) -> dict | None:
+    """Atomic requests per round counted by the L2 (rocprofv3 TCC_ATOMIC_sum, median per dispatch of
+    the allocation and the free kernel) in the committed counter pass of this configuration."""
+    try:
+        t = json.loads(TRAFFIC_FILE.read_text())
+    except (OSError, ValueError):
+        return None
+    a = t.get("configs", {}).get(f"{variant}/b{block}/t{tile}", {}).get("atomics")
+    if not a or "k_malloc_persist" not in a or "k_free_persist" not in a:
+        return None
+    per_kernel = {k: int(a[k].get("TCC_ATOMIC_sum", 0)) for k in ("k_malloc_persist", "k_free_persist")}
+    return {"per_round": sum(per_kernel.values()), "per_kernel": per_kernel,
+            "source": str(TRAFFIC_FILE.relative_to(ROOT)), "counter": "TCC_ATOMIC_sum"}
+
+
 def atomic_roofline(best, kind: str, sizes: np.ndarray, head: dict, steps: int, distinct: float,
                     same_wave: float) -> dict:
     """Atomic RMWs the headline form issues per round against the measured agent-scope u32
-    fetch-add rates. The block-wide form on standard page rings issues four per class present in a
-    tile (semaphore claim and front_ add on malloc, back_ add and signal on free) and hands ring
-    slots over with plain loads and stores; the per-thread forms issue at least two per malloc+free
-    pair (a slot or mask-bit RMW on each side)."""
+    fetch-add rates: the L2's own count from the committed counter pass where it covers the headline
+    configuration, else the count the protocol implies. The block-wide form on standard page rings
+    issues four per class present in a tile (semaphore claim and front_ add on malloc, back_ add and
+    signal on free) and hands ring slots over with plain loads and stores; the per-thread forms
+    issue at least two per malloc+free pair (a slot or mask-bit RMW on each side)."""
     round_s = (head["alloc_ms"] + head["free_ms"]) / steps * 1e-3
     if best.agg == 0 and kind == "PQ":
         per_tile = best.block * best.tile
@@ -226,9 +242,16 @@ def atomic_roofline(best, kind: str, sizes: np.ndarray, head: dict, steps: int, 
     else:
         issued = 2 * len(sizes)
         how = "at least 2 per malloc+free pair (slot or mask-bit RMW on each side)"
+    counted = counted_atomics(best.variant, best.block, best.tile) if best.agg == 0 else None
+    implied = issued
+    if counted:
+        issued = counted["per_round"]
+        how = f"counted by the L2 ({counted['counter']}, {counted['source']}); the protocol implies {implied}: " + how
     rate = issued / round_s if round_s else None
     return {
         "atomics_per_round": int(issued),
+        "atomics_per_round_implied": int(implied),
+        "counted": counted,
         "atomics_per_malloc_free_pair": round(issued / len(sizes), 5),
         "achieved_atomics_per_s": rate,
         "peak_atomics_per_s": distinct,
