@@ -212,7 +212,7 @@ int ouro_set_recycling(ouro_handle h, uint32_t min_queued);
 int ouro_statistics(ouro_handle h, uint64_t* out4);
 
 /* Bytes of a snapshot: the reference's pool partition (allocation_size) plus the chunk class bytes
- * that follow it in the same allocation. */
+ * and the linked queues' queue-chunk directories that follow it in the same allocation. */
 int ouro_snapshot_size(ouro_handle h, uint64_t* bytes);
 /* Copy the complete allocator state (the whole allocation, ouro_snapshot_size bytes) to dst, device
  * or host memory. Take it while no kernel uses the pool. (Ouroboros keeps all state in the pool and
