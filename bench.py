@@ -272,6 +272,18 @@ def probe(args, sizes: np.ndarray) -> int:
     plain launches, each kernel timed by its own dispatch's events), nothing else."""
     variant = args.variant or "PQ"
     gbs = ob.bench_copy(COPY_BYTES, 1)
+    if args.stress:
+        # the held-block stress (4,194,304 threads, Zipf 1.2, half held across rounds, 16 GiB) on the
+        # per-thread kernels at --agg (default 1), --variant (default VACQ)
+        n, pool, agg = 1 << 22, 16 << 30, args.agg or 1
+        variant = args.variant or "VACQ"
+        zs = ob.gen_sizes(SEED, n, "zipf", 1.2)
+        with ob.MemoryManager(variant, pool) as mm:
+            r = mm.run_trace(zs, args.steps, agg=agg, hold_permille=500, hold_seed=SEED).result
+        print(json.dumps({"probe": "stress", "variant": variant, "agg": agg, "copy_gbs": round(gbs, 1),
+                          "mops": round(mops(r, n, args.steps), 1), "alloc_ms": r["alloc_ms"] / args.steps,
+                          "free_ms": r["free_ms"] / args.steps}), flush=True)
+        return 0
     if args.agg:
         # the per-thread kernels (k_malloc / k_free, one request per kernel thread) at this width
         with ob.MemoryManager(variant, POOL) as mm:
@@ -323,6 +335,8 @@ def main() -> int:
                     "aggregation width instead of the batched form")
     ap.add_argument("--graph", type=int, default=1, help="with --probe: 1 = graph-replayed rounds as the headline, "
                     "0 = plain launches (counter passes)")
+    ap.add_argument("--stress", action="store_true", help="with --probe: the held-block stress (4M threads, Zipf "
+                    "1.2, half held, 16 GiB) on the per-thread kernels at --agg (default 1), --variant (default VACQ)")
     ap.add_argument("--no-table", action="store_true", help="skip the per-kind plain / wave / batched table")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-config4", action="store_true", help="skip the 256-workload batch set")
