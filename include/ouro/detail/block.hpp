@@ -483,7 +483,20 @@ __device__ inline bool block_release_chunks(Memory& m, I& inst, ReuseRing& base,
   Ctx<I> x{m, inst, base};
   ClassQueue* cq = &inst.q[here ? qi : 0u];
   ChunkHdr* h = hdr<ChunkHdr>(m, here ? chunk : 0u);
-  if (here) or_(&h->mask[page / 64u], 1ull << (page % 64u));
+  // the bits first (ChunkAccess::freePage, ChunkAccess_impl.cuh:7-25): one OR per mask word the wave
+  // frees pages of, carrying all of them (a tile frees the pages its malloc took together, so its
+  // lanes share words), instead of one per page
+  {
+    const uint64_t key = here ? ((uint64_t)chunk << 3) | (page / 64u) : ~0ull;
+    for (uint64_t todo = ballot(here); todo;) {
+      const int l = first(todo);
+      const uint64_t same = ballot(key == readlane64(key, l)) & todo;
+      todo &= ~same;
+      uint64_t bits = 0;
+      for (uint64_t r = same; r; r &= r - 1) bits |= 1ull << (readlane(page, first(r)) % 64u);
+      if ((int)lane() == l) or_(&h->mask[page / 64u], bits);
+    }
+  }
   order();
   const uint64_t grp = match<64>(here ? (uint64_t)chunk : ~0ull);
   const int L = first(grp);
