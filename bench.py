@@ -170,6 +170,7 @@ def stress_config3(rounds: int = 4) -> dict:
 
 
 SERVE_RATES = (2000.0, 5000.0, 10000.0, 20000.0, 50000.0)
+TABLE_TILES = (1, 2, 4)
 
 
 def serve_config5(kind: str, requests: int) -> dict:
@@ -399,13 +400,20 @@ def main() -> int:
                     log(f"table {v} agg={agg} ...")
                     r = run_trace(v, agg, sizes, max(4, steps // 4), 1)
                 row[f"agg{agg}"] = {"mops": round(r["mops"], 2), "errors": r["errors"], "verify_errors": r["verify_errors"]}
-            log(f"table {v} batched ...")
-            with ob.MemoryManager(v, POOL) as mm:
-                # like the headline: a validation round and two warm-up rounds before the timed ones
-                b = mm.run_batch(sizes, rounds=max(4, steps // 2), warmup=2, agg=0, block=1024, graph=True,
-                                 check=True, tile=table_tile, kernel_timing=True)
-            row["batched_block1024"] = {"mops": round(kernel_mops(b, len(sizes)), 2), "tile": table_tile,
-                                        "graph_replay_mops": round(b["mops"], 2), "errors": b["errors"]}
+            # the batched form at each request tile (a tuned parameter of the form; the headline's tile
+            # is tuned for PQ), reported at the fastest
+            by_tile, best_b, best_t = {}, None, table_tile
+            for t in TABLE_TILES:
+                log(f"table {v} batched tile={t} ...")
+                with ob.MemoryManager(v, POOL) as mm:
+                    # like the headline: a validation round and two warm-up rounds before the timed ones
+                    b = mm.run_batch(sizes, rounds=max(4, steps // 2), warmup=2, agg=0, block=1024, graph=True,
+                                     check=True, tile=t, kernel_timing=True)
+                by_tile[t] = round(kernel_mops(b, len(sizes)), 2)
+                if best_b is None or by_tile[t] > by_tile[best_t]:
+                    best_b, best_t = b, t
+            row["batched_block1024"] = {"mops": by_tile[best_t], "tile": best_t, "mops_by_tile": by_tile,
+                                        "graph_replay_mops": round(best_b["mops"], 2), "errors": best_b["errors"]}
             table[v] = row
     best_plain = max([plain["mops"]] + [table[v]["agg1"]["mops"] for v in table])
 
