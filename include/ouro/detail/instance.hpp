@@ -1938,8 +1938,21 @@ template <int W, class I>
 __device__ inline void free_chunk_pages(Ctx<I>& x, uint32_t qi, bool want, uint32_t chunk, uint32_t page) {
   ClassQueue* cq = &x.inst.q[want ? qi : 0];
   ChunkHdr* h = hdr<ChunkHdr>(x.m, want ? chunk : 0u);
-  // ChunkAccess::freePage (ChunkAccess_impl.cuh:7-25): set the bit, then count
-  if (want) or_(&h->mask[page / 64u], 1ull << (page % 64u));
+  // ChunkAccess::freePage (ChunkAccess_impl.cuh:7-25): set the bit, then count. Groups (W = 16 / 64)
+  // set their bits with one OR per mask word and segment, carrying all of them; W = 1 one per page.
+  if constexpr (W == 1) {
+    if (want) or_(&h->mask[page / 64u], 1ull << (page % 64u));
+  } else {
+    const uint64_t key = want ? ((uint64_t)chunk << 3) | (page / 64u) : ~0ull;
+    for (uint64_t todo = ballot(want); todo;) {
+      const int l = first(todo);
+      const uint64_t same = ballot(key == readlane64(key, l)) & todo & segment_of<W>(l);
+      todo &= ~same;
+      uint64_t bits = 0;
+      for (uint64_t r = same; r; r &= r - 1) bits |= 1ull << (readlane(page, first(r)) % 64u);
+      if ((int)lane() == l) or_(&h->mask[page / 64u], bits);
+    }
+  }
   order();
   const uint64_t key = want ? (uint64_t)chunk : ~0ull;
   const uint64_t grp = match<W>(key);  // W = 1: every lane on its own
