@@ -480,8 +480,10 @@ struct Ouroboros {
       out[t] = gc[t] == kMarker ? nullptr : memory.d_data + ((uint64_t)gc[t] << 27) + (uint64_t)r[t] * 16u;
   }
 
+  // Inlined into its kernel (free_tile_in too): compiled as a call, its register save and restore went
+  // through scratch (80 B per lane in the chunk kinds' persistent free) and the frees ran 10-17% slower.
   template <uint32_t T>
-  __device__ void free_tile(void* const (&ptr)[T], TileScratch<T>& sm) {
+  __device__ __forceinline__ void free_tile(void* const (&ptr)[T], TileScratch<T>& sm) {
     if constexpr (T > 1 && kVirtualized && !kAnyChunks) {
       // VA / VL page queues: a put at a queue-chunk boundary pre-allocates the next queue chunk, and
       // puts further on wait for it. put_lanes does a wave's pre-allocations before any of its waits,
@@ -500,7 +502,7 @@ struct Ouroboros {
   }
 
   template <uint32_t T, class S>
-  __device__ void free_tile_in(void* const (&ptr)[T], S& sm) {
+  __device__ __forceinline__ void free_tile_in(void* const (&ptr)[T], S& sm) {
     uint32_t chunk[T], page[T], gc[T], r[T];
 #pragma unroll
     for (uint32_t t = 0; t < T; ++t) {
