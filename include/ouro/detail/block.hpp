@@ -170,6 +170,32 @@ __device__ inline void block_put_ring_tile(Memory& m, I& inst, uint32_t off, con
   order();  // the slots are written before the block signals the pages
 }
 
+// Virtualized page queues (VA / VL): the tile's frees of one instance at the positions the block's
+// back_ fetch-adds gave them. Every sub-tile's queue-chunk pre-allocation runs before any put waits: a
+// put waits for the queue chunk (VL: the link to it) that the put at an earlier position allocates,
+// and ranks put sub-tile t's positions below sub-tile t + 1's, so no put waits on a pre-allocation
+// its own wave has still to make.
+template <uint32_t T, class I, class S>
+__device__ inline void block_put_pages_tile(Memory& m, I& inst, ReuseRing& base, uint32_t off, const S& sm,
+                                            const uint32_t (&gc)[T], const uint32_t (&r)[T],
+                                            const uint32_t (&chunk)[T], const uint32_t (&page)[T]) {
+  static_assert(!I::kChunks && !is_std(I::kKind), "virtualized page queues only");
+  Ctx<I> x{m, inst, base};
+  bool here[T];
+  uint32_t pos[T], qc[T];
+  ClassQueue* cq[T];
+#pragma unroll
+  for (uint32_t t = 0; t < T; ++t) {
+    here[t] = gc[t] != kMarker && gc[t] >= off && gc[t] < off + I::kNQ;
+    cq[t] = &inst.q[here[t] ? gc[t] - off : 0u];
+    pos[t] = here[t] ? sm.base[gc[t]] + r[t] : 0u;
+  }
+#pragma unroll
+  for (uint32_t t = 0; t < T; ++t) qc[t] = put_prealloc(x, cq[t], here[t], pos[t]);
+#pragma unroll
+  for (uint32_t t = 0; t < T; ++t) put_lanes_after<64>(x, cq[t], here[t], pos[t], make_index(chunk[t], page[t]), qc[t]);
+}
+
 // Wave 0: back_ fetch-add per class of the block's frees.
 template <class I, class S>
 __device__ inline void block_back(I& inst, uint32_t off, S& sm) {

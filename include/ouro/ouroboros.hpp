@@ -228,6 +228,18 @@ __device__ inline void chain_put_ring_tile(Memory& m, Chain<I, Rest...>& ch, S& 
   if constexpr (sizeof...(Rest) > 0) chain_put_ring_tile<T>(m, ch.next, sm, gc, r, chunk, page, off + I::kNQ);
 }
 
+// Whole-tile puts of chains of page queues, standard rings or virtualized (instance by instance, in
+// chain order on every workgroup).
+template <uint32_t T, class S, class I, class... Rest>
+__device__ inline void chain_put_pages_tile(Memory& m, Chain<I, Rest...>& ch, ReuseRing& base, S& sm,
+                                            const uint32_t (&gc)[T], const uint32_t (&r)[T],
+                                            const uint32_t (&chunk)[T], const uint32_t (&page)[T], uint32_t off = 0) {
+  static_assert(!I::kChunks, "page queues only");
+  if constexpr (is_std(I::kKind)) block_put_ring_tile<T>(m, ch.inst, off, sm, gc, r, chunk, page);
+  else block_put_pages_tile<T>(m, ch.inst, base, off, sm, gc, r, chunk, page);
+  if constexpr (sizeof...(Rest) > 0) chain_put_pages_tile<T>(m, ch.next, base, sm, gc, r, chunk, page, off + I::kNQ);
+}
+
 template <class S, class I, class... Rest>
 __device__ inline void chain_block_back(Chain<I, Rest...>& ch, S& sm, uint32_t off = 0) {
   if constexpr (!I::kChunks) block_back(ch.inst, off, sm);
@@ -484,21 +496,7 @@ struct Ouroboros {
   // through scratch (80 B per lane in the chunk kinds' persistent free) and the frees ran 10-17% slower.
   template <uint32_t T>
   __device__ __forceinline__ void free_tile(void* const (&ptr)[T], TileScratch<T>& sm) {
-    if constexpr (T > 1 && kVirtualized && !kAnyChunks) {
-      // VA / VL page queues: a put at a queue-chunk boundary pre-allocates the next queue chunk, and
-      // puts further on wait for it. put_lanes does a wave's pre-allocations before any of its waits,
-      // but sub-tiles released one after the other would wait (sub-tile 0) before pre-allocating
-      // (sub-tile 3), and two workgroups ordered differently in two class queues could then wait on
-      // each other. So the sub-tiles are freed as consecutive block-wide frees, each reserving its own
-      // back_ positions right before its puts.
-#pragma unroll
-      for (uint32_t t = 0; t < T; ++t) {
-        void* const p[1] = {ptr[t]};
-        free_tile_in<1>(p, sm);
-      }
-    } else {
-      free_tile_in<T>(ptr, sm);
-    }
+    free_tile_in<T>(ptr, sm);
   }
 
   template <uint32_t T, class S>
@@ -536,6 +534,10 @@ struct Ouroboros {
     __syncthreads();
     if constexpr (kRingPages) {
       detail::chain_put_ring_tile<T>(memory, chain, sm, gc, r, chunk, page);
+    } else if constexpr (!kAnyChunks) {
+      // VA / VL page queues: a put at a queue-chunk boundary pre-allocates the next queue chunk and
+      // puts further on wait for it, so every sub-tile's pre-allocations come before any put
+      detail::chain_put_pages_tile<T>(memory, chain, chain.inst.reuse, sm, gc, r, chunk, page);
     } else {
       // chunk queues without recycling defer the enqueues of the chunks the tile's frees bring back
       // to the queue, and enqueue them together: one count_ and one back_ fetch-add per class and
