@@ -192,8 +192,58 @@ __device__ inline void block_put_pages_tile(Memory& m, I& inst, ReuseRing& base,
   }
 #pragma unroll
   for (uint32_t t = 0; t < T; ++t) qc[t] = put_prealloc(x, cq[t], here[t], pos[t]);
+  if constexpr (is_va(I::kKind)) {
+    // the T puts' lookups, slot loads and stores each in flight together (put_lanes_after's VA branch
+    // for T positions at once): pointer-ring entries, queue-chunk checks, slots
+    uint32_t c[T];
 #pragma unroll
-  for (uint32_t t = 0; t < T; ++t) put_lanes_after<64>(x, cq[t], here[t], pos[t], make_index(chunk[t], page[t]), qc[t]);
+    for (uint32_t t = 0; t < T; ++t)
+      c[t] = here[t] ? ld(&ring_of(x, cq[t])[(pos[t] / kQCSpots) & (I::kQueueEntries - 1)]) : kMarker;
+    uint32_t vs[T];
+#pragma unroll
+    for (uint32_t t = 0; t < T; ++t) {
+      if (here[t] && (c[t] == kMarker || is_qc_tag(c[t]))) c[t] = va_wait_qc(x, cq[t], pos[t]);  // pre-allocation in flight
+      vs[t] = here[t] && chunk_ok(m, c[t]) ? ld(&hdr<QueueChunkHdr>(m, c[t])->virtual_start) : 0u;
+    }
+    uint32_t* slot[T];
+    uint32_t cur[T];
+#pragma unroll
+    for (uint32_t t = 0; t < T; ++t) {
+      if (here[t] && (!chunk_ok(m, c[t]) || vs[t] / kQCSpots != pos[t] / kQCSpots)) {
+        if (c[t] != kMarker) set_error(m, ERR_VSTART);
+        c[t] = kMarker;
+      }
+      slot[t] = c[t] != kMarker ? &qc_slots(m, c[t])[pos[t] % kQCSpots] : nullptr;
+      cur[t] = slot[t] ? ld(slot[t]) : kMarker;
+    }
+#pragma unroll
+    for (uint32_t t = 0; t < T; ++t) {
+      if (!slot[t]) continue;
+      // one writer and one reader per position, as a page ring: a slot still holding the entry of the
+      // previous lap waits for its take
+      if (cur[t] == kMarker) st(slot[t], make_index(chunk[t], page[t]));
+      else ring_slot_put(m, slot[t], make_index(chunk[t], page[t]));
+    }
+    order();  // the slots are written before the queue-chunk counts, whose last update may recycle one
+#pragma unroll
+    for (uint32_t t = 0; t < T; ++t) {
+      const uint64_t grp = match<64>(c[t] != kMarker ? (uint64_t)c[t] : ~0ull);
+      bool reclaim = false;
+      uint32_t victim = 0;
+      if (c[t] != kMarker && (int)lane() == first(grp)) {
+        const uint32_t added = popc(grp) * 0x10001u;
+        const uint32_t old = add(&hdr<QueueChunkHdr>(m, c[t])->count, added);
+        if (qc_empty_after_enqueue(old, added)) {
+          victim = xchg(&ring_of(x, cq[t])[(pos[t] / kQCSpots) & (I::kQueueEntries - 1)], qc_tag(pos[t] / kQCSpots));
+          reclaim = chunk_ok(m, victim);
+        }
+      }
+      ring_put(m, base, reclaim, victim);
+    }
+  } else {
+#pragma unroll
+    for (uint32_t t = 0; t < T; ++t) put_lanes_after<64>(x, cq[t], here[t], pos[t], make_index(chunk[t], page[t]), qc[t]);
+  }
 }
 
 // Wave 0: back_ fetch-add per class of the block's frees.
