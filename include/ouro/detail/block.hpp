@@ -144,6 +144,84 @@ __device__ inline void block_take_ring_tile(Memory& m, I& inst, uint32_t off, co
   }
 }
 
+// ---- VA page queues, a whole tile at once (take_lanes' VA branch for T positions)
+// Phase A: the tile's pointer-ring lookups, queue-chunk checks and slot takes each in flight together;
+// c[t] keeps the queue chunk each take emptied a slot of. Phase B (per sub-tile, in the order a
+// sub-tile-by-sub-tile take would run it, so emptied queue chunks reach the reuse ring in the same
+// order): the queue-chunk counts, one subtraction per queue chunk and wave, and the recycling of a
+// queue chunk the last take emptied.
+template <uint32_t T, class I, class S>
+__device__ inline void block_take_va_tile_slots(Memory& m, I& inst, ReuseRing& base, uint32_t off, const S& sm,
+                                                const uint32_t (&gc)[T], const uint32_t (&r)[T], void* (&out)[T],
+                                                uint32_t (&c)[T]) {
+  static_assert(!I::kChunks && is_va(I::kKind), "VA page queues only");
+  Ctx<I> x{m, inst, base};
+  bool here[T];
+  uint32_t pos[T];
+  ClassQueue* cq[T];
+#pragma unroll
+  for (uint32_t t = 0; t < T; ++t) {
+    here[t] = gc[t] != kMarker && gc[t] >= off && gc[t] < off + I::kNQ;
+    cq[t] = &inst.q[here[t] ? gc[t] - off : 0u];
+    pos[t] = here[t] ? sm.base[gc[t]] + r[t] : 0u;
+    c[t] = here[t] ? ld(&ring_of(x, cq[t])[(pos[t] / kQCSpots) & (I::kQueueEntries - 1)]) : kMarker;
+  }
+  uint32_t vs[T];
+#pragma unroll
+  for (uint32_t t = 0; t < T; ++t) {
+    if (here[t] && (c[t] == kMarker || is_qc_tag(c[t]))) c[t] = va_wait_qc(x, cq[t], pos[t]);
+    vs[t] = here[t] && chunk_ok(m, c[t]) ? ld(&hdr<QueueChunkHdr>(m, c[t])->virtual_start) : 0u;
+  }
+  uint32_t* slot[T];
+  uint32_t val[T];
+#pragma unroll
+  for (uint32_t t = 0; t < T; ++t) {
+    if (here[t] && (!chunk_ok(m, c[t]) || vs[t] / kQCSpots != pos[t] / kQCSpots)) {
+      if (c[t] != kMarker) set_error(m, ERR_VSTART);
+      c[t] = kMarker;
+    }
+    slot[t] = c[t] != kMarker ? &qc_slots(m, c[t])[pos[t] % kQCSpots] : nullptr;
+    val[t] = slot[t] ? ld(slot[t]) : kMarker;
+  }
+#pragma unroll
+  for (uint32_t t = 0; t < T; ++t) {
+    void* p = nullptr;
+    if (slot[t]) {
+      if (val[t] == kMarker) val[t] = ring_slot_take(m, slot[t]);  // not yet published: wait for it
+      else st(slot[t], kMarker);
+      const uint32_t qi = gc[t] - off;
+      if (val[t] != kMarker && val[t] != kNullPage) {
+        if (!chunk_ok(m, index_chunk(val[t])) || index_page(val[t]) >= ((I::kChunkSize >> I::kSmallLog2) >> qi))
+          set_error(m, ERR_CORRUPT);
+        else
+          p = chunk_addr(m, index_chunk(val[t])) + kMeta + (uint64_t)index_page(val[t]) * (I::kSmallest << qi);
+      }
+    }
+    if (here[t]) out[t] = p;
+  }
+}
+
+// Phase B for sub-tile t of one instance (after an order() that completes every slot take).
+template <uint32_t T, class I>
+__device__ inline void block_take_va_tile_count(Memory& m, I& inst, ReuseRing& base, uint32_t off,
+                                                const uint32_t (&gc)[T], uint32_t t, const uint32_t (&c)[T],
+                                                uint32_t pos_qc) {
+  const uint64_t grp = match<64>(c[t] != kMarker ? (uint64_t)c[t] : ~0ull);
+  bool reclaim = false;
+  uint32_t victim = 0;
+  if (c[t] != kMarker && (int)lane() == first(grp)) {
+    const uint32_t taken = popc(grp) << 16;
+    const uint32_t old = sub(&hdr<QueueChunkHdr>(m, c[t])->count, taken);
+    if (old - taken == kQCSpots) {  // QueueChunk::checkChunkEmptyDequeue
+      Ctx<I> x{m, inst, base};
+      ClassQueue* cq = &inst.q[gc[t] - off];
+      victim = xchg(&ring_of(x, cq)[pos_qc & (I::kQueueEntries - 1)], qc_tag(pos_qc));
+      reclaim = chunk_ok(m, victim);
+    }
+  }
+  ring_put(m, base, reclaim, victim);
+}
+
 template <uint32_t T, class I, class S>
 __device__ inline void block_put_ring_tile(Memory& m, I& inst, uint32_t off, const S& sm, const uint32_t (&gc)[T],
                                            const uint32_t (&r)[T], const uint32_t (&chunk)[T],

@@ -228,6 +228,26 @@ __device__ inline void chain_put_ring_tile(Memory& m, Chain<I, Rest...>& ch, S& 
   if constexpr (sizeof...(Rest) > 0) chain_put_ring_tile<T>(m, ch.next, sm, gc, r, chunk, page, off + I::kNQ);
 }
 
+// Whole-tile takes of chains whose instances are all VA page queues: every instance's slot takes
+// first (c[i] keeps instance i's queue chunks), then, after the caller's order(), the queue-chunk counts
+// sub-tile by sub-tile and instance by instance, the order of a take per sub-tile.
+template <uint32_t T, class S, class I, class... Rest>
+__device__ __forceinline__ void chain_take_va_slots(Memory& m, Chain<I, Rest...>& ch, ReuseRing& base, S& sm,
+                                                    const uint32_t (&gc)[T], const uint32_t (&r)[T], void* (&out)[T],
+                                                    uint32_t (*c)[T], uint32_t off = 0) {
+  block_take_va_tile_slots<T>(m, ch.inst, base, off, sm, gc, r, out, c[0]);
+  if constexpr (sizeof...(Rest) > 0) chain_take_va_slots<T>(m, ch.next, base, sm, gc, r, out, c + 1, off + I::kNQ);
+}
+template <uint32_t T, class S, class I, class... Rest>
+__device__ __forceinline__ void chain_take_va_count(Memory& m, Chain<I, Rest...>& ch, ReuseRing& base, S& sm,
+                                                    const uint32_t (&gc)[T], const uint32_t (&r)[T], uint32_t t,
+                                                    uint32_t (*c)[T], uint32_t off = 0) {
+  const bool here = gc[t] != kMarker && gc[t] >= off && gc[t] < off + I::kNQ;
+  const uint32_t pq = here ? (sm.base[gc[t]] + r[t]) / kQCSpots : 0u;
+  if (dev::ballot(c[0][t] != kMarker)) block_take_va_tile_count<T>(m, ch.inst, base, off, gc, t, c[0], pq);
+  if constexpr (sizeof...(Rest) > 0) chain_take_va_count<T>(m, ch.next, base, sm, gc, r, t, c + 1, off + I::kNQ);
+}
+
 // Whole-tile puts of chains of page queues, standard rings or virtualized (instance by instance, in
 // chain order on every workgroup).
 template <uint32_t T, class S, class I, class... Rest>
@@ -416,6 +436,7 @@ struct Ouroboros {
   static constexpr bool kAnyChunks = (Is::kChunks || ...);
   // every instance a standard page ring: block-wide takes and puts of a tile go out together
   static constexpr bool kRingPages = !kAnyChunks && !kVirtualized;
+  static constexpr bool kVAPages = ((is_va(Is::kKind) && !Is::kChunks) && ...);
   static_assert(kNumClasses <= kBlockMaxClasses, "too many size classes for the block scratch");
   template <uint32_t T>
   using TileScratch = BlockScratch<kAnyChunks, T>;
@@ -463,6 +484,14 @@ struct Ouroboros {
 #pragma unroll
       for (uint32_t t = 0; t < T; ++t) out[t] = nullptr;
       detail::chain_take_ring_tile<T>(memory, chain, sm, gc, r, out);
+    } else if constexpr (kVAPages) {
+#pragma unroll
+      for (uint32_t t = 0; t < T; ++t) out[t] = nullptr;
+      uint32_t c[sizeof...(Is)][T];
+      detail::chain_take_va_slots<T>(memory, chain, chain.inst.reuse, sm, gc, r, out, c);
+      dev::order();  // the slots are emptied before the queue-chunk counts, whose last update may recycle one
+#pragma unroll
+      for (uint32_t t = 0; t < T; ++t) detail::chain_take_va_count<T>(memory, chain, chain.inst.reuse, sm, gc, r, t, c);
     } else {
 #pragma unroll
       for (uint32_t t = 0; t < T; ++t)
