@@ -679,6 +679,61 @@ __device__ inline bool block_release_chunks(Memory& m, I& inst, ReuseRing& base,
   return false;
 }
 
+// block_release_chunks for a whole tile with the enqueues deferred (recycling off): the tile's bit
+// ORs in flight together, one order(), then the count adds of every sub-tile in flight, then the
+// enqueue claims. won[t] (gen_out[t]) as block_release_chunks returns them, for the lanes whose request
+// t belongs to this instance.
+template <uint32_t T, class I>
+__device__ inline void block_release_chunks_tile(Memory& m, I& inst, uint32_t off, const uint32_t (&gc)[T],
+                                                 const uint32_t (&chunk)[T], const uint32_t (&page)[T],
+                                                 bool (&won)[T], uint32_t (&gen_out)[T]) {
+  static_assert(I::kChunks, "chunk queues only");
+  bool here[T];
+  ChunkHdr* h[T];
+#pragma unroll
+  for (uint32_t t = 0; t < T; ++t) {
+    here[t] = gc[t] != kMarker && gc[t] >= off && gc[t] < off + I::kNQ;
+    h[t] = hdr<ChunkHdr>(m, here[t] ? chunk[t] : 0u);
+  }
+#pragma unroll
+  for (uint32_t t = 0; t < T; ++t) {
+    const uint64_t key = here[t] ? ((uint64_t)chunk[t] << 3) | (page[t] / 64u) : ~0ull;
+    for (uint64_t todo = ballot(here[t]); todo;) {
+      const int l = first(todo);
+      const uint64_t same = ballot(key == readlane64(key, l)) & todo;
+      todo &= ~same;
+      uint64_t bits = 0;
+      for (uint64_t r = same; r; r &= r - 1) bits |= 1ull << (readlane(page[t], first(r)) % 64u);
+      if ((int)lane() == l) or_(&h[t]->mask[page[t] / 64u], bits);
+    }
+  }
+  order();
+  bool lead[T];
+  int32_t old[T], n[T];
+#pragma unroll
+  for (uint32_t t = 0; t < T; ++t) {
+    const uint64_t grp = match<64>(here[t] ? (uint64_t)chunk[t] : ~0ull);
+    lead[t] = here[t] && (int)lane() == first(grp);
+    n[t] = (int32_t)popc(grp);
+    old[t] = lead[t] ? add(&h[t]->count, n[t]) : 1;
+  }
+  uint32_t gen[T];
+  bool crossing[T];
+#pragma unroll
+  for (uint32_t t = 0; t < T; ++t) {
+    crossing[t] = lead[t] && old[t] <= 0 && old[t] + n[t] > 0;
+    gen[t] = crossing[t] ? ld(&h[t]->gen) : 0u;
+  }
+#pragma unroll
+  for (uint32_t t = 0; t < T; ++t) {
+    const bool w = crossing[t] && cas(&h[t]->queue_pos, qp_not(gen[t]), qp_pending(gen[t])) == qp_not(gen[t]);
+    if (here[t]) {
+      won[t] = w;
+      if (w) gen_out[t] = gen[t];
+    }
+  }
+}
+
 // Wave 0, once the tile's deferred enqueues are ranked per class (sm.total, sm.off): one count_ and
 // one back_ fetch-add per class (ChunkQueue::enqueue, ChunkQueue_impl.cuh:20-38, for the whole tile).
 template <class I, class S>

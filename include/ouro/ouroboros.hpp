@@ -248,6 +248,19 @@ __device__ __forceinline__ void chain_take_va_count(Memory& m, Chain<I, Rest...>
   if constexpr (sizeof...(Rest) > 0) chain_take_va_count<T>(m, ch.next, base, sm, gc, r, t, c + 1, off + I::kNQ);
 }
 
+// Whole-tile chunk frees of chains of chunk queues with deferred enqueues (block_release_chunks_tile),
+// instance by instance.
+template <uint32_t T, class I, class... Rest>
+__device__ __forceinline__ void chain_release_chunks_tile(Memory& m, Chain<I, Rest...>& ch, const uint32_t (&gc)[T],
+                                                          const uint32_t (&chunk)[T], const uint32_t (&page)[T],
+                                                          bool (&won)[T], uint32_t (&gen)[T], uint32_t off = 0) {
+  bool any = false;
+#pragma unroll
+  for (uint32_t t = 0; t < T; ++t) any |= gc[t] != kMarker && gc[t] >= off && gc[t] < off + I::kNQ;
+  if (dev::ballot(any)) block_release_chunks_tile<T>(m, ch.inst, off, gc, chunk, page, won, gen);
+  if constexpr (sizeof...(Rest) > 0) chain_release_chunks_tile<T>(m, ch.next, gc, chunk, page, won, gen, off + I::kNQ);
+}
+
 // Whole-tile puts of chains of page queues, standard rings or virtualized (instance by instance, in
 // chain order on every workgroup).
 template <uint32_t T, class S, class I, class... Rest>
@@ -437,6 +450,7 @@ struct Ouroboros {
   // every instance a standard page ring: block-wide takes and puts of a tile go out together
   static constexpr bool kRingPages = !kAnyChunks && !kVirtualized;
   static constexpr bool kVAPages = ((is_va(Is::kKind) && !Is::kChunks) && ...);
+  static constexpr bool kAllChunks = (Is::kChunks && ...);
   static_assert(kNumClasses <= kBlockMaxClasses, "too many size classes for the block scratch");
   template <uint32_t T>
   using TileScratch = BlockScratch<kAnyChunks, T>;
@@ -573,12 +587,30 @@ struct Ouroboros {
       // tile instead of one per wave and sub-tile (same-address atomics serialise device-wide)
       const bool defer = kAnyChunks && memory.recycle_min == kRecycleOff;
       uint32_t ec[T], gen[T];
+      bool tiled = false;
+      if constexpr (kAllChunks) {
+        if (defer) {
+          // the whole tile's bits, counts and enqueue claims, each in flight together
+          bool won[T];
 #pragma unroll
-      for (uint32_t t = 0; t < T; ++t) {
-        gen[t] = 0;
-        const bool won =
-            detail::chain_block_release(memory, chain, chain.inst.reuse, sm, gc[t], r[t], chunk[t], page[t], defer, gen[t]);
-        ec[t] = won ? gc[t] : kMarker;
+          for (uint32_t t = 0; t < T; ++t) {
+            won[t] = false;
+            gen[t] = 0;
+          }
+          detail::chain_release_chunks_tile<T>(memory, chain, gc, chunk, page, won, gen);
+#pragma unroll
+          for (uint32_t t = 0; t < T; ++t) ec[t] = won[t] ? gc[t] : kMarker;
+          tiled = true;
+        }
+      }
+      if (!tiled) {
+#pragma unroll
+        for (uint32_t t = 0; t < T; ++t) {
+          gen[t] = 0;
+          const bool won =
+              detail::chain_block_release(memory, chain, chain.inst.reuse, sm, gc[t], r[t], chunk[t], page[t], defer, gen[t]);
+          ec[t] = won ? gc[t] : kMarker;
+        }
       }
       if constexpr (kAnyChunks) {
         if (defer) {
