@@ -248,6 +248,73 @@ __device__ inline void block_put_ring_tile(Memory& m, I& inst, uint32_t off, con
   order();  // the slots are written before the block signals the pages
 }
 
+// put_lanes_after's VA branch for T positions per lane at once (after put_prealloc of all of them): the
+// pointer-ring lookups, queue-chunk checks and slot writes each in flight together, then the
+// queue-chunk counts per position in order. Page queues hand a slot over with a load and a store (one
+// writer and one reader per position), chunk queues with a CAS on MARKER.
+template <uint32_t T, class I>
+__device__ inline void va_put_tile(Memory& m, I& inst, ReuseRing& base, ClassQueue* const (&cq)[T], const bool (&here)[T],
+                                   const uint32_t (&pos)[T], const uint32_t (&val)[T]) {
+  constexpr Kind K = I::kKind;
+  static_assert(is_va(K), "VA queues only");
+  Ctx<I> x{m, inst, base};
+  uint32_t c[T];
+#pragma unroll
+  for (uint32_t t = 0; t < T; ++t)
+    c[t] = here[t] ? ld(&ring_of(x, cq[t])[(pos[t] / kQCSpots) & (I::kQueueEntries - 1)]) : kMarker;
+  uint32_t vs[T];
+#pragma unroll
+  for (uint32_t t = 0; t < T; ++t) {
+    if (here[t] && (c[t] == kMarker || is_qc_tag(c[t]))) c[t] = va_wait_qc(x, cq[t], pos[t]);  // pre-allocation in flight
+    vs[t] = here[t] && chunk_ok(m, c[t]) ? ld(&hdr<QueueChunkHdr>(m, c[t])->virtual_start) : 0u;
+  }
+  uint32_t* slot[T];
+  uint32_t cur[T];
+#pragma unroll
+  for (uint32_t t = 0; t < T; ++t) {
+    if (here[t] && (!chunk_ok(m, c[t]) || vs[t] / kQCSpots != pos[t] / kQCSpots)) {
+      if (c[t] != kMarker) set_error(m, ERR_VSTART);
+      c[t] = kMarker;
+    }
+    slot[t] = c[t] != kMarker ? &qc_slots(m, c[t])[pos[t] % kQCSpots] : nullptr;
+    if constexpr (is_chunk_kind(K)) cur[t] = slot[t] ? cas(slot[t], kMarker, val[t]) : kMarker;
+    else cur[t] = slot[t] ? ld(slot[t]) : kMarker;
+  }
+#pragma unroll
+  for (uint32_t t = 0; t < T; ++t) {
+    if (!slot[t]) continue;
+    if constexpr (is_chunk_kind(K)) {
+      if (cur[t] != kMarker) {
+        Spin it(&m.error);
+        while (cas(slot[t], kMarker, val[t]) != kMarker) {
+          if (it.expired()) { OURO_TIMEOUT(m); break; }
+          backoff(it.it);
+        }
+      }
+    } else {
+      // a slot still holding the entry of the previous lap waits for its take
+      if (cur[t] == kMarker) st(slot[t], val[t]);
+      else ring_slot_put(m, slot[t], val[t]);
+    }
+  }
+  if constexpr (!is_chunk_kind(K)) order();  // plain slot stores complete before the counts
+#pragma unroll
+  for (uint32_t t = 0; t < T; ++t) {
+    const uint64_t grp = match<64>(c[t] != kMarker ? (uint64_t)c[t] : ~0ull);
+    bool reclaim = false;
+    uint32_t victim = 0;
+    if (c[t] != kMarker && (int)lane() == first(grp)) {
+      const uint32_t added = popc(grp) * 0x10001u;
+      const uint32_t old = add(&hdr<QueueChunkHdr>(m, c[t])->count, added);
+      if (qc_empty_after_enqueue(old, added)) {
+        victim = xchg(&ring_of(x, cq[t])[(pos[t] / kQCSpots) & (I::kQueueEntries - 1)], qc_tag(pos[t] / kQCSpots));
+        reclaim = chunk_ok(m, victim);
+      }
+    }
+    ring_put(m, base, reclaim, victim);
+  }
+}
+
 // Virtualized page queues (VA / VL): the tile's frees of one instance at the positions the block's
 // back_ fetch-adds gave them. Every sub-tile's queue-chunk pre-allocation runs before any put waits: a
 // put waits for the queue chunk (VL: the link to it) that the put at an earlier position allocates,
@@ -271,53 +338,10 @@ __device__ inline void block_put_pages_tile(Memory& m, I& inst, ReuseRing& base,
 #pragma unroll
   for (uint32_t t = 0; t < T; ++t) qc[t] = put_prealloc(x, cq[t], here[t], pos[t]);
   if constexpr (is_va(I::kKind)) {
-    // the T puts' lookups, slot loads and stores each in flight together (put_lanes_after's VA branch
-    // for T positions at once): pointer-ring entries, queue-chunk checks, slots
-    uint32_t c[T];
+    uint32_t val[T];
 #pragma unroll
-    for (uint32_t t = 0; t < T; ++t)
-      c[t] = here[t] ? ld(&ring_of(x, cq[t])[(pos[t] / kQCSpots) & (I::kQueueEntries - 1)]) : kMarker;
-    uint32_t vs[T];
-#pragma unroll
-    for (uint32_t t = 0; t < T; ++t) {
-      if (here[t] && (c[t] == kMarker || is_qc_tag(c[t]))) c[t] = va_wait_qc(x, cq[t], pos[t]);  // pre-allocation in flight
-      vs[t] = here[t] && chunk_ok(m, c[t]) ? ld(&hdr<QueueChunkHdr>(m, c[t])->virtual_start) : 0u;
-    }
-    uint32_t* slot[T];
-    uint32_t cur[T];
-#pragma unroll
-    for (uint32_t t = 0; t < T; ++t) {
-      if (here[t] && (!chunk_ok(m, c[t]) || vs[t] / kQCSpots != pos[t] / kQCSpots)) {
-        if (c[t] != kMarker) set_error(m, ERR_VSTART);
-        c[t] = kMarker;
-      }
-      slot[t] = c[t] != kMarker ? &qc_slots(m, c[t])[pos[t] % kQCSpots] : nullptr;
-      cur[t] = slot[t] ? ld(slot[t]) : kMarker;
-    }
-#pragma unroll
-    for (uint32_t t = 0; t < T; ++t) {
-      if (!slot[t]) continue;
-      // one writer and one reader per position, as a page ring: a slot still holding the entry of the
-      // previous lap waits for its take
-      if (cur[t] == kMarker) st(slot[t], make_index(chunk[t], page[t]));
-      else ring_slot_put(m, slot[t], make_index(chunk[t], page[t]));
-    }
-    order();  // the slots are written before the queue-chunk counts, whose last update may recycle one
-#pragma unroll
-    for (uint32_t t = 0; t < T; ++t) {
-      const uint64_t grp = match<64>(c[t] != kMarker ? (uint64_t)c[t] : ~0ull);
-      bool reclaim = false;
-      uint32_t victim = 0;
-      if (c[t] != kMarker && (int)lane() == first(grp)) {
-        const uint32_t added = popc(grp) * 0x10001u;
-        const uint32_t old = add(&hdr<QueueChunkHdr>(m, c[t])->count, added);
-        if (qc_empty_after_enqueue(old, added)) {
-          victim = xchg(&ring_of(x, cq[t])[(pos[t] / kQCSpots) & (I::kQueueEntries - 1)], qc_tag(pos[t] / kQCSpots));
-          reclaim = chunk_ok(m, victim);
-        }
-      }
-      ring_put(m, base, reclaim, victim);
-    }
+    for (uint32_t t = 0; t < T; ++t) val[t] = make_index(chunk[t], page[t]);
+    va_put_tile<T>(m, inst, base, cq, here, pos, val);
   } else {
 #pragma unroll
     for (uint32_t t = 0; t < T; ++t) put_lanes_after<64>(x, cq[t], here[t], pos[t], make_index(chunk[t], page[t]), qc[t]);
@@ -769,8 +793,12 @@ __device__ inline void block_enqueue_chunks(Memory& m, I& inst, ReuseRing& base,
   }
 #pragma unroll
   for (uint32_t t = 0; t < T; ++t) qc[t] = put_prealloc(x, cq[t], here[t], pos[t]);
+  if constexpr (is_va(I::kKind)) {
+    va_put_tile<T>(m, inst, base, cq, here, pos, chunk);
+  } else {
 #pragma unroll
-  for (uint32_t t = 0; t < T; ++t) put_lanes_after<64>(x, cq[t], here[t], pos[t], chunk[t], qc[t]);
+    for (uint32_t t = 0; t < T; ++t) put_lanes_after<64>(x, cq[t], here[t], pos[t], chunk[t], qc[t]);
+  }
   order();
 #pragma unroll
   for (uint32_t t = 0; t < T; ++t)
