@@ -64,8 +64,8 @@ def test_persistent_block_aggregated_invariants(kind, block, tile):
 def test_virtual_page_queue_tiles_at_1m(kind, block, tile):
     # 1M requests on one pool after rounds at one request per thread: with several requests per
     # thread, frees into VA / VL page queues must not wait on a queue chunk that another workgroup
-    # only pre-allocates in a later sub-tile (each sub-tile reserves its positions right before its
-    # puts), so no bounded wait may expire
+    # only pre-allocates later (every sub-tile's pre-allocations run before any put of the tile), so
+    # no bounded wait may expire
     print(f"virtual page queue tiles {kind} block={block} tile={tile}", flush=True)
     sizes = ob.gen_sizes(42, 1 << 20, "uniform")
     with ob.MemoryManager(kind, 4 << 30) as mm:
@@ -74,6 +74,22 @@ def test_virtual_page_queue_tiles_at_1m(kind, block, tile):
         res = mm.run_batch(sizes, rounds=4, warmup=1, agg=0, block=block, graph=True, check=True, tile=tile)
         assert_clean(res)
         assert mm.check_quiescent().violations == 0
+
+
+@pytest.mark.parametrize("kind", ["VAPQ", "VACQ", "VLPQ", "VLCQ", "VAPQ-c16k-q8"])
+def test_virtualized_tiles_over_many_rounds(kind):
+    # twelve 1M-request rounds of the tiled batched form: the tile-wide puts and takes recycle queue
+    # chunks round after round (their counts and pointer-ring entries updated sub-tile by sub-tile),
+    # with every round's live set checked and the queues quiescent at the end
+    print(f"virtualized tiles over rounds {kind}", flush=True)
+    sizes = ob.gen_sizes(7, 1 << 20, "uniform")
+    with ob.MemoryManager(kind, 4 << 30) as mm:
+        res = mm.run_batch(sizes, rounds=12, warmup=1, agg=0, block=1024, graph=True, check=True, tile=4)
+        assert_clean(res)
+        chk = mm.check_quiescent()
+        assert chk.violations == 0, chk.message.decode()
+        st = mm.query()
+        assert st.error == 0, ob.decode_errors(st.error)
 
 
 def test_kernel_only_timing_rounds():
