@@ -1571,9 +1571,12 @@ __device__ __forceinline__ uint32_t batch_at(uint32_t v, uint32_t b) {
 // cursor folded back into the window at some arbitrary position would leave old live chunks behind
 // and let the window grow round after round. A cursor that ran past back_ (claims ahead of the
 // chunks being queued) folds back into the window and keeps its value: moving it back would send
-// the claims after it over positions already consumed.
-constexpr uint32_t kRedrawAfter = 64;
-constexpr uint32_t kMaxRedraws = 4;
+// the claims after it over positions already consumed. The bounds were measured on 4M-request rounds
+// with half the blocks held (VACQ): 64 positions and 4 redraws let the spill run on (95 / 140 M ops/s
+// at W = 1 / 64), 24 and 12 cut it (231 / 610); 16 positions made the standard ring's late claims
+// redraw over its tombstoned stretches (CQ W = 64 mallocs 3x slower on 1M mixed requests).
+constexpr uint32_t kRedrawAfter = 24;
+constexpr uint32_t kMaxRedraws = 12;
 
 __device__ __forceinline__ uint32_t fold_start(ClassQueue* cq, uint32_t h) {
   const uint32_t f = ld(&cq->front), b = ld(&cq->back);

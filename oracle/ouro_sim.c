@@ -35,8 +35,8 @@
                                  thread of a 64-thread workgroup */
 #define COMPACT_STEPS 256u    /* CAS steps per call of the per-lane compaction */
 #define WAVE_COMPACT_STEPS 8192u /* front moves per call of the wave-wide compaction */
-#define REDRAW_AFTER 64u      /* positions a claim passes without a page before it draws a new start */
-#define MAX_REDRAWS 4u
+#define REDRAW_AFTER 24u      /* positions a claim passes without a page before it draws a new start */
+#define MAX_REDRAWS 12u
 #define RELOC_SLACK 64u       /* front relocation: window longer than 2 x queued chunks + this */
 #define MAX_RELOCATE 16u
 
