@@ -548,3 +548,23 @@ def test_preload_chunk_kinds(kind, first):
     assert (offs == np.array([first * STRIDE + 128 + 128 * (24 + l) for l in range(40)], np.uint64)).all()
     st = s.state()
     assert st.error == 0 and st.sem_count[3] == 0 and st.queue_count[3] == 0
+
+
+def test_oracle_constants_match_the_device():
+    # the oracle restates the device's scan, compaction and relocation bounds: a bound changed on one
+    # side only changes which chunk a claim finds first, which only a GPU trace that reaches the bound
+    # would show. Read both sources and compare the pairs.
+    import re
+    from pathlib import Path
+    root = Path(__file__).resolve().parents[1]
+    dev = (root / "include/ouro/detail/instance.hpp").read_text() + (root / "include/ouro/detail/layout.hpp").read_text()
+    sim = (root / "oracle/ouro_sim.c").read_text()
+    pairs = {"kPeek": "PEEK", "kCompactSteps": "COMPACT_STEPS", "kWaveCompactSteps": "WAVE_COMPACT_STEPS",
+             "kRedrawAfter": "REDRAW_AFTER", "kMaxRedraws": "MAX_REDRAWS", "kRelocSlack": "RELOC_SLACK",
+             "kMaxRelocate": "MAX_RELOCATE", "kOldHold": "OLD_HOLD", "kQCReserve": "QC_RESERVE",
+             "kVAEntries": "VA_ENTRIES", "kReuseEntries": "REUSE_ENTRIES"}
+    for d, o in pairs.items():
+        dv = re.search(rf"constexpr uint32_t {d} = (\d+);", dev)
+        ov = re.search(rf"#define {o} (\d+)u", sim)
+        assert dv and ov, (d, o)
+        assert int(dv.group(1)) == int(ov.group(1)), (d, dv.group(1), o, ov.group(1))
