@@ -612,11 +612,10 @@ __device__ inline void compact(Ctx<I>& x, ClassQueue* cq) {
 // retries from the new front_ (a few times at most).
 constexpr uint32_t kWaveCompactSteps = 8192;
 
-template <class I>
+template <uint32_t kPer = 16, class I>  // kPer: positions per lane and round
 __device__ inline bool compact_wave_pass(Ctx<I>& x, ClassQueue* cq) {
   constexpr Kind K = I::kKind;
   static_assert(is_chunk_kind(K), "page queues dequeue strictly at front_");
-  constexpr uint32_t kPer = 16;  // positions per lane and round
   const uint64_t act = active();
   const int L = first(act);
   const uint32_t nact = popc(act), me = rank_in(act);
@@ -1472,12 +1471,12 @@ __device__ inline bool relocate_front(Ctx<I>& x, ClassQueue* cq) {
 // Wave-wide compaction of a chunk queue's front_ (compact_wave_pass), with front relocation; a pass
 // that stopped at its step limit inside a long dead stretch is followed by another (the same budget
 // of kMaxRelocate more passes).
-template <class I>
+template <uint32_t kPer = 16, class I>
 __device__ inline void compact_wave(Ctx<I>& x, ClassQueue* cq) {
-  bool more = compact_wave_pass(x, cq);
+  bool more = compact_wave_pass<kPer>(x, cq);
   for (uint32_t r = 0; r < kMaxRelocate; ++r) {
     if (!more && !relocate_front(x, cq)) break;
-    more = compact_wave_pass(x, cq);
+    more = compact_wave_pass<kPer>(x, cq);
   }
 }
 
