@@ -628,9 +628,10 @@ __device__ inline void block_claim_chunks(Memory& m, I& inst, ReuseRing& base, u
     // front_ over the leading positions this claim emptied (one wave-wide compaction per claim). A
     // standard ring's compaction loads, passes and resets every dead position (a VA / VL one passes an
     // emptied queue chunk in one move), and after the block claims of a round it is what the slowest
-    // waves wait on: 32 positions per lane and round there (measured: CQ tile 1 malloc 0.84 -> 0.69 ms,
-    // tile 4 1.12 -> 0.93 ms; the per-thread paths and the VA / VL kinds measured slower with 32).
-    if (ballot(any_removed)) compact_wave<is_std(K) ? 32u : 16u>(x, cq);
+    // waves wait on: 64 positions per lane and round there (CQ malloc, 1M mixed requests, tiles 1 / 2 /
+    // 4: 0.84 / 0.76 / 1.12 ms with 16, 0.73 / 0.75 / 0.94 with 32, 0.82 / 0.62 / 0.71 with 64; the
+    // per-thread paths and the VA / VL kinds measured slower with wider rounds).
+    if (ballot(any_removed)) compact_wave<is_std(K) ? 64u : 16u>(x, cq);
     if (l == 0) {
       sm.base[gc] = start;
       sm.served[gc] = assigned;
